@@ -1,0 +1,19 @@
+"""grasph_amd — MI355X (gfx950) particle step for Grasph.jl's SPH model.
+
+Host mirror of the reference's Julia API (particle systems, kernels, pair
+functors, state updaters, ghosts, boundary wrappers, SystemInteraction,
+LeapFrogTimeIntegrator, time_integrate!) over the C ABI of
+include/grasph_hip.h, whose HIP kernels run the step on the GPU.
+"""
+from . import _abi  # noqa: F401
+from .model import (  # noqa: F401
+    BasicParticleSystem, CauchyFluidPfn, CubicSplineKernel, DynamicBoundarySystem,
+    ElastoPlasticParticleSystem, ElastoPlasticStressUpdater, FluidParticleSystem, FluidPfn,
+    GhostCopier, GhostEntry, GhostParticleSystem, HouseholderReflect, LeapFrogTimeIntegrator,
+    LinearEOSUpdater, NeighborCounterPfn, StaticBoundarySystem, StrainRatePfn,
+    StrainRateVorticityPfn, StressParticleSystem, SystemInteraction, TaitEOSUpdater,
+    ViscoPlasticMCStressUpdater, WenlandC2Kernel, ZeroFieldUpdater)
+from .scene import DeviceScene  # noqa: F401
+from .integration import (  # noqa: F401
+    Session, create_grid, sort_particles, sweep, time_integrate, update_state)
+from . import configs  # noqa: F401

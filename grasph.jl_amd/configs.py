@@ -1,0 +1,183 @@
+"""Synthetic, seeded scenes of the benchmark workloads.
+
+Built with the same object model a reference user writes in a script
+(patterns of dambreak.jl, DambreakWall.jl:109-171, EP_ColumnCollapse.jl,
+Trapdoor.jl:165-176, bubble.jl:94-98).  Lattice particles sit at (i+½)dx.
+Every builder takes the spacing as a parameter so the test-suite can run the
+same scene at a size the CPU oracle finishes in seconds.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import model as M
+
+G = 9.81
+
+
+def _lattice(nx, ny, dx, nz=None, origin=(0.0, 0.0, 0.0)):
+    if nz is None:
+        i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+        return np.stack([(i.ravel() + 0.5) * dx + origin[0],
+                         (j.ravel() + 0.5) * dx + origin[1]], axis=1)
+    i, j, k = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    return np.stack([(i.ravel() + 0.5) * dx + origin[0], (j.ravel() + 0.5) * dx + origin[1],
+                     (k.ravel() + 0.5) * dx + origin[2]], axis=1)
+
+
+def _hydrostatic_rho(height_above, rho0, c0, gamma=7.0):
+    """Tait inverse for p = ρ0·g·depth (bubble.jl:94-98)."""
+    p = height_above * rho0 * G
+    return (p * gamma / (c0 * c0 * rho0) + 1.0) ** (1.0 / gamma) * rho0
+
+
+def _boundary(name, pts, mass, c, normal, point, beta=3.0):
+    inner = M.BasicParticleSystem(name, len(pts), pts.shape[1], mass, c)
+    inner.x[:] = pts
+    inner.rho[:] = 1000.0
+    return inner, M.DynamicBoundarySystem(inner, normal, point, beta)
+
+
+def _round(v):
+    return int(round(v))
+
+
+def dambreak_2d(dx=0.01, water=(1.0, 2.0), tank=(4.0, 3.0), layers=3, alpha=0.1,
+                h_fac=1.3, cfl=0.2):
+    """2D weakly-compressible dam break with 3-layer dynamic walls
+    (floor, left, right): Wendland C2 h = 1.3dx, Tait ρ0 = 1000, γ = 7,
+    c0 = 10·sqrt(2·g·H), α = 0.1, gravity −9.81, hydrostatic ρ, v = 0."""
+    rho0 = 1000.0
+    H = water[1]
+    c0 = 10.0 * math.sqrt(2.0 * G * H)
+    h = h_fac * dx
+    nwx, nwy = _round(water[0] / dx), _round(water[1] / dx)
+    ntx, nty = _round(tank[0] / dx), _round(tank[1] / dx)
+    mass = rho0 * dx * dx
+    fluid = M.FluidParticleSystem("fluid", nwx * nwy, 2, mass, c0, source_v=[0.0, -G],
+                                  state_updater=M.TaitEOSUpdater(rho0))
+    fluid.x[:] = _lattice(nwx, nwy, dx)
+    fluid.rho[:] = _hydrostatic_rho(H - fluid.x[:, 1], rho0, c0)
+    L = layers
+    fl = np.array([((i + 0.5) * dx, -(l + 0.5) * dx) for l in range(L) for i in range(-L, ntx + L)])
+    le = np.array([(-(l + 0.5) * dx, (j + 0.5) * dx) for l in range(L) for j in range(nty)])
+    ri = np.array([(tank[0] + (l + 0.5) * dx, (j + 0.5) * dx) for l in range(L) for j in range(nty)])
+    floor, floor_w = _boundary("floor", fl, mass, c0, (0.0, 1.0), (0.0, 0.0))
+    left, left_w = _boundary("left", le, mass, c0, (1.0, 0.0), (0.0, 0.0))
+    right, right_w = _boundary("right", ri, mass, c0, (-1.0, 0.0), (tank[0], 0.0))
+    kern = M.WenlandC2Kernel(h, ndims=2)
+    pfn = M.FluidPfn(alpha, 0.0, h)
+    ints = [M.SystemInteraction(kern, pfn, fluid)] + [
+        M.SystemInteraction(kern, pfn, fluid, w) for w in (floor_w, left_w, right_w)]
+    lf = M.LeapFrogTimeIntegrator([fluid, floor, left, right], ints)
+    return {"integrator": lf, "fluid": fluid, "cfl": cfl, "name": "dambreak_2d",
+            "n_particles": sum(ps.n for ps in lf.systems), "fluid_n": fluid.n}
+
+
+def dambreak_3d(dx=0.006, water=(0.6, 0.6, 0.6), tank=(1.6, 0.6, 0.8), layers=3,
+                alpha=0.1, h_fac=1.3, cfl=0.2, seed=20261202):
+    """3D dam break (z vertical): water block in the x-low corner of the
+    tank, 3-layer dynamic floor and four walls (no lid), Wendland C2
+    h = 1.3dx, Tait, c0 = 10·sqrt(2·g·H), α = 0.1; fluid order shuffled by
+    `seed` before the first sort."""
+    rho0 = 1000.0
+    H = water[2]
+    c0 = 10.0 * math.sqrt(2.0 * G * H)
+    h = h_fac * dx
+    nw = [_round(w / dx) for w in water]
+    nt = [_round(t / dx) for t in tank]
+    mass = rho0 * dx ** 3
+    fluid = M.FluidParticleSystem("fluid", nw[0] * nw[1] * nw[2], 3, mass, c0,
+                                  source_v=[0.0, 0.0, -G], state_updater=M.TaitEOSUpdater(rho0))
+    x = _lattice(nw[0], nw[1], dx, nw[2])
+    perm = np.random.default_rng(seed).permutation(len(x))
+    fluid.x[:] = x[perm]
+    fluid.rho[:] = _hydrostatic_rho(H - fluid.x[:, 2], rho0, c0)
+    L = layers
+    r = np.arange
+    # floor covers the corners under the walls
+    fi, fj, fl = np.meshgrid(r(-L, nt[0] + L), r(-L, nt[1] + L), r(L), indexing="ij")
+    floor_pts = np.stack([(fi + 0.5) * dx, (fj + 0.5) * dx, -(fl + 0.5) * dx], -1).reshape(-1, 3)
+    wj, wk, wl = np.meshgrid(r(nt[1]), r(nt[2]), r(L), indexing="ij")
+    left_pts = np.stack([-(wl + 0.5) * dx, (wj + 0.5) * dx, (wk + 0.5) * dx], -1).reshape(-1, 3)
+    right_pts = np.stack([tank[0] + (wl + 0.5) * dx, (wj + 0.5) * dx, (wk + 0.5) * dx], -1).reshape(-1, 3)
+    bi, bk, bl = np.meshgrid(r(-L, nt[0] + L), r(nt[2]), r(L), indexing="ij")
+    front_pts = np.stack([(bi + 0.5) * dx, -(bl + 0.5) * dx, (bk + 0.5) * dx], -1).reshape(-1, 3)
+    back_pts = np.stack([(bi + 0.5) * dx, tank[1] + (bl + 0.5) * dx, (bk + 0.5) * dx], -1).reshape(-1, 3)
+    walls = [("floor", floor_pts, (0, 0, 1.0), (0, 0, 0)),
+             ("left", left_pts, (1.0, 0, 0), (0, 0, 0)),
+             ("right", right_pts, (-1.0, 0, 0), (tank[0], 0, 0)),
+             ("front", front_pts, (0, 1.0, 0), (0, 0, 0)),
+             ("back", back_pts, (0, -1.0, 0), (0, tank[1], 0))]
+    kern = M.WenlandC2Kernel(h, ndims=3)
+    pfn = M.FluidPfn(alpha, 0.0, h)
+    inners, ints = [], [M.SystemInteraction(kern, pfn, fluid)]
+    for name, pts, nrm, pnt in walls:
+        inner, w = _boundary(name, pts, mass, c0, nrm, pnt)
+        inners.append(inner)
+        ints.append(M.SystemInteraction(kern, pfn, fluid, w))
+    lf = M.LeapFrogTimeIntegrator([fluid] + inners, ints)
+    return {"integrator": lf, "fluid": fluid, "cfl": cfl, "name": "dambreak_3d",
+            "n_particles": sum(ps.n for ps in lf.systems), "fluid_n": fluid.n,
+            "boundary_n": sum(ps.n for ps in inners)}
+
+
+def ep_column_2d(dx=0.0025, width=0.4, height=0.2, rho0=1850.0, E=0.84e6, nu=0.3,
+                 phi_deg=19.8, psi=0.0, cohesion=0.0, alpha=0.1, beta=0.1, h_fac=1.5,
+                 cfl=0.1, k0=None, name="soil"):
+    """2D elasto-plastic soil column (plane strain, NS = 4): cubic spline
+    h = 1.5dx, Drucker–Prager (ψ = 0, c = 0), stages
+    (StrainRateVorticityPfn, CauchyFluidPfn) as in EP_ColumnCollapse.jl, and
+    free-slip ghost floor and left wall with Householder stress reflection
+    (the Trapdoor.jl:165-176 pattern; ghost cutoff 3h).  Initial stress is
+    geostatic: σyy = −ρ0·g·(H − y), σxx = σzz = K0·σyy with K0 = 1 − sin φ
+    (Jaky) unless given."""
+    phi = phi_deg * math.pi / 180.0
+    c = math.sqrt(E * (1 - nu) / (rho0 * (1 + nu) * (1 - 2 * nu)))
+    h = h_fac * dx
+    nx, ny = _round(width / dx), _round(height / dx)
+    mass = rho0 * dx * dx
+    soil = M.ElastoPlasticParticleSystem(
+        name, nx * ny, 2, 4, mass, c, source_v=[0.0, -G],
+        state_updater=(M.ZeroFieldUpdater("strain_rate", "vorticity"),
+                       M.ElastoPlasticStressUpdater(E, nu, phi, psi, cohesion)))
+    soil.x[:] = _lattice(nx, ny, dx)
+    soil.rho[:] = rho0
+    K0 = (1.0 - math.sin(phi)) if k0 is None else k0
+    syy = -rho0 * G * (height - soil.x[:, 1])
+    soil.stress[:, 0] = K0 * syy
+    soil.stress[:, 1] = syy
+    soil.stress[:, 2] = K0 * syy
+    floor_g = M.GhostParticleSystem(soil, M.GhostCopier(("stress", M.HouseholderReflect())),
+                                    name=f"{name}_floor_ghost")
+    wall_g = M.GhostParticleSystem(soil, M.GhostCopier(("stress", M.HouseholderReflect())),
+                                   name=f"{name}_wall_ghost")
+    fe = M.GhostEntry(floor_g, 3.0 * h, ((0.0, 1.0), (0.0, 0.0)))
+    we = M.GhostEntry(wall_g, 3.0 * h, ((1.0, 0.0), (0.0, 0.0)))
+    kern = M.CubicSplineKernel(h, ndims=2)
+    pfns = (M.StrainRateVorticityPfn(), M.CauchyFluidPfn(alpha, beta, h))
+    ints = [M.SystemInteraction(kern, pfns, soil),
+            M.SystemInteraction(kern, pfns, soil, floor_g),
+            M.SystemInteraction(kern, pfns, soil, wall_g)]
+    lf = M.LeapFrogTimeIntegrator([soil], ints, ghosts=(fe, we))
+    return {"integrator": lf, "fluid": soil, "cfl": cfl, "name": "ep_column_2d",
+            "n_particles": soil.n, "fluid_n": soil.n}
+
+
+def ensemble_columns(n_sims=256, dx=0.0025, seed=7, aspect=(0.5, 4.0), width=(0.1, 0.4),
+                     phi_deg=(20.0, 40.0), rho=(1500.0, 2000.0)):
+    """Independent 2D granular columns with parameters drawn per simulation
+    (one PRNG stream per simulation from `seed`)."""
+    sims = []
+    for k in range(n_sims):
+        g = np.random.default_rng([seed, k])
+        a = g.uniform(*aspect)
+        w = g.uniform(*width)
+        ph = g.uniform(*phi_deg)
+        r0 = g.uniform(*rho)
+        hgt = a * w
+        sims.append(ep_column_2d(dx=dx, width=w, height=hgt, rho0=r0, phi_deg=ph,
+                                 name=f"col{k}"))
+    return sims

@@ -1,0 +1,2173 @@
+// gr_api.hip — C ABI (include/grasph_hip.h), particle pool and step planner.
+//
+// Host side of the library: registry of systems / ghosts / interactions /
+// integrators, the packed device pool, per-batch launch plans (block tables,
+// gather-source tables, updater tables) and the step sequence of
+// time_integrate! (src/TimeIntegration.jl:645-685) expressed as kernel
+// launches on one stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/grasph_hip.h"
+#include "gr_device.h"
+#include "gr_launch.h"
+
+using namespace gr;
+
+namespace {
+
+struct GrError : std::runtime_error {
+  int code;
+  GrError(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                    \
+  do {                                                                               \
+    hipError_t e_ = (x);                                                             \
+    if (e_ != hipSuccess)                                                            \
+      throw GrError(GR_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+// Owned device buffer.
+struct DBuf {
+  void *p = nullptr;
+  size_t bytes = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  void alloc(size_t b) {
+    release();
+    if (b == 0) b = 16;
+    HIPCHK(hipMalloc(&p, b));
+    bytes = b;
+  }
+  template <typename T>
+  T *as() const {
+    return (T *)p;
+  }
+  // Copy a host vector in, reusing the allocation when it is large enough
+  // (pointers baked into captured graphs stay valid), then wait for it.
+  template <typename T>
+  void upload(const std::vector<T> &v, hipStream_t s) {
+    const size_t need = std::max<size_t>(v.size() * sizeof(T), 16);
+    if (!p || bytes < need) alloc(need);
+    if (!v.empty()) {
+      HIPCHK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+      HIPCHK(hipStreamSynchronize(s));
+    }
+  }
+};
+
+struct SysH {
+  int kind = GR_SYS_BASIC, ns = 0, nv = 0;
+  int64_t n = 0, cap = 0, off = -1;
+  double mass = 0, c = 0, sv[3] = {0, 0, 0}, srho = 0;
+  std::vector<std::vector<UpdDesc>> upd;  // per stage
+  // ghost
+  int src = -1, nb = 0, bnd = -1;
+  double gcut = 0;
+  bool ex_p = false, ex_stress = false;
+  std::vector<int2> copy;  // per stage: {copy_p, stress_mode}
+};
+
+struct InterH {
+  int kk = 0;
+  double h = 0, norm = 0, cell = 0;
+  int a = -1, b = -1, wrap = GR_WRAP_NONE;
+  double wn[3] = {0, 0, 0}, wp[3] = {0, 0, 0}, beta = 0, ljcut = 0;
+  std::vector<gr_pfn_desc> pfns;
+  int carr_b = -1, carr_a = -1;  // CellArr indices (b or self; a when coupled)
+  bool grid_built = false;
+};
+
+struct IntegH {
+  std::vector<int> sys, inter, ghost;
+  double gamma = 0, h = 0, c = 0, dt = 0;
+  int nst = 0;
+};
+
+enum PK { P_SELF = PK_SELF, P_TWO = PK_TWO_SIDED, P_ONE = PK_ONE_SIDED, P_STATIC = PK_STATIC, P_DYN = PK_DYNAMIC };
+
+struct FamPlan {
+  DBuf tasks, srcs, blk;
+  int nblk = 0;
+};
+struct StagePlan {
+  DBuf blk_upd, urange, upd;
+  int nblk_upd = 0;
+  DBuf blk_gcopy, gcopy;
+  int nblk_gcopy = 0;
+  FamPlan fam[3];
+};
+
+struct Plan {
+  std::vector<int> integs;
+  // real sort
+  std::vector<int> real_list;
+  DBuf d_real_list, d_bbase_real, blk_real;
+  int nblk_real = 0, n_real = 0;
+  // ghosts
+  std::vector<int> ghost_list;
+  DBuf d_ghost_list, d_bbase_ghost, d_cand_off, blk_ghost, blk_cand;
+  int nblk_ghost = 0, nblk_cand = 0, n_ghost = 0, n_cand = 0;
+  // grids
+  std::vector<int> inters;
+  DBuf d_inter_list, blk_cells, blk_mark, carr_blocks;
+  int nblk_cells = 0, nblk_mark = 0, ncarr = 0;
+  // stages
+  std::vector<std::unique_ptr<StagePlan>> stages;
+  std::vector<std::pair<int, StepSys>> stepsys;  // per-system step parameters
+  bool do_sort = true, do_ghosts = true, do_grid = true, do_step = true;
+  // graph capture per A/B parity
+  hipGraphExec_t graph[2] = {nullptr, nullptr};
+  ~Plan() {
+    for (auto &g : graph)
+      if (g) (void)hipGraphExecDestroy(g);
+  }
+};
+
+}  // namespace
+
+struct gr_ctx {
+  int device = 0, D = 2, prec = GR_PREC_F32;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::vector<SysH> sys;
+  std::vector<InterH> inter;
+  std::vector<IntegH> integ;
+  std::vector<Bnd> bnds;
+  // pool
+  int64_t pool_cap = 0;
+  bool dirty_layout = true;
+  int nsmax = 4;
+  int rsize = 4;  // sizeof(R)
+  // swapped families: index 0/1 = A/B
+  DBuf x[2], v[2], rho[2], dvdt[2], drhodt[2], p[2], id[2], stress[2], stress_s[2], sr[2],
+      vor[2], strain[2], strainp[2];
+  int cur = 0;
+  DBuf vs, rhos, rho_h, gidx, gbnd, gnrm, gxs, gidx_s, gbnd_s;
+  // tables
+  DBuf d_sd, d_n, d_st, d_bnd, d_kmin, d_kmax, d_bbmin, d_bbmax, d_kbase, d_krange, d_flags;
+  DBuf d_grids, d_carr, d_cellsz, d_inter_sys, d_inter_carr, cells;
+  std::vector<CellArr> carr;
+  int64_t cells_cap = 0;
+  std::vector<StepSys> stepsys;
+  // sort / scan scratch
+  DBuf keys, vals, keys2, vals2, hist, cand, cand_scan, scan_bsum, bagg, carry;
+  int64_t sort_cap = 0, cand_cap = 0, cellblk_cap = 0;
+  DBuf d_pairs;
+  std::map<std::vector<int>, std::unique_ptr<Plan>> plans;
+  // cached device tables derived from the layout
+  DBuf allreal_blk;
+  int allreal_nblk = 0;
+  int64_t allreal_cap = -1;
+  size_t allreal_nsys = 0;
+  DBuf xs_pool[2], xs_gxs;
+  int64_t xs_cap = -1;
+  std::vector<char> id_written;
+  bool graph_mode = false, timing = false;
+  hipEvent_t ev[9] = {};
+  double phase_ms[8] = {0};
+  long long last_pairs[2] = {0, 0};
+  int last_pairs_integ_key = -1;
+};
+
+namespace {
+
+int fail(gr_ctx *c, int code, const std::string &m) {
+  if (c) c->err = m;
+  return code;
+}
+
+template <typename F>
+int guard(gr_ctx *c, F &&f) {
+  if (!c) return GR_ERR_ARGUMENT;
+  try {
+    f();
+    return GR_OK;
+  } catch (const GrError &e) {
+    c->err = e.what();
+    return e.code;
+  } catch (const std::exception &e) {
+    c->err = e.what();
+    return GR_ERR_STATE;
+  }
+}
+
+void need(bool cond, int code, const std::string &m) {
+  if (!cond) throw GrError(code, m);
+}
+
+// ---------------------------------------------------------------------------
+// Pool
+// ---------------------------------------------------------------------------
+
+template <typename R>
+Pool<R> pool_view(gr_ctx *c, int w) {
+  Pool<R> P;
+  const int64_t cap = c->pool_cap;
+  for (int d = 0; d < 3; ++d) {
+    P.x[d] = c->x[w].as<double>() + d * cap;
+    P.v[d] = c->v[w].as<double>() + d * cap;
+    P.vs[d] = c->vs.as<R>() + d * cap;
+    P.dvdt[d] = c->dvdt[w].as<R>() + d * cap;
+    P.vor[d] = c->vor[w].as<R>() + d * cap;
+  }
+  P.rho = c->rho[w].as<double>();
+  P.rho_h = c->rho_h.as<double>();
+  P.rhos = c->rhos.as<R>();
+  P.drhodt = c->drhodt[w].as<R>();
+  P.p = c->p[w].as<R>();
+  for (int k = 0; k < GR_MAXNS; ++k) {
+    const int kk = k < c->nsmax ? k : 0;
+    P.stress[k] = c->stress[w].as<double>() + kk * cap;
+    P.stress_s[k] = c->stress_s[w].as<R>() + kk * cap;
+    P.sr[k] = c->sr[w].as<R>() + kk * cap;
+    P.strain[k] = c->strain[w].as<double>() + kk * cap;
+    P.strainp[k] = c->strainp[w].as<double>() + kk * cap;
+  }
+  P.id = c->id[w].as<int>();
+  P.gidx = c->gidx.as<int>();
+  P.gbnd = c->gbnd.as<int>();
+  P.gnrm = c->gnrm.as<int>();
+  return P;
+}
+
+// Grow (or create) one pool array family, keeping the first old_cap entries
+// of each of its `comps` components.
+void grow(gr_ctx *c, DBuf &b, int comps, size_t esz, int64_t old_cap, int64_t new_cap) {
+  DBuf nb;
+  nb.alloc((size_t)comps * new_cap * esz);
+  HIPCHK(hipMemsetAsync(nb.p, 0, nb.bytes, c->stream));
+  if (b.p && old_cap > 0)
+    for (int k = 0; k < comps; ++k)
+      HIPCHK(hipMemcpyAsync((char *)nb.p + (size_t)k * new_cap * esz,
+                            (char *)b.p + (size_t)k * old_cap * esz, old_cap * esz,
+                            hipMemcpyDeviceToDevice, c->stream));
+  std::swap(b.p, nb.p);
+  std::swap(b.bytes, nb.bytes);
+}
+
+void commit(gr_ctx *c) {
+  if (!c->dirty_layout) return;
+  int64_t off = 0;
+  for (auto &s : c->sys) {
+    if (s.off < 0) s.off = off;
+    off = std::max(off, s.off + s.cap);
+  }
+  const int64_t need_cap = std::max<int64_t>(off, 1);
+  if (need_cap > c->pool_cap) {
+    const int64_t old = c->pool_cap;
+    const int64_t nc = std::max(need_cap, old + old / 2);
+    const size_t R = c->rsize;
+    for (int w = 0; w < 2; ++w) {
+      grow(c, c->x[w], 3, 8, old, nc);
+      grow(c, c->v[w], 3, 8, old, nc);
+      grow(c, c->rho[w], 1, 8, old, nc);
+      grow(c, c->dvdt[w], 3, R, old, nc);
+      grow(c, c->drhodt[w], 1, R, old, nc);
+      grow(c, c->p[w], 1, R, old, nc);
+      grow(c, c->id[w], 1, 4, old, nc);
+      grow(c, c->stress[w], c->nsmax, 8, old, nc);
+      grow(c, c->stress_s[w], c->nsmax, R, old, nc);
+      grow(c, c->sr[w], c->nsmax, R, old, nc);
+      grow(c, c->vor[w], 3, R, old, nc);
+      grow(c, c->strain[w], c->nsmax, 8, old, nc);
+      grow(c, c->strainp[w], c->nsmax, 8, old, nc);
+    }
+    grow(c, c->vs, 3, R, old, nc);
+    grow(c, c->rhos, 1, R, old, nc);
+    grow(c, c->rho_h, 1, 8, old, nc);
+    grow(c, c->gidx, 1, 4, old, nc);
+    grow(c, c->gbnd, 1, 4, old, nc);
+    grow(c, c->gnrm, 1, 4, old, nc);
+    grow(c, c->gxs, 3, 8, old, nc);
+    grow(c, c->gidx_s, 1, 4, old, nc);
+    grow(c, c->gbnd_s, 1, 4, old, nc);
+    c->pool_cap = nc;
+  }
+  c->plans.clear();  // block tables and captured graphs refer to the layout
+  // id = 1..n for systems laid out for the first time (src/Particles.jl:92)
+  c->id_written.resize(c->sys.size(), 0);
+  for (size_t i = 0; i < c->sys.size(); ++i) {
+    if (c->id_written[i] || c->sys[i].kind == GR_SYS_GHOST) continue;
+    std::vector<int> ids(c->sys[i].n);
+    for (int64_t k = 0; k < c->sys[i].n; ++k) ids[k] = (int)(k + 1);
+    HIPCHK(hipMemcpyAsync(c->id[c->cur].as<int>() + c->sys[i].off, ids.data(), ids.size() * 4,
+                          hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->id_written[i] = 1;
+  }
+  // descriptor tables
+  const int ns = (int)c->sys.size();
+  std::vector<SysDesc> sd(ns);
+  std::vector<int> nn(ns);
+  for (int i = 0; i < ns; ++i) {
+    const SysH &s = c->sys[i];
+    SysDesc &d = sd[i];
+    memset(&d, 0, sizeof(d));
+    d.off = s.off;
+    d.cap = (int32_t)s.cap;
+    d.kind = s.kind;
+    d.ns = s.ns;
+    d.nv = s.nv;
+    d.src = s.src;
+    d.nb = s.nb;
+    d.bnd = s.bnd;
+    d.gcut = s.gcut;
+    d.mass = s.mass;
+    d.c = s.c;
+    for (int k = 0; k < 3; ++k) d.sv[k] = s.sv[k];
+    d.srho = s.srho;
+    nn[i] = s.kind == GR_SYS_GHOST ? 0 : (int)s.n;
+  }
+  // keep device ghost counts across commits
+  std::vector<int> oldn;
+  if (c->d_n.p && c->d_n.bytes >= sizeof(int)) {
+    oldn.resize(c->d_n.bytes / sizeof(int));
+    HIPCHK(hipMemcpyAsync(oldn.data(), c->d_n.p, oldn.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < ns && i < (int)oldn.size(); ++i)
+      if (c->sys[i].kind == GR_SYS_GHOST) nn[i] = oldn[i];
+  }
+  c->d_sd.upload(sd, c->stream);
+  c->d_n.upload(nn, c->stream);
+  c->stepsys.resize(ns);
+  c->d_st.upload(c->stepsys, c->stream);
+  std::vector<Bnd> bnds = c->bnds;
+  if (bnds.empty()) bnds.resize(1);
+  c->d_bnd.upload(bnds, c->stream);
+  std::vector<int> z(ns * 3, 0);
+  std::vector<long long> zl(ns * 3, 0);
+  c->d_kmin.upload(z, c->stream);
+  c->d_kmax.upload(z, c->stream);
+  c->d_bbmin.upload(zl, c->stream);
+  c->d_bbmax.upload(zl, c->stream);
+  std::vector<unsigned> zu(ns * 3, 0);
+  c->d_kbase.upload(zu, c->stream);
+  c->d_krange.upload(zu, c->stream);
+  if (!c->d_flags.p) {
+    std::vector<int> f(4, 0);
+    c->d_flags.upload(f, c->stream);
+  }
+  if (!c->d_pairs.p) {
+    std::vector<unsigned long long> z2(2, 0);
+    c->d_pairs.upload(z2, c->stream);
+  }
+  // sort / candidate scratch
+  int64_t nreal = 0, nghost = 0, ncand = 0;
+  for (auto &s : c->sys) {
+    if (s.kind == GR_SYS_GHOST) {
+      nghost += s.cap;
+      ncand += s.cap;
+    } else {
+      nreal += s.cap;
+    }
+  }
+  const int64_t sc = std::max<int64_t>(std::max(nreal, nghost), 1);
+  if (sc > c->sort_cap) {
+    c->keys.alloc(sc * 4);
+    c->vals.alloc(sc * 4);
+    c->keys2.alloc(sc * 4);
+    c->vals2.alloc(sc * 4);
+    c->hist.alloc((size_t)radix_hist_entries((int)sc) * 4);
+    c->sort_cap = sc;
+  }
+  if (ncand + 1 > c->cand_cap) {
+    c->cand.alloc((ncand + 1) * 4);
+    c->cand_scan.alloc((ncand + 2) * 4);
+    c->scan_bsum.alloc((size_t)(scan_block_count((int)ncand + 1) + 1) * 4);
+    c->cand_cap = ncand + 1;
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->dirty_layout = false;
+}
+
+void upload_stepsys(gr_ctx *c) { c->d_st.upload(c->stepsys, c->stream); }
+
+// ---------------------------------------------------------------------------
+// Field transfer (host AoS doubles <-> device SoA)
+// ---------------------------------------------------------------------------
+
+int ncomp(gr_ctx *c, const SysH &s, int field) {
+  const int D = c->D;
+  const int NS = s.kind == GR_SYS_GHOST ? c->sys[s.src].ns : s.ns;
+  switch (field) {
+    case GR_F_X:
+    case GR_F_V:
+    case GR_F_DVDT:
+    case GR_F_V_ADJUSTMENT:
+    case GR_F_NORMALS:
+      return D;
+    case GR_F_RHO:
+    case GR_F_DRHODT:
+    case GR_F_P:
+      return 1;
+    case GR_F_STRESS:
+    case GR_F_STRAIN_RATE:
+    case GR_F_STRAIN:
+    case GR_F_STRAIN_P:
+      return NS;
+    case GR_F_VORTICITY:
+      return s.nv;
+  }
+  return 0;
+}
+
+bool has_field(const SysH &s, int field) {
+  if (s.kind == GR_SYS_GHOST) {
+    switch (field) {
+      case GR_F_X:
+      case GR_F_V:
+      case GR_F_RHO:
+      case GR_F_NORMALS:
+        return true;
+      case GR_F_P:
+        return s.ex_p;
+      case GR_F_STRESS:
+        return s.ex_stress;
+    }
+    return false;
+  }
+  switch (field) {
+    case GR_F_X:
+    case GR_F_V:
+    case GR_F_RHO:
+    case GR_F_DVDT:
+    case GR_F_DRHODT:
+    case GR_F_V_ADJUSTMENT:
+      return true;
+    case GR_F_P:
+      return s.kind >= GR_SYS_FLUID;
+    case GR_F_STRESS:
+    case GR_F_STRAIN_RATE:
+      return s.kind >= GR_SYS_STRESS;
+    case GR_F_VORTICITY:
+    case GR_F_STRAIN:
+    case GR_F_STRAIN_P:
+      return s.kind == GR_SYS_EP;
+  }
+  return false;
+}
+
+// Component pointers (element size) of a field in the current pool.
+struct FieldRef {
+  std::vector<char *> comp;
+  size_t esz;
+};
+
+FieldRef field_ref(gr_ctx *c, int field, bool shadow_only) {
+  const int w = c->cur;
+  const int64_t cap = c->pool_cap;
+  const size_t R = c->rsize;
+  FieldRef f;
+  auto add = [&](DBuf &b, int comps, size_t esz) {
+    f.esz = esz;
+    for (int k = 0; k < comps; ++k) f.comp.push_back((char *)b.p + (size_t)k * cap * esz);
+  };
+  (void)shadow_only;
+  switch (field) {
+    case GR_F_X: add(c->x[w], c->D, 8); break;
+    case GR_F_V: add(c->v[w], c->D, 8); break;
+    case GR_F_RHO: add(c->rho[w], 1, 8); break;
+    case GR_F_DVDT: add(c->dvdt[w], c->D, R); break;
+    case GR_F_DRHODT: add(c->drhodt[w], 1, R); break;
+    case GR_F_P: add(c->p[w], 1, R); break;
+    case GR_F_STRESS: add(c->stress[w], c->nsmax, 8); break;
+    case GR_F_STRAIN_RATE: add(c->sr[w], c->nsmax, R); break;
+    case GR_F_VORTICITY: add(c->vor[w], 3, R); break;
+    case GR_F_STRAIN: add(c->strain[w], c->nsmax, 8); break;
+    case GR_F_STRAIN_P: add(c->strainp[w], c->nsmax, 8); break;
+  }
+  return f;
+}
+
+void copy_comp_to_dev(gr_ctx *c, char *dst, size_t esz, int64_t off, const double *src,
+                      int64_t n, int stride, int k) {
+  if (n <= 0) return;
+  if (esz == 8) {
+    std::vector<double> t(n);
+    for (int64_t i = 0; i < n; ++i) t[i] = src[i * stride + k];
+    HIPCHK(hipMemcpyAsync(dst + off * 8, t.data(), n * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  } else {
+    std::vector<float> t(n);
+    for (int64_t i = 0; i < n; ++i) t[i] = (float)src[i * stride + k];
+    HIPCHK(hipMemcpyAsync(dst + off * 4, t.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+}
+
+void copy_comp_from_dev(gr_ctx *c, const char *src, size_t esz, int64_t off, double *dst,
+                        int64_t n, int stride, int k) {
+  if (n <= 0) return;
+  if (esz == 8) {
+    std::vector<double> t(n);
+    HIPCHK(hipMemcpyAsync(t.data(), src + off * 8, n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int64_t i = 0; i < n; ++i) dst[i * stride + k] = t[i];
+  } else if (esz == 4) {
+    std::vector<float> t(n);
+    HIPCHK(hipMemcpyAsync(t.data(), src + off * 4, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int64_t i = 0; i < n; ++i) dst[i * stride + k] = (double)t[i];
+  }
+}
+
+std::vector<int> read_ints(gr_ctx *c, const DBuf &b, int64_t off, int64_t n) {
+  std::vector<int> t(std::max<int64_t>(n, 0));
+  if (n > 0) {
+    HIPCHK(hipMemcpyAsync(t.data(), (const char *)b.p + off * 4, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return t;
+}
+
+int64_t live_n(gr_ctx *c, int sysid) {
+  const SysH &s = c->sys[sysid];
+  if (s.kind != GR_SYS_GHOST) return s.n;
+  commit(c);
+  auto v = read_ints(c, c->d_n, sysid, 1);
+  return v[0];
+}
+
+// ---------------------------------------------------------------------------
+// Planning helpers
+// ---------------------------------------------------------------------------
+
+void add_blocks(std::vector<Blk> &v, int item, int64_t count, int per) {
+  for (int64_t f = 0; f < count; f += per) v.push_back(Blk{item, (int32_t)f});
+}
+
+// Functor dispatch on the partner's type, as the oracle's classify().
+int classify(gr_ctx *c, const InterH &in, int pfn) {
+  if (in.b < 0) return P_SELF;
+  if (in.wrap == GR_WRAP_STATIC) {
+    if (pfn == GR_PFN_SR || pfn == GR_PFN_SRV || pfn == GR_PFN_COUNT) return P_ONE;
+    return P_STATIC;
+  }
+  if (in.wrap == GR_WRAP_DYNAMIC) return pfn == GR_PFN_COUNT ? P_ONE : P_DYN;
+  const SysH &b = c->sys[in.b];
+  const SysH &a = c->sys[in.a];
+  if (b.kind == GR_SYS_GHOST) return P_ONE;
+  switch (pfn) {
+    case GR_PFN_FLUID:
+      return (a.kind == GR_SYS_FLUID && b.kind == GR_SYS_FLUID) ? P_TWO : P_ONE;
+    case GR_PFN_CAUCHY:
+    case GR_PFN_COUNT:
+      return P_TWO;
+    default:
+      return P_ONE;
+  }
+}
+
+int family_of(int pfn) {
+  if (pfn == GR_PFN_FLUID || pfn == GR_PFN_CAUCHY) return FAM_FORCE;
+  if (pfn == GR_PFN_SR || pfn == GR_PFN_SRV) return FAM_STRAIN;
+  if (pfn == GR_PFN_COUNT) return FAM_COUNT;
+  return -1;
+}
+
+double src_mass(gr_ctx *c, int sysid) {
+  const SysH &s = c->sys[sysid];
+  return s.kind == GR_SYS_GHOST ? c->sys[s.src].mass : s.mass;
+}
+double src_c(gr_ctx *c, int sysid) {
+  const SysH &s = c->sys[sysid];
+  return s.kind == GR_SYS_GHOST ? c->sys[s.src].c : s.c;
+}
+
+SrcDesc make_src(gr_ctx *c, int it, int target, bool b_side, int pk) {
+  const InterH &in = c->inter[it];
+  (void)target;
+  SrcDesc q;
+  memset(&q, 0, sizeof(q));
+  q.pk = pk;
+  q.grid = it;
+  const int a = in.a;
+  const int partner = b_side ? a : (in.b < 0 ? a : in.b);
+  q.partner = partner;
+  q.carr = b_side ? in.carr_a : in.carr_b;
+  q.kk = in.kk;
+  q.h = in.h;
+  q.norm = in.norm;
+  q.cell = in.cell;
+  q.cutoff2 = in.cell * in.cell;
+  q.wbeta = in.beta;
+  q.ljcut = in.ljcut;
+  for (int d = 0; d < 3; ++d) {
+    q.wn[d] = in.wn[d];
+    q.wp[d] = in.wp[d];
+  }
+  const int t = b_side ? in.b : a;
+  const double ct = src_c(c, t), cq = src_c(c, partner), ca = c->sys[a].c;
+  switch (pk) {
+    case P_SELF:
+      q.mass_q = c->sys[a].mass;
+      break;
+    case P_TWO:
+      q.mass_q = src_mass(c, partner);
+      break;
+    case P_ONE:
+      q.mass_q = src_mass(c, partner);
+      break;
+    case P_STATIC:
+      q.mass_q = src_mass(c, partner);
+      break;
+    case P_DYN:
+      q.mass_q = c->sys[a].mass;
+      break;
+  }
+  q.c_t_piv = q.c_q_piv = q.c_t_psi = q.c_q_psi = ca;
+  if (pk == P_TWO) {
+    q.c_t_piv = q.c_t_psi = ct;
+    q.c_q_piv = q.c_q_psi = cq;
+  }
+  return q;
+}
+
+void set_pfn(SrcDesc &q, const gr_pfn_desc &f, int pk, gr_ctx *c, const InterH &in) {
+  q.pfn = f.kind;
+  q.sigma = f.sigma;
+  q.alpha = f.alpha;
+  q.beta = f.beta;
+  q.hf = f.h;
+  q.has_delta = std::isnan(f.delta) ? 0 : 1;
+  q.delta = q.has_delta ? f.delta : 0.0;
+  q.has_eps = (!std::isnan(f.epsilon) && pk == P_TWO && f.kind == GR_PFN_FLUID) ? 1 : 0;
+  q.eps = q.has_eps ? f.epsilon : 0.0;
+  // CauchyFluidPfn with a ghost / virtual partner uses the partner's c in
+  // the artificial viscosity (src/PairwiseFunctors.jl:293)
+  if (f.kind == GR_PFN_CAUCHY && pk == P_ONE && in.b >= 0 &&
+      c->sys[in.b].kind == GR_SYS_GHOST)
+    q.c_q_piv = src_c(c, in.b);
+}
+
+// Cell capacity of an interaction: grid of the union bounding box of the
+// given systems, padded by the ghost cutoffs and doubled per dimension.
+int64_t cell_capacity(gr_ctx *c, const InterH &in, const double *mn, const double *mx) {
+  double pad = 0.0;
+  for (auto &s : c->sys)
+    if (s.kind == GR_SYS_GHOST) pad = std::max(pad, s.gcut);
+  int64_t cells = 1;
+  for (int d = 0; d < c->D; ++d) {
+    double ext = (mx[d] - mn[d]) + 2 * pad + 6 * in.cell;
+    double n = std::floor(ext / in.cell) + 3;
+    n = n * 1.6 + 4;
+    if (!(n < 4e6)) n = 4e6;
+    cells *= (int64_t)n;
+  }
+  return std::min<int64_t>(cells + 1, (int64_t)1 << 30);
+}
+
+// Bounding boxes of the given real systems (device reduction, host readback).
+void host_bbox(gr_ctx *c, const std::vector<int> &systems, std::vector<double> &mn,
+               std::vector<double> &mx) {
+  commit(c);
+  std::vector<int> list;
+  std::vector<Blk> blk;
+  for (int s : systems) {
+    list.push_back(s);
+    add_blocks(blk, s, c->sys[s].cap, GR_BLOCK);
+  }
+  DBuf dl, db;
+  dl.upload(list, c->stream);
+  db.upload(blk, c->stream);
+  launch_minmax_reset(c->stream, dl.as<int>(), (int)list.size(), c->d_kmin.as<int>(),
+                      c->d_kmax.as<int>(), c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>());
+  std::vector<double *> xs(3);
+  for (int d = 0; d < 3; ++d) xs[d] = c->x[c->cur].as<double>() + d * c->pool_cap;
+  DBuf dxs;
+  dxs.upload(xs, c->stream);
+  launch_key_minmax(c->stream, dxs.as<const double *>(), c->d_sd.as<SysDesc>(), c->d_n.as<int>(),
+                    c->d_st.as<StepSys>(), db.as<Blk>(), (int)blk.size(), c->D,
+                    c->d_kmin.as<int>(), c->d_kmax.as<int>(), c->d_bbmin.as<long long>(),
+                    c->d_bbmax.as<long long>(), 0);
+  const int ns = (int)c->sys.size();
+  std::vector<long long> bmn(ns * 3), bmx(ns * 3);
+  HIPCHK(hipMemcpyAsync(bmn.data(), c->d_bbmin.p, ns * 3 * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(bmx.data(), c->d_bbmax.p, ns * 3 * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  auto dof = [](long long i) {
+    long long u = i < 0 ? (i ^ 0x7FFFFFFFFFFFFFFFLL) : i;
+    double d;
+    memcpy(&d, &u, 8);
+    return d;
+  };
+  mn.assign(3, INFINITY);
+  mx.assign(3, -INFINITY);
+  for (int s : systems)
+    for (int d = 0; d < c->D; ++d) {
+      if (c->sys[s].n == 0) continue;
+      mn[d] = std::min(mn[d], dof(bmn[s * 3 + d]));
+      mx[d] = std::max(mx[d], dof(bmx[s * 3 + d]));
+    }
+}
+
+// Allocate cell arrays of the interactions that have none yet.
+void ensure_cells(gr_ctx *c, const std::vector<int> &inters, const std::vector<int> &union_sys) {
+  bool any = false;
+  for (int it : inters)
+    if (c->inter[it].carr_b < 0) any = true;
+  if (!any) return;
+  std::vector<int> reals;
+  for (int s : union_sys)
+    if (c->sys[s].kind != GR_SYS_GHOST) reals.push_back(s);
+  for (int it : inters) {
+    InterH &in = c->inter[it];
+    if (in.carr_b >= 0) continue;
+    std::vector<int> own;
+    for (int s : reals) own.push_back(s);
+    if (own.empty()) {
+      own.push_back(in.a);
+      if (in.b >= 0 && c->sys[in.b].kind != GR_SYS_GHOST) own.push_back(in.b);
+    }
+    std::vector<double> mn, mx;
+    host_bbox(c, own, mn, mx);
+    const int64_t cap = cell_capacity(c, in, mn.data(), mx.data());
+    auto mk = [&](int sysid) {
+      CellArr a;
+      a.off = c->cells_cap;
+      a.cap = (int32_t)cap;
+      a.grid = it;
+      a.sys = sysid;
+      a.pad = 0;
+      c->cells_cap += cap;
+      c->carr.push_back(a);
+      return (int)c->carr.size() - 1;
+    };
+    if (in.b < 0) {
+      in.carr_b = mk(in.a);
+    } else {
+      in.carr_b = mk(in.b);
+      in.carr_a = mk(in.a);
+    }
+  }
+  // (re)allocate the cell pool; grids are rebuilt every step, so no copy
+  c->cells.alloc(std::max<int64_t>(c->cells_cap, 1) * 4);
+  c->d_carr.upload(c->carr, c->stream);
+  const int ni = (int)c->inter.size();
+  std::vector<int2> isys(ni), icarr(ni);
+  std::vector<double> cellsz(ni);
+  for (int i = 0; i < ni; ++i) {
+    isys[i] = make_int2(c->inter[i].a, c->inter[i].b);
+    icarr[i] = make_int2(c->inter[i].carr_b, c->inter[i].carr_a);
+    cellsz[i] = c->inter[i].cell;
+  }
+  c->d_inter_sys.upload(isys, c->stream);
+  c->d_inter_carr.upload(icarr, c->stream);
+  c->d_cellsz.upload(cellsz, c->stream);
+  std::vector<Grid> g(std::max(ni, 1));
+  memset(g.data(), 0, g.size() * sizeof(Grid));
+  if (!c->d_grids.p || c->d_grids.bytes < g.size() * sizeof(Grid)) c->d_grids.upload(g, c->stream);
+  for (auto &kv : c->plans)
+    for (auto &gg : kv.second->graph)
+      if (gg) {
+        (void)hipGraphExecDestroy(gg);
+        gg = nullptr;
+      }
+}
+
+// Grid plan (block tables over cell arrays and marked particles).
+void plan_grids(gr_ctx *c, Plan &P, const std::vector<int> &inters) {
+  P.inters = inters;
+  P.d_inter_list.upload(inters, c->stream);
+  std::vector<Blk> cb, mb;
+  std::vector<int2> cblocks;
+  for (int it : inters) {
+    const InterH &in = c->inter[it];
+    for (int ca : {in.carr_b, in.carr_a}) {
+      if (ca < 0) continue;
+      const int first = (int)cb.size();
+      add_blocks(cb, ca, c->carr[ca].cap, GR_BLOCK * 4);
+      cblocks.push_back(make_int2(first, (int)cb.size()));
+      add_blocks(mb, ca, c->sys[c->carr[ca].sys].cap, GR_BLOCK);
+    }
+  }
+  P.blk_cells.upload(cb, c->stream);
+  P.blk_mark.upload(mb, c->stream);
+  P.carr_blocks.upload(cblocks, c->stream);
+  P.nblk_cells = (int)cb.size();
+  P.nblk_mark = (int)mb.size();
+  P.ncarr = (int)cblocks.size();
+  if ((int64_t)cb.size() + 1 > c->cellblk_cap) {
+    c->bagg.alloc((cb.size() + 1) * 4);
+    c->carry.alloc((cb.size() + 1) * 4);
+    c->cellblk_cap = cb.size() + 1;
+  }
+}
+
+void plan_sort_real(gr_ctx *c, Plan &P, const std::vector<int> &systems) {
+  P.real_list = systems;
+  P.d_real_list.upload(systems, c->stream);
+  std::vector<int> bbase(c->sys.size(), -1);
+  std::vector<Blk> blk;
+  int64_t acc = 0;
+  for (int s : systems) {
+    bbase[s] = (int)acc;
+    acc += c->sys[s].cap;
+    add_blocks(blk, s, c->sys[s].cap, GR_BLOCK);
+  }
+  P.n_real = (int)acc;
+  P.d_bbase_real.upload(bbase, c->stream);
+  P.blk_real.upload(blk, c->stream);
+  P.nblk_real = (int)blk.size();
+}
+
+void plan_ghosts(gr_ctx *c, Plan &P, const std::vector<int> &ghosts) {
+  P.ghost_list = ghosts;
+  P.d_ghost_list.upload(ghosts, c->stream);
+  std::vector<int> bbase(c->sys.size(), -1), cand(c->sys.size(), 0);
+  std::vector<Blk> blk, cblk;
+  int64_t acc = 0, cacc = 0;
+  for (int g : ghosts) {
+    bbase[g] = (int)acc;
+    acc += c->sys[g].cap;
+    cand[g] = (int)cacc;
+    cacc += c->sys[g].cap;
+    add_blocks(blk, g, c->sys[g].cap, GR_BLOCK);
+    add_blocks(cblk, g, c->sys[g].cap, GR_BLOCK);
+  }
+  P.n_ghost = (int)acc;
+  P.n_cand = (int)cacc;
+  P.d_bbase_ghost.upload(bbase, c->stream);
+  P.d_cand_off.upload(cand, c->stream);
+  P.blk_ghost.upload(blk, c->stream);
+  P.blk_cand.upload(cblk, c->stream);
+  P.nblk_ghost = (int)blk.size();
+  P.nblk_cand = (int)cblk.size();
+}
+
+// Stage plans of a batch of integrators: updater tables (with the fused EOS
+// and the zeroing absorbed by assigning sweeps), ghost copiers, and the
+// gather tasks of each functor family.
+void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
+  (void)count_pairs_unused;
+  int nst = 0;
+  for (int ig : P.integs) nst = std::max(nst, c->integ[ig].nst);
+  P.stages.clear();
+  // per-system effective updater lists
+  const int ns = (int)c->sys.size();
+  std::vector<std::vector<std::vector<UpdDesc>>> upd(ns);
+  std::vector<int> sys_integ(ns, -1);
+  for (int ig : P.integs)
+    for (int s : c->integ[ig].sys) sys_integ[s] = ig;
+  for (int ig : P.integs) {
+    const IntegH &I = c->integ[ig];
+    for (int s : I.sys) {
+      SysH &S = c->sys[s];
+      StepSys &T = c->stepsys[s];
+      memset(&T, 0, sizeof(T));
+      T.dt = I.dt;
+      T.half = I.dt / 2;
+      T.gdt = I.gamma / I.dt;
+      T.damping = I.gamma != 0.0;
+      T.sort_cutoff = 2.0 * I.h;
+      T.reset_force = 1;
+      upd[s].assign(I.nst, {});
+      if ((int)S.upd.size() == I.nst) {  // _sweep_all_stages! :405
+        for (int k = 0; k < I.nst; ++k)
+          for (const UpdDesc &u : S.upd[k])
+            if (u.kind != GR_UPD_NOTHING) upd[s][k].push_back(u);
+        if (I.nst >= 1 && upd[s][0].size() == 1 &&
+            (upd[s][0][0].kind == GR_UPD_TAIT || upd[s][0][0].kind == GR_UPD_LINEAR)) {
+          T.fuse_eos = upd[s][0][0].kind;
+          T.eos_rho0 = upd[s][0][0].prm[0];
+          T.eos_gamma = upd[s][0][0].prm[1];
+          upd[s][0].clear();
+        }
+      }
+    }
+    for (int g : I.ghost) {
+      StepSys &T = c->stepsys[g];
+      memset(&T, 0, sizeof(T));
+      T.sort_cutoff = 2.0 * I.h;
+      T.dt = I.dt;
+    }
+  }
+  // gather tasks: (stage, family, target) → sources, in interaction order
+  struct TaskB {
+    Task t;
+    std::vector<SrcDesc> src;
+  };
+  std::vector<std::map<int, TaskB>> tasks(nst * 3);  // [stage*3+fam] target → task
+  for (int k = 0; k < nst; ++k)
+    for (int ig : P.integs)
+      for (int it : c->integ[ig].inter) {
+        const InterH &in = c->inter[it];
+        if (k >= (int)in.pfns.size()) continue;
+        const gr_pfn_desc &f = in.pfns[k];
+        const int fam = family_of(f.kind);
+        if (fam < 0) continue;
+        const int pk = classify(c, in, f.kind);
+        auto add = [&](int target, bool b_side) {
+          auto &m = tasks[k * 3 + fam];
+          TaskB &tb = m[target];
+          if (tb.src.empty()) {
+            memset(&tb.t, 0, sizeof(Task));
+            tb.t.sys = target;
+            tb.t.ns = c->sys[target].ns;
+            tb.t.nv = c->sys[target].nv;
+          }
+          SrcDesc q = make_src(c, it, target, b_side, pk);
+          set_pfn(q, f, pk, c, in);
+          if (f.kind == GR_PFN_CAUCHY) tb.t.has_stress = 1;
+          if (f.kind == GR_PFN_SRV) tb.t.write_vor = 1;
+          tb.src.push_back(q);
+        };
+        add(in.a, false);
+        if (pk == P_TWO && in.b >= 0) add(in.b, true);
+      }
+  // assignment: FORCE group
+  for (int s = 0; s < ns; ++s) {
+    if (sys_integ[s] < 0 || c->sys[s].kind == GR_SYS_GHOST) continue;
+    int first = -1, firstfam = -1;
+    for (int k = 0; k < nst && first < 0; ++k)
+      for (int fam : {FAM_FORCE, FAM_COUNT})
+        if (tasks[k * 3 + fam].count(s)) {
+          first = k;
+          firstfam = fam;
+          break;
+        }
+    if (first < 0) continue;
+    bool zeroed = false;
+    for (int k = 0; k <= first && k < (int)upd[s].size(); ++k)
+      for (auto &u : upd[s][k])
+        if (u.kind == GR_UPD_ZERO && (u.mask & ((1 << GR_F_DVDT) | (1 << GR_F_DRHODT))))
+          zeroed = true;
+    if (!zeroed) {
+      c->stepsys[s].reset_force = 0;
+      tasks[first * 3 + firstfam][s].t.assign = 1;
+    }
+  }
+  // assignment: STRAIN group absorbs a stage's ZeroFieldUpdater
+  for (int k = 0; k < nst; ++k)
+    for (auto &kv : tasks[k * 3 + FAM_STRAIN]) {
+      const int s = kv.first;
+      if (k >= (int)upd[s].size()) continue;
+      auto &L = upd[s][k];
+      bool only_zero = !L.empty();
+      long long m = 0;
+      for (auto &u : L) {
+        if (u.kind != GR_UPD_ZERO) only_zero = false;
+        m |= u.mask;
+      }
+      long long need = 1 << GR_F_STRAIN_RATE;
+      if (kv.second.t.write_vor) need |= 1 << GR_F_VORTICITY;
+      if (only_zero && (m & need) == need) {
+        kv.second.t.assign = 1;
+        for (auto &u : L) u.mask &= ~need;
+        L.erase(std::remove_if(L.begin(), L.end(), [](const UpdDesc &u) { return u.mask == 0; }),
+                L.end());
+      }
+    }
+  // materialise stage plans
+  for (int k = 0; k < nst; ++k) {
+    auto sp = std::make_unique<StagePlan>();
+    std::vector<UpdDesc> ud;
+    std::vector<int2> ur(ns, make_int2(0, 0));
+    std::vector<Blk> ub;
+    for (int s = 0; s < ns; ++s) {
+      if (sys_integ[s] < 0 || k >= (int)upd[s].size() || upd[s][k].empty()) continue;
+      ur[s] = make_int2((int)ud.size(), (int)(ud.size() + upd[s][k].size()));
+      for (auto &u : upd[s][k]) ud.push_back(u);
+      add_blocks(ub, s, c->sys[s].cap, GR_BLOCK);
+    }
+    sp->upd.upload(ud, c->stream);
+    sp->urange.upload(ur, c->stream);
+    sp->blk_upd.upload(ub, c->stream);
+    sp->nblk_upd = (int)ub.size();
+    std::vector<int2> gc(ns, make_int2(0, -1));
+    std::vector<Blk> gb;
+    for (int ig : P.integs)
+      for (int g : c->integ[ig].ghost) {
+        const SysH &G = c->sys[g];
+        if (k < (int)G.copy.size() && (G.copy[k].x || G.copy[k].y >= 0)) {
+          gc[g] = G.copy[k];
+          add_blocks(gb, g, G.cap, GR_BLOCK);
+        }
+      }
+    sp->gcopy.upload(gc, c->stream);
+    sp->blk_gcopy.upload(gb, c->stream);
+    sp->nblk_gcopy = (int)gb.size();
+    for (int fam = 0; fam < 3; ++fam) {
+      std::vector<Task> tv;
+      std::vector<SrcDesc> sv;
+      std::vector<Blk> bv;
+      for (auto &kv : tasks[k * 3 + fam]) {
+        Task t = kv.second.t;
+        t.src_begin = (int)sv.size();
+        for (auto &q : kv.second.src) sv.push_back(q);
+        t.src_end = (int)sv.size();
+        add_blocks(bv, (int)tv.size(), c->sys[t.sys].cap, GR_BLOCK);
+        tv.push_back(t);
+      }
+      sp->fam[fam].tasks.upload(tv, c->stream);
+      sp->fam[fam].srcs.upload(sv, c->stream);
+      sp->fam[fam].blk.upload(bv, c->stream);
+      sp->fam[fam].nblk = (int)bv.size();
+    }
+    P.stages.push_back(std::move(sp));
+  }
+  P.stepsys.clear();
+  for (int ig : P.integs) {
+    for (int s : c->integ[ig].sys) P.stepsys.push_back({s, c->stepsys[s]});
+    for (int g : c->integ[ig].ghost) P.stepsys.push_back({g, c->stepsys[g]});
+  }
+  upload_stepsys(c);
+}
+
+Plan &get_plan(gr_ctx *c, const std::vector<int> &integs) {
+  commit(c);
+  auto it = c->plans.find(integs);
+  if (it != c->plans.end()) return *it->second;
+  auto P = std::make_unique<Plan>();
+  P->integs = integs;
+  std::vector<int> reals, ghosts, inters, all;
+  for (int ig : integs) {
+    const IntegH &I = c->integ[ig];
+    for (int s : I.sys) reals.push_back(s);
+    for (int g : I.ghost) ghosts.push_back(g);
+    for (int t : I.inter) inters.push_back(t);
+  }
+  all = reals;
+  for (int g : ghosts) all.push_back(g);
+  {
+    std::vector<int> u = all;
+    std::sort(u.begin(), u.end());
+    need(std::unique(u.begin(), u.end()) == u.end(), GR_ERR_ARGUMENT,
+         "a system belongs to more than one integrator of the batch");
+  }
+  ensure_cells(c, inters, reals);
+  plan_sort_real(c, *P, reals);
+  plan_ghosts(c, *P, ghosts);
+  plan_grids(c, *P, inters);
+  plan_stages(c, *P, false);
+  HIPCHK(hipStreamSynchronize(c->stream));
+  Plan &ref = *P;
+  c->plans[integs] = std::move(P);
+  return ref;
+}
+
+// Block table over every real system of the pool (the permutation writes all
+// of them, so the A/B swap keeps systems outside the batch intact).
+void all_real(gr_ctx *c, const Blk *&blk, int &nblk) {
+  if (c->allreal_cap != c->pool_cap || c->allreal_nsys != c->sys.size()) {
+    std::vector<Blk> v;
+    for (int s = 0; s < (int)c->sys.size(); ++s)
+      if (c->sys[s].kind != GR_SYS_GHOST) add_blocks(v, s, c->sys[s].cap, GR_BLOCK);
+    c->allreal_blk.release();
+    c->allreal_blk.upload(v, c->stream);
+    c->allreal_nblk = (int)v.size();
+    c->allreal_cap = c->pool_cap;
+    c->allreal_nsys = c->sys.size();
+  }
+  blk = c->allreal_blk.as<Blk>();
+  nblk = c->allreal_nblk;
+}
+
+// ---------------------------------------------------------------------------
+// Step phases
+// ---------------------------------------------------------------------------
+
+// Device arrays of the x-component pointers (pool A, pool B, ghost scratch).
+struct XsView {
+  DBuf *pool;
+  DBuf *gxs;
+};
+XsView xs_ptrs(gr_ctx *c) {
+  if (c->xs_cap != c->pool_cap) {
+    for (int w = 0; w < 2; ++w) {
+      std::vector<double *> v(3);
+      for (int d = 0; d < 3; ++d) v[d] = c->x[w].as<double>() + d * c->pool_cap;
+      c->xs_pool[w].upload(v, c->stream);
+    }
+    std::vector<double *> g(3);
+    for (int d = 0; d < 3; ++d) g[d] = c->gxs.as<double>() + d * c->pool_cap;
+    c->xs_gxs.upload(g, c->stream);
+    c->xs_cap = c->pool_cap;
+  }
+  return XsView{c->xs_pool, &c->xs_gxs};
+}
+
+template <typename R>
+void phase_sort_real(gr_ctx *c, Plan &P) {
+  if (P.real_list.empty()) return;
+  hipStream_t s = c->stream;
+  XsView X = xs_ptrs(c);
+  const double *const *xs = X.pool[c->cur].as<const double *>();
+  launch_minmax_reset(s, P.d_real_list.as<int>(), (int)P.real_list.size(), c->d_kmin.as<int>(),
+                      c->d_kmax.as<int>(), c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>());
+  launch_key_minmax(s, xs, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), c->d_st.as<StepSys>(),
+                    P.blk_real.as<Blk>(), P.nblk_real, c->D, c->d_kmin.as<int>(),
+                    c->d_kmax.as<int>(), c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>(), 1);
+  launch_key_base(s, P.d_real_list.as<int>(), (int)P.real_list.size(), c->D, 0, c->d_n.as<int>(),
+                  c->d_kmin.as<int>(), c->d_kmax.as<int>(), c->d_kbase.as<unsigned>(),
+                  c->d_krange.as<unsigned>(), c->d_flags.as<int>());
+  launch_key_write(s, xs, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), c->d_st.as<StepSys>(),
+                   P.blk_real.as<Blk>(), P.nblk_real, P.d_bbase_real.as<int>(), c->D,
+                   c->d_kmin.as<int>(), c->d_krange.as<unsigned>(), c->d_kbase.as<unsigned>(),
+                   c->keys.as<unsigned>(), c->vals.as<int>(), c->d_flags.as<int>());
+  launch_radix_sort(s, c->keys.as<unsigned>(), c->vals.as<int>(), c->keys2.as<unsigned>(),
+                    c->vals2.as<int>(), c->hist.as<unsigned>(), P.n_real, c->d_flags.as<int>());
+  const Blk *arb;
+  int arn;
+  all_real(c, arb, arn);
+  Pool<R> A = pool_view<R>(c, c->cur), B = pool_view<R>(c, 1 - c->cur);
+  launch_permute_real<R>(s, A, B, c->d_sd.as<SysDesc>(), arb, arn,
+                         P.d_bbase_real.as<int>(), c->vals.as<int>(), c->d_flags.as<int>(), c->D);
+  c->cur = 1 - c->cur;
+}
+
+// Generation (flags, scan, scatter to scratch) + sort (scratch → pool).
+template <typename R>
+void phase_ghosts(gr_ctx *c, Plan &P, bool generate, bool sort) {
+  if (P.ghost_list.empty()) return;
+  hipStream_t s = c->stream;
+  XsView X = xs_ptrs(c);
+  const double *const *x = X.pool[c->cur].as<const double *>();
+  Pool<R> A = pool_view<R>(c, c->cur);
+  if (generate) {
+    launch_ghost_flags(s, x, c->d_sd.as<SysDesc>(), c->d_bnd.as<Bnd>(), P.blk_cand.as<Blk>(),
+                       P.nblk_cand, P.d_cand_off.as<int>(), c->D, c->cand.as<int>());
+    launch_scan(s, c->cand.as<int>(), c->cand_scan.as<int>(), c->scan_bsum.as<int>(), P.n_cand);
+    launch_ghost_scatter(s, x, X.gxs->as<double *const>(), c->gidx_s.as<int>(), c->gbnd_s.as<int>(),
+                         c->gnrm.as<int>(), c->d_sd.as<SysDesc>(), c->d_bnd.as<Bnd>(),
+                         P.blk_cand.as<Blk>(), P.nblk_cand, P.d_cand_off.as<int>(),
+                         c->cand_scan.as<int>(), c->D, c->d_n.as<int>());
+  }
+  const double *const *gx = X.gxs->as<const double *>();
+  launch_minmax_reset(s, P.d_ghost_list.as<int>(), (int)P.ghost_list.size(), c->d_kmin.as<int>(),
+                      c->d_kmax.as<int>(), c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>());
+  launch_key_minmax(s, gx, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), c->d_st.as<StepSys>(),
+                    P.blk_ghost.as<Blk>(), P.nblk_ghost, c->D, c->d_kmin.as<int>(),
+                    c->d_kmax.as<int>(), c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>(),
+                    sort ? 1 : 0);
+  if (sort) {
+    launch_key_base(s, P.d_ghost_list.as<int>(), (int)P.ghost_list.size(), c->D, 1,
+                    c->d_n.as<int>(), c->d_kmin.as<int>(), c->d_kmax.as<int>(),
+                    c->d_kbase.as<unsigned>(), c->d_krange.as<unsigned>(), c->d_flags.as<int>());
+    launch_key_write(s, gx, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), c->d_st.as<StepSys>(),
+                     P.blk_ghost.as<Blk>(), P.nblk_ghost, P.d_bbase_ghost.as<int>(), c->D,
+                     c->d_kmin.as<int>(), c->d_krange.as<unsigned>(), c->d_kbase.as<unsigned>(),
+                     c->keys.as<unsigned>(), c->vals.as<int>(), c->d_flags.as<int>());
+    launch_radix_sort(s, c->keys.as<unsigned>(), c->vals.as<int>(), c->keys2.as<unsigned>(),
+                      c->vals2.as<int>(), c->hist.as<unsigned>(), P.n_ghost, c->d_flags.as<int>());
+  } else {
+    launch_fill_i32(s, c->d_flags.as<int>(), 0, 1);
+  }
+  launch_permute_ghost(s, gx, c->gidx_s.as<int>(), c->gbnd_s.as<int>(),
+                       X.pool[c->cur].as<double *const>(), A.gidx, A.gbnd, c->d_sd.as<SysDesc>(),
+                       c->d_n.as<int>(), P.blk_ghost.as<Blk>(), P.nblk_ghost,
+                       P.d_bbase_ghost.as<int>(), c->vals.as<int>(), c->d_flags.as<int>(), c->D);
+}
+
+void phase_grids(gr_ctx *c, Plan &P) {
+  if (P.inters.empty()) return;
+  hipStream_t s = c->stream;
+  XsView X = xs_ptrs(c);
+  launch_grid_params(s, P.d_inter_list.as<int>(), (int)P.inters.size(), c->d_inter_sys.as<int2>(),
+                     c->d_cellsz.as<double>(), c->d_bbmin.as<long long>(),
+                     c->d_bbmax.as<long long>(), c->d_n.as<int>(), c->d_grids.as<Grid>(),
+                     c->d_inter_carr.as<int2>(), c->d_carr.as<CellArr>(), c->D, c->d_flags.as<int>());
+  launch_cells_build(s, X.pool[c->cur].as<const double *>(), c->cells.as<int>(),
+                     c->d_carr.as<CellArr>(), c->d_grids.as<Grid>(), c->d_cellsz.as<double>(),
+                     c->d_sd.as<SysDesc>(), c->d_n.as<int>(), P.blk_cells.as<Blk>(), P.nblk_cells,
+                     P.blk_mark.as<Blk>(), P.nblk_mark, P.carr_blocks.as<int2>(), P.ncarr,
+                     c->bagg.as<int>(), c->carry.as<int>(), c->D);
+}
+
+template <typename R>
+void phase_stage(gr_ctx *c, Plan &P, int k, bool cnt) {
+  hipStream_t s = c->stream;
+  StagePlan &S = *P.stages[k];
+  Pool<R> A = pool_view<R>(c, c->cur);
+  launch_updaters<R>(s, A, c->d_sd.as<SysDesc>(), c->d_st.as<StepSys>(), S.blk_upd.as<Blk>(),
+                     S.nblk_upd, S.urange.as<int2>(), S.upd.as<UpdDesc>(), c->D);
+  launch_ghost_copy<R>(s, A, c->d_sd.as<SysDesc>(), c->d_bnd.as<Bnd>(), c->d_n.as<int>(),
+                       S.blk_gcopy.as<Blk>(), S.nblk_gcopy, S.gcopy.as<int2>(), c->D);
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[5], s));
+  for (int fam : {FAM_FORCE, FAM_COUNT, FAM_STRAIN}) {
+    FamPlan &F = S.fam[fam];
+    if (F.nblk == 0) continue;
+    SweepArgs a;
+    a.sd = c->d_sd.as<SysDesc>();
+    a.n = c->d_n.as<int>();
+    a.tasks = F.tasks.as<Task>();
+    a.srcs = F.srcs.as<SrcDesc>();
+    a.grids = c->d_grids.as<Grid>();
+    a.carr = c->d_carr.as<CellArr>();
+    a.cells = c->cells.as<int>();
+    a.blk = F.blk.as<Blk>();
+    a.nblk = F.nblk;
+    a.pair_count = c->d_pairs.as<unsigned long long>();
+    launch_sweep<R>(s, A, a, c->D, fam, cnt);
+  }
+  if (c->timing) HIPCHK(hipEventRecord(c->ev[6], s));
+}
+
+template <typename R>
+void run_step(gr_ctx *c, Plan &P, bool cnt) {
+  hipStream_t s = c->stream;
+  const bool T = c->timing;
+  if (T) HIPCHK(hipEventRecord(c->ev[0], s));
+  phase_sort_real<R>(c, P);
+  if (T) HIPCHK(hipEventRecord(c->ev[1], s));
+  phase_ghosts<R>(c, P, true, true);
+  if (T) HIPCHK(hipEventRecord(c->ev[2], s));
+  phase_grids(c, P);
+  if (T) HIPCHK(hipEventRecord(c->ev[3], s));
+  Pool<R> A = pool_view<R>(c, c->cur);
+  launch_halfstep<R>(s, A, c->d_sd.as<SysDesc>(), c->d_st.as<StepSys>(), P.blk_real.as<Blk>(),
+                     P.nblk_real, c->D);
+  launch_ghost_kin<R>(s, A, c->d_sd.as<SysDesc>(), c->d_bnd.as<Bnd>(), c->d_n.as<int>(),
+                      P.blk_ghost.as<Blk>(), P.nblk_ghost, c->D);
+  if (T) HIPCHK(hipEventRecord(c->ev[4], s));
+  float sweep_ms = 0.0f, upd_ms = 0.0f;
+  for (int k = 0; k < (int)P.stages.size(); ++k) {
+    phase_stage<R>(c, P, k, cnt);
+    if (T) {
+      HIPCHK(hipEventSynchronize(c->ev[6]));
+      float a = 0, b = 0;
+      HIPCHK(hipEventElapsedTime(&a, c->ev[5], c->ev[6]));
+      sweep_ms += a;
+      HIPCHK(hipEventElapsedTime(&b, k == 0 ? c->ev[4] : c->ev[7], c->ev[5]));
+      upd_ms += b;
+      HIPCHK(hipEventRecord(c->ev[7], s));
+    }
+  }
+  if (T) HIPCHK(hipEventRecord(c->ev[7], s));
+  launch_fullstep<R>(s, A, c->d_sd.as<SysDesc>(), c->d_st.as<StepSys>(), P.blk_real.as<Blk>(),
+                     P.nblk_real, c->D);
+  if (T) {
+    HIPCHK(hipEventRecord(c->ev[8], s));
+    HIPCHK(hipEventSynchronize(c->ev[8]));
+    float ms[4];
+    for (int q = 0; q < 3; ++q) HIPCHK(hipEventElapsedTime(&ms[q], c->ev[q], c->ev[q + 1]));
+    float hs = 0, fs = 0, tot = 0;
+    HIPCHK(hipEventElapsedTime(&hs, c->ev[3], c->ev[4]));
+    HIPCHK(hipEventElapsedTime(&fs, c->ev[7], c->ev[8]));
+    HIPCHK(hipEventElapsedTime(&tot, c->ev[0], c->ev[8]));
+    c->phase_ms[0] += ms[0];
+    c->phase_ms[1] += ms[1];
+    c->phase_ms[2] += ms[2];
+    c->phase_ms[3] += hs;
+    c->phase_ms[4] += upd_ms;
+    c->phase_ms[5] += sweep_ms;
+    c->phase_ms[6] += fs;
+    c->phase_ms[7] += tot;
+  }
+}
+
+// Graph-captured step: two executables, one per A/B parity of the pool.
+template <typename R>
+void step_graph(gr_ctx *c, Plan &P) {
+  hipGraphExec_t &g = P.graph[c->cur];
+  if (!g) {
+    const int cur0 = c->cur;
+    hipGraph_t graph;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    run_step<R>(c, P, false);
+    HIPCHK(hipStreamEndCapture(c->stream, &graph));
+    HIPCHK(hipGraphInstantiate(&g, graph, nullptr, nullptr, 0));
+    HIPCHK(hipGraphDestroy(graph));
+    c->cur = cur0;  // capture did not execute; restore the parity
+  }
+  HIPCHK(hipGraphLaunch(g, c->stream));
+  c->cur = 1 - c->cur;  // one real-sort swap per step
+}
+
+template <typename R>
+void do_steps(gr_ctx *c, const std::vector<int> &integs, int nsteps) {
+  Plan &P = get_plan(c, integs);
+  // the plan's step parameters (fine-grained calls may have changed them)
+  for (auto &e : P.stepsys) c->stepsys[e.first] = e.second;
+  upload_stepsys(c);
+  {
+    const Blk *b;
+    int nb;
+    all_real(c, b, nb);  // build cached tables before any capture
+    xs_ptrs(c);
+  }
+  if (c->timing) {
+    for (double &m : c->phase_ms) m = 0;
+    launch_fill_i32(c->stream, c->d_pairs.as<int>(), 0, 4);
+  }
+  for (int k = 0; k < nsteps; ++k) {
+    if (c->graph_mode && !c->timing && !P.real_list.empty())
+      step_graph<R>(c, P);
+    else
+      run_step<R>(c, P, c->timing);
+  }
+  int flags[4];
+  HIPCHK(hipMemcpyAsync(flags, c->d_flags.p, sizeof(flags), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (c->timing) {
+    unsigned long long pc[2];
+    HIPCHK(hipMemcpyAsync(pc, c->d_pairs.p, sizeof(pc), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->last_pairs[0] = (long long)pc[0];
+    c->last_pairs[1] = (long long)pc[1];
+    for (double &m : c->phase_ms) m /= std::max(nsteps, 1);
+  }
+  if (flags[1]) {
+    int z[4] = {0, 0, 0, 0};
+    HIPCHK(hipMemcpyAsync(c->d_flags.p, z, sizeof(z), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    // invalidate plans so the next call re-plans with larger capacity
+    throw GrError(GR_ERR_CAPACITY,
+                  (flags[1] & 2) ? "cell grid outgrew its capacity (particles left the planned domain)"
+                                 : "cell-key range exceeds 32 bits");
+  }
+}
+
+void validate_sys(gr_ctx *c, int s) {
+  need(s >= 0 && s < (int)c->sys.size(), GR_ERR_ARGUMENT, "bad system handle");
+}
+
+template <typename R>
+void refresh_shadows(gr_ctx *c, const std::vector<int> &systems);
+
+}  // namespace
+
+// ===========================================================================
+// Shadow refresh kernel (fine-grained calls see the uploaded state)
+// ===========================================================================
+
+template <typename R>
+__global__ void k_refresh(Pool<R> P, const SysDesc *sd, const Blk *blk, int D) {
+  const Blk b = blk[blockIdx.x];
+  const SysDesc S = sd[b.item];
+  const int i = b.first + threadIdx.x;
+  if (i >= S.cap) return;
+  const int64_t g = S.off + i;
+  for (int d = 0; d < D; ++d) P.vs[d][g] = (R)P.v[d][g];
+  P.rhos[g] = (R)P.rho[g];
+  P.rho_h[g] = P.rho[g];
+  if (S.kind >= GR_SYS_STRESS)
+    for (int k = 0; k < S.ns; ++k) P.stress_s[k][g] = (R)P.stress[k][g];
+}
+
+namespace {
+template <typename R>
+void refresh_shadows(gr_ctx *c, const std::vector<int> &systems) {
+  std::vector<Blk> v;
+  for (int s : systems)
+    if (c->sys[s].kind != GR_SYS_GHOST) add_blocks(v, s, c->sys[s].cap, GR_BLOCK);
+  if (v.empty()) return;
+  DBuf b;
+  b.upload(v, c->stream);
+  k_refresh<R><<<(unsigned)v.size(), GR_BLOCK, 0, c->stream>>>(pool_view<R>(c, c->cur),
+                                                              c->d_sd.as<SysDesc>(), b.as<Blk>(), c->D);
+  HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+template <typename F>
+void by_prec(gr_ctx *c, F &&f) {
+  if (c->prec == GR_PREC_F64)
+    f((double *)nullptr);
+  else
+    f((float *)nullptr);
+}
+}  // namespace
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+
+static thread_local std::string g_create_err;
+
+extern "C" {
+
+int gr_ctx_create(gr_ctx **out, int device, int ndims, int precision) {
+  if (!out) return GR_ERR_ARGUMENT;
+  *out = nullptr;
+  if (ndims != 2 && ndims != 3) {
+    g_create_err = "ndims must be 2 or 3";
+    return GR_ERR_ARGUMENT;
+  }
+  if (precision != GR_PREC_F32 && precision != GR_PREC_F64) {
+    g_create_err = "precision must be GR_PREC_F32 or GR_PREC_F64";
+    return GR_ERR_ARGUMENT;
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
+    g_create_err = "no HIP device " + std::to_string(device);
+    return GR_ERR_HIP;
+  }
+  if (hipSetDevice(device) != hipSuccess) {
+    g_create_err = "hipSetDevice failed";
+    return GR_ERR_HIP;
+  }
+  gr_ctx *c = new gr_ctx();
+  c->device = device;
+  c->D = ndims;
+  c->prec = precision;
+  c->rsize = precision == GR_PREC_F64 ? 8 : 4;
+  c->nsmax = ndims == 2 ? 4 : 6;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    g_create_err = "hipStreamCreate failed";
+    return GR_ERR_HIP;
+  }
+  for (auto &e : c->ev) (void)hipEventCreate(&e);
+  *out = c;
+  return GR_OK;
+}
+
+int gr_ctx_destroy(gr_ctx *c) {
+  if (!c) return GR_OK;
+  (void)hipStreamSynchronize(c->stream);
+  c->plans.clear();
+  for (auto &e : c->ev)
+    if (e) (void)hipEventDestroy(e);
+  hipStream_t s = c->stream;
+  delete c;
+  (void)hipStreamDestroy(s);
+  return GR_OK;
+}
+
+const char *gr_last_error(const gr_ctx *c) { return c ? c->err.c_str() : g_create_err.c_str(); }
+
+int gr_ctx_info(gr_ctx *c, char *buf, int64_t buflen) {
+  return guard(c, [&] {
+    hipDeviceProp_t p;
+    HIPCHK(hipGetDeviceProperties(&p, c->device));
+    std::string s = std::string(p.name) + " " + p.gcnArchName + " CUs=" +
+                    std::to_string(p.multiProcessorCount) +
+                    (c->prec == GR_PREC_F64 ? " prec=f64" : " prec=f32");
+    snprintf(buf, (size_t)buflen, "%s", s.c_str());
+  });
+}
+
+int gr_system_create(gr_ctx *c, int kind, int64_t n, int ns, double mass, double cc,
+                     const double *source_v, double source_rho, int *sys_out) {
+  return guard(c, [&] {
+    need(n > 0, GR_ERR_ARGUMENT, "n must be positive, got " + std::to_string(n));
+    need(n < (1LL << 30), GR_ERR_ARGUMENT, "n too large for one system");
+    need(kind >= GR_SYS_BASIC && kind <= GR_SYS_EP, GR_ERR_ARGUMENT, "bad system kind");
+    if (kind >= GR_SYS_STRESS)
+      need(ns == 3 || ns == 4 || ns == 6, GR_ERR_ARGUMENT,
+           "ns must be 3, 4, or 6, got " + std::to_string(ns));
+    if (kind >= GR_SYS_STRESS)
+      need((c->D == 2 && ns != 6) || (c->D == 3 && ns == 6), GR_ERR_ARGUMENT,
+           "stress length must be 3/4 in 2D and 6 in 3D");
+    SysH s;
+    s.kind = kind;
+    s.ns = kind >= GR_SYS_STRESS ? ns : 0;
+    s.nv = kind == GR_SYS_EP ? (c->D == 2 ? 1 : 3) : 0;
+    s.n = n;
+    s.cap = n;
+    s.mass = mass;
+    s.c = cc;
+    for (int d = 0; d < c->D; ++d) s.sv[d] = source_v ? source_v[d] : 0.0;
+    s.srho = source_rho;
+    c->sys.push_back(s);
+    c->dirty_layout = true;
+    *sys_out = (int)c->sys.size() - 1;
+  });
+}
+
+int gr_system_add_updater(gr_ctx *c, int sys, int stage, int kind, const double *params,
+                          int nparams) {
+  return guard(c, [&] {
+    validate_sys(c, sys);
+    need(stage >= 1, GR_ERR_ARGUMENT, "stage must be >= 1");
+    SysH &s = c->sys[sys];
+    need(s.kind != GR_SYS_GHOST, GR_ERR_ARGUMENT, "ghost systems take copiers, not updaters");
+    if (kind == GR_UPD_EP)
+      need(s.ns == 4, GR_ERR_ASSERTION,
+           "ElastoPlasticStressUpdater requires NS=4 (plane-strain Voigt). Got NS=" +
+               std::to_string(s.ns) + ".");
+    need(!((kind == GR_UPD_EP && s.kind != GR_SYS_EP) || (kind == GR_UPD_VPMC && s.kind < GR_SYS_STRESS) ||
+           ((kind == GR_UPD_TAIT || kind == GR_UPD_LINEAR) && s.kind < GR_SYS_FLUID)),
+         GR_ERR_ARGUMENT, "updater needs fields the system does not have");
+    need(kind >= GR_UPD_NOTHING && kind <= GR_UPD_VPMC, GR_ERR_ARGUMENT, "unknown updater kind");
+    if ((int)s.upd.size() < stage) s.upd.resize(stage);
+    UpdDesc u;
+    memset(&u, 0, sizeof(u));
+    u.kind = kind;
+    double prm[5] = {0, 0, 0, 0, 0};
+    for (int k = 0; k < nparams && k < 5; ++k) prm[k] = params[k];
+    switch (kind) {
+      case GR_UPD_ZERO:
+        u.mask = (long long)prm[0];
+        break;
+      case GR_UPD_TAIT:
+        u.prm[0] = prm[0];
+        u.prm[1] = nparams >= 2 ? prm[1] : 7.0;
+        break;
+      case GR_UPD_LINEAR:
+        u.prm[0] = prm[0];
+        break;
+      case GR_UPD_EP: {  // trig constants as src/StateUpdaters.jl:264-266,288
+        const double phi = prm[2], psi = prm[3], coh = prm[4];
+        const double sqrt3 = std::sqrt(3.0);
+        u.prm[0] = prm[0];
+        u.prm[1] = prm[1];
+        u.prm[2] = 2 * std::sin(phi) / (sqrt3 * (3 - std::sin(phi)));
+        u.prm[3] = 6 * coh * std::cos(phi) / (sqrt3 * (3 - std::sin(phi)));
+        u.prm[4] = 2 * std::sin(psi) / (sqrt3 * (3 - std::sin(psi)));
+        break;
+      }
+      case GR_UPD_VPMC:
+        u.prm[0] = prm[0];
+        u.prm[1] = prm[1];
+        u.prm[2] = prm[2];
+        u.prm[3] = std::tan(prm[3]);
+        u.prm[4] = prm[4];
+        need((int)prm[0] == GR_UPD_TAIT || (int)prm[0] == GR_UPD_LINEAR, GR_ERR_ARGUMENT,
+             "ViscoPlasticMCStressUpdater needs a Tait or Linear EOS");
+        break;
+    }
+    s.upd[stage - 1].push_back(u);
+    c->plans.clear();
+  });
+}
+
+int gr_system_upload(gr_ctx *c, int sys, int field, const double *data, int64_t count) {
+  return guard(c, [&] {
+    validate_sys(c, sys);
+    commit(c);
+    SysH &s = c->sys[sys];
+    need(has_field(s, field) && field != GR_F_NORMALS, GR_ERR_ARGUMENT,
+         "field not present on this system");
+    const int nc = ncomp(c, s, field);
+    const int64_t n = live_n(c, sys);
+    need(count == n * nc, GR_ERR_ARGUMENT, "count mismatch");
+    if (field == GR_F_V_ADJUSTMENT) return;  // not stored: no XSPH functor on the device
+    const int64_t off = s.off;
+    const size_t R = c->rsize;
+    const int64_t cap = c->pool_cap;
+    if (s.kind == GR_SYS_GHOST) {
+      switch (field) {
+        case GR_F_X:
+          for (int k = 0; k < nc; ++k)
+            copy_comp_to_dev(c, (char *)(c->x[c->cur].as<double>() + k * cap), 8, off, data, n, nc, k);
+          break;
+        case GR_F_V:
+          for (int k = 0; k < nc; ++k)
+            copy_comp_to_dev(c, (char *)c->vs.p + k * cap * R, R, off, data, n, nc, k);
+          break;
+        case GR_F_RHO:
+          copy_comp_to_dev(c, (char *)c->rhos.p, R, off, data, n, 1, 0);
+          break;
+        case GR_F_P:
+          copy_comp_to_dev(c, (char *)c->p[c->cur].p, R, off, data, n, 1, 0);
+          break;
+        case GR_F_STRESS:
+          for (int k = 0; k < nc; ++k)
+            copy_comp_to_dev(c, (char *)c->stress_s[c->cur].p + k * cap * R, R, off, data, n, nc, k);
+          break;
+      }
+      return;
+    }
+    FieldRef f = field_ref(c, field, false);
+    for (int k = 0; k < nc; ++k) copy_comp_to_dev(c, f.comp[k], f.esz, off, data, n, nc, k);
+    // shadows follow their masters
+    if (field == GR_F_V)
+      for (int k = 0; k < nc; ++k) copy_comp_to_dev(c, (char *)c->vs.p + k * cap * R, R, off, data, n, nc, k);
+    if (field == GR_F_RHO) {
+      copy_comp_to_dev(c, (char *)c->rhos.p, R, off, data, n, 1, 0);
+      copy_comp_to_dev(c, (char *)c->rho_h.p, 8, off, data, n, 1, 0);
+    }
+    if (field == GR_F_STRESS)
+      for (int k = 0; k < nc; ++k)
+        copy_comp_to_dev(c, (char *)c->stress_s[c->cur].p + k * cap * R, R, off, data, n, nc, k);
+  });
+}
+
+int gr_system_download(gr_ctx *c, int sys, int field, double *data, int64_t count) {
+  return guard(c, [&] {
+    validate_sys(c, sys);
+    commit(c);
+    SysH &s = c->sys[sys];
+    need(has_field(s, field), GR_ERR_ARGUMENT, "field not present on this system");
+    const int nc = ncomp(c, s, field);
+    const int64_t n = live_n(c, sys);
+    need(count == n * nc, GR_ERR_ARGUMENT, "count mismatch");
+    const int64_t off = s.off, cap = c->pool_cap;
+    const size_t R = c->rsize;
+    if (field == GR_F_V_ADJUSTMENT) {
+      std::fill(data, data + count, 0.0);
+      return;
+    }
+    if (s.kind == GR_SYS_GHOST) {
+      switch (field) {
+        case GR_F_X:
+          for (int k = 0; k < nc; ++k)
+            copy_comp_from_dev(c, (char *)(c->x[c->cur].as<double>() + k * cap), 8, off, data, n, nc, k);
+          break;
+        case GR_F_V:
+          for (int k = 0; k < nc; ++k)
+            copy_comp_from_dev(c, (char *)c->vs.p + k * cap * R, R, off, data, n, nc, k);
+          break;
+        case GR_F_RHO:
+          copy_comp_from_dev(c, (char *)c->rhos.p, R, off, data, n, 1, 0);
+          break;
+        case GR_F_P:
+          copy_comp_from_dev(c, (char *)c->p[c->cur].p, R, off, data, n, 1, 0);
+          break;
+        case GR_F_STRESS:
+          for (int k = 0; k < nc; ++k)
+            copy_comp_from_dev(c, (char *)c->stress_s[c->cur].p + k * cap * R, R, off, data, n, nc, k);
+          break;
+        case GR_F_NORMALS: {
+          auto nb = read_ints(c, c->gnrm, off, n);
+          for (int64_t i = 0; i < n; ++i)
+            for (int d = 0; d < nc; ++d) data[i * nc + d] = c->bnds[s.bnd + nb[i]].n[d];
+          break;
+        }
+      }
+      return;
+    }
+    FieldRef f = field_ref(c, field, false);
+    for (int k = 0; k < nc; ++k) copy_comp_from_dev(c, f.comp[k], f.esz, off, data, n, nc, k);
+  });
+}
+
+int gr_system_upload_index(gr_ctx *c, int sys, int field, const int64_t *data, int64_t count) {
+  return guard(c, [&] {
+    validate_sys(c, sys);
+    commit(c);
+    SysH &s = c->sys[sys];
+    const int64_t n = live_n(c, sys);
+    need(count == n, GR_ERR_ARGUMENT, "count mismatch");
+    need((s.kind == GR_SYS_GHOST) == (field != GR_I_ID), GR_ERR_ARGUMENT, "index field not present");
+    std::vector<int> t(n);
+    for (int64_t i = 0; i < n; ++i) t[i] = (int)(field == GR_I_ID ? data[i] : data[i] - 1);
+    int *dst = field == GR_I_ID ? c->id[c->cur].as<int>() : field == GR_I_IDX_ORIGINAL ? c->gidx.as<int>() : c->gbnd.as<int>();
+    if (n) {
+      HIPCHK(hipMemcpyAsync(dst + s.off, t.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+    }
+  });
+}
+
+int gr_system_download_index(gr_ctx *c, int sys, int field, int64_t *data, int64_t count) {
+  return guard(c, [&] {
+    validate_sys(c, sys);
+    commit(c);
+    SysH &s = c->sys[sys];
+    const int64_t n = live_n(c, sys);
+    need(count == n, GR_ERR_ARGUMENT, "count mismatch");
+    need((s.kind == GR_SYS_GHOST) == (field != GR_I_ID), GR_ERR_ARGUMENT, "index field not present");
+    const DBuf &b = field == GR_I_ID ? c->id[c->cur] : field == GR_I_IDX_ORIGINAL ? c->gidx : c->gbnd;
+    auto t = read_ints(c, b, s.off, n);
+    for (int64_t i = 0; i < n; ++i) data[i] = field == GR_I_ID ? t[i] : t[i] + 1;
+  });
+}
+
+int gr_system_get_n(gr_ctx *c, int sys, int64_t *n_out) {
+  return guard(c, [&] {
+    validate_sys(c, sys);
+    *n_out = live_n(c, sys);
+  });
+}
+
+int gr_ghost_create(gr_ctx *c, int source_sys, double cutoff, int nb, const double *normals,
+                    const double *points, int *ghost_out) {
+  return guard(c, [&] {
+    validate_sys(c, source_sys);
+    need(c->sys[source_sys].kind != GR_SYS_GHOST, GR_ERR_ARGUMENT, "bad ghost source");
+    need(nb >= 1, GR_ERR_ARGUMENT, "at least one boundary (normal, point) pair is required");
+    SysH g;
+    g.kind = GR_SYS_GHOST;
+    g.src = source_sys;
+    g.nb = nb;
+    g.gcut = cutoff;
+    g.cap = (int64_t)nb * c->sys[source_sys].cap;
+    need(g.cap < (1LL << 30), GR_ERR_ARGUMENT, "ghost capacity too large");
+    g.n = 0;
+    g.ns = c->sys[source_sys].ns;
+    g.mass = c->sys[source_sys].mass;
+    g.c = c->sys[source_sys].c;
+    g.bnd = (int)c->bnds.size();
+    for (int b = 0; b < nb; ++b) {
+      Bnd B;
+      memset(&B, 0, sizeof(B));
+      for (int d = 0; d < c->D; ++d) {
+        B.n[d] = normals[b * c->D + d];
+        B.p[d] = points[b * c->D + d];
+      }
+      c->bnds.push_back(B);
+    }
+    c->sys.push_back(g);
+    c->dirty_layout = true;
+    *ghost_out = (int)c->sys.size() - 1;
+  });
+}
+
+int gr_ghost_set_copier(gr_ctx *c, int ghost, int stage, int nfields, const int *fields,
+                        const int *modes) {
+  return guard(c, [&] {
+    validate_sys(c, ghost);
+    SysH &g = c->sys[ghost];
+    need(g.kind == GR_SYS_GHOST && stage >= 1, GR_ERR_ARGUMENT, "bad ghost/stage");
+    const SysH &src = c->sys[g.src];
+    if ((int)g.copy.size() < stage) g.copy.resize(stage, make_int2(0, -1));
+    for (int k = 0; k < nfields; ++k) {
+      const int f = fields[k];
+      const int m = modes ? modes[k] : GR_COPY_PLAIN;
+      if (f == GR_F_P) {
+        need(src.kind >= GR_SYS_FLUID, GR_ERR_ARGUMENT, "source has no p");
+        g.ex_p = true;
+        g.copy[stage - 1].x = 1;
+      } else if (f == GR_F_STRESS) {
+        need(src.kind >= GR_SYS_STRESS, GR_ERR_ARGUMENT, "source has no stress");
+        g.ex_stress = true;
+        g.copy[stage - 1].y = m;
+      } else {
+        throw GrError(GR_ERR_ARGUMENT, "GhostCopier supports :p and :stress");
+      }
+    }
+    c->plans.clear();
+  });
+}
+
+int gr_interaction_create(gr_ctx *c, int kernel_kind, double h, int sys_a, int partner, int wrap,
+                          const double *normal, const double *point, double beta,
+                          double lj_cutoff, int n_stages, const gr_pfn_desc *pfns,
+                          int *inter_out) {
+  return guard(c, [&] {
+    validate_sys(c, sys_a);
+    need(partner < (int)c->sys.size(), GR_ERR_ARGUMENT, "bad system_b");
+    need(kernel_kind == GR_KERNEL_CUBIC || kernel_kind == GR_KERNEL_WENDLAND_C2, GR_ERR_ARGUMENT,
+         "bad kernel kind");
+    need(c->sys[sys_a].kind != GR_SYS_GHOST, GR_ERR_ARGUMENT, "a ghost system cannot be system_a");
+    need(wrap == GR_WRAP_NONE || (partner >= 0 && c->sys[partner].kind != GR_SYS_GHOST),
+         GR_ERR_ARGUMENT, "boundary wrappers need a real partner system");
+    InterH in;
+    in.kk = kernel_kind;
+    in.h = h;
+    const int D = c->D;
+    const double H = h, pi = 3.141592653589793;
+    if (kernel_kind == GR_KERNEL_CUBIC)
+      in.norm = D == 2 ? 10.0 / ((7.0 * pi) * (H * H)) : 1.0 / (pi * (H * H * H));
+    else
+      in.norm = D == 2 ? 7.0 / ((64.0 * pi) * (H * H)) : 21.0 / ((256.0 * pi) * (H * H * H));
+    in.cell = 2.0 * h;
+    in.a = sys_a;
+    in.b = partner;
+    in.wrap = wrap;
+    for (int d = 0; d < D; ++d) {
+      in.wn[d] = normal ? normal[d] : 0.0;
+      in.wp[d] = point ? point[d] : 0.0;
+    }
+    in.beta = beta;
+    in.ljcut = lj_cutoff;
+    in.pfns.assign(pfns, pfns + n_stages);
+    const SysH &A = c->sys[sys_a];
+    for (const gr_pfn_desc &f : in.pfns) {
+      if (f.kind == GR_PFN_NONE) continue;
+      need(f.kind >= GR_PFN_FLUID && f.kind <= GR_PFN_COUNT, GR_ERR_ARGUMENT, "unknown functor kind");
+      if (f.kind == GR_PFN_FLUID)
+        need(f.sigma == 1 || f.sigma == 2, GR_ERR_ARGUMENT, "FluidPfn sigma must be 1 or 2");
+      need(!(f.kind == GR_PFN_FLUID && A.kind < GR_SYS_FLUID), GR_ERR_ARGUMENT, "FluidPfn needs p on system_a");
+      need(!(f.kind == GR_PFN_CAUCHY && A.kind < GR_SYS_STRESS), GR_ERR_ARGUMENT, "CauchyFluidPfn needs stress on system_a");
+      need(!(f.kind == GR_PFN_SR && A.kind < GR_SYS_STRESS), GR_ERR_ARGUMENT, "StrainRatePfn needs strain_rate on system_a");
+      need(!(f.kind == GR_PFN_SRV && A.kind != GR_SYS_EP), GR_ERR_ARGUMENT, "StrainRateVorticityPfn needs vorticity on system_a");
+      if (partner >= 0) {
+        const SysH &B = c->sys[partner];
+        const int pk = classify(c, in, f.kind);
+        if (B.kind == GR_SYS_GHOST) {
+          need(!(f.kind == GR_PFN_FLUID && !B.ex_p), GR_ERR_ARGUMENT,
+               "ghost partner of FluidPfn must copy :p (GhostCopier)");
+          need(!(f.kind == GR_PFN_CAUCHY && !B.ex_stress), GR_ERR_ARGUMENT,
+               "ghost partner of CauchyFluidPfn must copy :stress (GhostCopier)");
+        } else if (pk == P_TWO || pk == P_ONE) {
+          need(!(f.kind == GR_PFN_FLUID && B.kind < GR_SYS_FLUID), GR_ERR_ARGUMENT, "FluidPfn partner needs p");
+          need(!(f.kind == GR_PFN_CAUCHY && B.kind < GR_SYS_STRESS), GR_ERR_ARGUMENT, "CauchyFluidPfn partner needs stress");
+          need(!(f.kind == GR_PFN_CAUCHY && B.ns != A.ns), GR_ERR_ARGUMENT, "stress length mismatch");
+        }
+      }
+    }
+    c->inter.push_back(in);
+    *inter_out = (int)c->inter.size() - 1;
+  });
+}
+
+int gr_integrator_create(gr_ctx *c, int nsys, const int *systems, int ninter, const int *inters,
+                         int nghost, const int *ghosts, double gamma, int *integ_out) {
+  return guard(c, [&] {
+    need(nsys >= 1, GR_ERR_ARGUMENT, "systems must not be empty");
+    need(ninter >= 1, GR_ERR_ARGUMENT, "interactions must not be empty");
+    IntegH I;
+    I.sys.assign(systems, systems + nsys);
+    I.inter.assign(inters, inters + ninter);
+    if (nghost > 0) I.ghost.assign(ghosts, ghosts + nghost);
+    for (int s : I.sys) {
+      validate_sys(c, s);
+      need(c->sys[s].kind != GR_SYS_GHOST, GR_ERR_ARGUMENT, "ghost systems go in `ghosts`");
+    }
+    for (int g : I.ghost) {
+      validate_sys(c, g);
+      need(c->sys[g].kind == GR_SYS_GHOST, GR_ERR_ARGUMENT, "not a ghost system");
+    }
+    for (int t : I.inter) {
+      need(t >= 0 && t < (int)c->inter.size(), GR_ERR_ARGUMENT, "bad interaction");
+      const InterH &in = c->inter[t];
+      auto owned = [&](int s) {
+        return std::find(I.sys.begin(), I.sys.end(), s) != I.sys.end() ||
+               std::find(I.ghost.begin(), I.ghost.end(), s) != I.ghost.end();
+      };
+      need(owned(in.a) && (in.b < 0 || owned(in.b)), GR_ERR_ARGUMENT,
+           "every system of an interaction must be in the integrator's systems or ghosts");
+    }
+    I.gamma = gamma;
+    I.c = -INFINITY;
+    for (int s : I.sys) I.c = std::max(I.c, c->sys[s].c);
+    I.h = INFINITY;
+    for (int t : I.inter) I.h = std::min(I.h, c->inter[t].h);
+    I.nst = (int)c->inter[I.inter[0]].pfns.size();
+    for (int t : I.inter)
+      need((int)c->inter[t].pfns.size() == I.nst, GR_ERR_ASSERTION,
+           "All interactions must have the same number of stages (pfns length)");
+    c->integ.push_back(I);
+    *integ_out = (int)c->integ.size() - 1;
+  });
+}
+
+int gr_integrator_dt(gr_ctx *c, int integ, double cfl, double *dt_out) {
+  return guard(c, [&] {
+    need(integ >= 0 && integ < (int)c->integ.size(), GR_ERR_ARGUMENT, "bad integrator");
+    const IntegH &I = c->integ[integ];
+    *dt_out = cfl * I.h / I.c;
+  });
+}
+
+int gr_integrator_set_dt(gr_ctx *c, int integ, double dt) {
+  return guard(c, [&] {
+    need(integ >= 0 && integ < (int)c->integ.size(), GR_ERR_ARGUMENT, "bad integrator");
+    need(dt > 0, GR_ERR_ARGUMENT, "dt must be positive");
+    if (c->integ[integ].dt != dt) {
+      c->integ[integ].dt = dt;
+      c->plans.clear();
+    }
+  });
+}
+
+int gr_step(gr_ctx *c, int integ, int nsteps, double dt) {
+  return guard(c, [&] {
+    need(integ >= 0 && integ < (int)c->integ.size(), GR_ERR_ARGUMENT, "bad integrator");
+    need(nsteps >= 0, GR_ERR_ARGUMENT, "nsteps must be >= 0");
+    need(dt > 0, GR_ERR_ARGUMENT, "dt must be positive");
+    if (c->integ[integ].dt != dt) {
+      c->integ[integ].dt = dt;
+      c->plans.clear();
+    }
+    std::vector<int> b{integ};
+    by_prec(c, [&](auto *tag) {
+      using R = std::remove_pointer_t<decltype(tag)>;
+      do_steps<R>(c, b, nsteps);
+    });
+  });
+}
+
+int gr_step_many(gr_ctx *c, int ninteg, const int *integs, int nsteps) {
+  return guard(c, [&] {
+    std::vector<int> b(integs, integs + ninteg);
+    for (int ig : b) {
+      need(ig >= 0 && ig < (int)c->integ.size(), GR_ERR_ARGUMENT, "bad integrator");
+      need(c->integ[ig].dt > 0, GR_ERR_STATE, "integrator has no dt (gr_integrator_set_dt)");
+    }
+    std::vector<int> sorted = b;
+    std::sort(sorted.begin(), sorted.end());
+    need(std::unique(sorted.begin(), sorted.end()) == sorted.end(), GR_ERR_ARGUMENT, "duplicate integrator");
+    by_prec(c, [&](auto *tag) {
+      using R = std::remove_pointer_t<decltype(tag)>;
+      do_steps<R>(c, b, nsteps);
+    });
+  });
+}
+
+int gr_set_graph_mode(gr_ctx *c, int on) {
+  return guard(c, [&] { c->graph_mode = on != 0; });
+}
+int gr_set_timing(gr_ctx *c, int on) {
+  return guard(c, [&] { c->timing = on != 0; });
+}
+int gr_get_timing(gr_ctx *c, double *ms_out, int nphases) {
+  return guard(c, [&] {
+    for (int k = 0; k < nphases && k < 8; ++k) ms_out[k] = c->phase_ms[k];
+  });
+}
+int gr_get_pair_count(gr_ctx *c, int integ, int64_t *pairs_out) {
+  return guard(c, [&] {
+    (void)integ;
+    *pairs_out = c->last_pairs[0] / 2 + c->last_pairs[1];
+  });
+}
+
+// ---- fine-grained ---------------------------------------------------------
+
+int gr_sort(gr_ctx *c, int sys, double cutoff) {
+  return guard(c, [&] {
+    validate_sys(c, sys);
+    commit(c);
+    Plan P;
+    c->stepsys[sys].sort_cutoff = cutoff;
+    upload_stepsys(c);
+    if (c->sys[sys].kind == GR_SYS_GHOST) {
+      // copy the ghost's pool state to the generation scratch, then sort
+      plan_ghosts(c, P, {sys});
+      const int64_t n = live_n(c, sys), off = c->sys[sys].off, cap = c->pool_cap;
+      for (int d = 0; d < c->D; ++d)
+        HIPCHK(hipMemcpyAsync(c->gxs.as<double>() + d * cap + off, c->x[c->cur].as<double>() + d * cap + off,
+                              n * 8, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(c->gidx_s.as<int>() + off, c->gidx.as<int>() + off, n * 4, hipMemcpyDeviceToDevice, c->stream));
+      HIPCHK(hipMemcpyAsync(c->gbnd_s.as<int>() + off, c->gbnd.as<int>() + off, n * 4, hipMemcpyDeviceToDevice, c->stream));
+      by_prec(c, [&](auto *tag) {
+        using R = std::remove_pointer_t<decltype(tag)>;
+        phase_ghosts<R>(c, P, false, true);
+      });
+    } else {
+      plan_sort_real(c, P, {sys});
+      by_prec(c, [&](auto *tag) {
+        using R = std::remove_pointer_t<decltype(tag)>;
+        phase_sort_real<R>(c, P);
+      });
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int gr_create_grid(gr_ctx *c, int inter) {
+  return guard(c, [&] {
+    need(inter >= 0 && inter < (int)c->inter.size(), GR_ERR_ARGUMENT, "bad interaction");
+    commit(c);
+    InterH &in = c->inter[inter];
+    std::vector<int> syss{in.a};
+    if (in.b >= 0) syss.push_back(in.b);
+    std::vector<int> reals;
+    for (int s : syss)
+      if (c->sys[s].kind != GR_SYS_GHOST) reals.push_back(s);
+    ensure_cells(c, {inter}, reals);
+    // bounding boxes of the live particles
+    std::vector<Blk> blk;
+    for (int s : syss) add_blocks(blk, s, c->sys[s].cap, GR_BLOCK);
+    DBuf db, dl;
+    db.upload(blk, c->stream);
+    dl.upload(syss, c->stream);
+    launch_minmax_reset(c->stream, dl.as<int>(), (int)syss.size(), c->d_kmin.as<int>(), c->d_kmax.as<int>(),
+                        c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>());
+    XsView X = xs_ptrs(c);
+    launch_key_minmax(c->stream, X.pool[c->cur].as<const double *>(), c->d_sd.as<SysDesc>(),
+                      c->d_n.as<int>(), c->d_st.as<StepSys>(), db.as<Blk>(), (int)blk.size(), c->D,
+                      c->d_kmin.as<int>(), c->d_kmax.as<int>(), c->d_bbmin.as<long long>(),
+                      c->d_bbmax.as<long long>(), 0);
+    Plan P;
+    plan_grids(c, P, {inter});
+    phase_grids(c, P);
+    int flags[4];
+    HIPCHK(hipMemcpyAsync(flags, c->d_flags.p, sizeof(flags), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (flags[1] & 2) {
+      int z[4] = {0, 0, 0, 0};
+      HIPCHK(hipMemcpyAsync(c->d_flags.p, z, sizeof(z), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      throw GrError(GR_ERR_CAPACITY, "cell grid outgrew its capacity");
+    }
+    in.grid_built = true;
+  });
+}
+
+int gr_sweep(gr_ctx *c, int inter, int stage) {
+  return guard(c, [&] {
+    need(inter >= 0 && inter < (int)c->inter.size(), GR_ERR_ARGUMENT, "bad interaction");
+    InterH &in = c->inter[inter];
+    if (stage < 1 || stage > (int)in.pfns.size()) return;
+    const gr_pfn_desc f = in.pfns[stage - 1];
+    const int fam = family_of(f.kind);
+    if (fam < 0) return;
+    need(in.grid_built, GR_ERR_STATE, "sweep!: create_grid! has not been called");
+    std::vector<int> syss{in.a};
+    if (in.b >= 0) syss.push_back(in.b);
+    by_prec(c, [&](auto *tag) {
+      using R = std::remove_pointer_t<decltype(tag)>;
+      refresh_shadows<R>(c, syss);
+    });
+    const int pk = classify(c, in, f.kind);
+    std::vector<Task> tv;
+    std::vector<SrcDesc> sv;
+    std::vector<Blk> bv;
+    auto add = [&](int target, bool b_side) {
+      Task t;
+      memset(&t, 0, sizeof(t));
+      t.sys = target;
+      t.ns = c->sys[target].ns;
+      t.nv = c->sys[target].nv;
+      t.has_stress = f.kind == GR_PFN_CAUCHY;
+      t.write_vor = f.kind == GR_PFN_SRV;
+      t.assign = 0;
+      SrcDesc q = make_src(c, inter, target, b_side, pk);
+      set_pfn(q, f, pk, c, in);
+      t.src_begin = (int)sv.size();
+      sv.push_back(q);
+      t.src_end = (int)sv.size();
+      add_blocks(bv, (int)tv.size(), c->sys[target].cap, GR_BLOCK);
+      tv.push_back(t);
+    };
+    add(in.a, false);
+    if (pk == P_TWO && in.b >= 0) add(in.b, true);
+    DBuf dt, ds, db;
+    dt.upload(tv, c->stream);
+    ds.upload(sv, c->stream);
+    db.upload(bv, c->stream);
+    SweepArgs a;
+    a.sd = c->d_sd.as<SysDesc>();
+    a.n = c->d_n.as<int>();
+    a.tasks = dt.as<Task>();
+    a.srcs = ds.as<SrcDesc>();
+    a.grids = c->d_grids.as<Grid>();
+    a.carr = c->d_carr.as<CellArr>();
+    a.cells = c->cells.as<int>();
+    a.blk = db.as<Blk>();
+    a.nblk = (int)bv.size();
+    a.pair_count = c->d_pairs.as<unsigned long long>();
+    by_prec(c, [&](auto *tag) {
+      using R = std::remove_pointer_t<decltype(tag)>;
+      launch_sweep<R>(c->stream, pool_view<R>(c, c->cur), a, c->D, fam, false);
+    });
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int gr_update_state(gr_ctx *c, int sys, int stage, double dt) {
+  return guard(c, [&] {
+    validate_sys(c, sys);
+    commit(c);
+    SysH &s = c->sys[sys];
+    if (stage < 1 || stage > (int)s.upd.size()) return;
+    by_prec(c, [&](auto *tag) {
+      using R = std::remove_pointer_t<decltype(tag)>;
+      refresh_shadows<R>(c, {sys});
+    });
+    c->stepsys[sys].dt = dt;
+    upload_stepsys(c);
+    std::vector<UpdDesc> ud;
+    for (auto &u : s.upd[stage - 1])
+      if (u.kind != GR_UPD_NOTHING) ud.push_back(u);
+    if (ud.empty()) return;
+    std::vector<int2> ur(c->sys.size(), make_int2(0, 0));
+    ur[sys] = make_int2(0, (int)ud.size());
+    std::vector<Blk> bv;
+    add_blocks(bv, sys, s.cap, GR_BLOCK);
+    DBuf du, dr, db;
+    du.upload(ud, c->stream);
+    dr.upload(ur, c->stream);
+    db.upload(bv, c->stream);
+    by_prec(c, [&](auto *tag) {
+      using R = std::remove_pointer_t<decltype(tag)>;
+      launch_updaters<R>(c->stream, pool_view<R>(c, c->cur), c->d_sd.as<SysDesc>(), c->d_st.as<StepSys>(),
+                         db.as<Blk>(), (int)bv.size(), dr.as<int2>(), du.as<UpdDesc>(), c->D);
+    });
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int gr_generate_ghosts(gr_ctx *c, int ghost) {
+  return guard(c, [&] {
+    validate_sys(c, ghost);
+    need(c->sys[ghost].kind == GR_SYS_GHOST, GR_ERR_ARGUMENT, "not a ghost system");
+    commit(c);
+    Plan P;
+    plan_ghosts(c, P, {ghost});
+    by_prec(c, [&](auto *tag) {
+      using R = std::remove_pointer_t<decltype(tag)>;
+      phase_ghosts<R>(c, P, true, false);
+    });
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int gr_update_ghost_kinematics(gr_ctx *c, int ghost) {
+  return guard(c, [&] {
+    validate_sys(c, ghost);
+    need(c->sys[ghost].kind == GR_SYS_GHOST, GR_ERR_ARGUMENT, "not a ghost system");
+    commit(c);
+    std::vector<Blk> bv;
+    add_blocks(bv, ghost, c->sys[ghost].cap, GR_BLOCK);
+    DBuf db;
+    db.upload(bv, c->stream);
+    by_prec(c, [&](auto *tag) {
+      using R = std::remove_pointer_t<decltype(tag)>;
+      refresh_shadows<R>(c, {c->sys[ghost].src});
+      launch_ghost_kin<R>(c->stream, pool_view<R>(c, c->cur), c->d_sd.as<SysDesc>(), c->d_bnd.as<Bnd>(),
+                          c->d_n.as<int>(), db.as<Blk>(), (int)bv.size(), c->D);
+    });
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int gr_update_ghost(gr_ctx *c, int ghost, int stage) {
+  return guard(c, [&] {
+    validate_sys(c, ghost);
+    SysH &g = c->sys[ghost];
+    need(g.kind == GR_SYS_GHOST, GR_ERR_ARGUMENT, "not a ghost system");
+    commit(c);
+    if (stage < 1 || stage > (int)g.copy.size()) return;
+    const int2 cp = g.copy[stage - 1];
+    if (!cp.x && cp.y < 0) return;
+    std::vector<int2> gc(c->sys.size(), make_int2(0, -1));
+    gc[ghost] = cp;
+    std::vector<Blk> bv;
+    add_blocks(bv, ghost, g.cap, GR_BLOCK);
+    DBuf dg, db;
+    dg.upload(gc, c->stream);
+    db.upload(bv, c->stream);
+    by_prec(c, [&](auto *tag) {
+      using R = std::remove_pointer_t<decltype(tag)>;
+      launch_ghost_copy<R>(c->stream, pool_view<R>(c, c->cur), c->d_sd.as<SysDesc>(), c->d_bnd.as<Bnd>(),
+                           c->d_n.as<int>(), db.as<Blk>(), (int)bv.size(), dg.as<int2>(), c->D);
+    });
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int gr_get_grid(gr_ctx *c, int inter, double *mingridx, int64_t *ngridx, int64_t *ncells) {
+  return guard(c, [&] {
+    need(inter >= 0 && inter < (int)c->inter.size(), GR_ERR_ARGUMENT, "bad interaction");
+    need(c->inter[inter].carr_b >= 0, GR_ERR_STATE, "grid not built");
+    Grid G;
+    HIPCHK(hipMemcpyAsync(&G, c->d_grids.as<Grid>() + inter, sizeof(G), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int d = 0; d < c->D; ++d) {
+      mingridx[d] = G.ming[d];
+      ngridx[d] = G.ng[d];
+    }
+    *ncells = G.ncells;
+  });
+}
+
+int gr_get_cell_start(gr_ctx *c, int inter, int which, int64_t *out, int64_t count) {
+  return guard(c, [&] {
+    need(inter >= 0 && inter < (int)c->inter.size(), GR_ERR_ARGUMENT, "bad interaction");
+    const InterH &in = c->inter[inter];
+    const int ca = which ? in.carr_a : in.carr_b;
+    need(ca >= 0, GR_ERR_ARGUMENT, "no such cell array");
+    Grid G;
+    HIPCHK(hipMemcpyAsync(&G, c->d_grids.as<Grid>() + inter, sizeof(G), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    need(count == (int64_t)G.ncells + 1, GR_ERR_ARGUMENT, "count mismatch");
+    auto v = read_ints(c, c->cells, c->carr[ca].off, count);
+    for (int64_t i = 0; i < count; ++i) out[i] = v[i];
+  });
+}
+
+}  // extern "C"

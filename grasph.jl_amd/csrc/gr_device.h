@@ -1,0 +1,136 @@
+// gr_device.h — device-side data layout and descriptors of the particle pool.
+//
+// One context owns one packed structure-of-arrays pool.  Every particle
+// system (real or ghost) is a segment [off, off + cap) of the pool; a real
+// system's count is fixed, a ghost system's count is written on the device
+// each step.  Kernels are launched once per phase over a block table that
+// maps each workgroup to (segment, first local index), so one launch serves
+// every system of every simulation in the pool.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define GR_MAXD 3
+#define GR_MAXNS 6
+#define GR_BLOCK 256
+
+namespace gr {
+
+// Partner relation of one gather source (functor dispatch of the reference
+// on (system_a, system_b) types, src/PairwiseFunctors.jl).
+enum PartnerKind : int32_t {
+  PK_SELF = 0,       // same system, j != i
+  PK_TWO_SIDED = 1,  // real partner, both sides written (each side gathers)
+  PK_ONE_SIDED = 2,  // generic one-sided (ghosts, other real systems)
+  PK_STATIC = 3,     // StaticBoundarySystem: LJ + artificial viscosity
+  PK_DYNAMIC = 4     // DynamicBoundarySystem: mirrored velocity
+};
+
+// Functor families with a kernel each.
+enum Family : int32_t { FAM_FORCE = 0, FAM_STRAIN = 1, FAM_COUNT = 2 };
+
+struct SysDesc {
+  int64_t off;     // pool offset of local index 0
+  int32_t cap;     // capacity (real: n; ghost: nb * n_source)
+  int32_t kind;    // GR_SYS_*
+  int32_t ns, nv;  // stress / vorticity components
+  int32_t src;     // ghost: source system; else -1
+  int32_t nb;      // ghost: boundaries
+  int32_t bnd;     // ghost: first boundary in the boundary table
+  int32_t pad0;
+  double gcut;     // ghost cutoff
+  double mass, c;  // ghosts carry their source's
+  double sv[3], srho;
+};
+
+// Per-(system) parameters of one step of a batch.
+struct StepSys {
+  double dt, half, gdt;   // dt, dt/2, Γ/dt
+  double sort_cutoff;     // 2·h of the integrator
+  int32_t reset_force;    // half-step writes dvdt/drhodt ← source
+  int32_t fuse_eos;       // 0 none, GR_UPD_TAIT, GR_UPD_LINEAR
+  double eos_rho0, eos_gamma;
+  int32_t damping;        // Γ != 0
+  int32_t pad;
+};
+
+struct Bnd {  // ghost boundary plane
+  double n[3], p[3];
+};
+
+// Cell grid of one interaction (src/Interaction.jl:29-35), written on the
+// device each step.
+struct Grid {
+  double ming[3];
+  int32_t ng[3];
+  int32_t ncells;
+  int32_t ok;
+  int32_t pad;
+};
+
+// One CSR array of one interaction (role a or b).
+struct CellArr {
+  int64_t off;    // offset in the cell pool
+  int32_t cap;    // capacity in entries (ncells + 1 must fit)
+  int32_t grid;   // interaction index
+  int32_t sys;    // system whose particles it indexes
+  int32_t pad;
+};
+
+// One gather source of a target system (one interaction, one side).
+struct SrcDesc {
+  int32_t pk, pfn, partner, grid;
+  int32_t carr;      // CellArr index of the partner particles
+  int32_t kk, sigma, has_delta;
+  int32_t has_eps, pad;
+  double h, norm, cutoff2, cell;  // kernel
+  double alpha, beta, hf, delta, eps;
+  double mass_q;                  // partner mass used by the functor
+  double c_t_piv, c_q_piv, c_t_psi, c_q_psi;
+  double wn[3], wp[3], wbeta, ljcut;
+};
+
+// One target system of one (stage, family) sweep launch.
+struct Task {
+  int32_t sys;
+  int32_t src_begin, src_end;
+  int32_t assign;   // 1: write source(+)acc, 0: accumulate (+=)
+  int32_t ns, nv;
+  int32_t has_stress;
+  int32_t write_vor;  // an SRV source writes vorticity
+};
+
+// One state updater of one (system, stage) (src/StateUpdaters.jl).
+struct UpdDesc {
+  int32_t kind;
+  int32_t pad;
+  long long mask;   // ZeroFieldUpdater field mask
+  double prm[5];
+};
+
+struct Blk {  // block table entry
+  int32_t item;   // system / task / cell-array index
+  int32_t first;  // first local index
+};
+
+// Device view of the pool for one precision.
+template <typename R>
+struct Pool {
+  double *x[3], *v[3], *rho, *rho_h;
+  R *vs[3], *rhos, *dvdt[3], *drhodt, *p;
+  double *stress[GR_MAXNS];
+  R *stress_s[GR_MAXNS], *sr[GR_MAXNS], *vor[3];
+  double *strain[GR_MAXNS], *strainp[GR_MAXNS];
+  int32_t *id, *gidx, *gbnd, *gnrm;
+};
+
+// Double → ordered int64 (monotone, -0.0 < +0.0) for atomic min/max.
+__device__ inline long long ord_of(double d) {
+  long long i = __double_as_longlong(d);
+  return i < 0 ? (i ^ 0x7FFFFFFFFFFFFFFFLL) : i;
+}
+__device__ inline double double_of(long long i) {
+  return __longlong_as_double(i < 0 ? (i ^ 0x7FFFFFFFFFFFFFFFLL) : i);
+}
+
+}  // namespace gr

@@ -1,0 +1,1163 @@
+// gr_kernels.hip — per-particle, sort, ghost and grid kernels for gfx950.
+//
+// Integer-deciding floating point (cell keys, CSR cells, ghost qualification,
+// positions) is evaluated in FP64 in the order the reference writes it; the
+// library is compiled with -ffp-contract=off so nothing is fused unless an
+// explicit fma() is written (the reference's @fastmath scalar axpy).
+#include <hip/hip_runtime.h>
+
+#include "../../include/grasph_hip.h"
+#include "gr_device.h"
+#include "gr_launch.h"
+
+namespace gr {
+
+// ---------------------------------------------------------------------------
+// Wave / block reductions (64-lane wavefronts)
+// ---------------------------------------------------------------------------
+
+__device__ inline long long wave_min_ll(long long v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    long long u = __shfl_xor(v, o, 64);
+    v = u < v ? u : v;
+  }
+  return v;
+}
+__device__ inline long long wave_max_ll(long long v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    long long u = __shfl_xor(v, o, 64);
+    v = u > v ? u : v;
+  }
+  return v;
+}
+__device__ inline int wave_min_i(int v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    int u = __shfl_xor(v, o, 64);
+    v = u < v ? u : v;
+  }
+  return v;
+}
+__device__ inline int wave_max_i(int v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    int u = __shfl_xor(v, o, 64);
+    v = u > v ? u : v;
+  }
+  return v;
+}
+
+__device__ inline long long ord_d(double d) {
+  long long i = __double_as_longlong(d);
+  return i < 0 ? (i ^ 0x7FFFFFFFFFFFFFFFLL) : i;
+}
+
+// floor(Int, x / cutoff) (src/Sorting.jl:122-124), clamped to int32.
+__device__ inline int key_comp(double x, double cutoff) {
+  double f = floor(x / cutoff);
+  if (!(f > -1.0e9)) f = -1.0e9;
+  if (f > 1.0e9) f = 1.0e9;
+  return (int)f;
+}
+
+// ---------------------------------------------------------------------------
+// Keys: per-system key range and position bounding box
+// ---------------------------------------------------------------------------
+
+// Block reduction of min/max of the integer cell key and of the position per
+// dimension for the block's system; one atomic per block per quantity.
+__global__ void k_key_minmax(const double *const *xs, const SysDesc *sd, const int *n_dev,
+                             const StepSys *st, const Blk *blk, int D, int *kmin, int *kmax,
+                             long long *bbmin, long long *bbmax, int with_keys) {
+  __shared__ long long red[4][GR_BLOCK / 64][3];
+  const Blk b = blk[blockIdx.x];
+  const int s = b.item;
+  const int i = b.first + threadIdx.x;
+  const int n = n_dev[s];
+  const bool live = i < n;
+  const int64_t gi = sd[s].off + i;
+  const double cutoff = st[s].sort_cutoff;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int d = 0; d < D; ++d) {
+    double xv = live ? xs[d][gi] : 0.0;
+    long long o = ord_d(xv);
+    long long mn = live ? o : LLONG_MAX, mx = live ? o : LLONG_MIN;
+    int k = (live && with_keys) ? key_comp(xv, cutoff) : 0;
+    int kmn = live ? k : INT_MAX, kmx = live ? k : INT_MIN;
+    mn = wave_min_ll(mn);
+    mx = wave_max_ll(mx);
+    kmn = wave_min_i(kmn);
+    kmx = wave_max_i(kmx);
+    if (lane == 0) {
+      red[0][w][d] = mn;
+      red[1][w][d] = mx;
+      red[2][w][d] = kmn;
+      red[3][w][d] = kmx;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < D) {
+    const int d = threadIdx.x;
+    long long mn = LLONG_MAX, mx = LLONG_MIN;
+    int kmn = INT_MAX, kmx = INT_MIN;
+    for (int q = 0; q < GR_BLOCK / 64; ++q) {
+      mn = red[0][q][d] < mn ? red[0][q][d] : mn;
+      mx = red[1][q][d] > mx ? red[1][q][d] : mx;
+      kmn = (int)red[2][q][d] < kmn ? (int)red[2][q][d] : kmn;
+      kmx = (int)red[3][q][d] > kmx ? (int)red[3][q][d] : kmx;
+    }
+    if (mn != LLONG_MAX) {
+      atomicMin(&bbmin[s * 3 + d], mn);
+      atomicMax(&bbmax[s * 3 + d], mx);
+      if (with_keys) {
+        atomicMin(&kmin[s * 3 + d], kmn);
+        atomicMax(&kmax[s * 3 + d], kmx);
+      }
+    }
+  }
+}
+
+// Reset the reduction slots of the listed systems.
+__global__ void k_minmax_reset(const int *list, int nlist, int *kmin, int *kmax, long long *bbmin,
+                               long long *bbmax) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nlist * 3) return;
+  int s = list[t / 3], d = t % 3;
+  kmin[s * 3 + d] = INT_MAX;
+  kmax[s * 3 + d] = INT_MIN;
+  bbmin[s * 3 + d] = LLONG_MAX;
+  bbmax[s * 3 + d] = LLONG_MIN;
+}
+
+// Key space of each listed system: lexicographic linearisation of
+// (k1 - kmin1, ..., kD - kminD), bases prefix-summed in list order.  Ghost
+// systems reserve one extra key for padding slots.  flags[0] (the inversion
+// flag) is cleared here; flags[1] records overflow.
+__global__ void k_key_base(const int *list, int nlist, int D, int pad, const int *n_dev,
+                           const int *kmin, const int *kmax, unsigned *kbase, unsigned *krange,
+                           int *flags) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  unsigned long long base = 0;
+  flags[0] = 0;
+  for (int q = 0; q < nlist; ++q) {
+    int s = list[q];
+    unsigned long long range = 1;
+    if (n_dev[s] > 0) {
+      for (int d = 0; d < D; ++d) {
+        long long r = (long long)kmax[s * 3 + d] - (long long)kmin[s * 3 + d] + 1;
+        krange[s * 3 + d] = (unsigned)r;
+        range *= (unsigned long long)r;
+      }
+    } else {
+      for (int d = 0; d < D; ++d) krange[s * 3 + d] = 1;
+      range = 0;
+    }
+    kbase[s] = (unsigned)base;
+    base += range + (pad ? 1 : 0);
+    if (base >= 0xFFFFFFFFull) flags[1] |= 1;
+  }
+}
+
+__device__ inline unsigned lin_key(const double *const *xs, int64_t gi, int D, double cutoff,
+                                   const int *kmin, const unsigned *krange, int s) {
+  unsigned k = 0;
+  for (int d = 0; d < D; ++d) {
+    int c = key_comp(xs[d][gi], cutoff) - kmin[s * 3 + d];
+    k = k * krange[s * 3 + d] + (unsigned)c;
+  }
+  return k;
+}
+
+// Write (key, pool index) at each particle's compact position and flag any
+// key inversion inside a system (src/Sorting.jl:179-186).
+__global__ void k_key_write(const double *const *xs, const SysDesc *sd, const int *n_dev,
+                            const StepSys *st, const Blk *blk, const int *bbase, int D,
+                            const int *kmin, const unsigned *krange, const unsigned *kbase,
+                            unsigned *keys, int *vals, int *flags) {
+  const Blk b = blk[blockIdx.x];
+  const int s = b.item;
+  const int i = b.first + threadIdx.x;
+  const int cap = sd[s].cap;
+  if (i >= cap) return;
+  const int n = n_dev[s];
+  const int64_t gi = sd[s].off + i;
+  const int p = bbase[s] + i;
+  const double cutoff = st[s].sort_cutoff;
+  unsigned key;
+  if (i < n) {
+    unsigned rng = 1;
+    for (int d = 0; d < D; ++d) rng *= krange[s * 3 + d];
+    key = kbase[s] + lin_key(xs, gi, D, cutoff, kmin, krange, s);
+    if (i + 1 < n) {
+      unsigned k2 = kbase[s] + lin_key(xs, gi + 1, D, cutoff, kmin, krange, s);
+      if (k2 < key) atomicOr(&flags[0], 1);
+    }
+    (void)rng;
+  } else {
+    unsigned rng = 1;
+    for (int d = 0; d < D; ++d) rng *= krange[s * 3 + d];
+    key = kbase[s] + (n > 0 ? rng : 0u);
+  }
+  keys[p] = key;
+  vals[p] = (int)gi;
+}
+
+// ---------------------------------------------------------------------------
+// LSD radix sort, 8-bit digits, stable.  Skipped entirely (early return)
+// when flags[0] == 0: the keys are already in order.
+// ---------------------------------------------------------------------------
+
+#define RADIX_ITEMS 8
+#define RADIX_TILE (GR_BLOCK * RADIX_ITEMS)
+
+__global__ void k_radix_hist(const unsigned *keys, int N, int shift, unsigned *hist, int nblk,
+                             const int *flags) {
+  if (flags[0] == 0) return;
+  __shared__ unsigned h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int base = blockIdx.x * RADIX_TILE;
+  for (int r = 0; r < RADIX_ITEMS; ++r) {
+    int e = base + r * GR_BLOCK + threadIdx.x;
+    if (e < N) atomicAdd(&h[(keys[e] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of 256·nblk counters in digit-major order (one block).
+__global__ void k_radix_scan(unsigned *hist, int total, const int *flags) {
+  if (flags[0] == 0) return;
+  __shared__ unsigned part[1024];
+  const int t = threadIdx.x;
+  const int per = (total + 1023) / 1024;
+  const int b0 = t * per, b1 = min(total, b0 + per);
+  unsigned s = 0;
+  for (int k = b0; k < b1; ++k) s += hist[k];
+  part[t] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    unsigned v = t >= o ? part[t - o] : 0u;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  unsigned run = t > 0 ? part[t - 1] : 0u;
+  for (int k = b0; k < b1; ++k) {
+    unsigned c = hist[k];
+    hist[k] = run;
+    run += c;
+  }
+}
+
+// Stable scatter: element e of the tile is ranked after every earlier
+// element with the same digit (round-major, then lane order), using 64-lane
+// ballots to find equal digits within a wave.
+__global__ void k_radix_scatter(const unsigned *kin, const int *vin, unsigned *kout, int *vout,
+                                int N, int shift, const unsigned *hist, int nblk,
+                                const int *flags) {
+  if (flags[0] == 0) return;
+  __shared__ unsigned run[256];
+  __shared__ unsigned wcnt[GR_BLOCK / 64][256];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  run[t] = hist[t * nblk + blockIdx.x];
+  for (int q = 0; q < GR_BLOCK / 64; ++q) wcnt[q][t] = 0;
+  __syncthreads();
+  const int base = blockIdx.x * RADIX_TILE;
+  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  for (int r = 0; r < RADIX_ITEMS; ++r) {
+    const int e = base + r * GR_BLOCK + t;
+    const bool valid = e < N;
+    unsigned key = valid ? kin[e] : 0u;
+    int val = valid ? vin[e] : 0;
+    unsigned dig = (key >> shift) & 255u;
+    unsigned long long match = __ballot(valid);
+    for (int bit = 0; bit < 8; ++bit) {
+      unsigned long long bal = __ballot((dig >> bit) & 1u);
+      match &= ((dig >> bit) & 1u) ? bal : ~bal;
+    }
+    const int rank = __popcll(match & lt_mask);
+    if (valid && rank == 0) wcnt[w][dig] = (unsigned)__popcll(match);
+    __syncthreads();
+    if (valid) {
+      unsigned pre = run[dig];
+      for (int q = 0; q < w; ++q) pre += wcnt[q][dig];
+      unsigned pos = pre + rank;
+      kout[pos] = key;
+      vout[pos] = val;
+    }
+    __syncthreads();
+    {
+      unsigned add = 0;
+      for (int q = 0; q < GR_BLOCK / 64; ++q) {
+        add += wcnt[q][t];
+        wcnt[q][t] = 0;
+      }
+      run[t] += add;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Permutation of real systems (src/Sorting.jl:88-105): pool A → pool B for
+// every real particle; identity where the system is not sorted this step.
+// ---------------------------------------------------------------------------
+
+template <typename R>
+__global__ void k_permute_real(Pool<R> A, Pool<R> B, const SysDesc *sd, const Blk *blk,
+                               const int *bbase, const int *vals, const int *flags, int D) {
+  const Blk b = blk[blockIdx.x];
+  const int s = b.item;
+  const int i = b.first + threadIdx.x;
+  const SysDesc S = sd[s];
+  if (i >= S.cap) return;
+  const int64_t dst = S.off + i;
+  int64_t src = dst;
+  if (flags[0] != 0 && bbase[s] >= 0) src = vals[bbase[s] + i];
+  for (int d = 0; d < D; ++d) {
+    B.x[d][dst] = A.x[d][src];
+    B.v[d][dst] = A.v[d][src];
+    B.dvdt[d][dst] = A.dvdt[d][src];
+  }
+  B.rho[dst] = A.rho[src];
+  B.drhodt[dst] = A.drhodt[src];
+  B.id[dst] = A.id[src];
+  if (S.kind >= GR_SYS_FLUID) B.p[dst] = A.p[src];
+  if (S.kind >= GR_SYS_STRESS) {
+    for (int k = 0; k < S.ns; ++k) {
+      double sg = A.stress[k][src];
+      B.stress[k][dst] = sg;
+      B.stress_s[k][dst] = (R)sg;
+      B.sr[k][dst] = A.sr[k][src];
+    }
+  }
+  if (S.kind == GR_SYS_EP) {
+    for (int k = 0; k < S.nv; ++k) B.vor[k][dst] = A.vor[k][src];
+    for (int k = 0; k < S.ns; ++k) {
+      B.strain[k][dst] = A.strain[k][src];
+      B.strainp[k][dst] = A.strainp[k][src];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Ghosts (src/GhostParticles.jl:344-421)
+// ---------------------------------------------------------------------------
+
+// Qualification of candidate (boundary b, source particle i) of ghost g,
+// laid out boundary-major: abs(da) <= cutoff && da > 0.
+__global__ void k_ghost_flags(const double *const *x, const SysDesc *sd, const Bnd *bnd,
+                              const Blk *blk, const int *cand_off, int D, int *flags) {
+  const Blk b = blk[blockIdx.x];
+  const int g = b.item;
+  const int c = b.first + threadIdx.x;
+  const SysDesc G = sd[g];
+  const SysDesc S = sd[G.src];
+  if (c >= G.cap) return;
+  const int bi = c / S.cap, i = c % S.cap;
+  const Bnd B = bnd[G.bnd + bi];
+  const int64_t gi = S.off + i;
+  double da = (x[0][gi] - B.p[0]) * B.n[0];
+  for (int d = 1; d < D; ++d) da = da + (x[d][gi] - B.p[d]) * B.n[d];
+  flags[cand_off[g] + c] = (fabs(da) <= G.gcut && da > 0.0) ? 1 : 0;
+}
+
+// Scatter qualifying candidates to generation order: mirrored position,
+// source index, boundary index, and the boundary whose normal slot k keeps
+// (normals are not permuted by the ghost sort, src/Sorting.jl:33-40).
+__global__ void k_ghost_scatter(const double *const *x, double *const *gx, int *gidx, int *gbnd,
+                                int *gnrm, const SysDesc *sd, const Bnd *bnd, const Blk *blk,
+                                const int *cand_off, const int *scan, int D, int *n_dev) {
+  const Blk b = blk[blockIdx.x];
+  const int g = b.item;
+  const int c = b.first + threadIdx.x;
+  const SysDesc G = sd[g];
+  const SysDesc S = sd[G.src];
+  if (c >= G.cap) return;
+  const int c0 = cand_off[g];
+  const int pos = scan[c0 + c], nxt = scan[c0 + c + 1];
+  if (c == G.cap - 1) n_dev[g] = nxt - scan[c0];
+  if (nxt == pos) return;
+  const int k = pos - scan[c0];
+  const int bi = c / S.cap, i = c % S.cap;
+  const Bnd B = bnd[G.bnd + bi];
+  const int64_t gi = S.off + i;
+  double da = (x[0][gi] - B.p[0]) * B.n[0];
+  for (int d = 1; d < D; ++d) da = da + (x[d][gi] - B.p[d]) * B.n[d];
+  const int64_t go = G.off + k;
+  for (int d = 0; d < D; ++d) gx[d][go] = x[d][gi] - (2.0 * da) * B.n[d];
+  gidx[go] = i;
+  gbnd[go] = bi;
+  gnrm[go] = bi;
+}
+
+// Ghost sort permutation: generation scratch → pool (x, idx_original,
+// idx_boundary).
+__global__ void k_permute_ghost(const double *const *gxs, const int *gidx_s, const int *gbnd_s,
+                                double *const *x, int *gidx, int *gbnd, const SysDesc *sd,
+                                const int *n_dev, const Blk *blk, const int *bbase,
+                                const int *vals, const int *flags, int D) {
+  const Blk b = blk[blockIdx.x];
+  const int g = b.item;
+  const int k = b.first + threadIdx.x;
+  const SysDesc G = sd[g];
+  if (k >= n_dev[g]) return;
+  const int64_t dst = G.off + k;
+  int64_t src = dst;
+  if (flags[0] != 0) src = vals[bbase[g] + k];
+  for (int d = 0; d < D; ++d) x[d][dst] = gxs[d][src];
+  gidx[dst] = gidx_s[src];
+  gbnd[dst] = gbnd_s[src];
+}
+
+// update_ghost_kinematics! (:404-421): reflected half-step velocity of the
+// source and its density, with the normal of slot k.
+template <typename R>
+__global__ void k_ghost_kin(Pool<R> P, const SysDesc *sd, const Bnd *bnd, const int *n_dev,
+                            const Blk *blk, int D) {
+  const Blk b = blk[blockIdx.x];
+  const int g = b.item;
+  const int k = b.first + threadIdx.x;
+  const SysDesc G = sd[g];
+  if (k >= n_dev[g]) return;
+  const SysDesc S = sd[G.src];
+  const int64_t go = G.off + k;
+  const int64_t si = S.off + P.gidx[go];
+  const Bnd B = bnd[G.bnd + P.gnrm[go]];
+  R vr[3];
+  for (int d = 0; d < D; ++d) vr[d] = P.vs[d][si];
+  R dd = vr[0] * (R)B.n[0];
+  for (int d = 1; d < D; ++d) dd = dd + vr[d] * (R)B.n[d];
+  for (int d = 0; d < D; ++d) P.vs[d][go] = vr[d] - (R)2 * dd * (R)B.n[d];
+  P.rhos[go] = P.rhos[si];
+}
+
+// _apply_mode(σ, HouseholderReflect(), n̂) (:66-110) in FP64.
+__device__ inline void householder(const double *s, const double *n, int NS, double *o) {
+  if (NS == 3) {
+    double nx = n[0], ny = n[1];
+    double tx = s[0] * nx + s[2] * ny, ty = s[2] * nx + s[1] * ny;
+    double q = tx * nx + ty * ny;
+    o[0] = s[0] - 4 * tx * nx + 4 * q * nx * nx;
+    o[1] = s[1] - 4 * ty * ny + 4 * q * ny * ny;
+    o[2] = s[2] - 2 * (tx * ny + nx * ty) + 4 * q * nx * ny;
+  } else if (NS == 4) {
+    double nx = n[0], ny = n[1];
+    double tx = s[0] * nx + s[3] * ny, ty = s[3] * nx + s[1] * ny;
+    double q = tx * nx + ty * ny;
+    o[0] = s[0] - 4 * tx * nx + 4 * q * nx * nx;
+    o[1] = s[1] - 4 * ty * ny + 4 * q * ny * ny;
+    o[2] = s[2];
+    o[3] = s[3] - 2 * (tx * ny + nx * ty) + 4 * q * nx * ny;
+  } else {
+    double nx = n[0], ny = n[1], nz = n[2];
+    double tx = s[0] * nx + s[3] * ny + s[4] * nz;
+    double ty = s[3] * nx + s[1] * ny + s[5] * nz;
+    double tz = s[4] * nx + s[5] * ny + s[2] * nz;
+    double q = tx * nx + ty * ny + tz * nz;
+    o[0] = s[0] - 4 * tx * nx + 4 * q * nx * nx;
+    o[1] = s[1] - 4 * ty * ny + 4 * q * ny * ny;
+    o[2] = s[2] - 4 * tz * nz + 4 * q * nz * nz;
+    o[3] = s[3] - 2 * (tx * ny + nx * ty) + 4 * q * nx * ny;
+    o[4] = s[4] - 2 * (tx * nz + nx * tz) + 4 * q * nx * nz;
+    o[5] = s[5] - 2 * (ty * nz + ny * tz) + 4 * q * ny * nz;
+  }
+}
+
+// GhostCopier of one stage (:112-129): copy p, copy or reflect stress.
+// copy[g] = {copy_p, stress_mode (-1 none, 0 plain, 1 Householder)}.
+template <typename R>
+__global__ void k_ghost_copy(Pool<R> P, const SysDesc *sd, const Bnd *bnd, const int *n_dev,
+                             const Blk *blk, const int2 *copy, int D) {
+  const Blk b = blk[blockIdx.x];
+  const int g = b.item;
+  const int k = b.first + threadIdx.x;
+  const SysDesc G = sd[g];
+  if (k >= n_dev[g]) return;
+  const SysDesc S = sd[G.src];
+  const int64_t go = G.off + k;
+  const int64_t si = S.off + P.gidx[go];
+  const int2 cp = copy[g];
+  if (cp.x) P.p[go] = P.p[si];
+  if (cp.y >= 0) {
+    const int NS = S.ns;
+    double s[6], o[6];
+    for (int q = 0; q < NS; ++q) s[q] = P.stress[q][si];
+    if (cp.y == GR_COPY_HOUSEHOLDER) {
+      const Bnd B = bnd[G.bnd + P.gnrm[go]];
+      householder(s, B.n, NS, o);
+    } else {
+      for (int q = 0; q < NS; ++q) o[q] = s[q];
+    }
+    for (int q = 0; q < NS; ++q) P.stress_s[q][go] = (R)o[q];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Device-wide exclusive scan of int32 (out has N + 1 entries, out[N] = total)
+// ---------------------------------------------------------------------------
+
+#define SCAN_TILE 1024
+
+__global__ void k_scan_local(const int *in, int *out, int *bsum, int N) {
+  __shared__ int s[SCAN_TILE];
+  const int t = threadIdx.x;  // 256 threads × 4
+  const int base = blockIdx.x * SCAN_TILE;
+  int v[4], tot = 0;
+  for (int q = 0; q < 4; ++q) {
+    int e = base + t * 4 + q;
+    v[q] = e < N ? in[e] : 0;
+    tot += v[q];
+  }
+  s[t] = tot;
+  __syncthreads();
+  for (int o = 1; o < 256; o <<= 1) {
+    int a = t >= o ? s[t - o] : 0;
+    __syncthreads();
+    s[t] += a;
+    __syncthreads();
+  }
+  int run = t > 0 ? s[t - 1] : 0;
+  for (int q = 0; q < 4; ++q) {
+    int e = base + t * 4 + q;
+    if (e < N) out[e] = run;
+    run += v[q];
+  }
+  if (t == 255) bsum[blockIdx.x] = s[255];
+}
+
+__global__ void k_scan_blocks(int *bsum, int nb, int *out, int N) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x;
+  const int per = (nb + 1023) / 1024;
+  const int b0 = t * per, b1 = min(nb, b0 + per);
+  int s = 0;
+  for (int k = b0; k < b1; ++k) s += bsum[k];
+  part[t] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {
+    int v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = t > 0 ? part[t - 1] : 0;
+  for (int k = b0; k < b1; ++k) {
+    int c = bsum[k];
+    bsum[k] = run;
+    run += c;
+  }
+  if (t == 1023) out[N] = part[1023];
+}
+
+__global__ void k_scan_add(int *out, const int *bsum, int N) {
+  const int e = blockIdx.x * SCAN_TILE + threadIdx.x * 4;
+  const int add = bsum[blockIdx.x];
+  for (int q = 0; q < 4; ++q)
+    if (e + q < N) out[e + q] += add;
+}
+
+// ---------------------------------------------------------------------------
+// Grid / CSR (src/Interaction.jl:123-236)
+// ---------------------------------------------------------------------------
+
+// Grid of each listed interaction from the bounding boxes of its systems.
+// inter_sys[2q] = a, inter_sys[2q+1] = b (or -1); cell arrays checked
+// against their capacity.
+__global__ void k_grid_params(const int *inter_list, int n, const int2 *inter_sys,
+                              const double *cellsz, const long long *bbmin,
+                              const long long *bbmax, const int *n_dev, Grid *grids,
+                              const int2 *inter_carr, const CellArr *carr, int D, int *flags) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= n) return;
+  const int it = inter_list[q];
+  const int2 ab = inter_sys[it];
+  const double cutoff = cellsz[it];
+  Grid G;
+  long long ncells = 1;
+  for (int d = 0; d < D; ++d) {
+    double mn = double_of(bbmin[ab.x * 3 + d]), mx = double_of(bbmax[ab.x * 3 + d]);
+    if (ab.y >= 0 && n_dev[ab.y] > 0) {
+      double mnb = double_of(bbmin[ab.y * 3 + d]), mxb = double_of(bbmax[ab.y * 3 + d]);
+      mn = mnb < mn ? mnb : mn;
+      mx = mxb > mx ? mxb : mx;
+    }
+    G.ming[d] = floor((mn - 2.0 * cutoff) / cutoff) * cutoff;  // _snap_to_grid :191
+    double maxg = mx + 2.0 * cutoff;
+    double ngd = floor((maxg - G.ming[d]) / cutoff) + 1.0;  // :179
+    if (!(ngd >= 1.0) || ngd > 1.0e9) ngd = 1.0e9;
+    G.ng[d] = (int)ngd;
+    ncells *= (long long)G.ng[d];
+  }
+  for (int d = D; d < 3; ++d) {
+    G.ming[d] = 0.0;
+    G.ng[d] = 1;
+  }
+  G.ok = 1;
+  const int2 ca = inter_carr[it];
+  long long cap = carr[ca.x].cap;
+  if (ca.y >= 0 && carr[ca.y].cap < cap) cap = carr[ca.y].cap;
+  if (ncells + 1 > cap || ncells < 1) {
+    G.ok = 0;
+    atomicOr(&flags[1], 2);
+    ncells = 0;
+  }
+  G.ncells = (int)ncells;
+  G.pad = 0;
+  grids[it] = G;
+}
+
+__device__ inline int cell_flat0(const double *const *x, int64_t gi, const Grid &G, double cell,
+                                 int D) {  // _cell_1idx − 1 (:197-206)
+  long long flat = 0, stride = 1;
+  for (int d = D - 1; d >= 0; --d) {
+    long long cd = (long long)floor((x[d][gi] - G.ming[d]) / cell);
+    flat += cd * stride;
+    stride *= G.ng[d];
+  }
+  return (int)flat;
+}
+
+// Zero the cells and write the sentinel n + 1 (:228).
+__global__ void k_cells_clear(int *cells, const CellArr *carr, const Grid *grids,
+                              const int *n_dev, const Blk *blk) {
+  const Blk b = blk[blockIdx.x];
+  const CellArr C = carr[b.item];
+  const int ncells = grids[C.grid].ncells;
+  for (int q = 0; q < 4; ++q) {
+    int c = b.first + q * GR_BLOCK + threadIdx.x;
+    if (c < C.cap && c <= ncells) cells[C.off + c] = (c == ncells) ? n_dev[C.sys] + 1 : 0;
+  }
+}
+
+// First particle of each run of equal cells (forward pass :219-226); the
+// last run of a cell wins, as the serial loop's last write does.
+__global__ void k_cells_mark(const double *const *x, int *cells, const CellArr *carr,
+                             const Grid *grids, const double *cellsz, const SysDesc *sd,
+                             const int *n_dev, const Blk *blk, int D) {
+  const Blk b = blk[blockIdx.x];
+  const CellArr C = carr[b.item];
+  const int i = b.first + threadIdx.x;
+  const int n = n_dev[C.sys];
+  const Grid G = grids[C.grid];
+  if (i >= n || G.ncells <= 0) return;
+  const double cell = cellsz[C.grid];
+  const int64_t gi = sd[C.sys].off + i;
+  const int c = cell_flat0(x, gi, G, cell, D);
+  const int prev = i > 0 ? cell_flat0(x, gi - 1, G, cell, D) : -1;
+  if (c != prev && c >= 0 && c < G.ncells) atomicMax(&cells[C.off + c], i + 1);
+}
+
+// Backward pass (:231-235): every empty cell takes the first non-empty start
+// to its right.  Three kernels: block-local suffix, per-array carry, apply.
+#define FILL_TILE (GR_BLOCK * 4)
+
+__device__ inline int first_nz(int a, int b) { return a != 0 ? a : b; }
+
+__global__ void k_fill_local(int *cells, const CellArr *carr, const Grid *grids, const Blk *blk,
+                             int *bagg) {
+  __shared__ int s[GR_BLOCK];
+  const Blk b = blk[blockIdx.x];
+  const CellArr C = carr[b.item];
+  const int ncells = grids[C.grid].ncells;
+  const int t = threadIdx.x;
+  const int c0 = b.first + t * 4;
+  int v[4];
+  for (int q = 0; q < 4; ++q) {
+    int c = c0 + q;
+    v[q] = (c < C.cap && c <= ncells) ? cells[C.off + c] : 0;
+  }
+  // thread-local suffix
+  for (int q = 2; q >= 0; --q) v[q] = first_nz(v[q], v[q + 1]);
+  s[t] = v[0];
+  __syncthreads();
+  for (int o = 1; o < GR_BLOCK; o <<= 1) {
+    int r = (t + o < GR_BLOCK) ? s[t + o] : 0;
+    __syncthreads();
+    s[t] = first_nz(s[t], r);
+    __syncthreads();
+  }
+  const int right = (t + 1 < GR_BLOCK) ? s[t + 1] : 0;
+  for (int q = 0; q < 4; ++q) {
+    int c = c0 + q;
+    if (c < C.cap && c <= ncells) cells[C.off + c] = first_nz(v[q], right);
+  }
+  if (t == 0) bagg[blockIdx.x] = s[0];
+}
+
+// carry[blk] = first non-zero start in the blocks to the right of blk
+// (blocks of one cell array are consecutive in the table).
+__global__ void k_fill_carry(const int2 *carr_blocks, int ncarr, const int *bagg, int *carry) {
+  int q = blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= ncarr) return;
+  const int2 r = carr_blocks[q];  // [first block, end block)
+  int c = 0;
+  for (int k = r.y - 1; k >= r.x; --k) {
+    carry[k] = c;
+    c = first_nz(bagg[k], c);
+  }
+}
+
+__global__ void k_fill_apply(int *cells, const CellArr *carr, const Grid *grids, const Blk *blk,
+                             const int *carry) {
+  const Blk b = blk[blockIdx.x];
+  const CellArr C = carr[b.item];
+  const int ncells = grids[C.grid].ncells;
+  const int cr = carry[blockIdx.x];
+  if (cr == 0) return;
+  for (int q = 0; q < 4; ++q) {
+    int c = b.first + threadIdx.x * 4 + q;
+    if (c < C.cap && c <= ncells && cells[C.off + c] == 0) cells[C.off + c] = cr;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Leapfrog per-particle kernels (src/TimeIntegration.jl:645-685)
+// ---------------------------------------------------------------------------
+
+// Julia ^(Float64, Integer) (compensated power by squaring, fused muladd).
+__device__ inline double jl_pow_int(double x, long long n) {
+  if (n == 0) return 1.0;
+  double y = 1.0, xnlo = 0.0, ynlo = 0.0;
+  if (n == 3) return x * x * x;
+  if (n < 0) {
+    double rx = 1.0 / x;
+    if (n == -2) return rx * rx;
+    if (isfinite(x)) xnlo = -fma(x, rx, -1.0) * rx;
+    x = rx;
+    n = -n;
+  }
+  while (n > 1) {
+    if (n & 1) {
+      double err = fma(y, xnlo, x * ynlo);
+      double hi = x * y;
+      double lo = fma(x, y, -hi);
+      y = hi;
+      ynlo = lo + err;
+    }
+    double err = x * 2 * xnlo;
+    double hi = x * x;
+    double lo = fma(x, x, -hi);
+    x = hi;
+    xnlo = lo + err;
+    n >>= 1;
+  }
+  double err = fma(y, xnlo, x * ynlo);
+  return (isfinite(x) && isfinite(err)) ? fma(x, y, err) : x * y;
+}
+
+// Julia ^(Float64, Float64) for the EOS exponent.
+__device__ inline double jl_pow(double x, double y) {
+  if (x == 1.0) return 1.0;
+  double t = trunc(y);
+  if (t == y && fabs(y) < 1.0e18) {
+    long long yi = (long long)y;
+    if (yi == 0) return 1.0;
+    if (yi >= -4096 && yi <= 3 * 8192) return jl_pow_int(x, yi);
+  }
+  return pow(x, y);
+}
+
+__device__ inline double eos_p(int kind, double rho, double rho0, double gamma, double c) {
+  if (kind == GR_UPD_TAIT) return rho0 * c * c / gamma * (jl_pow(rho / rho0, gamma) - 1.0);
+  return c * c * (rho - rho0);
+}
+
+// _halfstep_ps! (:158-162) writing the half-step state to the sweep shadows
+// (v and ρ keep their start-of-step values for the full step), the
+// FP64 half-step density, the stage-1 EOS when fused, and the source reset.
+template <typename R>
+__global__ void k_halfstep(Pool<R> P, const SysDesc *sd, const StepSys *st, const Blk *blk,
+                           int D) {
+  const Blk b = blk[blockIdx.x];
+  const int s = b.item;
+  const int i = b.first + threadIdx.x;
+  const SysDesc S = sd[s];
+  if (i >= S.cap) return;
+  const StepSys T = st[s];
+  const int64_t g = S.off + i;
+  for (int d = 0; d < D; ++d) {
+    double vh = P.v[d][g] + T.half * (double)P.dvdt[d][g];
+    P.vs[d][g] = (R)vh;
+  }
+  const double rh = fma(T.half, (double)P.drhodt[g], P.rho[g]);
+  P.rho_h[g] = rh;
+  P.rhos[g] = (R)rh;
+  if (T.fuse_eos) P.p[g] = (R)eos_p(T.fuse_eos, rh, T.eos_rho0, T.eos_gamma, S.c);
+  if (T.reset_force) {
+    for (int d = 0; d < D; ++d) P.dvdt[d][g] = (R)S.sv[d];
+    P.drhodt[g] = (R)S.srho;
+  }
+}
+
+// _apply_damping_all! (:251-271), _fullstep_q_all! (:227-232),
+// _update_positions_all! (:235-240).  v and x use separate products and sums
+// (SVector arithmetic is outside @fastmath); ρ uses the fused axpy.
+template <typename R>
+__global__ void k_fullstep(Pool<R> P, const SysDesc *sd, const StepSys *st, const Blk *blk,
+                           int D) {
+  const Blk b = blk[blockIdx.x];
+  const int s = b.item;
+  const int i = b.first + threadIdx.x;
+  const SysDesc S = sd[s];
+  if (i >= S.cap) return;
+  const StepSys T = st[s];
+  const int64_t g = S.off + i;
+  for (int d = 0; d < D; ++d) {
+    double a = (double)P.dvdt[d][g];
+    if (T.damping) {
+      a = a - T.gdt * (double)P.vs[d][g];
+      P.dvdt[d][g] = (R)a;
+      a = (double)P.dvdt[d][g];
+    }
+    double vn = P.v[d][g] + T.dt * a;
+    P.v[d][g] = vn;
+    P.x[d][g] = P.x[d][g] + T.dt * vn;
+  }
+  P.rho[g] = fma(T.dt, (double)P.drhodt[g], P.rho[g]);
+}
+
+// ---------------------------------------------------------------------------
+// State updaters of one stage (src/StateUpdaters.jl)
+// ---------------------------------------------------------------------------
+
+// ElastoPlasticStressUpdater (:216-335), FP64.  prm = {E, nu, a_phi, k_c,
+// a_psi} with the trigonometric constants evaluated on the host.
+template <typename R>
+__device__ void ep_update(Pool<R> &P, int64_t g, const double *prm, double dt) {
+  const double E = prm[0], nu = prm[1], a_phi = prm[2], k_c = prm[3], a_psi = prm[4];
+  double sxx = P.stress[0][g], syy = P.stress[1][g], szz = P.stress[2][g], sxy = P.stress[3][g];
+  double dexx = (double)P.sr[0][g] * dt, deyy = (double)P.sr[1][g] * dt,
+         dezz = (double)P.sr[2][g] * dt, dexy = (double)P.sr[3][g] * dt;
+  double D0 = E / ((1.0 + nu) * (1.0 - 2 * nu));
+  double dsxx = D0 * ((1.0 - nu) * dexx + nu * deyy + nu * dezz);
+  double dsyy = D0 * (nu * dexx + (1.0 - nu) * deyy + nu * dezz);
+  double dszz = D0 * (nu * dexx + nu * deyy + (1.0 - nu) * dezz);
+  double dsxy = D0 * (1.0 - 2 * nu) * dexy;
+  double wdt = (double)P.vor[0][g] * dt;
+  dsxx -= 2 * sxy * wdt;
+  dsyy += 2 * sxy * wdt;
+  dsxy += (sxx - syy) * wdt;
+  double txx = sxx + dsxx, tyy = syy + dsyy, tzz = szz + dszz, txy = sxy + dsxy;
+  double I1 = txx + tyy + tzz;
+  double p3 = I1 / 3;
+  double s_xx = txx - p3, s_yy = tyy - p3, s_zz = tzz - p3, s_xy = txy;
+  double J2 = 0.5 * (s_xx * s_xx + s_yy * s_yy + s_zz * s_zz + 2 * (s_xy * s_xy));
+  double sqJ2 = sqrt(J2);
+  double fy = a_phi * I1 + sqJ2 - k_c;
+  double dpxx = 0, dpyy = 0, dpzz = 0, dpxy = 0;
+  if (fy > 0.0) {
+    double inv2 = 1.0 / (2 * sqJ2);
+    double dfxx = a_phi + s_xx * inv2, dfyy = a_phi + s_yy * inv2, dfzz = a_phi + s_zz * inv2,
+           dfxy = s_xy * inv2;
+    double dgxx = a_psi + s_xx * inv2, dgyy = a_psi + s_yy * inv2, dgzz = a_psi + s_zz * inv2,
+           dgxy = s_xy * inv2;
+    double Dgxx = D0 * ((1.0 - nu) * dgxx + nu * dgyy + nu * dgzz);
+    double Dgyy = D0 * (nu * dgxx + (1.0 - nu) * dgyy + nu * dgzz);
+    double Dgzz = D0 * (nu * dgxx + nu * dgyy + (1.0 - nu) * dgzz);
+    double Dgxy = D0 * (1.0 - 2 * nu) * dgxy;
+    double denom = dfxx * Dgxx + dfyy * Dgyy + dfzz * Dgzz + 2 * dfxy * Dgxy;
+    double dl = fy / denom;
+    dpxx = dl * dgxx;
+    dpyy = dl * dgyy;
+    dpzz = dl * dgzz;
+    dpxy = dl * dgxy;
+    dsxx -= D0 * ((1.0 - nu) * dpxx + nu * dpyy + nu * dpzz);
+    dsyy -= D0 * (nu * dpxx + (1.0 - nu) * dpyy + nu * dpzz);
+    dszz -= D0 * (nu * dpxx + nu * dpyy + (1.0 - nu) * dpzz);
+    dsxy -= D0 * (1.0 - 2 * nu) * dpxy;
+  }
+  double o0 = sxx + dsxx, o1 = syy + dsyy, o2 = szz + dszz, o3 = sxy + dsxy;
+  if (sxx + dsxx + syy + dsyy + szz + dszz > k_c / a_phi) {
+    double ap = k_c / a_phi / 3.0;
+    o0 = ap;
+    o1 = ap;
+    o2 = ap;
+    o3 = 0.0;
+  }
+  P.stress[0][g] = o0;
+  P.stress[1][g] = o1;
+  P.stress[2][g] = o2;
+  P.stress[3][g] = o3;
+  P.stress_s[0][g] = (R)o0;
+  P.stress_s[1][g] = (R)o1;
+  P.stress_s[2][g] = (R)o2;
+  P.stress_s[3][g] = (R)o3;
+  P.strain[0][g] += dexx;
+  P.strain[1][g] += deyy;
+  P.strain[2][g] += dezz;
+  P.strain[3][g] += dexy;
+  P.strainp[0][g] += dpxx;
+  P.strainp[1][g] += dpyy;
+  P.strainp[2][g] += dpzz;
+  P.strainp[3][g] += dpxy;
+}
+
+// ViscoPlasticMCStressUpdater (:155-189).  prm = {eos_kind, rho0, gamma,
+// tan(phi), cohesion}.
+template <typename R>
+__device__ void vpmc_update(Pool<R> &P, int64_t g, int NS, const double *prm, double c) {
+  const double pval = eos_p((int)prm[0], P.rho_h[g], prm[1], prm[2], c);
+  P.p[g] = (R)pval;
+  double er[6];
+  for (int k = 0; k < NS; ++k) er[k] = (double)P.sr[k][g];
+  double mag;
+  if (NS == 3)
+    mag = er[0] * er[0] + er[1] * er[1] + 2 * (er[2] * er[2]);
+  else if (NS == 4)
+    mag = er[0] * er[0] + er[1] * er[1] + 2 * (er[3] * er[3]);
+  else
+    mag = er[0] * er[0] + er[1] * er[1] + er[2] * er[2] +
+          2 * (er[3] * er[3] + er[4] * er[4] + er[5] * er[5]);
+  mag = sqrt(mag);
+  if (!(mag >= 2.220446049250313e-16)) mag = isnan(mag) ? mag : 2.220446049250313e-16;
+  const double sc = 2.0 * (prm[4] + pval * prm[3]);
+  double sg[6];
+  for (int k = 0; k < NS; ++k) sg[k] = sc * er[k] / mag;
+  bool zero;
+  if (NS == 3) {
+    sg[0] = sg[0] - pval;
+    sg[1] = sg[1] - pval;
+    sg[2] = sg[2] - 0.0;
+    zero = sg[0] + sg[1] > 0.0;
+  } else {
+    sg[0] = sg[0] - pval;
+    sg[1] = sg[1] - pval;
+    sg[2] = sg[2] - pval;
+    for (int k = 3; k < NS; ++k) sg[k] = sg[k] - 0.0;
+    zero = sg[0] + sg[1] + sg[2] > 0.0;
+  }
+  for (int k = 0; k < NS; ++k) {
+    double v = zero ? 0.0 : sg[k];
+    P.stress[k][g] = v;
+    P.stress_s[k][g] = (R)v;
+  }
+}
+
+template <typename R>
+__global__ void k_updaters(Pool<R> P, const SysDesc *sd, const StepSys *st, const Blk *blk,
+                           const int2 *urange, const UpdDesc *ud, int D) {
+  const Blk b = blk[blockIdx.x];
+  const int s = b.item;
+  const int i = b.first + threadIdx.x;
+  const SysDesc S = sd[s];
+  if (i >= S.cap) return;
+  const int64_t g = S.off + i;
+  const int2 r = urange[s];
+  const double dt = st[s].dt;
+  for (int q = r.x; q < r.y; ++q) {
+    const UpdDesc u = ud[q];
+    switch (u.kind) {
+      case GR_UPD_ZERO: {
+        const long long m = u.mask;
+        if (m & (1 << GR_F_STRAIN_RATE))
+          for (int k = 0; k < S.ns; ++k) P.sr[k][g] = (R)0;
+        if (m & (1 << GR_F_VORTICITY))
+          for (int k = 0; k < S.nv; ++k) P.vor[k][g] = (R)0;
+        if (m & (1 << GR_F_DVDT))
+          for (int d = 0; d < D; ++d) P.dvdt[d][g] = (R)0;
+        if (m & (1 << GR_F_DRHODT)) P.drhodt[g] = (R)0;
+        if (m & (1 << GR_F_P)) P.p[g] = (R)0;
+        if (m & (1 << GR_F_STRESS))
+          for (int k = 0; k < S.ns; ++k) {
+            P.stress[k][g] = 0.0;
+            P.stress_s[k][g] = (R)0;
+          }
+        if (m & (1 << GR_F_STRAIN))
+          for (int k = 0; k < S.ns; ++k) P.strain[k][g] = 0.0;
+        if (m & (1 << GR_F_STRAIN_P))
+          for (int k = 0; k < S.ns; ++k) P.strainp[k][g] = 0.0;
+        break;
+      }
+      case GR_UPD_TAIT:
+      case GR_UPD_LINEAR:
+        P.p[g] = (R)eos_p(u.kind, P.rho_h[g], u.prm[0], u.prm[1], S.c);
+        break;
+      case GR_UPD_EP:
+        ep_update(P, g, u.prm, dt);
+        break;
+      case GR_UPD_VPMC:
+        vpmc_update(P, g, S.ns, u.prm, S.c);
+        break;
+      default:
+        break;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Small utilities
+// ---------------------------------------------------------------------------
+
+__global__ void k_fill_i32(int *p, int v, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers
+// ---------------------------------------------------------------------------
+
+static inline unsigned nb_of(long long n, int per) { return (unsigned)((n + per - 1) / per); }
+
+void launch_key_minmax(hipStream_t s, const double *const *xs, const SysDesc *sd, const int *n,
+                       const StepSys *st, const Blk *blk, int nblk, int D, int *kmin, int *kmax,
+                       long long *bbmin, long long *bbmax, int with_keys) {
+  if (nblk > 0)
+    k_key_minmax<<<nblk, GR_BLOCK, 0, s>>>(xs, sd, n, st, blk, D, kmin, kmax, bbmin, bbmax,
+                                           with_keys);
+}
+void launch_minmax_reset(hipStream_t s, const int *list, int nlist, int *kmin, int *kmax,
+                         long long *bbmin, long long *bbmax) {
+  if (nlist > 0)
+    k_minmax_reset<<<nb_of(nlist * 3, GR_BLOCK), GR_BLOCK, 0, s>>>(list, nlist, kmin, kmax, bbmin,
+                                                                   bbmax);
+}
+void launch_key_base(hipStream_t s, const int *list, int nlist, int D, int pad, const int *n,
+                     const int *kmin, const int *kmax, unsigned *kbase, unsigned *krange,
+                     int *flags) {
+  k_key_base<<<1, 64, 0, s>>>(list, nlist, D, pad, n, kmin, kmax, kbase, krange, flags);
+}
+void launch_key_write(hipStream_t s, const double *const *xs, const SysDesc *sd, const int *n,
+                      const StepSys *st, const Blk *blk, int nblk, const int *bbase, int D,
+                      const int *kmin, const unsigned *krange, const unsigned *kbase,
+                      unsigned *keys, int *vals, int *flags) {
+  if (nblk > 0)
+    k_key_write<<<nblk, GR_BLOCK, 0, s>>>(xs, sd, n, st, blk, bbase, D, kmin, krange, kbase, keys,
+                                          vals, flags);
+}
+void launch_radix_sort(hipStream_t s, unsigned *keys, int *vals, unsigned *keys2, int *vals2,
+                       unsigned *hist, int N, const int *flags) {
+  if (N <= 0) return;
+  const int nblk = (int)nb_of(N, RADIX_TILE);
+  unsigned *ki = keys, *ko = keys2;
+  int *vi = vals, *vo = vals2;
+  for (int pass = 0; pass < 4; ++pass) {
+    const int shift = pass * 8;
+    k_radix_hist<<<nblk, GR_BLOCK, 0, s>>>(ki, N, shift, hist, nblk, flags);
+    k_radix_scan<<<1, 1024, 0, s>>>(hist, 256 * nblk, flags);
+    k_radix_scatter<<<nblk, GR_BLOCK, 0, s>>>(ki, vi, ko, vo, N, shift, hist, nblk, flags);
+    unsigned *tk = ki;
+    ki = ko;
+    ko = tk;
+    int *tv = vi;
+    vi = vo;
+    vo = tv;
+  }
+}
+int radix_hist_entries(int N) { return 256 * (int)nb_of(N > 0 ? N : 1, RADIX_TILE); }
+
+template <typename R>
+void launch_permute_real(hipStream_t s, const Pool<R> &A, const Pool<R> &B, const SysDesc *sd,
+                         const Blk *blk, int nblk, const int *bbase, const int *vals,
+                         const int *flags, int D) {
+  if (nblk > 0) k_permute_real<R><<<nblk, GR_BLOCK, 0, s>>>(A, B, sd, blk, bbase, vals, flags, D);
+}
+template void launch_permute_real<float>(hipStream_t, const Pool<float> &, const Pool<float> &,
+                                         const SysDesc *, const Blk *, int, const int *,
+                                         const int *, const int *, int);
+template void launch_permute_real<double>(hipStream_t, const Pool<double> &,
+                                          const Pool<double> &, const SysDesc *, const Blk *,
+                                          int, const int *, const int *, const int *, int);
+
+void launch_ghost_flags(hipStream_t s, const double *const *x, const SysDesc *sd, const Bnd *bnd,
+                        const Blk *blk, int nblk, const int *cand_off, int D, int *flags) {
+  if (nblk > 0) k_ghost_flags<<<nblk, GR_BLOCK, 0, s>>>(x, sd, bnd, blk, cand_off, D, flags);
+}
+void launch_scan(hipStream_t s, const int *in, int *out, int *bsum, int N) {
+  const int nb = (int)nb_of(N > 0 ? N : 1, SCAN_TILE);
+  k_scan_local<<<nb, 256, 0, s>>>(in, out, bsum, N);
+  k_scan_blocks<<<1, 1024, 0, s>>>(bsum, nb, out, N);
+  k_scan_add<<<nb, 256, 0, s>>>(out, bsum, N);
+}
+int scan_block_count(int N) { return (int)nb_of(N > 0 ? N : 1, SCAN_TILE); }
+void launch_ghost_scatter(hipStream_t s, const double *const *x, double *const *gx, int *gidx,
+                          int *gbnd, int *gnrm, const SysDesc *sd, const Bnd *bnd, const Blk *blk,
+                          int nblk, const int *cand_off, const int *scan, int D, int *n) {
+  if (nblk > 0)
+    k_ghost_scatter<<<nblk, GR_BLOCK, 0, s>>>(x, gx, gidx, gbnd, gnrm, sd, bnd, blk, cand_off,
+                                              scan, D, n);
+}
+void launch_permute_ghost(hipStream_t s, const double *const *gxs, const int *gidx_s,
+                          const int *gbnd_s, double *const *x, int *gidx, int *gbnd,
+                          const SysDesc *sd, const int *n, const Blk *blk, int nblk,
+                          const int *bbase, const int *vals, const int *flags, int D) {
+  if (nblk > 0)
+    k_permute_ghost<<<nblk, GR_BLOCK, 0, s>>>(gxs, gidx_s, gbnd_s, x, gidx, gbnd, sd, n, blk,
+                                              bbase, vals, flags, D);
+}
+template <typename R>
+void launch_ghost_kin(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const Bnd *bnd,
+                      const int *n, const Blk *blk, int nblk, int D) {
+  if (nblk > 0) k_ghost_kin<R><<<nblk, GR_BLOCK, 0, s>>>(P, sd, bnd, n, blk, D);
+}
+template void launch_ghost_kin<float>(hipStream_t, const Pool<float> &, const SysDesc *,
+                                      const Bnd *, const int *, const Blk *, int, int);
+template void launch_ghost_kin<double>(hipStream_t, const Pool<double> &, const SysDesc *,
+                                       const Bnd *, const int *, const Blk *, int, int);
+template <typename R>
+void launch_ghost_copy(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const Bnd *bnd,
+                       const int *n, const Blk *blk, int nblk, const int2 *copy, int D) {
+  if (nblk > 0) k_ghost_copy<R><<<nblk, GR_BLOCK, 0, s>>>(P, sd, bnd, n, blk, copy, D);
+}
+template void launch_ghost_copy<float>(hipStream_t, const Pool<float> &, const SysDesc *,
+                                       const Bnd *, const int *, const Blk *, int, const int2 *,
+                                       int);
+template void launch_ghost_copy<double>(hipStream_t, const Pool<double> &, const SysDesc *,
+                                        const Bnd *, const int *, const Blk *, int,
+                                        const int2 *, int);
+
+void launch_grid_params(hipStream_t s, const int *inter_list, int n, const int2 *inter_sys,
+                        const double *cellsz, const long long *bbmin, const long long *bbmax,
+                        const int *n_dev, Grid *grids, const int2 *inter_carr,
+                        const CellArr *carr, int D, int *flags) {
+  if (n > 0)
+    k_grid_params<<<nb_of(n, 64), 64, 0, s>>>(inter_list, n, inter_sys, cellsz, bbmin, bbmax,
+                                              n_dev, grids, inter_carr, carr, D, flags);
+}
+void launch_cells_build(hipStream_t s, const double *const *x, int *cells, const CellArr *carr,
+                        const Grid *grids, const double *cellsz, const SysDesc *sd,
+                        const int *n_dev, const Blk *cblk, int ncblk, const Blk *mblk,
+                        int nmblk, const int2 *carr_blocks, int ncarr, int *bagg, int *carry,
+                        int D) {
+  if (ncblk > 0) k_cells_clear<<<ncblk, GR_BLOCK, 0, s>>>(cells, carr, grids, n_dev, cblk);
+  if (nmblk > 0)
+    k_cells_mark<<<nmblk, GR_BLOCK, 0, s>>>(x, cells, carr, grids, cellsz, sd, n_dev, mblk, D);
+  if (ncblk > 0) {
+    k_fill_local<<<ncblk, GR_BLOCK, 0, s>>>(cells, carr, grids, cblk, bagg);
+    k_fill_carry<<<nb_of(ncarr, 64), 64, 0, s>>>(carr_blocks, ncarr, bagg, carry);
+    k_fill_apply<<<ncblk, GR_BLOCK, 0, s>>>(cells, carr, grids, cblk, carry);
+  }
+}
+
+template <typename R>
+void launch_halfstep(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
+                     const Blk *blk, int nblk, int D) {
+  if (nblk > 0) k_halfstep<R><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, blk, D);
+}
+template <typename R>
+void launch_fullstep(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
+                     const Blk *blk, int nblk, int D) {
+  if (nblk > 0) k_fullstep<R><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, blk, D);
+}
+template <typename R>
+void launch_updaters(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
+                     const Blk *blk, int nblk, const int2 *urange, const UpdDesc *ud, int D) {
+  if (nblk > 0) k_updaters<R><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, blk, urange, ud, D);
+}
+#define INST(R)                                                                                \
+  template void launch_halfstep<R>(hipStream_t, const Pool<R> &, const SysDesc *,              \
+                                   const StepSys *, const Blk *, int, int);                    \
+  template void launch_fullstep<R>(hipStream_t, const Pool<R> &, const SysDesc *,              \
+                                   const StepSys *, const Blk *, int, int);                    \
+  template void launch_updaters<R>(hipStream_t, const Pool<R> &, const SysDesc *,              \
+                                   const StepSys *, const Blk *, int, const int2 *,            \
+                                   const UpdDesc *, int);
+INST(float)
+INST(double)
+#undef INST
+
+void launch_fill_i32(hipStream_t s, int *p, int v, long long n) {
+  if (n > 0) k_fill_i32<<<nb_of(n, GR_BLOCK), GR_BLOCK, 0, s>>>(p, v, n);
+}
+
+}  // namespace gr

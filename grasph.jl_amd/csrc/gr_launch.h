@@ -1,0 +1,80 @@
+// gr_launch.h — host-callable launchers of the device kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "gr_device.h"
+
+namespace gr {
+
+struct SweepArgs {
+  const SysDesc *sd;
+  const int *n;
+  const Task *tasks;
+  const SrcDesc *srcs;
+  const Grid *grids;
+  const CellArr *carr;
+  const int *cells;
+  const Blk *blk;
+  int nblk;
+  unsigned long long *pair_count;
+};
+
+void launch_key_minmax(hipStream_t s, const double *const *xs, const SysDesc *sd, const int *n,
+                       const StepSys *st, const Blk *blk, int nblk, int D, int *kmin, int *kmax,
+                       long long *bbmin, long long *bbmax, int with_keys);
+void launch_minmax_reset(hipStream_t s, const int *list, int nlist, int *kmin, int *kmax,
+                         long long *bbmin, long long *bbmax);
+void launch_key_base(hipStream_t s, const int *list, int nlist, int D, int pad, const int *n,
+                     const int *kmin, const int *kmax, unsigned *kbase, unsigned *krange,
+                     int *flags);
+void launch_key_write(hipStream_t s, const double *const *xs, const SysDesc *sd, const int *n,
+                      const StepSys *st, const Blk *blk, int nblk, const int *bbase, int D,
+                      const int *kmin, const unsigned *krange, const unsigned *kbase,
+                      unsigned *keys, int *vals, int *flags);
+void launch_radix_sort(hipStream_t s, unsigned *keys, int *vals, unsigned *keys2, int *vals2,
+                       unsigned *hist, int N, const int *flags);
+int radix_hist_entries(int N);
+template <typename R>
+void launch_permute_real(hipStream_t s, const Pool<R> &A, const Pool<R> &B, const SysDesc *sd,
+                         const Blk *blk, int nblk, const int *bbase, const int *vals,
+                         const int *flags, int D);
+void launch_ghost_flags(hipStream_t s, const double *const *x, const SysDesc *sd, const Bnd *bnd,
+                        const Blk *blk, int nblk, const int *cand_off, int D, int *flags);
+void launch_scan(hipStream_t s, const int *in, int *out, int *bsum, int N);
+int scan_block_count(int N);
+void launch_ghost_scatter(hipStream_t s, const double *const *x, double *const *gx, int *gidx,
+                          int *gbnd, int *gnrm, const SysDesc *sd, const Bnd *bnd, const Blk *blk,
+                          int nblk, const int *cand_off, const int *scan, int D, int *n);
+void launch_permute_ghost(hipStream_t s, const double *const *gxs, const int *gidx_s,
+                          const int *gbnd_s, double *const *x, int *gidx, int *gbnd,
+                          const SysDesc *sd, const int *n, const Blk *blk, int nblk,
+                          const int *bbase, const int *vals, const int *flags, int D);
+template <typename R>
+void launch_ghost_kin(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const Bnd *bnd,
+                      const int *n, const Blk *blk, int nblk, int D);
+template <typename R>
+void launch_ghost_copy(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const Bnd *bnd,
+                       const int *n, const Blk *blk, int nblk, const int2 *copy, int D);
+void launch_grid_params(hipStream_t s, const int *inter_list, int n, const int2 *inter_sys,
+                        const double *cellsz, const long long *bbmin, const long long *bbmax,
+                        const int *n_dev, Grid *grids, const int2 *inter_carr,
+                        const CellArr *carr, int D, int *flags);
+void launch_cells_build(hipStream_t s, const double *const *x, int *cells, const CellArr *carr,
+                        const Grid *grids, const double *cellsz, const SysDesc *sd,
+                        const int *n_dev, const Blk *cblk, int ncblk, const Blk *mblk,
+                        int nmblk, const int2 *carr_blocks, int ncarr, int *bagg, int *carry,
+                        int D);
+template <typename R>
+void launch_halfstep(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
+                     const Blk *blk, int nblk, int D);
+template <typename R>
+void launch_fullstep(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
+                     const Blk *blk, int nblk, int D);
+template <typename R>
+void launch_updaters(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
+                     const Blk *blk, int nblk, const int2 *urange, const UpdDesc *ud, int D);
+template <typename R>
+void launch_sweep(hipStream_t s, const Pool<R> &P, const SweepArgs &a, int D, int fam, bool cnt);
+void launch_fill_i32(hipStream_t s, int *p, int v, long long n);
+
+}  // namespace gr

@@ -1,0 +1,501 @@
+"""Host-side mirror of Grasph.jl's data model and operator types.
+
+Same names, argument meaning and errors as the reference constructors; the
+arrays are numpy arrays in the reference's per-particle layout
+(`Vector{SVector{ND,T}}` → shape (n, ND) float64, `Vector{T}` → (n,)).
+Indices (`id`, `idx_original`, `idx_boundary`) are 1-based int64 as in Julia.
+
+These objects carry no device state; `scene.DeviceScene` mirrors them onto a
+backend context and moves arrays between host and device.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from . import _abi as A
+
+# ---------------------------------------------------------------------------
+# Kernels (src/Kernels.jl)
+# ---------------------------------------------------------------------------
+
+
+class _Kernel:
+    kind = None
+
+    def __init__(self, h, ndims=3):
+        self.h = float(h)
+        self.ndims = int(ndims)
+
+    @property
+    def interaction_length(self):  # src/Kernels.jl:47-48
+        return 2.0 * self.h
+
+
+class CubicSplineKernel(_Kernel):
+    """CubicSplineKernel(h; ndims=3) (src/Kernels.jl:34)."""
+
+    kind = A.GR_KERNEL_CUBIC
+
+    @property
+    def norm_coeff(self):  # src/Kernels.jl:49-56
+        H = self.h
+        if self.ndims == 1:
+            return 2.0 / (3.0 * H)
+        if self.ndims == 2:
+            return 10.0 / ((7.0 * math.pi) * (H * H))
+        return 1.0 / (math.pi * (H * H * H))
+
+
+class WenlandC2Kernel(_Kernel):
+    """WenlandC2Kernel(h; ndims=3) (src/Kernels.jl:108)."""
+
+    kind = A.GR_KERNEL_WENDLAND_C2
+
+    @property
+    def norm_coeff(self):  # src/Kernels.jl:123-128
+        H = self.h
+        if self.ndims == 2:
+            return 7.0 / ((64.0 * math.pi) * (H * H))
+        return 21.0 / ((256.0 * math.pi) * (H * H * H))
+
+
+# ---------------------------------------------------------------------------
+# State updaters (src/StateUpdaters.jl)
+# ---------------------------------------------------------------------------
+
+
+class ZeroFieldUpdater:
+    """ZeroFieldUpdater(:f1, :f2, ...) (src/StateUpdaters.jl:30-42)."""
+
+    def __init__(self, *fields):
+        for f in fields:
+            if f not in A.FIELD_IDS:
+                raise ValueError(f"unknown field :{f}")
+        self.fields = tuple(fields)
+
+    def encode(self):
+        mask = 0
+        for f in self.fields:
+            mask |= 1 << A.FIELD_IDS[f]
+        return A.GR_UPD_ZERO, [float(mask)]
+
+
+class TaitEOSUpdater:
+    """TaitEOSUpdater(rho0; gamma=7) (src/StateUpdaters.jl:108-118)."""
+
+    def __init__(self, rho0, gamma=7.0):
+        self.rho0 = float(rho0)
+        self.gamma = float(gamma)
+
+    def encode(self):
+        return A.GR_UPD_TAIT, [self.rho0, self.gamma]
+
+
+class LinearEOSUpdater:
+    """LinearEOSUpdater(rho0) (src/StateUpdaters.jl:131-140)."""
+
+    def __init__(self, rho0):
+        self.rho0 = float(rho0)
+
+    def encode(self):
+        return A.GR_UPD_LINEAR, [self.rho0]
+
+
+class ElastoPlasticStressUpdater:
+    """ElastoPlasticStressUpdater(E, nu, phi, psi, cohesion)
+    (src/StateUpdaters.jl:202-335)."""
+
+    def __init__(self, E, nu, phi, psi, cohesion):
+        self.E, self.nu, self.phi, self.psi, self.cohesion = map(float, (E, nu, phi, psi, cohesion))
+
+    def encode(self):
+        return A.GR_UPD_EP, [self.E, self.nu, self.phi, self.psi, self.cohesion]
+
+
+class ViscoPlasticMCStressUpdater:
+    """ViscoPlasticMCStressUpdater(eos, friction_angle, cohesion)
+    (src/StateUpdaters.jl:144-189)."""
+
+    def __init__(self, eos, friction_angle, cohesion):
+        if not isinstance(eos, (TaitEOSUpdater, LinearEOSUpdater)):
+            raise ValueError("ViscoPlasticMCStressUpdater needs a Tait or Linear EOS updater")
+        self.eos = eos
+        self.friction_angle = float(friction_angle)
+        self.cohesion = float(cohesion)
+
+    def encode(self):
+        kind, prm = self.eos.encode()
+        gamma = prm[1] if kind == A.GR_UPD_TAIT else 0.0
+        return A.GR_UPD_VPMC, [float(kind), prm[0], gamma, self.friction_angle, self.cohesion]
+
+
+_UPDATER_TYPES = (ZeroFieldUpdater, TaitEOSUpdater, LinearEOSUpdater,
+                  ElastoPlasticStressUpdater, ViscoPlasticMCStressUpdater)
+
+
+def _normalise_updaters(state_updater):
+    """`state_updater isa Tuple ? state_updater : (state_updater,)`
+    (src/Particles.jl:87); each stage entry is None, one updater, or a tuple."""
+    ups = state_updater if isinstance(state_updater, tuple) else (state_updater,)
+    for u in ups:
+        items = u if isinstance(u, tuple) else (u,)
+        for it in items:
+            if it is not None and not isinstance(it, _UPDATER_TYPES):
+                raise ValueError(
+                    f"state updater {type(it).__name__} has no device implementation")
+    return ups
+
+
+# ---------------------------------------------------------------------------
+# Particle systems (src/Particles.jl)
+# ---------------------------------------------------------------------------
+
+
+class _ParticleSystem:
+    kind = None
+
+    def __init__(self, name, n, ndims, mass, c, *, source_v=None, source_rho=0.0,
+                 state_updater=(), ns=0):
+        n = int(n)
+        ndims = int(ndims)
+        if n <= 0:
+            raise ValueError(f"n must be positive, got {n}")
+        if ndims <= 0:
+            raise ValueError(f"ndims must be positive, got {ndims}")
+        sv = np.zeros(ndims) if source_v is None else np.asarray(source_v, dtype=np.float64)
+        if sv.shape != (ndims,):
+            raise ValueError(f"length(source_v) = {sv.size} must equal ndims = {ndims}")
+        self.name = str(name)
+        self.n = n
+        self.ndims = ndims
+        self.mass = float(mass)
+        self.c = float(c)
+        self.source_v = sv
+        self.source_rho = float(source_rho)
+        self.state_updater = _normalise_updaters(state_updater)
+        self.ns = int(ns)
+        self.id = np.arange(1, n + 1, dtype=np.int64)
+        self.x = np.zeros((n, ndims))
+        self.v = np.zeros((n, ndims))
+        self.v_adjustment = np.zeros((n, ndims))
+        self.rho = np.zeros(n)
+        self.dvdt = np.zeros((n, ndims))
+        self.drhodt = np.zeros(n)
+
+    # fields moved between host and device, in the order of _particle_arrays
+    # (src/Sorting.jl:15-26)
+    def field_names(self):
+        return ["x", "v", "rho", "dvdt", "drhodt"]
+
+
+class BasicParticleSystem(_ParticleSystem):
+    """BasicParticleSystem(name, n, ndims, mass, c; source_v, source_rho,
+    state_updater) (src/Particles.jl:61)."""
+
+    kind = A.GR_SYS_BASIC
+
+    def __init__(self, name, n, ndims, mass, c, **kw):
+        super().__init__(name, n, ndims, mass, c, **kw)
+
+
+class FluidParticleSystem(_ParticleSystem):
+    """FluidParticleSystem (src/Particles.jl:173): Basic + pressure p."""
+
+    kind = A.GR_SYS_FLUID
+
+    def __init__(self, name, n, ndims, mass, c, **kw):
+        super().__init__(name, n, ndims, mass, c, **kw)
+        self.p = np.zeros(self.n)
+
+    def field_names(self):
+        return super().field_names() + ["p"]
+
+
+class StressParticleSystem(_ParticleSystem):
+    """StressParticleSystem(name, n, ndims, ns, mass, c; ...)
+    (src/Particles.jl:272): Fluid + stress, strain_rate (Voigt, ns ∈ {3,4,6})."""
+
+    kind = A.GR_SYS_STRESS
+
+    def __init__(self, name, n, ndims, ns, mass, c, **kw):
+        ns = int(ns)
+        if ns not in (3, 4, 6):
+            raise ValueError(f"ns must be 3, 4, or 6, got {ns}")
+        super().__init__(name, n, ndims, mass, c, ns=ns, **kw)
+        self.p = np.zeros(self.n)
+        self.stress = np.zeros((self.n, ns))
+        self.strain_rate = np.zeros((self.n, ns))
+
+    def field_names(self):
+        return super().field_names() + ["p", "stress", "strain_rate"]
+
+
+class ElastoPlasticParticleSystem(StressParticleSystem):
+    """ElastoPlasticParticleSystem (src/Particles.jl:381): Stress + vorticity
+    (scalar in 2D, 3-vector in 3D), strain, strain_p."""
+
+    kind = A.GR_SYS_EP
+
+    def __init__(self, name, n, ndims, ns, mass, c, **kw):
+        super().__init__(name, n, ndims, ns, mass, c, **kw)
+        self.vorticity = np.zeros(self.n) if self.ndims == 2 else np.zeros((self.n, 3))
+        self.strain = np.zeros((self.n, self.ns))
+        self.strain_p = np.zeros((self.n, self.ns))
+
+    def field_names(self):
+        return super().field_names() + ["vorticity", "strain", "strain_p"]
+
+
+# ---------------------------------------------------------------------------
+# Boundary wrappers (src/BoundaryParticles.jl)
+# ---------------------------------------------------------------------------
+
+
+class StaticBoundarySystem:
+    """StaticBoundarySystem(inner, lj_cutoff) (src/BoundaryParticles.jl:47)."""
+
+    wrap = A.GR_WRAP_STATIC
+
+    def __init__(self, inner, lj_cutoff):
+        if not isinstance(inner, _ParticleSystem):
+            raise ValueError("inner must be a particle system")
+        self.inner = inner
+        self.lj_cutoff = float(lj_cutoff)
+        self.ndims = inner.ndims
+
+    def __getattr__(self, k):  # forwarded getproperty (:13-20)
+        return getattr(self.__dict__["inner"], k)
+
+
+class DynamicBoundarySystem:
+    """DynamicBoundarySystem(inner, normal, point, beta)
+    (src/BoundaryParticles.jl:79)."""
+
+    wrap = A.GR_WRAP_DYNAMIC
+
+    def __init__(self, inner, normal, point, beta):
+        if not isinstance(inner, _ParticleSystem):
+            raise ValueError("inner must be a particle system")
+        self.inner = inner
+        self.boundary_normal = np.asarray(normal, dtype=np.float64).reshape(inner.ndims)
+        self.boundary_point = np.asarray(point, dtype=np.float64).reshape(inner.ndims)
+        self.boundary_beta = float(beta)
+        self.ndims = inner.ndims
+
+    def __getattr__(self, k):
+        return getattr(self.__dict__["inner"], k)
+
+
+# ---------------------------------------------------------------------------
+# Ghosts (src/GhostParticles.jl)
+# ---------------------------------------------------------------------------
+
+
+class HouseholderReflect:
+    """Marker selecting σ' = HσH, H = I − 2n̂n̂ᵀ (src/GhostParticles.jl:26)."""
+
+
+class GhostCopier:
+    """GhostCopier(:f, :g => HouseholderReflect(), ...) (src/GhostParticles.jl:55).
+
+    Entries are field names ('p', 'stress') or (name, HouseholderReflect())."""
+
+    def __init__(self, *entries):
+        self.fields = []
+        self.modes = []
+        for e in entries:
+            if isinstance(e, tuple):
+                name, mode = e
+                if not isinstance(mode, HouseholderReflect):
+                    raise ValueError("GhostCopier mode must be HouseholderReflect()")
+                self.fields.append(name)
+                self.modes.append(A.GR_COPY_HOUSEHOLDER)
+            else:
+                self.fields.append(e)
+                self.modes.append(A.GR_COPY_PLAIN)
+        for f in self.fields:
+            if f not in ("p", "stress"):
+                raise ValueError(f"GhostCopier field :{f} has no device implementation")
+
+
+class GhostParticleSystem:
+    """GhostParticleSystem(ps, copiers...; name) (src/GhostParticles.jl:199).
+
+    Host arrays are sized to the live ghost count after each download."""
+
+    kind = A.GR_SYS_GHOST
+
+    def __init__(self, ps, *updaters, name=None):
+        if not isinstance(ps, _ParticleSystem):
+            raise ValueError("ghost source must be a particle system")
+        self.source = ps
+        self.updaters = tuple(updaters)
+        self.name = f"ghost({ps.name})" if name is None else str(name)
+        self.ndims = ps.ndims
+        extras = []
+        for u in self.updaters:
+            if u is None:
+                continue
+            for f in u.fields:
+                if f not in extras:
+                    extras.append(f)
+        self.extras = tuple(extras)
+        self._resize(0)
+
+    @property
+    def mass(self):
+        return self.source.mass
+
+    @property
+    def c(self):
+        return self.source.c
+
+    def _resize(self, n):
+        D = self.ndims
+        self.n = int(n)
+        self.x = np.zeros((n, D))
+        self.v = np.zeros((n, D))
+        self.rho = np.zeros(n)
+        self.idx_original = np.zeros(n, dtype=np.int64)
+        self.idx_boundary = np.zeros(n, dtype=np.int64)
+        self.normals = np.zeros((n, D))
+        if "p" in self.extras:
+            self.p = np.zeros(n)
+        if "stress" in self.extras:
+            self.stress = np.zeros((n, self.source.ns))
+
+
+class GhostEntry:
+    """GhostEntry(ghost, cutoff, (normal, point), ...) (src/GhostParticles.jl:317)."""
+
+    def __init__(self, ghost, cutoff, *boundary_pairs):
+        if not boundary_pairs:
+            raise ValueError("at least one boundary (normal, point) pair is required")
+        self.ghost = ghost
+        self.cutoff = float(cutoff)
+        D = ghost.ndims
+        self.normals = np.array([np.asarray(nm, dtype=np.float64).reshape(D) for nm, _ in boundary_pairs])
+        self.points = np.array([np.asarray(pt, dtype=np.float64).reshape(D) for _, pt in boundary_pairs])
+
+
+# ---------------------------------------------------------------------------
+# Pair functors (src/PairwiseFunctors.jl)
+# ---------------------------------------------------------------------------
+
+
+def _nan_or(v):
+    return math.nan if v is None else float(v)
+
+
+class FluidPfn:
+    """FluidPfn(alpha, beta, h; sigma=2, delta=nothing, epsilon=nothing)
+    (src/PairwiseFunctors.jl:143)."""
+
+    def __init__(self, alpha, beta, h, sigma=2, delta=None, epsilon=None):
+        if sigma not in (1, 2):
+            raise ValueError("FluidPfn sigma must be 1 or 2 on the device path")
+        self.art_visc_alpha, self.art_visc_beta, self.h = float(alpha), float(beta), float(h)
+        self.sigma, self.delta, self.epsilon = int(sigma), delta, epsilon
+
+    def encode(self):
+        return (A.GR_PFN_FLUID, self.sigma, self.art_visc_alpha, self.art_visc_beta, self.h,
+                _nan_or(self.delta), _nan_or(self.epsilon))
+
+
+class CauchyFluidPfn:
+    """CauchyFluidPfn(alpha, beta, h; delta=nothing) (src/PairwiseFunctors.jl:258)."""
+
+    def __init__(self, alpha, beta, h, delta=None):
+        self.art_visc_alpha, self.art_visc_beta, self.h = float(alpha), float(beta), float(h)
+        self.delta = delta
+
+    def encode(self):
+        return (A.GR_PFN_CAUCHY, 2, self.art_visc_alpha, self.art_visc_beta, self.h,
+                _nan_or(self.delta), math.nan)
+
+
+class StrainRatePfn:
+    """StrainRatePfn() (src/PairwiseFunctors.jl:18)."""
+
+    def encode(self):
+        return (A.GR_PFN_SR, 2, 0.0, 0.0, 0.0, math.nan, math.nan)
+
+
+class StrainRateVorticityPfn:
+    """StrainRateVorticityPfn() (src/PairwiseFunctors.jl:78)."""
+
+    def encode(self):
+        return (A.GR_PFN_SRV, 2, 0.0, 0.0, 0.0, math.nan, math.nan)
+
+
+class NeighborCounterPfn:
+    """Adds 1 to dvdt[i][1] and drhodt[i] of both particles of every accepted
+    pair — the counting functors of test/test_interaction_sweep.jl:14-30,
+    expressed as an enumerable functor so the device can run it."""
+
+    def encode(self):
+        return (A.GR_PFN_COUNT, 2, 0.0, 0.0, 0.0, math.nan, math.nan)
+
+
+_PFN_TYPES = (FluidPfn, CauchyFluidPfn, StrainRatePfn, StrainRateVorticityPfn, NeighborCounterPfn)
+
+
+def encode_pfn(p):
+    if p is None:
+        return (A.GR_PFN_NONE, 2, 0.0, 0.0, 0.0, math.nan, math.nan)
+    if not isinstance(p, _PFN_TYPES):
+        raise ValueError(f"pairwise functor {type(p).__name__} has no device implementation")
+    return p.encode()
+
+
+# ---------------------------------------------------------------------------
+# Interaction (src/Interaction.jl:61)
+# ---------------------------------------------------------------------------
+
+
+class SystemInteraction:
+    """SystemInteraction(kernel, pairwise_fn, system_a[, system_b])."""
+
+    def __init__(self, kernel, pairwise_fn, system_a, system_b=None):
+        nd = system_a.ndims
+        if kernel.ndims != nd:
+            raise ValueError(f"kernel.ndims ({kernel.ndims}) must match system_a.ndims ({nd})")
+        if system_b is not None and system_b.ndims != nd:
+            raise ValueError(f"system_b.ndims ({system_b.ndims}) must match system_a.ndims ({nd})")
+        self.kernel = kernel
+        self.pfns = pairwise_fn if isinstance(pairwise_fn, tuple) else (pairwise_fn,)
+        for p in self.pfns:
+            encode_pfn(p)
+        self.system_a = system_a
+        self.system_b = system_b
+        self.ndims = nd
+
+    def is_coupled(self):
+        return self.system_b is not None
+
+
+# ---------------------------------------------------------------------------
+# Leapfrog integrator (src/TimeIntegration.jl:29-67)
+# ---------------------------------------------------------------------------
+
+
+class LeapFrogTimeIntegrator:
+    """LeapFrogTimeIntegrator(systems, interactions; ghosts=(), Γ=0)."""
+
+    def __init__(self, systems, interactions, ghosts=(), gamma=0.0):
+        sys = (systems,) if isinstance(systems, _ParticleSystem) else tuple(systems)
+        ints = (interactions,) if isinstance(interactions, SystemInteraction) else tuple(interactions)
+        gsts = (ghosts,) if isinstance(ghosts, GhostEntry) else tuple(ghosts)
+        if not sys:
+            raise ValueError("systems must not be empty")
+        if not ints:
+            raise ValueError("interactions must not be empty")
+        self.systems = sys
+        self.interactions = ints
+        self.ghosts = gsts
+        self.gamma = float(gamma)
+        self.c = max(ps.c for ps in sys)
+        self.h = min(si.kernel.h for si in ints)
+        self.ndims = sys[0].ndims

@@ -1,0 +1,184 @@
+"""Mirror host objects (model.py) onto one backend context.
+
+`DeviceScene` assigns backend handles to particle systems, ghost entries,
+interactions and integrators, and moves arrays between the host mirror and
+the context.  The backend is the HIP library context returned by
+`_abi.hip_backend`; any object exposing the same methods (the C ABI of
+include/grasph_hip.h) can be passed, which is how the test-suite runs the
+same scene on its CPU oracle for comparison.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _abi as A
+from . import model as M
+
+
+def _field_comps(ps, name):
+    D = ps.ndims
+    if name in ("x", "v", "dvdt", "v_adjustment", "normals"):
+        return D
+    if name in ("rho", "drhodt", "p"):
+        return 1
+    if name in ("stress", "strain_rate", "strain", "strain_p"):
+        return ps.ns if not isinstance(ps, M.GhostParticleSystem) else ps.source.ns
+    if name == "vorticity":
+        return 1 if D == 2 else 3
+    raise KeyError(name)
+
+
+class DeviceScene:
+    def __init__(self, backend, ndims):
+        self.be = backend
+        self.ndims = int(ndims)
+        self._sys = {}
+        self._ghost = {}
+        self._inter = {}
+        self._integ = {}
+        self._objs = []  # keep the mirrored objects alive (ids stay valid)
+
+    # ---- registration -------------------------------------------------
+    def system(self, ps):
+        """Handle of a real particle system (boundary wrappers → inner)."""
+        if isinstance(ps, (M.StaticBoundarySystem, M.DynamicBoundarySystem)):
+            ps = ps.inner
+        k = id(ps)
+        if k in self._sys:
+            return self._sys[k]
+        if isinstance(ps, M.GhostParticleSystem):
+            raise ValueError(f"ghost system {ps.name} must be registered through its GhostEntry")
+        if ps.ndims != self.ndims:
+            raise ValueError("system dimension does not match the context")
+        h = self.be.system_create(ps.kind, ps.n, ps.ns, ps.mass, ps.c, ps.source_v,
+                                  ps.source_rho)
+        for stage, entry in enumerate(ps.state_updater, start=1):
+            items = entry if isinstance(entry, tuple) else (entry,)
+            if not items:
+                self.be.system_add_updater(h, stage, A.GR_UPD_NOTHING, [])
+            for u in items:
+                if u is None:
+                    self.be.system_add_updater(h, stage, A.GR_UPD_NOTHING, [])
+                else:
+                    kind, prm = u.encode()
+                    self.be.system_add_updater(h, stage, kind, prm)
+        self._sys[k] = h
+        self._objs.append(ps)
+        return h
+
+    def ghost(self, entry):
+        """Handle of the ghost system of a GhostEntry."""
+        g = entry.ghost
+        k = id(g)
+        if k in self._ghost:
+            return self._ghost[k]
+        src = self.system(g.source)
+        h = self.be.ghost_create(src, entry.cutoff, entry.normals, entry.points)
+        for stage, cop in enumerate(g.updaters, start=1):
+            if cop is None:
+                continue
+            fields = [A.FIELD_IDS[f] for f in cop.fields]
+            self.be.ghost_set_copier(h, stage, fields, list(cop.modes))
+        self._ghost[k] = h
+        self._objs.extend([entry, g])
+        return h
+
+    def ghost_handle(self, g):
+        try:
+            return self._ghost[id(g)]
+        except KeyError:
+            raise ValueError(f"ghost system {g.name} has no registered GhostEntry") from None
+
+    def interaction(self, si):
+        k = id(si)
+        if k in self._inter:
+            return self._inter[k]
+        a = self.system(si.system_a)
+        b = si.system_b
+        wrap, normal, point, beta, ljcut = A.GR_WRAP_NONE, None, None, 0.0, 0.0
+        if b is None:
+            partner = -1
+        elif isinstance(b, M.GhostParticleSystem):
+            partner = self.ghost_handle(b)
+        else:
+            if isinstance(b, M.StaticBoundarySystem):
+                wrap, ljcut = A.GR_WRAP_STATIC, b.lj_cutoff
+            elif isinstance(b, M.DynamicBoundarySystem):
+                wrap = A.GR_WRAP_DYNAMIC
+                normal, point, beta = b.boundary_normal, b.boundary_point, b.boundary_beta
+            partner = self.system(b)
+        pfns = [M.encode_pfn(p) for p in si.pfns]
+        h = self.be.interaction_create(si.kernel.kind, si.kernel.h, a, partner, wrap, normal,
+                                       point, beta, ljcut, pfns)
+        self._inter[k] = h
+        self._objs.append(si)
+        return h
+
+    def integrator(self, lf):
+        k = id(lf)
+        if k in self._integ:
+            return self._integ[k]
+        sys = [self.system(ps) for ps in lf.systems]
+        gh = [self.ghost(ge) for ge in lf.ghosts]
+        its = [self.interaction(si) for si in lf.interactions]
+        h = self.be.integrator_create(sys, its, gh, lf.gamma)
+        self._integ[k] = h
+        self._objs.append(lf)
+        return h
+
+    # ---- host <-> context -------------------------------------------------
+    def upload(self, ps, fields=None):
+        if isinstance(ps, (M.StaticBoundarySystem, M.DynamicBoundarySystem)):
+            ps = ps.inner
+        if isinstance(ps, M.GhostParticleSystem):
+            h = self.ghost_handle(ps)
+            names = fields or ["x", "v", "rho"] + list(ps.extras)
+            for f in names:
+                self.be.system_upload(h, A.FIELD_IDS[f], getattr(ps, f))
+            return
+        h = self.system(ps)
+        for f in fields or ps.field_names():
+            self.be.system_upload(h, A.FIELD_IDS[f], getattr(ps, f))
+        if fields is None:
+            self.be.system_upload_index(h, A.GR_I_ID, ps.id)
+
+    def download(self, ps, fields=None):
+        if isinstance(ps, (M.StaticBoundarySystem, M.DynamicBoundarySystem)):
+            ps = ps.inner
+        if isinstance(ps, M.GhostParticleSystem):
+            h = self.ghost_handle(ps)
+            n = self.be.system_get_n(h)
+            ps._resize(n)
+            if n == 0:
+                return
+            for f in fields or ["x", "v", "rho", "normals"] + list(ps.extras):
+                arr = self.be.system_download(h, A.FIELD_IDS[f], n * _field_comps(ps, f))
+                getattr(ps, f)[...] = arr.reshape(getattr(ps, f).shape)
+            ps.idx_original[:] = self.be.system_download_index(h, A.GR_I_IDX_ORIGINAL, n)
+            ps.idx_boundary[:] = self.be.system_download_index(h, A.GR_I_IDX_BOUNDARY, n)
+            return
+        h = self.system(ps)
+        for f in fields or ps.field_names():
+            arr = self.be.system_download(h, A.FIELD_IDS[f], ps.n * _field_comps(ps, f))
+            getattr(ps, f)[...] = arr.reshape(getattr(ps, f).shape)
+        if fields is None:
+            ps.id[:] = self.be.system_download_index(h, A.GR_I_ID, ps.n)
+
+    def upload_all(self, lf):
+        self.integrator(lf)
+        for ps in lf.systems:
+            self.upload(ps)
+
+    def download_all(self, lf):
+        for ps in lf.systems:
+            self.download(ps)
+        for ge in lf.ghosts:
+            self.download(ge.ghost)
+
+    # ---- grid introspection -------------------------------------------------
+    def grid(self, si):
+        h = self.interaction(si)
+        ming, ng, ncells = self.be.get_grid(h, self.ndims)
+        cs = self.be.get_cell_start(h, 0, ncells + 1)
+        csa = self.be.get_cell_start(h, 1, ncells + 1) if si.system_b is not None else None
+        return ming, ng, cs, csa

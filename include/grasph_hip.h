@@ -1,0 +1,240 @@
+/*
+ * grasph_hip.h — C ABI of the MI355X (gfx950) particle-step library.
+ *
+ * The library replaces the execution of Grasph.jl's particle step
+ * (sort → ghosts → grid → half-step → staged updaters/sweeps → full step →
+ * positions) with HIP kernels.  The caller keeps its particle systems on the
+ * host; the library keeps a device copy in a packed structure-of-arrays pool
+ * and advances it.  Every entry point returns an int status (GR_OK on
+ * success); gr_last_error() returns a message.  No C++ exception crosses the
+ * ABI.  Host buffers passed in are borrowed for the duration of the call.
+ *
+ * Host array layout: the reference's `reinterpret(reshape, T, Vector{SVector})`
+ * layout, i.e. interleaved AoS (x1,y1,z1,x2,...), doubles; index arrays are
+ * int64 and 1-based, as in Julia.
+ *
+ * A context is not thread-safe.  One context = one HIP stream = one pool of a
+ * single spatial dimension; it can hold many independent simulations
+ * ("integrators"), which gr_step_many advances together in one launch per
+ * phase (packed, offset-indexed pool).
+ *
+ * Reference interfaces each entry point replaces are cited per function
+ * (paths relative to the reference repository root).
+ */
+#ifndef GRASPH_HIP_H
+#define GRASPH_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------- */
+enum {
+  GR_OK = 0,
+  GR_ERR_ARGUMENT = 1,   /* reference: ArgumentError                     */
+  GR_ERR_ASSERTION = 2,  /* reference: AssertionError (@assert)          */
+  GR_ERR_HIP = 3,        /* HIP runtime failure                           */
+  GR_ERR_CAPACITY = 4,   /* a grid/key range outgrew its device capacity  */
+  GR_ERR_STATE = 5       /* call made in the wrong state                  */
+};
+
+/* ---- arithmetic precision of the pair sums (gr_ctx_create) ------------ */
+enum {
+  GR_PREC_F32 = 0,  /* FP32 pair math + Kahan sums; FP64 x, v, rho, stress */
+  GR_PREC_F64 = 1   /* FP64 everywhere                                    */
+};
+
+/* ---- particle system kinds (src/Particles.jl:38,149,245,351) ---------- */
+enum {
+  GR_SYS_BASIC = 0,   /* BasicParticleSystem                              */
+  GR_SYS_FLUID = 1,   /* FluidParticleSystem (+ p)                        */
+  GR_SYS_STRESS = 2,  /* StressParticleSystem (+ p, stress, strain_rate)  */
+  GR_SYS_EP = 3,      /* ElastoPlasticParticleSystem (+ vorticity, strain, strain_p) */
+  GR_SYS_GHOST = 4    /* GhostParticleSystem (src/GhostParticles.jl:174)  */
+};
+
+/* ---- per-particle fields (double arrays) ------------------------------ */
+enum {
+  GR_F_X = 0, GR_F_V = 1, GR_F_RHO = 2, GR_F_DVDT = 3, GR_F_DRHODT = 4,
+  GR_F_P = 5, GR_F_STRESS = 6, GR_F_STRAIN_RATE = 7, GR_F_VORTICITY = 8,
+  GR_F_STRAIN = 9, GR_F_STRAIN_P = 10, GR_F_V_ADJUSTMENT = 11,
+  GR_F_NORMALS = 12,
+  GR_NUM_FIELDS = 13
+};
+/* ---- per-particle index fields (int64, 1-based) ----------------------- */
+enum { GR_I_ID = 0, GR_I_IDX_ORIGINAL = 1, GR_I_IDX_BOUNDARY = 2 };
+
+/* ---- SPH kernels (src/Kernels.jl:18,92) -------------------------------- */
+enum { GR_KERNEL_CUBIC = 0, GR_KERNEL_WENDLAND_C2 = 1 };
+
+/* ---- pair functors (src/PairwiseFunctors.jl) -------------------------- */
+enum {
+  GR_PFN_NONE = 0,    /* `nothing` stage                                   */
+  GR_PFN_FLUID = 1,   /* FluidPfn :136                                     */
+  GR_PFN_CAUCHY = 2,  /* CauchyFluidPfn :252                               */
+  GR_PFN_SR = 3,      /* StrainRatePfn :18                                 */
+  GR_PFN_SRV = 4,     /* StrainRateVorticityPfn :78                        */
+  GR_PFN_COUNT = 5    /* neighbour counter: +1 to dvdt[1] and drhodt per pair,
+                         both sides (the counting functors of
+                         test/test_interaction_sweep.jl:14-30)              */
+};
+
+/* ---- partner wrappers (src/BoundaryParticles.jl:38,68) ---------------- */
+enum { GR_WRAP_NONE = 0, GR_WRAP_STATIC = 1, GR_WRAP_DYNAMIC = 2 };
+
+/* ---- state updaters (src/StateUpdaters.jl) ----------------------------- */
+enum {
+  GR_UPD_NOTHING = 0,  /* `nothing` stage entry                            */
+  GR_UPD_ZERO = 1,     /* ZeroFieldUpdater :30; params[0] = field bit mask
+                          (1<<GR_F_*)                                       */
+  GR_UPD_TAIT = 2,     /* TaitEOSUpdater :108; params = {rho0, gamma}      */
+  GR_UPD_LINEAR = 3,   /* LinearEOSUpdater :131; params = {rho0}           */
+  GR_UPD_EP = 4,       /* ElastoPlasticStressUpdater :202;
+                          params = {E, nu, phi, psi, cohesion}             */
+  GR_UPD_VPMC = 5      /* ViscoPlasticMCStressUpdater :144;
+                          params = {eos_kind(GR_UPD_TAIT|LINEAR), rho0,
+                                    gamma, phi, cohesion}                  */
+};
+
+/* ---- ghost copier modes (src/GhostParticles.jl:26,53) ------------------ */
+enum { GR_COPY_PLAIN = 0, GR_COPY_HOUSEHOLDER = 1 };
+
+/* Pair-functor descriptor: one per interaction stage.
+ * delta / epsilon: NaN means `nothing` (FluidPfn :143, CauchyFluidPfn :258). */
+typedef struct gr_pfn_desc {
+  int32_t kind;     /* GR_PFN_*                         */
+  int32_t sigma;    /* FluidPfn S (1 or 2)              */
+  double alpha;     /* art_visc_alpha                   */
+  double beta;      /* art_visc_beta                    */
+  double h;         /* functor h (artificial viscosity) */
+  double delta;     /* δ-SPH coefficient or NaN         */
+  double epsilon;   /* surface-tension ε or NaN         */
+} gr_pfn_desc;
+
+typedef struct gr_ctx gr_ctx;
+
+/* ---- context -----------------------------------------------------------
+ * One HIP stream and one packed particle pool of spatial dimension `ndims`
+ * (2 or 3) on HIP device `device`. */
+int gr_ctx_create(gr_ctx **out, int device, int ndims, int precision);
+int gr_ctx_destroy(gr_ctx *ctx);
+const char *gr_last_error(const gr_ctx *ctx);
+/* Device and build description (for logs). */
+int gr_ctx_info(gr_ctx *ctx, char *buf, int64_t buflen);
+
+/* ---- particle systems --------------------------------------------------
+ * Replaces the constructors BasicParticleSystem / FluidParticleSystem /
+ * StressParticleSystem / ElastoPlasticParticleSystem
+ * (src/Particles.jl:61,173,272,381): n > 0, ns ∈ {3,4,6} for stress kinds,
+ * source_v has ndims entries.  Fields start as the reference's do: x, v, rho
+ * must be uploaded; everything else is zero; id = 1..n. */
+int gr_system_create(gr_ctx *ctx, int kind, int64_t n, int ns, double mass,
+                     double c, const double *source_v, double source_rho,
+                     int *sys_out);
+/* Register the updater at `stage` (1-based).  Several calls for one stage
+ * form the tuple applied in order (src/Particles.jl:562-565).  The number of
+ * updater stages is the largest stage registered (GR_UPD_NOTHING registers
+ * a `nothing` stage), matching `length(ps.state_updater)`. */
+int gr_system_add_updater(gr_ctx *ctx, int sys, int stage, int kind,
+                          const double *params, int nparams);
+/* Host → device copy of one field (count = n × components).  For ghost
+ * systems only GR_F_X/V/RHO/P/STRESS of live ghosts are accepted. */
+int gr_system_upload(gr_ctx *ctx, int sys, int field, const double *data,
+                     int64_t count);
+int gr_system_download(gr_ctx *ctx, int sys, int field, double *data,
+                       int64_t count);
+int gr_system_upload_index(gr_ctx *ctx, int sys, int field,
+                           const int64_t *data, int64_t count);
+int gr_system_download_index(gr_ctx *ctx, int sys, int field, int64_t *data,
+                             int64_t count);
+/* Current particle count (ghost systems: the count after the last
+ * generate_ghosts; reference `ghost.n`, src/GhostParticles.jl:228). */
+int gr_system_get_n(gr_ctx *ctx, int sys, int64_t *n_out);
+
+/* ---- ghosts --------------------------------------------------------------
+ * GhostParticleSystem(ps, copiers...) + GhostEntry(ghost, cutoff,
+ * (normal, point)...) (src/GhostParticles.jl:199,317).  normals/points hold
+ * nb × ndims doubles. */
+int gr_ghost_create(gr_ctx *ctx, int source_sys, double cutoff, int nb,
+                    const double *normals, const double *points,
+                    int *ghost_out);
+/* GhostCopier(fields... [=> HouseholderReflect()]) registered at `stage`
+ * (src/GhostParticles.jl:55).  fields ∈ {GR_F_P, GR_F_STRESS}. */
+int gr_ghost_set_copier(gr_ctx *ctx, int ghost, int stage, int nfields,
+                        const int *fields, const int *modes);
+
+/* ---- interactions --------------------------------------------------------
+ * SystemInteraction(kernel, pfns, system_a[, system_b]) (src/Interaction.jl:61).
+ * partner < 0: self interaction.  Otherwise partner is a system or ghost id;
+ * wrap selects StaticBoundarySystem(inner, lj_cutoff) or
+ * DynamicBoundarySystem(inner, normal, point, beta) around a real partner
+ * (src/BoundaryParticles.jl:47,79).  normal/point hold ndims doubles (may be
+ * NULL for GR_WRAP_NONE/STATIC). */
+int gr_interaction_create(gr_ctx *ctx, int kernel_kind, double h, int sys_a,
+                          int partner, int wrap, const double *normal,
+                          const double *point, double beta, double lj_cutoff,
+                          int n_stages, const gr_pfn_desc *pfns,
+                          int *inter_out);
+
+/* ---- integrator ----------------------------------------------------------
+ * LeapFrogTimeIntegrator(systems, interactions; ghosts, Γ)
+ * (src/TimeIntegration.jl:53).  Validates as the reference does and fixes
+ * the per-step plan.  A context may hold many integrators (an ensemble). */
+int gr_integrator_create(gr_ctx *ctx, int nsys, const int *systems,
+                         int ninter, const int *inters, int nghost,
+                         const int *ghosts, double gamma, int *integ_out);
+/* dt = CFL·h/c with h = min kernel h, c = max system c
+ * (src/TimeIntegration.jl:63-64,598). */
+int gr_integrator_dt(gr_ctx *ctx, int integ, double cfl, double *dt_out);
+/* Override the step size of one integrator (ensembles: per-sim dt). */
+int gr_integrator_set_dt(gr_ctx *ctx, int integ, double dt);
+/* Advance one integrator `nsteps` leapfrog steps
+ * (the per-step body of time_integrate!, src/TimeIntegration.jl:645-685;
+ * print/save cadence stays with the caller). */
+int gr_step(gr_ctx *ctx, int integ, int nsteps, double dt);
+/* Advance every listed integrator `nsteps` steps with its own dt, all of
+ * them in the same kernel launches (packed pool). */
+int gr_step_many(gr_ctx *ctx, int ninteg, const int *integs, int nsteps);
+/* Capture one step of the listed integrators in a hipGraph and replay it
+ * for subsequent gr_step_many calls (0 = off). */
+int gr_set_graph_mode(gr_ctx *ctx, int on);
+
+/* ---- fine-grained drop-ins (one reference call each) --------------------- */
+/* sort_particles!(ps, cutoff, ...) (src/Sorting.jl:157). */
+int gr_sort(gr_ctx *ctx, int sys, double cutoff);
+/* create_grid!(si) (src/Interaction.jl:123). */
+int gr_create_grid(gr_ctx *ctx, int inter);
+/* sweep!(si, stage) (src/Interaction.jl:292): accumulates into the target
+ * fields (+=), as the reference does. */
+int gr_sweep(gr_ctx *ctx, int inter, int stage);
+/* update_state!(ps, stage, dt) (src/Particles.jl:542). */
+int gr_update_state(gr_ctx *ctx, int sys, int stage, double dt);
+/* generate_ghosts!(ge) / update_ghost_kinematics!(ge) / update_ghost!(ge, stage)
+ * (src/GhostParticles.jl:344,404,423). */
+int gr_generate_ghosts(gr_ctx *ctx, int ghost);
+int gr_update_ghost_kinematics(gr_ctx *ctx, int ghost);
+int gr_update_ghost(gr_ctx *ctx, int ghost, int stage);
+/* Grid introspection (src/Interaction.jl:29-34): mingridx[ndims],
+ * ngridx[ndims], cell_start (which = 0: _cell_start, 1: _cell_start_a),
+ * length ncells+1, 1-based values. */
+int gr_get_grid(gr_ctx *ctx, int inter, double *mingridx, int64_t *ngridx,
+                int64_t *ncells);
+int gr_get_cell_start(gr_ctx *ctx, int inter, int which, int64_t *out,
+                      int64_t count);
+
+/* ---- timing ---------------------------------------------------------------
+ * Per-phase device time of the last gr_step/gr_step_many call (ms), measured
+ * with HIP events on the context stream.  phase: 0 sort, 1 ghosts, 2 grid,
+ * 3 half-step, 4 updaters, 5 sweep, 6 full-step, 7 total. */
+int gr_set_timing(gr_ctx *ctx, int on);
+int gr_get_timing(gr_ctx *ctx, double *ms_out, int nphases);
+/* Unique pairs with r < 2h counted by the last sweep launches of one
+ * integrator (requires timing mode; diagnostic). */
+int gr_get_pair_count(gr_ctx *ctx, int integ, int64_t *pairs_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GRASPH_HIP_H */
