@@ -1,0 +1,197 @@
+"""Benchmark of the particle step on one MI355X.
+
+Workload: the 3D dam break — 1.0M fluid particles (0.6 m cube, dx = 0.006 m)
+in a 1.6 × 0.6 × 0.8 m tank with 3-layer dynamic floor and walls (~0.38M
+boundary particles), Wendland C2 h = 1.3dx, Tait EOS, FluidPfn α = 0.1,
+fluid order shuffled by seed.  Synthetic, seeded input built in-process.
+
+A "step" is one leapfrog step of the whole scene (sort, grids, half step,
+fused sweep, full step).  The timed region holds inputs already resident in
+HBM; the library synchronises its stream on return of every gr_step call.
+
+Prints one JSON line (see the fields below).  --gpus N > 1 is not supported:
+this build runs on one GPU.
+"""
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+import __graft_entry__  # noqa: E402
+
+METRIC = "particle-pair updates/s & % HBM roofline; served sims/h + p50/p99 vs plain"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--dx", type=float, default=0.006)
+    ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--cpu-steps", type=int, default=8,
+                    help="timed oracle steps on the host (0 disables the CPU baseline)")
+    ap.add_argument("--graph", type=int, default=1, help="replay the step as a hipGraph")
+    return ap.parse_args()
+
+
+def sweep_bytes(cfg, grids):
+    """Algorithmic bytes of one sweep launch (FP32 state, FP64 positions):
+    fluid reads x (8D), v (4D), ρ (4), p (4) and writes dvdt (4D), drhodt (4);
+    each dynamic-boundary particle is read for x (8D); every cell array the
+    gather reads is read once (4 B per entry)."""
+    D = 3
+    nf = cfg["fluid_n"]
+    nb = cfg["boundary_n"]
+    cells = sum(nc + 1 for nc in grids)
+    return nf * (8 * D + 4 * D + 4 + 4) + nf * (4 * D + 4) + nb * 8 * D + 4 * cells
+
+
+def load_traffic():
+    """HBM bytes per sweep launch from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE + WRITE_SIZE, KiB per dispatch, calibrated on a plain copy
+    kernel of known size), or None if no summary is committed."""
+    p = ROOT / "profiles" / "sweep_dambreak3d_pmc.json"
+    if not p.exists():
+        return None
+    try:
+        d = json.loads(p.read_text())
+        return float(d["traffic_bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def cpu_baseline(cfg_builder, kw, nsteps, pairs_per_step):
+    """The FP64 oracle (reference-algorithm restatement, OpenMP over cell
+    colours) on the same workload, timed on this host's cores."""
+    if nsteps <= 0:
+        return None
+    g = __graft_entry__.package()
+    A = g._abi
+    lib_path = ROOT / "oracle" / "libgrasph_oracle.so"
+    lib = C.CDLL(str(lib_path))
+    lib.orc_scene_create.argtypes = [C.POINTER(C.c_void_p), C.c_int]
+    lib.orc_set_threads.argtypes = [C.c_void_p, C.c_int]
+    ctx = C.c_void_p()
+    assert lib.orc_scene_create(C.byref(ctx), 3) == 0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    lib.orc_set_threads(ctx, threads)
+    be = A.CBackend(lib, "orc_", ctx, "orc_last_error", "orc_scene_destroy")
+    cfg = cfg_builder(**kw)
+    lf = cfg["integrator"]
+    sc = g.DeviceScene(be, 3)
+    h = sc.integrator(lf)
+    sc.upload_all(lf)
+    dt = be.integrator_dt(h, cfg["cfl"])
+    t0 = time.perf_counter()
+    be.step(h, 1, dt)  # first step: sorts the shuffled order
+    t1 = time.perf_counter()
+    be.step(h, nsteps, dt)
+    t2 = time.perf_counter()
+    el = t2 - t1
+    be.close()
+    return {"value": pairs_per_step * nsteps / el, "unit": "pair-updates/s", "cores": threads,
+            "kind": "port",
+            "sample": f"oracle (FP64 C++ restatement of the reference algorithm) on the same "
+                      f"3D dam break, {nsteps} steps after 1 warm-up step "
+                      f"({t1 - t0:.2f} s incl. the first sort); {el / nsteps * 1e3:.1f} ms/step"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported",
+              file=sys.stderr)
+        sys.exit(1)
+    g = __graft_entry__.package()
+    kw = dict(dx=args.dx)
+    cfg = g.configs.dambreak_3d(**kw)
+    lf = cfg["integrator"]
+    s = g.Session(3, args.precision)
+    be, sc = s.backend, s.scene
+    h = sc.integrator(lf)
+    sc.upload_all(lf)
+    dt = be.integrator_dt(h, cfg["cfl"])
+    be.integrator_set_dt(h, dt)
+    be.set_graph_mode(bool(args.graph))
+    info = be.ctx_info()
+    # warm-up (includes the first sort of the shuffled order and graph capture)
+    if args.warmup:
+        be.step_many([h], args.warmup)
+    # pair count of the current configuration (diagnostic counting kernels)
+    be.set_timing(2)
+    be.step_many([h], 1)
+    pairs = be.get_pair_count(h)
+    # timed region: HIP events at every phase, no extra synchronisation
+    be.set_timing(1)
+    t0 = time.perf_counter()
+    be.step_many([h], args.steps)
+    t1 = time.perf_counter()
+    phases = be.get_timing()
+    be.set_timing(0)
+    # graph replay without events (the production path), same step count
+    t2 = time.perf_counter()
+    be.step_many([h], args.steps)
+    t3 = time.perf_counter()
+    el_events, el_graph = t1 - t0, t3 - t2
+    el = min(el_events, el_graph) if args.graph else el_events
+    ms_step = el / args.steps * 1e3
+    grids = []
+    for si in lf.interactions:
+        _, _, nc = be.get_grid(sc.interaction(si), 3)
+        grids.append(nc)
+    bytes_sweep = sweep_bytes(cfg, grids)
+    sweep_ms = phases[5]
+    achieved = bytes_sweep / (sweep_ms * 1e-3) / 1e9
+    traffic = load_traffic()
+    cpu = cpu_baseline(g.configs.dambreak_3d, kw, args.cpu_steps, pairs)
+    total_pairs = pairs * args.steps
+    value = total_pairs / el
+    res = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "pair-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 pair math + fp64 positions/velocity/density" if args.precision == "f32"
+        else "fp64",
+        "data": "synthetic (seeded lattice, shuffled order)",
+        "config": {"workload": "dambreak_3d", "n_fluid": cfg["fluid_n"],
+                   "n_boundary": cfg["boundary_n"], "dx": args.dx, "kernel": "WendlandC2",
+                   "h_over_dx": 1.3, "boundary": "3-layer DynamicBoundarySystem x5",
+                   "pairs_per_step": pairs, "graph": bool(args.graph)},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic,
+                     "kernel": "k_sweep<3,float,FAM_FORCE> (fused gather: self + 5 dynamic walls)",
+                     "algorithmic_bytes_per_launch": bytes_sweep,
+                     "launch_ms": sweep_ms},
+        "cpu_baseline": cpu,
+        "steps_per_s": 1e3 / ms_step,
+        "phase_ms": {k: round(v, 4) for k, v in zip(
+            ("sort", "ghosts", "grid", "halfstep", "updaters", "sweep", "fullstep", "step"),
+            phases)},
+        "ms_per_step_events": el_events / args.steps * 1e3,
+        "ms_per_step_graph": el_graph / args.steps * 1e3,
+        "device": info,
+    }
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

@@ -92,6 +92,7 @@ _GPU_SIGS = {
     "get_timing": [_dp, _i],
     "get_pair_count": [_i, _i64p],
     "ctx_info": [C.c_char_p, _i64],
+    "bench_copy": [_i64, _i],
 }
 
 PKG_DIR = Path(__file__).resolve().parent
@@ -281,8 +282,12 @@ class CBackend:
     def set_graph_mode(self, on):
         self.call("set_graph_mode", int(on))
 
-    def set_timing(self, on):
-        self.call("set_timing", int(on))
+    def set_timing(self, mode):
+        """mode bit 0: per-phase HIP events; bit 1: pair counting."""
+        self.call("set_timing", int(mode))
+
+    def bench_copy(self, nbytes, reps):
+        self.call("bench_copy", int(nbytes), int(reps))
 
     def get_timing(self):
         a = np.zeros(8)

@@ -180,8 +180,9 @@ struct gr_ctx {
   DBuf xs_pool[2], xs_gxs;
   int64_t xs_cap = -1;
   std::vector<char> id_written;
-  bool graph_mode = false, timing = false;
+  bool graph_mode = false, timing = false, count_pairs = false;
   hipEvent_t ev[9] = {};
+  std::vector<std::vector<hipEvent_t>> tev;  // per-step phase events (timing mode)
   double phase_ms[8] = {0};
   long long last_pairs[2] = {0, 0};
   int last_pairs_integ_key = -1;
@@ -1193,7 +1194,7 @@ void phase_grids(gr_ctx *c, Plan &P) {
 }
 
 template <typename R>
-void phase_stage(gr_ctx *c, Plan &P, int k, bool cnt) {
+void phase_stage(gr_ctx *c, Plan &P, int k, bool cnt, hipEvent_t *mid) {
   hipStream_t s = c->stream;
   StagePlan &S = *P.stages[k];
   Pool<R> A = pool_view<R>(c, c->cur);
@@ -1201,7 +1202,7 @@ void phase_stage(gr_ctx *c, Plan &P, int k, bool cnt) {
                      S.nblk_upd, S.urange.as<int2>(), S.upd.as<UpdDesc>(), c->D);
   launch_ghost_copy<R>(s, A, c->d_sd.as<SysDesc>(), c->d_bnd.as<Bnd>(), c->d_n.as<int>(),
                        S.blk_gcopy.as<Blk>(), S.nblk_gcopy, S.gcopy.as<int2>(), c->D);
-  if (c->timing) HIPCHK(hipEventRecord(c->ev[5], s));
+  if (mid) HIPCHK(hipEventRecord(*mid, s));
   for (int fam : {FAM_FORCE, FAM_COUNT, FAM_STRAIN}) {
     FamPlan &F = S.fam[fam];
     if (F.nblk == 0) continue;
@@ -1218,60 +1219,77 @@ void phase_stage(gr_ctx *c, Plan &P, int k, bool cnt) {
     a.pair_count = c->d_pairs.as<unsigned long long>();
     launch_sweep<R>(s, A, a, c->D, fam, cnt);
   }
-  if (c->timing) HIPCHK(hipEventRecord(c->ev[6], s));
+}
+
+// Phase events of one step (timing mode): 0 start, 1 sorted, 2 ghosts,
+// 3 grids, 4 half-step, then per stage (updaters end, sweeps end), last
+// full-step end.  Recorded without synchronisation; read after the call.
+std::vector<hipEvent_t> &step_events(gr_ctx *c, int k, int nst) {
+  if ((int)c->tev.size() <= k) c->tev.resize(k + 1);
+  auto &v = c->tev[k];
+  const size_t need = 6 + 2 * nst;
+  while (v.size() < need) {
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    v.push_back(e);
+  }
+  return v;
 }
 
 template <typename R>
-void run_step(gr_ctx *c, Plan &P, bool cnt) {
+void run_step(gr_ctx *c, Plan &P, bool cnt, std::vector<hipEvent_t> *ev) {
   hipStream_t s = c->stream;
-  const bool T = c->timing;
-  if (T) HIPCHK(hipEventRecord(c->ev[0], s));
+  auto rec = [&](size_t q) {
+    if (ev) HIPCHK(hipEventRecord((*ev)[q], s));
+  };
+  rec(0);
   phase_sort_real<R>(c, P);
-  if (T) HIPCHK(hipEventRecord(c->ev[1], s));
+  rec(1);
   phase_ghosts<R>(c, P, true, true);
-  if (T) HIPCHK(hipEventRecord(c->ev[2], s));
+  rec(2);
   phase_grids(c, P);
-  if (T) HIPCHK(hipEventRecord(c->ev[3], s));
+  rec(3);
   Pool<R> A = pool_view<R>(c, c->cur);
   launch_halfstep<R>(s, A, c->d_sd.as<SysDesc>(), c->d_st.as<StepSys>(), P.blk_real.as<Blk>(),
                      P.nblk_real, c->D);
   launch_ghost_kin<R>(s, A, c->d_sd.as<SysDesc>(), c->d_bnd.as<Bnd>(), c->d_n.as<int>(),
                       P.blk_ghost.as<Blk>(), P.nblk_ghost, c->D);
-  if (T) HIPCHK(hipEventRecord(c->ev[4], s));
-  float sweep_ms = 0.0f, upd_ms = 0.0f;
-  for (int k = 0; k < (int)P.stages.size(); ++k) {
-    phase_stage<R>(c, P, k, cnt);
-    if (T) {
-      HIPCHK(hipEventSynchronize(c->ev[6]));
-      float a = 0, b = 0;
-      HIPCHK(hipEventElapsedTime(&a, c->ev[5], c->ev[6]));
-      sweep_ms += a;
-      HIPCHK(hipEventElapsedTime(&b, k == 0 ? c->ev[4] : c->ev[7], c->ev[5]));
-      upd_ms += b;
-      HIPCHK(hipEventRecord(c->ev[7], s));
-    }
+  rec(4);
+  const int nst = (int)P.stages.size();
+  for (int k = 0; k < nst; ++k) {
+    phase_stage<R>(c, P, k, cnt, ev ? &(*ev)[5 + 2 * k] : nullptr);
+    rec(6 + 2 * k);
   }
-  if (T) HIPCHK(hipEventRecord(c->ev[7], s));
   launch_fullstep<R>(s, A, c->d_sd.as<SysDesc>(), c->d_st.as<StepSys>(), P.blk_real.as<Blk>(),
                      P.nblk_real, c->D);
-  if (T) {
-    HIPCHK(hipEventRecord(c->ev[8], s));
-    HIPCHK(hipEventSynchronize(c->ev[8]));
-    float ms[4];
-    for (int q = 0; q < 3; ++q) HIPCHK(hipEventElapsedTime(&ms[q], c->ev[q], c->ev[q + 1]));
-    float hs = 0, fs = 0, tot = 0;
-    HIPCHK(hipEventElapsedTime(&hs, c->ev[3], c->ev[4]));
-    HIPCHK(hipEventElapsedTime(&fs, c->ev[7], c->ev[8]));
-    HIPCHK(hipEventElapsedTime(&tot, c->ev[0], c->ev[8]));
-    c->phase_ms[0] += ms[0];
-    c->phase_ms[1] += ms[1];
-    c->phase_ms[2] += ms[2];
-    c->phase_ms[3] += hs;
-    c->phase_ms[4] += upd_ms;
-    c->phase_ms[5] += sweep_ms;
-    c->phase_ms[6] += fs;
-    c->phase_ms[7] += tot;
+  rec(5 + 2 * nst);
+}
+
+// Sum the phase times of `nsteps` recorded steps (ms per step).
+void collect_timing(gr_ctx *c, int nsteps, int nst) {
+  double acc[8] = {0};
+  for (int k = 0; k < nsteps; ++k) {
+    auto &v = c->tev[k];
+    HIPCHK(hipEventSynchronize(v[5 + 2 * nst]));
+    auto el = [&](size_t a, size_t b) {
+      float ms = 0.0f;
+      HIPCHK(hipEventElapsedTime(&ms, v[a], v[b]));
+      return (double)ms;
+    };
+    acc[0] += el(0, 1);
+    acc[1] += el(1, 2);
+    acc[2] += el(2, 3);
+    acc[3] += el(3, 4);
+    size_t prev = 4;
+    for (int q = 0; q < nst; ++q) {
+      acc[4] += el(prev, 5 + 2 * q);
+      acc[5] += el(5 + 2 * q, 6 + 2 * q);
+      prev = 6 + 2 * q;
+    }
+    acc[6] += el(prev, 5 + 2 * nst);
+    acc[7] += el(0, 5 + 2 * nst);
   }
+  for (int q = 0; q < 8; ++q) c->phase_ms[q] = acc[q] / std::max(nsteps, 1);
 }
 
 // Graph-captured step: two executables, one per A/B parity of the pool.
@@ -1282,7 +1300,7 @@ void step_graph(gr_ctx *c, Plan &P) {
     const int cur0 = c->cur;
     hipGraph_t graph;
     HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-    run_step<R>(c, P, false);
+    run_step<R>(c, P, false, nullptr);
     HIPCHK(hipStreamEndCapture(c->stream, &graph));
     HIPCHK(hipGraphInstantiate(&g, graph, nullptr, nullptr, 0));
     HIPCHK(hipGraphDestroy(graph));
@@ -1304,32 +1322,31 @@ void do_steps(gr_ctx *c, const std::vector<int> &integs, int nsteps) {
     all_real(c, b, nb);  // build cached tables before any capture
     xs_ptrs(c);
   }
-  if (c->timing) {
-    for (double &m : c->phase_ms) m = 0;
-    launch_fill_i32(c->stream, c->d_pairs.as<int>(), 0, 4);
-  }
+  if (c->count_pairs) launch_fill_i32(c->stream, c->d_pairs.as<int>(), 0, 4);
+  const int nst = (int)P.stages.size();
   for (int k = 0; k < nsteps; ++k) {
-    if (c->graph_mode && !c->timing && !P.real_list.empty())
+    if (c->timing)
+      run_step<R>(c, P, c->count_pairs, &step_events(c, k, nst));
+    else if (c->graph_mode && !c->count_pairs && !P.real_list.empty())
       step_graph<R>(c, P);
     else
-      run_step<R>(c, P, c->timing);
+      run_step<R>(c, P, c->count_pairs, nullptr);
   }
   int flags[4];
   HIPCHK(hipMemcpyAsync(flags, c->d_flags.p, sizeof(flags), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  if (c->timing) {
+  if (c->timing) collect_timing(c, nsteps, nst);
+  if (c->count_pairs) {
     unsigned long long pc[2];
     HIPCHK(hipMemcpyAsync(pc, c->d_pairs.p, sizeof(pc), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    c->last_pairs[0] = (long long)pc[0];
-    c->last_pairs[1] = (long long)pc[1];
-    for (double &m : c->phase_ms) m /= std::max(nsteps, 1);
+    c->last_pairs[0] = (long long)(pc[0] / std::max(nsteps, 1));
+    c->last_pairs[1] = (long long)(pc[1] / std::max(nsteps, 1));
   }
   if (flags[1]) {
     int z[4] = {0, 0, 0, 0};
     HIPCHK(hipMemcpyAsync(c->d_flags.p, z, sizeof(z), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    // invalidate plans so the next call re-plans with larger capacity
     throw GrError(GR_ERR_CAPACITY,
                   (flags[1] & 2) ? "cell grid outgrew its capacity (particles left the planned domain)"
                                  : "cell-key range exceeds 32 bits");
@@ -1356,11 +1373,16 @@ __global__ void k_refresh(Pool<R> P, const SysDesc *sd, const Blk *blk, int D) {
   const int i = b.first + threadIdx.x;
   if (i >= S.cap) return;
   const int64_t g = S.off + i;
-  for (int d = 0; d < D; ++d) P.vs[d][g] = (R)P.v[d][g];
+#pragma unroll
+  for (int d = 0; d < 3; ++d)
+    if (d < D) P.vs[d][g] = (R)P.v[d][g];
   P.rhos[g] = (R)P.rho[g];
   P.rho_h[g] = P.rho[g];
-  if (S.kind >= GR_SYS_STRESS)
-    for (int k = 0; k < S.ns; ++k) P.stress_s[k][g] = (R)P.stress[k][g];
+  if (S.kind >= GR_SYS_STRESS) {
+#pragma unroll
+    for (int k = 0; k < GR_MAXNS; ++k)
+      if (k < S.ns) P.stress_s[k][g] = (R)P.stress[k][g];
+  }
 }
 
 namespace {
@@ -1436,6 +1458,8 @@ int gr_ctx_destroy(gr_ctx *c) {
   c->plans.clear();
   for (auto &e : c->ev)
     if (e) (void)hipEventDestroy(e);
+  for (auto &v : c->tev)
+    for (auto &e : v) (void)hipEventDestroy(e);
   hipStream_t s = c->stream;
   delete c;
   (void)hipStreamDestroy(s);
@@ -1900,7 +1924,10 @@ int gr_set_graph_mode(gr_ctx *c, int on) {
   return guard(c, [&] { c->graph_mode = on != 0; });
 }
 int gr_set_timing(gr_ctx *c, int on) {
-  return guard(c, [&] { c->timing = on != 0; });
+  return guard(c, [&] {
+    c->timing = (on & 1) != 0;
+    c->count_pairs = (on & 2) != 0;
+  });
 }
 int gr_get_timing(gr_ctx *c, double *ms_out, int nphases) {
   return guard(c, [&] {
@@ -1910,7 +1937,7 @@ int gr_get_timing(gr_ctx *c, double *ms_out, int nphases) {
 int gr_get_pair_count(gr_ctx *c, int integ, int64_t *pairs_out) {
   return guard(c, [&] {
     (void)integ;
-    *pairs_out = c->last_pairs[0] / 2 + c->last_pairs[1];
+    *pairs_out = c->last_pairs[0] / 2 + c->last_pairs[1];  // per step
   });
 }
 
@@ -2171,3 +2198,25 @@ int gr_get_cell_start(gr_ctx *c, int inter, int which, int64_t *out, int64_t cou
 }
 
 }  // extern "C"
+
+// ---- calibration copy kernel ------------------------------------------------
+
+__global__ void k_bench_copy(const float4 *__restrict__ a, float4 *__restrict__ b, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) b[i] = a[i];
+}
+
+extern "C" int gr_bench_copy(gr_ctx *c, int64_t bytes, int reps) {
+  return guard(c, [&] {
+    need(bytes >= 16 && reps >= 1, GR_ERR_ARGUMENT, "bytes >= 16 and reps >= 1");
+    DBuf a, b;
+    a.alloc(bytes);
+    b.alloc(bytes);
+    HIPCHK(hipMemsetAsync(a.p, 0, bytes, c->stream));
+    const long long n = bytes / 16;
+    for (int r = 0; r < reps; ++r)
+      k_bench_copy<<<1024, 256, 0, c->stream>>>(a.as<const float4>(), b.as<float4>(), n);
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}

@@ -302,9 +302,9 @@ __global__ void k_radix_scatter(const unsigned *kin, const int *vin, unsigned *k
 // every real particle; identity where the system is not sorted this step.
 // ---------------------------------------------------------------------------
 
-template <typename R>
+template <typename R, int D>
 __global__ void k_permute_real(Pool<R> A, Pool<R> B, const SysDesc *sd, const Blk *blk,
-                               const int *bbase, const int *vals, const int *flags, int D) {
+                               const int *bbase, const int *vals, const int *flags) {
   const Blk b = blk[blockIdx.x];
   const int s = b.item;
   const int i = b.first + threadIdx.x;
@@ -313,6 +313,7 @@ __global__ void k_permute_real(Pool<R> A, Pool<R> B, const SysDesc *sd, const Bl
   const int64_t dst = S.off + i;
   int64_t src = dst;
   if (flags[0] != 0 && bbase[s] >= 0) src = vals[bbase[s] + i];
+#pragma unroll
   for (int d = 0; d < D; ++d) {
     B.x[d][dst] = A.x[d][src];
     B.v[d][dst] = A.v[d][src];
@@ -323,19 +324,25 @@ __global__ void k_permute_real(Pool<R> A, Pool<R> B, const SysDesc *sd, const Bl
   B.id[dst] = A.id[src];
   if (S.kind >= GR_SYS_FLUID) B.p[dst] = A.p[src];
   if (S.kind >= GR_SYS_STRESS) {
-    for (int k = 0; k < S.ns; ++k) {
-      double sg = A.stress[k][src];
-      B.stress[k][dst] = sg;
-      B.stress_s[k][dst] = (R)sg;
-      B.sr[k][dst] = A.sr[k][src];
-    }
+#pragma unroll
+    for (int k = 0; k < GR_MAXNS; ++k)
+      if (k < S.ns) {
+        const double sg = A.stress[k][src];
+        B.stress[k][dst] = sg;
+        B.stress_s[k][dst] = (R)sg;
+        B.sr[k][dst] = A.sr[k][src];
+      }
   }
   if (S.kind == GR_SYS_EP) {
-    for (int k = 0; k < S.nv; ++k) B.vor[k][dst] = A.vor[k][src];
-    for (int k = 0; k < S.ns; ++k) {
-      B.strain[k][dst] = A.strain[k][src];
-      B.strainp[k][dst] = A.strainp[k][src];
-    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      if (k < S.nv) B.vor[k][dst] = A.vor[k][src];
+#pragma unroll
+    for (int k = 0; k < GR_MAXNS; ++k)
+      if (k < S.ns) {
+        B.strain[k][dst] = A.strain[k][src];
+        B.strainp[k][dst] = A.strainp[k][src];
+      }
   }
 }
 
@@ -411,9 +418,9 @@ __global__ void k_permute_ghost(const double *const *gxs, const int *gidx_s, con
 
 // update_ghost_kinematics! (:404-421): reflected half-step velocity of the
 // source and its density, with the normal of slot k.
-template <typename R>
+template <typename R, int D>
 __global__ void k_ghost_kin(Pool<R> P, const SysDesc *sd, const Bnd *bnd, const int *n_dev,
-                            const Blk *blk, int D) {
+                            const Blk *blk) {
   const Blk b = blk[blockIdx.x];
   const int g = b.item;
   const int k = b.first + threadIdx.x;
@@ -423,51 +430,81 @@ __global__ void k_ghost_kin(Pool<R> P, const SysDesc *sd, const Bnd *bnd, const 
   const int64_t go = G.off + k;
   const int64_t si = S.off + P.gidx[go];
   const Bnd B = bnd[G.bnd + P.gnrm[go]];
-  R vr[3];
+  R vr[D];
+#pragma unroll
   for (int d = 0; d < D; ++d) vr[d] = P.vs[d][si];
   R dd = vr[0] * (R)B.n[0];
+#pragma unroll
   for (int d = 1; d < D; ++d) dd = dd + vr[d] * (R)B.n[d];
+#pragma unroll
   for (int d = 0; d < D; ++d) P.vs[d][go] = vr[d] - (R)2 * dd * (R)B.n[d];
   P.rhos[go] = P.rhos[si];
 }
 
-// _apply_mode(σ, HouseholderReflect(), n̂) (:66-110) in FP64.
-__device__ inline void householder(const double *s, const double *n, int NS, double *o) {
-  if (NS == 3) {
-    double nx = n[0], ny = n[1];
-    double tx = s[0] * nx + s[2] * ny, ty = s[2] * nx + s[1] * ny;
-    double q = tx * nx + ty * ny;
-    o[0] = s[0] - 4 * tx * nx + 4 * q * nx * nx;
-    o[1] = s[1] - 4 * ty * ny + 4 * q * ny * ny;
-    o[2] = s[2] - 2 * (tx * ny + nx * ty) + 4 * q * nx * ny;
-  } else if (NS == 4) {
-    double nx = n[0], ny = n[1];
-    double tx = s[0] * nx + s[3] * ny, ty = s[3] * nx + s[1] * ny;
-    double q = tx * nx + ty * ny;
-    o[0] = s[0] - 4 * tx * nx + 4 * q * nx * nx;
-    o[1] = s[1] - 4 * ty * ny + 4 * q * ny * ny;
-    o[2] = s[2];
-    o[3] = s[3] - 2 * (tx * ny + nx * ty) + 4 * q * nx * ny;
-  } else {
-    double nx = n[0], ny = n[1], nz = n[2];
-    double tx = s[0] * nx + s[3] * ny + s[4] * nz;
-    double ty = s[3] * nx + s[1] * ny + s[5] * nz;
-    double tz = s[4] * nx + s[5] * ny + s[2] * nz;
-    double q = tx * nx + ty * ny + tz * nz;
-    o[0] = s[0] - 4 * tx * nx + 4 * q * nx * nx;
-    o[1] = s[1] - 4 * ty * ny + 4 * q * ny * ny;
-    o[2] = s[2] - 4 * tz * nz + 4 * q * nz * nz;
-    o[3] = s[3] - 2 * (tx * ny + nx * ty) + 4 * q * nx * ny;
-    o[4] = s[4] - 2 * (tx * nz + nx * tz) + 4 * q * nx * nz;
-    o[5] = s[5] - 2 * (ty * nz + ny * tz) + 4 * q * ny * nz;
-  }
+// _apply_mode(σ, HouseholderReflect(), n̂) (:66-110) in FP64, one function
+// per Voigt length so every index is static.
+__device__ inline void householder3(double *s, const double *n) {
+  const double nx = n[0], ny = n[1];
+  const double tx = s[0] * nx + s[2] * ny, ty = s[2] * nx + s[1] * ny;
+  const double q = tx * nx + ty * ny;
+  const double o0 = s[0] - 4 * tx * nx + 4 * q * nx * nx;
+  const double o1 = s[1] - 4 * ty * ny + 4 * q * ny * ny;
+  const double o2 = s[2] - 2 * (tx * ny + nx * ty) + 4 * q * nx * ny;
+  s[0] = o0;
+  s[1] = o1;
+  s[2] = o2;
+}
+__device__ inline void householder4(double *s, const double *n) {
+  const double nx = n[0], ny = n[1];
+  const double tx = s[0] * nx + s[3] * ny, ty = s[3] * nx + s[1] * ny;
+  const double q = tx * nx + ty * ny;
+  const double o0 = s[0] - 4 * tx * nx + 4 * q * nx * nx;
+  const double o1 = s[1] - 4 * ty * ny + 4 * q * ny * ny;
+  const double o3 = s[3] - 2 * (tx * ny + nx * ty) + 4 * q * nx * ny;
+  s[0] = o0;
+  s[1] = o1;
+  s[3] = o3;
+}
+__device__ inline void householder6(double *s, const double *n) {
+  const double nx = n[0], ny = n[1], nz = n[2];
+  const double tx = s[0] * nx + s[3] * ny + s[4] * nz;
+  const double ty = s[3] * nx + s[1] * ny + s[5] * nz;
+  const double tz = s[4] * nx + s[5] * ny + s[2] * nz;
+  const double q = tx * nx + ty * ny + tz * nz;
+  const double o0 = s[0] - 4 * tx * nx + 4 * q * nx * nx;
+  const double o1 = s[1] - 4 * ty * ny + 4 * q * ny * ny;
+  const double o2 = s[2] - 4 * tz * nz + 4 * q * nz * nz;
+  const double o3 = s[3] - 2 * (tx * ny + nx * ty) + 4 * q * nx * ny;
+  const double o4 = s[4] - 2 * (tx * nz + nx * tz) + 4 * q * nx * nz;
+  const double o5 = s[5] - 2 * (ty * nz + ny * tz) + 4 * q * ny * nz;
+  s[0] = o0;
+  s[1] = o1;
+  s[2] = o2;
+  s[3] = o3;
+  s[4] = o4;
+  s[5] = o5;
 }
 
 // GhostCopier of one stage (:112-129): copy p, copy or reflect stress.
 // copy[g] = {copy_p, stress_mode (-1 none, 0 plain, 1 Householder)}.
+template <typename R, int NS>
+__device__ __forceinline__ void ghost_copy_stress(Pool<R> P, int64_t si, int64_t go, int mode,
+                                         const double *n) {
+  double s[NS];
+#pragma unroll
+  for (int q = 0; q < NS; ++q) s[q] = P.stress[q][si];
+  if (mode == GR_COPY_HOUSEHOLDER) {
+    if (NS == 3) householder3(s, n);
+    if (NS == 4) householder4(s, n);
+    if (NS == 6) householder6(s, n);
+  }
+#pragma unroll
+  for (int q = 0; q < NS; ++q) P.stress_s[q][go] = (R)s[q];
+}
+
 template <typename R>
 __global__ void k_ghost_copy(Pool<R> P, const SysDesc *sd, const Bnd *bnd, const int *n_dev,
-                             const Blk *blk, const int2 *copy, int D) {
+                             const Blk *blk, const int2 *copy) {
   const Blk b = blk[blockIdx.x];
   const int g = b.item;
   const int k = b.first + threadIdx.x;
@@ -479,16 +516,10 @@ __global__ void k_ghost_copy(Pool<R> P, const SysDesc *sd, const Bnd *bnd, const
   const int2 cp = copy[g];
   if (cp.x) P.p[go] = P.p[si];
   if (cp.y >= 0) {
-    const int NS = S.ns;
-    double s[6], o[6];
-    for (int q = 0; q < NS; ++q) s[q] = P.stress[q][si];
-    if (cp.y == GR_COPY_HOUSEHOLDER) {
-      const Bnd B = bnd[G.bnd + P.gnrm[go]];
-      householder(s, B.n, NS, o);
-    } else {
-      for (int q = 0; q < NS; ++q) o[q] = s[q];
-    }
-    for (int q = 0; q < NS; ++q) P.stress_s[q][go] = (R)o[q];
+    const Bnd B = bnd[G.bnd + P.gnrm[go]];
+    if (S.ns == 3) ghost_copy_stress<R, 3>(P, si, go, cp.y, B.n);
+    if (S.ns == 4) ghost_copy_stress<R, 4>(P, si, go, cp.y, B.n);
+    if (S.ns == 6) ghost_copy_stress<R, 6>(P, si, go, cp.y, B.n);
   }
 }
 
@@ -765,9 +796,8 @@ __device__ inline double eos_p(int kind, double rho, double rho0, double gamma, 
 // _halfstep_ps! (:158-162) writing the half-step state to the sweep shadows
 // (v and ρ keep their start-of-step values for the full step), the
 // FP64 half-step density, the stage-1 EOS when fused, and the source reset.
-template <typename R>
-__global__ void k_halfstep(Pool<R> P, const SysDesc *sd, const StepSys *st, const Blk *blk,
-                           int D) {
+template <typename R, int D>
+__global__ void k_halfstep(Pool<R> P, const SysDesc *sd, const StepSys *st, const Blk *blk) {
   const Blk b = blk[blockIdx.x];
   const int s = b.item;
   const int i = b.first + threadIdx.x;
@@ -775,6 +805,7 @@ __global__ void k_halfstep(Pool<R> P, const SysDesc *sd, const StepSys *st, cons
   if (i >= S.cap) return;
   const StepSys T = st[s];
   const int64_t g = S.off + i;
+#pragma unroll
   for (int d = 0; d < D; ++d) {
     double vh = P.v[d][g] + T.half * (double)P.dvdt[d][g];
     P.vs[d][g] = (R)vh;
@@ -784,6 +815,7 @@ __global__ void k_halfstep(Pool<R> P, const SysDesc *sd, const StepSys *st, cons
   P.rhos[g] = (R)rh;
   if (T.fuse_eos) P.p[g] = (R)eos_p(T.fuse_eos, rh, T.eos_rho0, T.eos_gamma, S.c);
   if (T.reset_force) {
+#pragma unroll
     for (int d = 0; d < D; ++d) P.dvdt[d][g] = (R)S.sv[d];
     P.drhodt[g] = (R)S.srho;
   }
@@ -792,9 +824,8 @@ __global__ void k_halfstep(Pool<R> P, const SysDesc *sd, const StepSys *st, cons
 // _apply_damping_all! (:251-271), _fullstep_q_all! (:227-232),
 // _update_positions_all! (:235-240).  v and x use separate products and sums
 // (SVector arithmetic is outside @fastmath); ρ uses the fused axpy.
-template <typename R>
-__global__ void k_fullstep(Pool<R> P, const SysDesc *sd, const StepSys *st, const Blk *blk,
-                           int D) {
+template <typename R, int D>
+__global__ void k_fullstep(Pool<R> P, const SysDesc *sd, const StepSys *st, const Blk *blk) {
   const Blk b = blk[blockIdx.x];
   const int s = b.item;
   const int i = b.first + threadIdx.x;
@@ -802,6 +833,7 @@ __global__ void k_fullstep(Pool<R> P, const SysDesc *sd, const StepSys *st, cons
   if (i >= S.cap) return;
   const StepSys T = st[s];
   const int64_t g = S.off + i;
+#pragma unroll
   for (int d = 0; d < D; ++d) {
     double a = (double)P.dvdt[d][g];
     if (T.damping) {
@@ -823,8 +855,8 @@ __global__ void k_fullstep(Pool<R> P, const SysDesc *sd, const StepSys *st, cons
 // ElastoPlasticStressUpdater (:216-335), FP64.  prm = {E, nu, a_phi, k_c,
 // a_psi} with the trigonometric constants evaluated on the host.
 template <typename R>
-__device__ void ep_update(Pool<R> &P, int64_t g, const double *prm, double dt) {
-  const double E = prm[0], nu = prm[1], a_phi = prm[2], k_c = prm[3], a_psi = prm[4];
+__device__ __forceinline__ void ep_update(Pool<R> P, int64_t g, double E, double nu, double a_phi,
+                                          double k_c, double a_psi, double dt) {
   double sxx = P.stress[0][g], syy = P.stress[1][g], szz = P.stress[2][g], sxy = P.stress[3][g];
   double dexx = (double)P.sr[0][g] * dt, deyy = (double)P.sr[1][g] * dt,
          dezz = (double)P.sr[2][g] * dt, dexy = (double)P.sr[3][g] * dt;
@@ -895,11 +927,14 @@ __device__ void ep_update(Pool<R> &P, int64_t g, const double *prm, double dt) {
 // ViscoPlasticMCStressUpdater (:155-189).  prm = {eos_kind, rho0, gamma,
 // tan(phi), cohesion}.
 template <typename R>
-__device__ void vpmc_update(Pool<R> &P, int64_t g, int NS, const double *prm, double c) {
+__device__ __forceinline__ void vpmc_update(Pool<R> P, int64_t g, int NS, double p0, double p1,
+                                            double p2, double p3, double p4, double c) {
+  const double prm[5] = {p0, p1, p2, p3, p4};
   const double pval = eos_p((int)prm[0], P.rho_h[g], prm[1], prm[2], c);
   P.p[g] = (R)pval;
-  double er[6];
-  for (int k = 0; k < NS; ++k) er[k] = (double)P.sr[k][g];
+  double er[GR_MAXNS];
+#pragma unroll
+  for (int k = 0; k < GR_MAXNS; ++k) er[k] = k < NS ? (double)P.sr[k][g] : 0.0;
   double mag;
   if (NS == 3)
     mag = er[0] * er[0] + er[1] * er[1] + 2 * (er[2] * er[2]);
@@ -911,8 +946,9 @@ __device__ void vpmc_update(Pool<R> &P, int64_t g, int NS, const double *prm, do
   mag = sqrt(mag);
   if (!(mag >= 2.220446049250313e-16)) mag = isnan(mag) ? mag : 2.220446049250313e-16;
   const double sc = 2.0 * (prm[4] + pval * prm[3]);
-  double sg[6];
-  for (int k = 0; k < NS; ++k) sg[k] = sc * er[k] / mag;
+  double sg[GR_MAXNS];
+#pragma unroll
+  for (int k = 0; k < GR_MAXNS; ++k) sg[k] = sc * er[k] / mag;
   bool zero;
   if (NS == 3) {
     sg[0] = sg[0] - pval;
@@ -923,14 +959,17 @@ __device__ void vpmc_update(Pool<R> &P, int64_t g, int NS, const double *prm, do
     sg[0] = sg[0] - pval;
     sg[1] = sg[1] - pval;
     sg[2] = sg[2] - pval;
-    for (int k = 3; k < NS; ++k) sg[k] = sg[k] - 0.0;
+#pragma unroll
+    for (int k = 3; k < GR_MAXNS; ++k) sg[k] = sg[k] - 0.0;
     zero = sg[0] + sg[1] + sg[2] > 0.0;
   }
-  for (int k = 0; k < NS; ++k) {
-    double v = zero ? 0.0 : sg[k];
-    P.stress[k][g] = v;
-    P.stress_s[k][g] = (R)v;
-  }
+#pragma unroll
+  for (int k = 0; k < GR_MAXNS; ++k)
+    if (k < NS) {
+      const double v = zero ? 0.0 : sg[k];
+      P.stress[k][g] = v;
+      P.stress_s[k][g] = (R)v;
+    }
 }
 
 template <typename R>
@@ -949,23 +988,24 @@ __global__ void k_updaters(Pool<R> P, const SysDesc *sd, const StepSys *st, cons
     switch (u.kind) {
       case GR_UPD_ZERO: {
         const long long m = u.mask;
-        if (m & (1 << GR_F_STRAIN_RATE))
-          for (int k = 0; k < S.ns; ++k) P.sr[k][g] = (R)0;
-        if (m & (1 << GR_F_VORTICITY))
-          for (int k = 0; k < S.nv; ++k) P.vor[k][g] = (R)0;
-        if (m & (1 << GR_F_DVDT))
-          for (int d = 0; d < D; ++d) P.dvdt[d][g] = (R)0;
-        if (m & (1 << GR_F_DRHODT)) P.drhodt[g] = (R)0;
-        if (m & (1 << GR_F_P)) P.p[g] = (R)0;
-        if (m & (1 << GR_F_STRESS))
-          for (int k = 0; k < S.ns; ++k) {
+#pragma unroll
+        for (int k = 0; k < GR_MAXNS; ++k) {
+          if (k >= S.ns) continue;
+          if (m & (1 << GR_F_STRAIN_RATE)) P.sr[k][g] = (R)0;
+          if (m & (1 << GR_F_STRESS)) {
             P.stress[k][g] = 0.0;
             P.stress_s[k][g] = (R)0;
           }
-        if (m & (1 << GR_F_STRAIN))
-          for (int k = 0; k < S.ns; ++k) P.strain[k][g] = 0.0;
-        if (m & (1 << GR_F_STRAIN_P))
-          for (int k = 0; k < S.ns; ++k) P.strainp[k][g] = 0.0;
+          if (m & (1 << GR_F_STRAIN)) P.strain[k][g] = 0.0;
+          if (m & (1 << GR_F_STRAIN_P)) P.strainp[k][g] = 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+          if ((m & (1 << GR_F_VORTICITY)) && k < S.nv) P.vor[k][g] = (R)0;
+          if ((m & (1 << GR_F_DVDT)) && k < D) P.dvdt[k][g] = (R)0;
+        }
+        if (m & (1 << GR_F_DRHODT)) P.drhodt[g] = (R)0;
+        if (m & (1 << GR_F_P)) P.p[g] = (R)0;
         break;
       }
       case GR_UPD_TAIT:
@@ -973,10 +1013,10 @@ __global__ void k_updaters(Pool<R> P, const SysDesc *sd, const StepSys *st, cons
         P.p[g] = (R)eos_p(u.kind, P.rho_h[g], u.prm[0], u.prm[1], S.c);
         break;
       case GR_UPD_EP:
-        ep_update(P, g, u.prm, dt);
+        ep_update(P, g, u.prm[0], u.prm[1], u.prm[2], u.prm[3], u.prm[4], dt);
         break;
       case GR_UPD_VPMC:
-        vpmc_update(P, g, S.ns, u.prm, S.c);
+        vpmc_update(P, g, S.ns, u.prm[0], u.prm[1], u.prm[2], u.prm[3], u.prm[4], S.c);
         break;
       default:
         break;
@@ -1050,7 +1090,11 @@ template <typename R>
 void launch_permute_real(hipStream_t s, const Pool<R> &A, const Pool<R> &B, const SysDesc *sd,
                          const Blk *blk, int nblk, const int *bbase, const int *vals,
                          const int *flags, int D) {
-  if (nblk > 0) k_permute_real<R><<<nblk, GR_BLOCK, 0, s>>>(A, B, sd, blk, bbase, vals, flags, D);
+  if (nblk <= 0) return;
+  if (D == 2)
+    k_permute_real<R, 2><<<nblk, GR_BLOCK, 0, s>>>(A, B, sd, blk, bbase, vals, flags);
+  else
+    k_permute_real<R, 3><<<nblk, GR_BLOCK, 0, s>>>(A, B, sd, blk, bbase, vals, flags);
 }
 template void launch_permute_real<float>(hipStream_t, const Pool<float> &, const Pool<float> &,
                                          const SysDesc *, const Blk *, int, const int *,
@@ -1088,7 +1132,11 @@ void launch_permute_ghost(hipStream_t s, const double *const *gxs, const int *gi
 template <typename R>
 void launch_ghost_kin(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const Bnd *bnd,
                       const int *n, const Blk *blk, int nblk, int D) {
-  if (nblk > 0) k_ghost_kin<R><<<nblk, GR_BLOCK, 0, s>>>(P, sd, bnd, n, blk, D);
+  if (nblk <= 0) return;
+  if (D == 2)
+    k_ghost_kin<R, 2><<<nblk, GR_BLOCK, 0, s>>>(P, sd, bnd, n, blk);
+  else
+    k_ghost_kin<R, 3><<<nblk, GR_BLOCK, 0, s>>>(P, sd, bnd, n, blk);
 }
 template void launch_ghost_kin<float>(hipStream_t, const Pool<float> &, const SysDesc *,
                                       const Bnd *, const int *, const Blk *, int, int);
@@ -1097,7 +1145,8 @@ template void launch_ghost_kin<double>(hipStream_t, const Pool<double> &, const 
 template <typename R>
 void launch_ghost_copy(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const Bnd *bnd,
                        const int *n, const Blk *blk, int nblk, const int2 *copy, int D) {
-  if (nblk > 0) k_ghost_copy<R><<<nblk, GR_BLOCK, 0, s>>>(P, sd, bnd, n, blk, copy, D);
+  (void)D;
+  if (nblk > 0) k_ghost_copy<R><<<nblk, GR_BLOCK, 0, s>>>(P, sd, bnd, n, blk, copy);
 }
 template void launch_ghost_copy<float>(hipStream_t, const Pool<float> &, const SysDesc *,
                                        const Bnd *, const int *, const Blk *, int, const int2 *,
@@ -1132,12 +1181,20 @@ void launch_cells_build(hipStream_t s, const double *const *x, int *cells, const
 template <typename R>
 void launch_halfstep(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
                      const Blk *blk, int nblk, int D) {
-  if (nblk > 0) k_halfstep<R><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, blk, D);
+  if (nblk <= 0) return;
+  if (D == 2)
+    k_halfstep<R, 2><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, blk);
+  else
+    k_halfstep<R, 3><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, blk);
 }
 template <typename R>
 void launch_fullstep(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
                      const Blk *blk, int nblk, int D) {
-  if (nblk > 0) k_fullstep<R><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, blk, D);
+  if (nblk <= 0) return;
+  if (D == 2)
+    k_fullstep<R, 2><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, blk);
+  else
+    k_fullstep<R, 3><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, blk);
 }
 template <typename R>
 void launch_updaters(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,

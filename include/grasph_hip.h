@@ -224,15 +224,21 @@ int gr_get_grid(gr_ctx *ctx, int inter, double *mingridx, int64_t *ngridx,
 int gr_get_cell_start(gr_ctx *ctx, int inter, int which, int64_t *out,
                       int64_t count);
 
-/* ---- timing ---------------------------------------------------------------
- * Per-phase device time of the last gr_step/gr_step_many call (ms), measured
- * with HIP events on the context stream.  phase: 0 sort, 1 ghosts, 2 grid,
- * 3 half-step, 4 updaters, 5 sweep, 6 full-step, 7 total. */
-int gr_set_timing(gr_ctx *ctx, int on);
+/* ---- timing (replaces the TimerOutputs sections of src/TimeIntegration.jl) --
+ * mode bit 0: record HIP events on the context stream at every phase of
+ * every step of the following gr_step/gr_step_many calls (no extra
+ * synchronisation inside a call); gr_get_timing returns the mean ms per step
+ * of phases 0 sort, 1 ghosts, 2 grid, 3 half-step, 4 updaters + ghost
+ * copies, 5 sweeps, 6 full step, 7 whole step.
+ * mode bit 1: count pairs with r < 2h in the sweeps; gr_get_pair_count
+ * returns unique pairs per step (self pairs once, coupled pairs once). */
+int gr_set_timing(gr_ctx *ctx, int mode);
 int gr_get_timing(gr_ctx *ctx, double *ms_out, int nphases);
-/* Unique pairs with r < 2h counted by the last sweep launches of one
- * integrator (requires timing mode; diagnostic). */
 int gr_get_pair_count(gr_ctx *ctx, int integ, int64_t *pairs_out);
+/* Device-to-device copy of `bytes` bytes by a plain streaming kernel, `reps`
+ * times (calibration of HBM traffic counters; no counterpart in the
+ * reference). */
+int gr_bench_copy(gr_ctx *ctx, int64_t bytes, int reps);
 
 #ifdef __cplusplus
 }
