@@ -42,14 +42,17 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=8,
                     help="timed oracle steps on the host (0 disables the CPU baseline)")
     ap.add_argument("--graph", type=int, default=1, help="replay the step as a hipGraph")
+    ap.add_argument("--calibrate-copy", type=int, default=0,
+                    help="also run the plain copy kernel on this many MiB (counter calibration)")
     return ap.parse_args()
 
 
 def sweep_bytes(cfg, grids):
-    """Algorithmic bytes of one sweep launch (FP32 state, FP64 positions):
-    fluid reads x (8D), v (4D), ρ (4), p (4) and writes dvdt (4D), drhodt (4);
-    each dynamic-boundary particle is read for x (8D); every cell array the
-    gather reads is read once (4 B per entry)."""
+    """Algorithmic bytes of the pair sweep of one step (FP32 state, FP64
+    positions): fluid reads x (8D), v (4D), ρ (4), p (4) and writes dvdt
+    (4D), drhodt (4); each dynamic-boundary particle is read for x (8D);
+    every cell array is read once (4 B per entry).  The neighbour search
+    (k_nlist) and the functor pass (k_sweep_list) together do this work."""
     D = 3
     nf = cfg["fluid_n"]
     nb = cfg["boundary_n"]
@@ -58,7 +61,8 @@ def sweep_bytes(cfg, grids):
 
 
 def load_traffic():
-    """HBM bytes per sweep launch from the committed rocprofv3 --pmc summary
+    """HBM bytes of the sweep per step (the k_nlist and k_sweep_list
+    dispatches of one step) from the committed rocprofv3 --pmc summary
     (FETCH_SIZE + WRITE_SIZE, KiB per dispatch, calibrated on a plain copy
     kernel of known size), or None if no summary is committed."""
     p = ROOT / "profiles" / "sweep_dambreak3d_pmc.json"
@@ -66,7 +70,7 @@ def load_traffic():
         return None
     try:
         d = json.loads(p.read_text())
-        return float(d["traffic_bytes_per_launch"])
+        return float(d["traffic_bytes_per_step"])
     except Exception:
         return None
 
@@ -132,6 +136,7 @@ def main():
     be.set_timing(2)
     be.step_many([h], 1)
     pairs = be.get_pair_count(h)
+    self_ordered, coupled = be.get_pair_stats()
     # timed region: HIP events at every phase, no extra synchronisation
     be.set_timing(1)
     t0 = time.perf_counter()
@@ -144,6 +149,8 @@ def main():
     be.step_many([h], args.steps)
     t3 = time.perf_counter()
     el_events, el_graph = t1 - t0, t3 - t2
+    if args.calibrate_copy:
+        be.bench_copy(args.calibrate_copy << 20, 3)
     el = min(el_events, el_graph) if args.graph else el_events
     ms_step = el / args.steps * 1e3
     grids = []
@@ -151,7 +158,7 @@ def main():
         _, _, nc = be.get_grid(sc.interaction(si), 3)
         grids.append(nc)
     bytes_sweep = sweep_bytes(cfg, grids)
-    sweep_ms = phases[5]
+    sweep_ms = phases[5] + phases[8]
     achieved = bytes_sweep / (sweep_ms * 1e-3) / 1e9
     traffic = load_traffic()
     cpu = cpu_baseline(g.configs.dambreak_3d, kw, args.cpu_steps, pairs)
@@ -178,13 +185,17 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic,
-                     "kernel": "k_sweep<3,float,FAM_FORCE> (fused gather: self + 5 dynamic walls)",
-                     "algorithmic_bytes_per_launch": bytes_sweep,
-                     "launch_ms": sweep_ms},
+                     "kernel": "pair sweep = k_nlist<3,float> + k_sweep_list<3,float,FORCE> "
+                               "(fused gather: self + 5 dynamic walls), one launch each per step",
+                     "algorithmic_bytes_per_step": bytes_sweep,
+                     "launch_ms": sweep_ms,
+                     "k_nlist_ms": phases[8], "k_sweep_list_ms": phases[5],
+                     "ordered_pairs": self_ordered + coupled},
         "cpu_baseline": cpu,
         "steps_per_s": 1e3 / ms_step,
         "phase_ms": {k: round(v, 4) for k, v in zip(
-            ("sort", "ghosts", "grid", "halfstep", "updaters", "sweep", "fullstep", "step"),
+            ("sort", "ghosts", "grid", "halfstep", "updaters", "sweep", "fullstep", "step",
+             "nlist"),
             phases)},
         "ms_per_step_events": el_events / args.steps * 1e3,
         "ms_per_step_graph": el_graph / args.steps * 1e3,

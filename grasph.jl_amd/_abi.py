@@ -91,6 +91,7 @@ _GPU_SIGS = {
     "set_timing": [_i],
     "get_timing": [_dp, _i],
     "get_pair_count": [_i, _i64p],
+    "get_pair_stats": [_i64p, _i64p],
     "ctx_info": [C.c_char_p, _i64],
     "bench_copy": [_i64, _i],
 }
@@ -290,9 +291,14 @@ class CBackend:
         self.call("bench_copy", int(nbytes), int(reps))
 
     def get_timing(self):
-        a = np.zeros(8)
-        self.call("get_timing", _dptr(a), 8)
+        a = np.zeros(9)
+        self.call("get_timing", _dptr(a), 9)
         return a
+
+    def get_pair_stats(self):
+        a, b = C.c_int64(), C.c_int64()
+        self.call("get_pair_stats", C.byref(a), C.byref(b))
+        return a.value, b.value
 
     def get_pair_count(self, integ):
         out = C.c_int64()

@@ -132,6 +132,9 @@ struct Plan {
   // stages
   std::vector<std::unique_ptr<StagePlan>> stages;
   std::vector<std::pair<int, StepSys>> stepsys;  // per-system step parameters
+  // neighbour lists
+  DBuf ltasks, lsrcs, blk_list;
+  int nblk_list = 0;
   bool do_sort = true, do_ghosts = true, do_grid = true, do_step = true;
   // graph capture per A/B parity
   hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -171,6 +174,10 @@ struct gr_ctx {
   DBuf keys, vals, keys2, vals2, hist, cand, cand_scan, scan_bsum, bagg, carry;
   int64_t sort_cap = 0, cand_cap = 0, cellblk_cap = 0;
   DBuf d_pairs;
+  DBuf lidx, lcnt;
+  int64_t lidx_cap = 0, lcnt_cap = 0;
+  int use_lists = 1;
+  int list_maxnb = 0;  // partners kept per target (0: 192 in 3D, 96 in 2D)
   std::map<std::vector<int>, std::unique_ptr<Plan>> plans;
   // cached device tables derived from the layout
   DBuf allreal_blk;
@@ -183,7 +190,7 @@ struct gr_ctx {
   bool graph_mode = false, timing = false, count_pairs = false;
   hipEvent_t ev[9] = {};
   std::vector<std::vector<hipEvent_t>> tev;  // per-step phase events (timing mode)
-  double phase_ms[8] = {0};
+  double phase_ms[9] = {0};
   long long last_pairs[2] = {0, 0};
   int last_pairs_integ_key = -1;
 };
@@ -604,6 +611,8 @@ SrcDesc make_src(gr_ctx *c, int it, int target, bool b_side, int pk) {
   const int partner = b_side ? a : (in.b < 0 ? a : in.b);
   q.partner = partner;
   q.carr = b_side ? in.carr_a : in.carr_b;
+  q.side = b_side ? 1 : 0;
+  q.lsrc = 0;
   q.kk = in.kk;
   q.h = in.h;
   q.norm = in.norm;
@@ -977,6 +986,64 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
                 L.end());
       }
     }
+  // neighbour lists: one per target, over the union of its sources
+  std::map<int, std::vector<SrcDesc>> tl;  // target → list sources
+  for (int k = 0; k < nst; ++k)
+    for (int fam = 0; fam < 3; ++fam)
+      for (auto &kv : tasks[k * 3 + fam]) {
+        auto &L = tl[kv.first];
+        for (auto &q : kv.second.src) {
+          int idx = -1;
+          for (int l = 0; l < (int)L.size(); ++l)
+            if (L[l].grid == q.grid && L[l].side == q.side) idx = l;
+          if (idx < 0) {
+            idx = (int)L.size();
+            L.push_back(q);
+          }
+          q.lsrc = idx;
+        }
+      }
+  std::vector<LTask> ltv;
+  std::vector<SrcDesc> lsv;
+  std::vector<Blk> lbv;
+  std::map<int, int> ltask_of;
+  int64_t lacc = 0, cacc = 0;
+  const int maxnb = c->list_maxnb > 0 ? c->list_maxnb : (c->D == 3 ? 192 : 96);
+  for (auto &kv : tl) {
+    LTask L;
+    memset(&L, 0, sizeof(L));
+    L.sys = kv.first;
+    L.cap = (int32_t)c->sys[kv.first].cap;
+    L.maxnb = maxnb;
+    L.src_begin = (int)lsv.size();
+    for (auto &q : kv.second) lsv.push_back(q);
+    L.src_end = (int)lsv.size();
+    L.list_off = lacc;
+    L.cnt_off = cacc;
+    lacc += (int64_t)L.cap * maxnb;
+    cacc += (int64_t)L.cap * (L.src_end - L.src_begin + 1);
+    add_blocks(lbv, (int)ltv.size(), L.cap, GR_BLOCK);
+    ltask_of[kv.first] = (int)ltv.size();
+    ltv.push_back(L);
+  }
+  if (c->use_lists) {
+    P.ltasks.upload(ltv, c->stream);
+    P.lsrcs.upload(lsv, c->stream);
+    P.blk_list.upload(lbv, c->stream);
+    P.nblk_list = (int)lbv.size();
+    if (lacc > c->lidx_cap || cacc > c->lcnt_cap) {
+      c->lidx.alloc(std::max<int64_t>(lacc, 1) * 4);
+      c->lcnt.alloc(std::max<int64_t>(cacc, 1) * 4);
+      c->lidx_cap = lacc;
+      c->lcnt_cap = cacc;
+      for (auto &pk : c->plans)
+        for (auto &gg : pk.second->graph)
+          if (gg) {
+            (void)hipGraphExecDestroy(gg);
+            gg = nullptr;
+          }
+    }
+  }
   // materialise stage plans
   for (int k = 0; k < nst; ++k) {
     auto sp = std::make_unique<StagePlan>();
@@ -1012,6 +1079,7 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
       std::vector<Blk> bv;
       for (auto &kv : tasks[k * 3 + fam]) {
         Task t = kv.second.t;
+        t.ltask = c->use_lists ? ltask_of[kv.first] : -1;
         t.src_begin = (int)sv.size();
         for (auto &q : kv.second.src) sv.push_back(q);
         t.src_end = (int)sv.size();
@@ -1217,6 +1285,11 @@ void phase_stage(gr_ctx *c, Plan &P, int k, bool cnt, hipEvent_t *mid) {
     a.blk = F.blk.as<Blk>();
     a.nblk = F.nblk;
     a.pair_count = c->d_pairs.as<unsigned long long>();
+    if (P.nblk_list > 0) {
+      a.lt = P.ltasks.as<LTask>();
+      a.lcnt = c->lcnt.as<int>();
+      a.lidx = c->lidx.as<int>();
+    }
     launch_sweep<R>(s, A, a, c->D, fam, cnt);
   }
 }
@@ -1227,7 +1300,7 @@ void phase_stage(gr_ctx *c, Plan &P, int k, bool cnt, hipEvent_t *mid) {
 std::vector<hipEvent_t> &step_events(gr_ctx *c, int k, int nst) {
   if ((int)c->tev.size() <= k) c->tev.resize(k + 1);
   auto &v = c->tev[k];
-  const size_t need = 6 + 2 * nst;
+  const size_t need = 7 + 2 * nst;
   while (v.size() < need) {
     hipEvent_t e;
     HIPCHK(hipEventCreate(&e));
@@ -1250,27 +1323,43 @@ void run_step(gr_ctx *c, Plan &P, bool cnt, std::vector<hipEvent_t> *ev) {
   phase_grids(c, P);
   rec(3);
   Pool<R> A = pool_view<R>(c, c->cur);
+  if (P.nblk_list > 0) {
+    NlistArgs a;
+    a.sd = c->d_sd.as<SysDesc>();
+    a.n = c->d_n.as<int>();
+    a.lt = P.ltasks.as<LTask>();
+    a.srcs = P.lsrcs.as<SrcDesc>();
+    a.grids = c->d_grids.as<Grid>();
+    a.carr = c->d_carr.as<CellArr>();
+    a.cells = c->cells.as<int>();
+    a.blk = P.blk_list.as<Blk>();
+    a.nblk = P.nblk_list;
+    a.lcnt = c->lcnt.as<int>();
+    a.lidx = c->lidx.as<int>();
+    launch_nlist<R>(s, A, a, c->D);
+  }
+  rec(4);
   launch_halfstep<R>(s, A, c->d_sd.as<SysDesc>(), c->d_st.as<StepSys>(), P.blk_real.as<Blk>(),
                      P.nblk_real, c->D);
   launch_ghost_kin<R>(s, A, c->d_sd.as<SysDesc>(), c->d_bnd.as<Bnd>(), c->d_n.as<int>(),
                       P.blk_ghost.as<Blk>(), P.nblk_ghost, c->D);
-  rec(4);
+  rec(5);
   const int nst = (int)P.stages.size();
   for (int k = 0; k < nst; ++k) {
-    phase_stage<R>(c, P, k, cnt, ev ? &(*ev)[5 + 2 * k] : nullptr);
-    rec(6 + 2 * k);
+    phase_stage<R>(c, P, k, cnt, ev ? &(*ev)[6 + 2 * k] : nullptr);
+    rec(7 + 2 * k);
   }
   launch_fullstep<R>(s, A, c->d_sd.as<SysDesc>(), c->d_st.as<StepSys>(), P.blk_real.as<Blk>(),
                      P.nblk_real, c->D);
-  rec(5 + 2 * nst);
+  rec(6 + 2 * nst);
 }
 
 // Sum the phase times of `nsteps` recorded steps (ms per step).
 void collect_timing(gr_ctx *c, int nsteps, int nst) {
-  double acc[8] = {0};
+  double acc[9] = {0};
   for (int k = 0; k < nsteps; ++k) {
     auto &v = c->tev[k];
-    HIPCHK(hipEventSynchronize(v[5 + 2 * nst]));
+    HIPCHK(hipEventSynchronize(v[6 + 2 * nst]));
     auto el = [&](size_t a, size_t b) {
       float ms = 0.0f;
       HIPCHK(hipEventElapsedTime(&ms, v[a], v[b]));
@@ -1279,17 +1368,18 @@ void collect_timing(gr_ctx *c, int nsteps, int nst) {
     acc[0] += el(0, 1);
     acc[1] += el(1, 2);
     acc[2] += el(2, 3);
-    acc[3] += el(3, 4);
-    size_t prev = 4;
+    acc[8] += el(3, 4);
+    acc[3] += el(4, 5);
+    size_t prev = 5;
     for (int q = 0; q < nst; ++q) {
-      acc[4] += el(prev, 5 + 2 * q);
-      acc[5] += el(5 + 2 * q, 6 + 2 * q);
-      prev = 6 + 2 * q;
+      acc[4] += el(prev, 6 + 2 * q);
+      acc[5] += el(6 + 2 * q, 7 + 2 * q);
+      prev = 7 + 2 * q;
     }
-    acc[6] += el(prev, 5 + 2 * nst);
-    acc[7] += el(0, 5 + 2 * nst);
+    acc[6] += el(prev, 6 + 2 * nst);
+    acc[7] += el(0, 6 + 2 * nst);
   }
-  for (int q = 0; q < 8; ++q) c->phase_ms[q] = acc[q] / std::max(nsteps, 1);
+  for (int q = 0; q < 9; ++q) c->phase_ms[q] = acc[q] / std::max(nsteps, 1);
 }
 
 // Graph-captured step: two executables, one per A/B parity of the pool.
@@ -1448,6 +1538,10 @@ int gr_ctx_create(gr_ctx **out, int device, int ndims, int precision) {
     return GR_ERR_HIP;
   }
   for (auto &e : c->ev) (void)hipEventCreate(&e);
+  // GRASPH_NO_NLIST=1: walk the cells in every sweep; GRASPH_NLIST_MAXNB=k:
+  // keep at most k partners per target (targets with more walk the cells).
+  if (const char *e = getenv("GRASPH_NO_NLIST")) c->use_lists = atoi(e) ? 0 : 1;
+  if (const char *e = getenv("GRASPH_NLIST_MAXNB")) c->list_maxnb = std::max(0, atoi(e));
   *out = c;
   return GR_OK;
 }
@@ -1931,13 +2025,19 @@ int gr_set_timing(gr_ctx *c, int on) {
 }
 int gr_get_timing(gr_ctx *c, double *ms_out, int nphases) {
   return guard(c, [&] {
-    for (int k = 0; k < nphases && k < 8; ++k) ms_out[k] = c->phase_ms[k];
+    for (int k = 0; k < nphases && k < 9; ++k) ms_out[k] = c->phase_ms[k];
   });
 }
 int gr_get_pair_count(gr_ctx *c, int integ, int64_t *pairs_out) {
   return guard(c, [&] {
     (void)integ;
     *pairs_out = c->last_pairs[0] / 2 + c->last_pairs[1];  // per step
+  });
+}
+int gr_get_pair_stats(gr_ctx *c, int64_t *self_ordered, int64_t *coupled) {
+  return guard(c, [&] {
+    *self_ordered = c->last_pairs[0];
+    *coupled = c->last_pairs[1];
   });
 }
 
@@ -2042,6 +2142,7 @@ int gr_sweep(gr_ctx *c, int inter, int stage) {
       t.has_stress = f.kind == GR_PFN_CAUCHY;
       t.write_vor = f.kind == GR_PFN_SRV;
       t.assign = 0;
+      t.ltask = -1;
       SrcDesc q = make_src(c, inter, target, b_side, pk);
       set_pfn(q, f, pk, c, in);
       t.src_begin = (int)sv.size();

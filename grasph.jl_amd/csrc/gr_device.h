@@ -82,7 +82,10 @@ struct SrcDesc {
   int32_t pk, pfn, partner, grid;
   int32_t carr;      // CellArr index of the partner particles
   int32_t kk, sigma, has_delta;
-  int32_t has_eps, pad;
+  int32_t has_eps;
+  int32_t side;      // 0: gathers for system_a, 1: for system_b (two-sided)
+  int32_t lsrc;      // index of this source in the target's neighbour list
+  int32_t pad;
   double h, norm, cutoff2, cell;  // kernel
   double alpha, beta, hf, delta, eps;
   double mass_q;                  // partner mass used by the functor
@@ -98,6 +101,20 @@ struct Task {
   int32_t ns, nv;
   int32_t has_stress;
   int32_t write_vor;  // an SRV source writes vorticity
+  int32_t ltask;      // neighbour-list task of the target (-1: walk cells)
+  int32_t pad;
+};
+
+// Neighbour list of one target system: sources [src_begin, src_end) of the
+// list source table; partners at lidx[list_off + k·cap + i], counts at
+// lcnt[cnt_off + q·cap + i], overflow flag at lcnt[cnt_off + nsrc·cap + i].
+struct LTask {
+  int32_t sys;
+  int32_t src_begin, src_end;
+  int32_t cap;
+  int32_t maxnb;
+  int32_t pad;
+  int64_t list_off, cnt_off;
 };
 
 // One state updater of one (system, stage) (src/StateUpdaters.jl).

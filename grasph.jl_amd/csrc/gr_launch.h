@@ -17,6 +17,23 @@ struct SweepArgs {
   const Blk *blk;
   int nblk;
   unsigned long long *pair_count;
+  const LTask *lt = nullptr;  // non-null: use the neighbour lists
+  const int *lcnt = nullptr;
+  const int *lidx = nullptr;
+};
+
+struct NlistArgs {
+  const SysDesc *sd;
+  const int *n;
+  const LTask *lt;
+  const SrcDesc *srcs;
+  const Grid *grids;
+  const CellArr *carr;
+  const int *cells;
+  const Blk *blk;
+  int nblk;
+  int *lcnt;
+  int *lidx;
 };
 
 void launch_key_minmax(hipStream_t s, const double *const *xs, const SysDesc *sd, const int *n,
@@ -75,6 +92,8 @@ void launch_updaters(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const S
                      const Blk *blk, int nblk, const int2 *urange, const UpdDesc *ud, int D);
 template <typename R>
 void launch_sweep(hipStream_t s, const Pool<R> &P, const SweepArgs &a, int D, int fam, bool cnt);
+template <typename R>
+void launch_nlist(hipStream_t s, const Pool<R> &P, const NlistArgs &a, int D);
 void launch_fill_i32(hipStream_t s, int *p, int v, long long n);
 
 }  // namespace gr

@@ -1,4 +1,4 @@
-// gr_sweep.hip — fused gather sweep of the pair functors (gfx950).
+// gr_sweep.hip — neighbour search and fused pair-functor gather (gfx950).
 //
 // The reference visits each unordered pair once (half-shell colour sweep,
 // src/Interaction.jl:393-541) and writes to both particles; the coupled sweep
@@ -10,8 +10,16 @@
 // accumulates in registers (Kahan-compensated in FP32) and writes once.  No
 // atomics; the result is independent of scheduling.
 //
-// Pair selection is exact: dx and r² are formed in FP64 from FP64 positions,
-// in the reference's order, so the accepted pair set equals the reference's.
+// Two passes per step:
+//  * k_nlist: neighbour search.  A wavefront walks the union of its lanes'
+//    candidate ranges with a wave-uniform index (candidate positions come
+//    through scalar loads, one per wavefront) and each lane keeps the
+//    candidates in its own CSR ranges with r² < (2h)², evaluated in FP64 in
+//    the reference's order, so the accepted pair set equals the reference's.
+//  * k_sweep_list: the functors over each lane's accepted partners, once per
+//    stage, so every lane does useful arithmetic on every iteration.
+// k_sweep walks the cells directly (fine-grained sweep! calls, and targets
+// whose list overflowed).
 #include <hip/hip_runtime.h>
 
 #include "../../include/grasph_hip.h"
@@ -83,282 +91,544 @@ __device__ inline R jmin0(R v) {  // Julia min(v, 0)
   return v < (R)0 ? v : (isnan(v) ? v : (R)0);
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous run of logical blocks so neighbouring cells share an L2.
+__device__ inline int xcd_block(int nblk) {
+  const int b = blockIdx.x;
+  const int full = (nblk / 8) * 8;
+  if (b >= full) return b;
+  return (b % 8) * (nblk / 8) + b / 8;
+}
+
 // Per-source constants in the arithmetic type.
 template <typename R>
-struct SrcR {
+struct SrcK {
   R h, norm, hf, alpha, beta, mc, h2, m_q, psi_coef, eps, ljcut, ljc2;
   int sigma, has_delta, has_eps;
+  __device__ inline void load(const SrcDesc &Q) {
+    h = (R)Q.h;
+    norm = (R)Q.norm;
+    hf = (R)Q.hf;
+    alpha = (R)Q.alpha;
+    beta = (R)Q.beta;
+    mc = (R)(0.5 * (Q.c_t_piv + Q.c_q_piv));
+    h2 = (R)(Q.hf * Q.hf * 0.01);
+    m_q = (R)Q.mass_q;
+    psi_coef = (R)(2.0 * Q.delta * (0.5 * (Q.c_t_psi + Q.c_q_psi)) * Q.hf);
+    eps = (R)Q.eps;
+    ljcut = (R)Q.ljcut;
+    ljc2 = (R)(0.01 * Q.c_t_piv * Q.c_t_piv);
+    sigma = Q.sigma;
+    has_delta = Q.has_delta;
+    has_eps = Q.has_eps;
+  }
 };
+
+// Target particle: its state and its sums.
+template <int D, typename R, int FAM>
+struct Gather {
+  double xi[D];
+  R vi[D], ri, pi;
+  R si[GR_MAXNS];
+  int NS;
+  Acc<R> adv[D], adr;
+  Acc<R> asr[GR_MAXNS], avo[3];
+
+  __device__ inline void load(const Pool<R> P, const SysDesc &S, const Task &T, int64_t gi) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      xi[d] = P.x[d][gi];
+      vi[d] = P.vs[d][gi];
+    }
+    ri = P.rhos[gi];
+    pi = 0;
+    NS = T.ns;
+    if (FAM == FAM_FORCE) {
+      if (S.kind >= GR_SYS_FLUID) pi = P.p[gi];
+      if (T.has_stress) {
+#pragma unroll
+        for (int k = 0; k < GR_MAXNS; ++k) si[k] = k < NS ? P.stress_s[k][gi] : (R)0;
+      }
+    }
+  }
+
+  __device__ inline double da_of(const SrcDesc &Q) const {
+    double da = (xi[0] - Q.wp[0]) * Q.wn[0];
+#pragma unroll
+    for (int d = 1; d < D; ++d) da = da + (xi[d] - Q.wp[d]) * Q.wn[d];
+    return da;
+  }
+
+  // One accepted pair (target i, partner j at pool index gj), dx = x_i − x_j
+  // and r² as evaluated by _pair_self!/_pair_coupled! (src/Interaction.jl:546-553).
+  __device__ __forceinline__ void pair(const Pool<R> P, const SrcDesc &Q, const SrcK<R> &K,
+                                       int64_t gj, const double *dx, double r2, double da) {
+    R dxr[D], gx[D];
+    const R r2r = (R)r2;
+    const R r = sqrt(r2r);
+    const R qq = r / K.h;
+    const R g = dwdq<R>(Q.kk, K.norm, qq) / (r * K.h);
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      dxr[d] = (R)dx[d];
+      gx[d] = g * dxr[d];
+    }
+    if (FAM == FAM_COUNT) {
+      adv[0].add((R)1);
+      adr.add((R)1);
+      return;
+    }
+    R vj[D], rj;
+    const R m = K.m_q;
+    if (Q.pk == PK_DYNAMIC) {  // DynamicBoundarySystem: mirrored velocity
+      double db = (P.x[0][gj] - Q.wp[0]) * Q.wn[0];
+#pragma unroll
+      for (int d = 1; d < D; ++d) db = db + (P.x[d][gj] - Q.wp[d]) * Q.wn[d];
+      const double rat = fabs(db / da);
+      const double mn = isnan(rat) ? rat : (rat < Q.wbeta ? rat : Q.wbeta);
+      const R fac = (R)(-mn);
+#pragma unroll
+      for (int d = 0; d < D; ++d) vj[d] = fac * vi[d];
+      rj = ri;
+    } else {
+#pragma unroll
+      for (int d = 0; d < D; ++d) vj[d] = P.vs[d][gj];
+      rj = P.rhos[gj];
+    }
+    if (FAM == FAM_STRAIN) {  // strain_rate_tensor / vorticity_tensor
+      R dv[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) dv[d] = vj[d] - vi[d];
+      const R f = m / rj;
+      if (D == 2) {
+        asr[0].add(dv[0] * gx[0] * f);
+        asr[1].add(dv[1] * gx[1] * f);
+        const R sh = (R)0.5 * (dv[0] * gx[1] + dv[1] * gx[0]) * f;
+        if (NS == 4)
+          asr[3].add(sh);
+        else
+          asr[2].add(sh);
+      } else {
+        asr[0].add(dv[0] * gx[0] * f);
+        asr[1].add(dv[1] * gx[1] * f);
+        asr[2].add(dv[D - 1] * gx[D - 1] * f);
+        asr[3].add((R)0.5 * (dv[0] * gx[1] + dv[1] * gx[0]) * f);
+        asr[4].add((R)0.5 * (dv[0] * gx[D - 1] + dv[D - 1] * gx[0]) * f);
+        asr[5].add((R)0.5 * (dv[1] * gx[D - 1] + dv[D - 1] * gx[1]) * f);
+      }
+      if (Q.pfn == GR_PFN_SRV) {
+        avo[0].add((R)0.5 * (dv[0] * gx[1] - dv[1] * gx[0]) * f);
+        if (D == 3) {
+          avo[1].add((R)0.5 * (dv[0] * gx[D - 1] - dv[D - 1] * gx[0]) * f);
+          avo[2].add((R)0.5 * (dv[1] * gx[D - 1] - dv[D - 1] * gx[1]) * f);
+        }
+      }
+      return;
+    }
+    // FAM_FORCE: FluidPfn / CauchyFluidPfn
+    R dv[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) dv[d] = vi[d] - vj[d];
+    R vr = dxr[0] * dv[0];
+#pragma unroll
+    for (int d = 1; d < D; ++d) vr = vr + dxr[d] * dv[d];
+    vr = jmin0(vr);
+    const R muv = K.hf * vr / (r2r + K.h2);
+    const R piv = (K.beta * muv - K.alpha * K.mc) * muv / ((R)0.5 * ri * rj);
+    if (Q.pk == PK_STATIC) {  // lennard_jones + artificial viscosity
+      const R lr = K.ljcut / r;
+      const R f = (r < K.ljcut ? (R)1 : (R)0) * (pow_i(lr, 12) - pow_i(lr, 6)) / r2r;
+      const R rf = K.ljc2 * f;
+      const R t = -m * piv;
+#pragma unroll
+      for (int d = 0; d < D; ++d) adv[d].add(t * gx[d] + rf * dxr[d]);
+      return;
+    }
+    R dr = dv[0] * gx[0];
+#pragma unroll
+    for (int d = 1; d < D; ++d) dr = dr + dv[d] * gx[d];
+    R psi = (R)0;
+    if (K.has_delta) {
+      R dxg = dxr[0] * gx[0];
+#pragma unroll
+      for (int d = 1; d < D; ++d) dxg = dxg + dxr[d] * gx[d];
+      psi = K.psi_coef * (ri - rj) * dxg / r2r;
+    }
+    if (Q.pfn == GR_PFN_FLUID) {
+      const R pj = (Q.pk == PK_DYNAMIC) ? pi : P.p[gj];
+      R dh;
+      if (K.sigma == 1)
+        dh = -(pi + pj) / (ri * rj);
+      else
+        dh = -(pi / (ri * ri) + pj / (rj * rj));
+      if (Q.pk == PK_TWO_SIDED) {
+        R ast = (R)0;
+        if (K.has_eps) ast = K.eps * (-(fabs(pi) + fabs(pj)) / (ri * rj));
+        const R t = dh + ast - piv;
+#pragma unroll
+        for (int d = 0; d < D; ++d) adv[d].add(m * (t * gx[d]));
+      } else {
+        const R t = m * (dh - piv);
+#pragma unroll
+        for (int d = 0; d < D; ++d) adv[d].add(t * gx[d]);
+      }
+      const R coef = (K.sigma == 1) ? ri / rj : (R)1;
+      adr.add(m * (dr * coef + psi / rj));
+    } else {  // cauchy_stress_force
+      R sj[GR_MAXNS];
+      if (Q.pk == PK_DYNAMIC) {
+#pragma unroll
+        for (int k = 0; k < GR_MAXNS; ++k) sj[k] = si[k];
+      } else {
+#pragma unroll
+        for (int k = 0; k < GR_MAXNS; ++k) sj[k] = k < NS ? P.stress_s[k][gj] : (R)0;
+      }
+      const R ci = (R)1 / (ri * ri), cj = (R)1 / (rj * rj);
+      R hv[D];
+      if (D == 2) {
+        R a0, a1;
+        if (NS == 3) {
+          a0 = (si[0] * gx[0] + si[2] * gx[1]) * ci + (sj[0] * gx[0] + sj[2] * gx[1]) * cj;
+          a1 = (si[2] * gx[0] + si[1] * gx[1]) * ci + (sj[2] * gx[0] + sj[1] * gx[1]) * cj;
+        } else {
+          a0 = (si[0] * gx[0] + si[3] * gx[1]) * ci + (sj[0] * gx[0] + sj[3] * gx[1]) * cj;
+          a1 = (si[3] * gx[0] + si[1] * gx[1]) * ci + (sj[3] * gx[0] + sj[1] * gx[1]) * cj;
+        }
+        hv[0] = a0;
+        hv[D - 1] = a1;
+      } else {
+        hv[0] = (si[0] * gx[0] + si[3] * gx[1] + si[4] * gx[D - 1]) * ci +
+                (sj[0] * gx[0] + sj[3] * gx[1] + sj[4] * gx[D - 1]) * cj;
+        hv[1] = (si[3] * gx[0] + si[1] * gx[1] + si[5] * gx[D - 1]) * ci +
+                (sj[3] * gx[0] + sj[1] * gx[1] + sj[5] * gx[D - 1]) * cj;
+        hv[D - 1] = (si[4] * gx[0] + si[5] * gx[1] + si[2] * gx[D - 1]) * ci +
+                    (sj[4] * gx[0] + sj[5] * gx[1] + sj[2] * gx[D - 1]) * cj;
+      }
+#pragma unroll
+      for (int d = 0; d < D; ++d) adv[d].add(m * (hv[d] - piv * gx[d]));
+      adr.add(m * (dr + psi / rj));
+    }
+  }
+
+  __device__ inline void store(const Pool<R> P, const SysDesc &S, const Task &T, int64_t gi) {
+    if (FAM == FAM_STRAIN) {
+#pragma unroll
+      for (int k = 0; k < GR_MAXNS; ++k)
+        if (k < NS) {
+          const R v = asr[k].get();
+          P.sr[k][gi] = T.assign ? v : P.sr[k][gi] + v;
+        }
+      if (T.write_vor) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          if (k < T.nv) {
+            const R v = avo[k].get();
+            P.vor[k][gi] = T.assign ? v : P.vor[k][gi] + v;
+          }
+      }
+      return;
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const R v = adv[d].get();
+      P.dvdt[d][gi] = T.assign ? (R)S.sv[d] + v : P.dvdt[d][gi] + v;
+    }
+    const R v = adr.get();
+    P.drhodt[gi] = T.assign ? (R)S.srho + v : P.drhodt[gi] + v;
+  }
+};
+
+// Target cell in the grid of a source, clamped so every strip lies inside.
+template <int D>
+__device__ inline void target_cell(const double *xi, const Grid &G, double cell, int *c) {
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    int cd = (int)floor((xi[d] - G.ming[d]) / cell);
+    cd = cd < 1 ? 1 : cd;
+    cd = cd > G.ng[d] - 2 ? G.ng[d] - 2 : cd;
+    c[d] = cd;
+  }
+}
+
+// First cell of strip st (3 consecutive cells along the fastest dimension).
+template <int D>
+__device__ inline int strip_base(const int *c, const Grid &G, int st) {
+  if (D == 3) {
+    const int ddx = st / 3 - 1, ddy = st % 3 - 1;
+    return ((c[0] + ddx) * G.ng[1] + (c[1] + ddy)) * G.ng[2] + (c[D - 1] - 1);
+  }
+  return (c[0] + st - 1) * G.ng[1] + (c[D - 1] - 1);
+}
+
+// Direct cell walk of every source of the task (no list).
+template <int D, typename R, int FAM>
+__device__ void gather_direct(Gather<D, R, FAM> &Gt, const Pool<R> P, const SysDesc *sd,
+                              const Task &T, const SrcDesc *srcs, const Grid *grids,
+                              const CellArr *carr, const int *cells, int i,
+                              unsigned long long &ns, unsigned long long &no) {
+  for (int q = T.src_begin; q < T.src_end; ++q) {
+    const SrcDesc Q = srcs[q];
+    const Grid G = grids[Q.grid];
+    if (G.ncells <= 0) continue;
+    const int *cs = cells + carr[Q.carr].off;
+    const int64_t poff = sd[Q.partner].off;
+    SrcK<R> K;
+    K.load(Q);
+    int c[D];
+    target_cell<D>(Gt.xi, G, Q.cell, c);
+    const double da = (Q.pk == PK_DYNAMIC) ? Gt.da_of(Q) : 0.0;
+    for (int st = 0; st < ((D == 3) ? 9 : 3); ++st) {
+      const int base = strip_base<D>(c, G, st);
+      const int j0 = cs[base] - 1, j1 = cs[base + 3] - 1;
+      for (int j = j0; j < j1; ++j) {
+        if (Q.pk == PK_SELF && j == i) continue;
+        const int64_t gj = poff + j;
+        double dx[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) dx[d] = Gt.xi[d] - P.x[d][gj];
+        double r2 = dx[0] * dx[0];
+#pragma unroll
+        for (int d = 1; d < D; ++d) r2 = r2 + dx[d] * dx[d];
+        if (!(r2 < Q.cutoff2)) continue;
+        if (Q.pk == PK_SELF)
+          ++ns;
+        else
+          ++no;
+        Gt.pair(P, Q, K, gj, dx, r2, da);
+      }
+    }
+  }
+}
+
+template <bool CNT>
+__device__ inline void count_pairs(unsigned long long *pc, unsigned long long a,
+                                   unsigned long long o2) {
+  if (!CNT) return;
+  // [0]: ordered self pairs (each unordered pair seen from both sides),
+  // [1]: coupled pairs seen from the gathering side.
+  for (int o = 32; o > 0; o >>= 1) {
+    a += __shfl_xor(a, o, 64);
+    o2 += __shfl_xor(o2, o, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    if (a) atomicAdd(&pc[0], a);
+    if (o2) atomicAdd(&pc[1], o2);
+  }
+}
 
 template <int D, typename R, int FAM, bool CNT>
 __global__ void __launch_bounds__(GR_BLOCK)
     k_sweep(Pool<R> P, const SysDesc *__restrict__ sd, const int *__restrict__ n_dev,
             const Task *__restrict__ tasks, const SrcDesc *__restrict__ srcs,
             const Grid *__restrict__ grids, const CellArr *__restrict__ carr,
-            const int *__restrict__ cells, const Blk *__restrict__ blk,
+            const int *__restrict__ cells, const Blk *__restrict__ blk, int nblk,
             unsigned long long *pair_count) {
-  const Blk b = blk[blockIdx.x];
+  const Blk b = blk[xcd_block(nblk)];
   const Task T = tasks[b.item];
   const int i = b.first + threadIdx.x;
   const SysDesc S = sd[T.sys];
   const bool live = i < n_dev[T.sys];
   const int64_t gi = S.off + (live ? i : 0);
-  double xi[D];
-  R vi[D];
-  for (int d = 0; d < D; ++d) {
-    xi[d] = P.x[d][gi];
-    vi[d] = P.vs[d][gi];
-  }
-  const R ri = P.rhos[gi];
-  R pi = 0;
-  R si[GR_MAXNS];
-  const int NS = T.ns;
-  if (FAM == FAM_FORCE) {
-    if (S.kind >= GR_SYS_FLUID) pi = P.p[gi];
-    if (T.has_stress)
-      for (int k = 0; k < NS; ++k) si[k] = P.stress_s[k][gi];
-  }
-  Acc<R> adv[D], adr;
-  Acc<R> asr[GR_MAXNS], avo[3];
-  unsigned long long npairs_self = 0, npairs_other = 0;
+  Gather<D, R, FAM> Gt;
+  Gt.load(P, S, T, gi);
+  unsigned long long ns = 0, no = 0;
+  if (live) gather_direct<D, R, FAM>(Gt, P, sd, T, srcs, grids, carr, cells, i, ns, no);
+  count_pairs<CNT>(pair_count, ns, no);
+  if (live) Gt.store(P, S, T, gi);
+}
 
-  for (int q = T.src_begin; q < T.src_end; ++q) {
+// ---------------------------------------------------------------------------
+// Neighbour lists
+// ---------------------------------------------------------------------------
+
+__device__ inline int wave_min_i(int v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    int u = __shfl_xor(v, o, 64);
+    v = u < v ? u : v;
+  }
+  return v;
+}
+__device__ inline int wave_max_i(int v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    int u = __shfl_xor(v, o, 64);
+    v = u > v ? u : v;
+  }
+  return v;
+}
+
+// Accepted partners of every target of a list task, per source, in the
+// order the sources are listed: lidx[L.list_off + k·L.cap + i] holds the
+// k-th partner (source-local index) of target i, lcnt[L.cnt_off + q·L.cap + i]
+// the number from source q.  Counts above L.maxnb mean "overflow" (the
+// physics pass then walks the cells directly).
+template <int D, typename R>
+__global__ void __launch_bounds__(GR_BLOCK)
+    k_nlist(Pool<R> P, const SysDesc *__restrict__ sd, const int *__restrict__ n_dev,
+            const LTask *__restrict__ lt, const SrcDesc *__restrict__ srcs,
+            const Grid *__restrict__ grids, const CellArr *__restrict__ carr,
+            const int *__restrict__ cells, const Blk *__restrict__ blk, int nblk, int *lcnt,
+            int *lidx) {
+  const Blk b = blk[xcd_block(nblk)];
+  const LTask L = lt[b.item];
+  const int i = b.first + threadIdx.x;
+  const SysDesc S = sd[L.sys];
+  const bool live = i < n_dev[L.sys];
+  const int64_t gi = S.off + (live ? i : 0);
+  double xi[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) xi[d] = P.x[d][gi];
+  int *my = lidx + L.list_off + i;
+  int cnt = 0;
+  for (int q = L.src_begin; q < L.src_end; ++q) {
     const SrcDesc Q = srcs[q];
     const Grid G = grids[Q.grid];
-    if (G.ncells <= 0 || !live) continue;
-    const int *cs = cells + carr[Q.carr].off;
-    const int64_t poff = sd[Q.partner].off;
-    SrcR<R> K;
-    K.h = (R)Q.h;
-    K.norm = (R)Q.norm;
-    K.hf = (R)Q.hf;
-    K.alpha = (R)Q.alpha;
-    K.beta = (R)Q.beta;
-    K.mc = (R)(0.5 * (Q.c_t_piv + Q.c_q_piv));
-    K.h2 = (R)(Q.hf * Q.hf * 0.01);
-    K.m_q = (R)Q.mass_q;
-    K.psi_coef = (R)(2.0 * Q.delta * (0.5 * (Q.c_t_psi + Q.c_q_psi)) * Q.hf);
-    K.eps = (R)Q.eps;
-    K.ljcut = (R)Q.ljcut;
-    K.ljc2 = (R)(0.01 * Q.c_t_piv * Q.c_t_piv);
-    K.sigma = Q.sigma;
-    K.has_delta = Q.has_delta;
-    K.has_eps = Q.has_eps;
-    // target cell in this grid (same formula as the CSR build)
-    int c[D];
-    for (int d = 0; d < D; ++d) {
-      int cd = (int)floor((xi[d] - G.ming[d]) / Q.cell);
-      cd = cd < 1 ? 1 : cd;
-      cd = cd > G.ng[d] - 2 ? G.ng[d] - 2 : cd;
-      c[d] = cd;
-    }
-    // dynamic boundary: da of the target (FP64)
-    double da = 0.0;
-    if (Q.pk == PK_DYNAMIC) {
-      da = (xi[0] - Q.wp[0]) * Q.wn[0];
-      for (int d = 1; d < D; ++d) da = da + (xi[d] - Q.wp[d]) * Q.wn[d];
-    }
-    const int nstrip = (D == 3) ? 9 : 3;
-    for (int st = 0; st < nstrip; ++st) {
-      int base;
-      if (D == 3) {
-        const int ddx = st / 3 - 1, ddy = st % 3 - 1;
-        base = ((c[0] + ddx) * G.ng[1] + (c[1] + ddy)) * G.ng[2] + (c[D - 1] - 1);
-      } else {
-        base = (c[0] + st - 1) * G.ng[1] + (c[1] - 1);
+    int cq = 0;
+    if (G.ncells > 0) {
+      const int *cs = cells + carr[Q.carr].off;
+      const int64_t poff = sd[Q.partner].off;
+      const bool self = Q.pk == PK_SELF;
+      int c[D];
+      target_cell<D>(xi, G, Q.cell, c);
+      for (int st = 0; st < ((D == 3) ? 9 : 3); ++st) {
+        int j0 = 0, j1 = 0;
+        if (live) {
+          const int base = strip_base<D>(c, G, st);
+          j0 = cs[base] - 1;
+          j1 = cs[base + 3] - 1;
+        }
+        const int lo = __builtin_amdgcn_readfirstlane(wave_min_i(j0 < j1 ? j0 : INT_MAX));
+        const int hi = __builtin_amdgcn_readfirstlane(wave_max_i(j0 < j1 ? j1 : INT_MIN));
+        const int wmax = __builtin_amdgcn_readfirstlane(wave_max_i(j1 - j0));
+        if (hi <= lo) continue;
+        if (hi - lo <= 3 * wmax) {
+          // wave-uniform walk of the union of the lanes' ranges
+          for (int j = lo; j < hi; ++j) {
+            const int64_t gj = poff + j;
+            double xj[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) xj[d] = P.x[d][gj];
+            if (j >= j0 && j < j1 && !(self && j == i)) {
+              double dx[D];
+#pragma unroll
+              for (int d = 0; d < D; ++d) dx[d] = xi[d] - xj[d];
+              double r2 = dx[0] * dx[0];
+#pragma unroll
+              for (int d = 1; d < D; ++d) r2 = r2 + dx[d] * dx[d];
+              if (r2 < Q.cutoff2) {
+                if (cnt < L.maxnb) my[(int64_t)cnt * L.cap] = j;
+                ++cnt;
+                ++cq;
+              }
+            }
+          }
+        } else {
+          // lanes in distant columns: each lane walks its own range
+          for (int j = j0; j < j1; ++j) {
+            if (self && j == i) continue;
+            const int64_t gj = poff + j;
+            double dx[D];
+#pragma unroll
+            for (int d = 0; d < D; ++d) dx[d] = xi[d] - P.x[d][gj];
+            double r2 = dx[0] * dx[0];
+#pragma unroll
+            for (int d = 1; d < D; ++d) r2 = r2 + dx[d] * dx[d];
+            if (r2 < Q.cutoff2) {
+              if (cnt < L.maxnb) my[(int64_t)cnt * L.cap] = j;
+              ++cnt;
+              ++cq;
+            }
+          }
+        }
       }
-      const int j0 = cs[base] - 1, j1 = cs[base + 3] - 1;
-      for (int j = j0; j < j1; ++j) {
-        if (Q.pk == PK_SELF && j == i) continue;
+    }
+    if (live) lcnt[L.cnt_off + (int64_t)(q - L.src_begin) * L.cap + i] = cq;
+  }
+  if (live && cnt > L.maxnb) lcnt[L.cnt_off + (int64_t)(L.src_end - L.src_begin) * L.cap + i] = 1;
+  else if (live) lcnt[L.cnt_off + (int64_t)(L.src_end - L.src_begin) * L.cap + i] = 0;
+}
+
+// Functors over the accepted partners of each target (one stage).
+template <int D, typename R, int FAM, bool CNT>
+__global__ void __launch_bounds__(GR_BLOCK)
+    k_sweep_list(Pool<R> P, const SysDesc *__restrict__ sd, const int *__restrict__ n_dev,
+                 const Task *__restrict__ tasks, const SrcDesc *__restrict__ srcs,
+                 const Grid *__restrict__ grids, const CellArr *__restrict__ carr,
+                 const int *__restrict__ cells, const Blk *__restrict__ blk, int nblk,
+                 unsigned long long *pair_count, const LTask *__restrict__ lt,
+                 const int *__restrict__ lcnt, const int *__restrict__ lidx) {
+  const Blk b = blk[xcd_block(nblk)];
+  const Task T = tasks[b.item];
+  const LTask L = lt[T.ltask];
+  const int i = b.first + threadIdx.x;
+  const SysDesc S = sd[T.sys];
+  const bool live = i < n_dev[T.sys];
+  const int64_t gi = S.off + (live ? i : 0);
+  Gather<D, R, FAM> Gt;
+  Gt.load(P, S, T, gi);
+  unsigned long long ns = 0, no = 0;
+  const int nls = L.src_end - L.src_begin;
+  const bool overflow = live && lcnt[L.cnt_off + (int64_t)nls * L.cap + i] != 0;
+  if (overflow) {
+    gather_direct<D, R, FAM>(Gt, P, sd, T, srcs, grids, carr, cells, i, ns, no);
+  } else if (live) {
+    const int *my = lidx + L.list_off + i;
+    for (int q = T.src_begin; q < T.src_end; ++q) {
+      const SrcDesc Q = srcs[q];
+      const int64_t poff = sd[Q.partner].off;
+      SrcK<R> K;
+      K.load(Q);
+      int off = 0;
+      for (int l = 0; l < Q.lsrc; ++l) off += lcnt[L.cnt_off + (int64_t)l * L.cap + i];
+      const int cq = lcnt[L.cnt_off + (int64_t)Q.lsrc * L.cap + i];
+      const double da = (Q.pk == PK_DYNAMIC) ? Gt.da_of(Q) : 0.0;
+      for (int k = 0; k < cq; ++k) {
+        const int j = my[(int64_t)(off + k) * L.cap];
         const int64_t gj = poff + j;
         double dx[D];
-        for (int d = 0; d < D; ++d) dx[d] = xi[d] - P.x[d][gj];
+#pragma unroll
+        for (int d = 0; d < D; ++d) dx[d] = Gt.xi[d] - P.x[d][gj];
         double r2 = dx[0] * dx[0];
+#pragma unroll
         for (int d = 1; d < D; ++d) r2 = r2 + dx[d] * dx[d];
-        if (!(r2 < Q.cutoff2)) continue;
         if (CNT) {
           if (Q.pk == PK_SELF)
-            ++npairs_self;
+            ++ns;
           else
-            ++npairs_other;
+            ++no;
         }
-        R dxr[D], gx[D];
-        const R r2r = (R)r2;
-        const R r = sqrt(r2r);
-        const R qq = r / K.h;
-        const R g = dwdq<R>(Q.kk, K.norm, qq) / (r * K.h);
-        for (int d = 0; d < D; ++d) {
-          dxr[d] = (R)dx[d];
-          gx[d] = g * dxr[d];
-        }
-        if (FAM == FAM_COUNT) {
-          adv[0].add((R)1);
-          adr.add((R)1);
-          continue;
-        }
-        // partner state
-        R vj[D], rj, m = K.m_q;
-        if (Q.pk == PK_DYNAMIC) {
-          double db = (P.x[0][gj] - Q.wp[0]) * Q.wn[0];
-          for (int d = 1; d < D; ++d) db = db + (P.x[d][gj] - Q.wp[d]) * Q.wn[d];
-          double rat = fabs(db / da);
-          double mn = isnan(rat) ? rat : (rat < Q.wbeta ? rat : Q.wbeta);
-          const R fac = (R)(-mn);
-          for (int d = 0; d < D; ++d) vj[d] = fac * vi[d];
-          rj = ri;
-        } else {
-          for (int d = 0; d < D; ++d) vj[d] = P.vs[d][gj];
-          rj = P.rhos[gj];
-        }
-        if (FAM == FAM_STRAIN) {
-          R dv[D];
-          for (int d = 0; d < D; ++d) dv[d] = vj[d] - vi[d];
-          const R f = m / rj;
-          R srv[GR_MAXNS];
-          if (D == 2) {
-            srv[0] = dv[0] * gx[0];
-            srv[1] = dv[1] * gx[1];
-            if (NS == 4) {
-              srv[2] = (R)0;
-              srv[3] = (R)0.5 * (dv[0] * gx[1] + dv[1] * gx[0]);
-            } else {
-              srv[2] = (R)0.5 * (dv[0] * gx[1] + dv[1] * gx[0]);
-            }
-          } else {
-            srv[0] = dv[0] * gx[0];
-            srv[1] = dv[1] * gx[1];
-            srv[2] = dv[D - 1] * gx[D - 1];
-            srv[3] = (R)0.5 * (dv[0] * gx[1] + dv[1] * gx[0]);
-            srv[4] = (R)0.5 * (dv[0] * gx[D - 1] + dv[D - 1] * gx[0]);
-            srv[5] = (R)0.5 * (dv[1] * gx[D - 1] + dv[D - 1] * gx[1]);
-          }
-          for (int k = 0; k < NS; ++k) asr[k].add(srv[k] * f);
-          if (Q.pfn == GR_PFN_SRV) {
-            avo[0].add((R)0.5 * (dv[0] * gx[1] - dv[1] * gx[0]) * f);
-            if (D == 3) {
-              avo[1].add((R)0.5 * (dv[0] * gx[D - 1] - dv[D - 1] * gx[0]) * f);
-              avo[2].add((R)0.5 * (dv[1] * gx[D - 1] - dv[D - 1] * gx[1]) * f);
-            }
-          }
-          continue;
-        }
-        // FAM_FORCE: FluidPfn / CauchyFluidPfn
-        R dv[D];
-        for (int d = 0; d < D; ++d) dv[d] = vi[d] - vj[d];
-        R vr = dxr[0] * dv[0];
-        for (int d = 1; d < D; ++d) vr = vr + dxr[d] * dv[d];
-        vr = jmin0(vr);
-        const R muv = K.hf * vr / (r2r + K.h2);
-        const R piv = (K.beta * muv - K.alpha * K.mc) * muv / ((R)0.5 * ri * rj);
-        if (Q.pk == PK_STATIC) {
-          const R lr = K.ljcut / r;
-          R f = (r < K.ljcut ? (R)1 : (R)0) * (pow_i(lr, 12) - pow_i(lr, 6)) / r2r;
-          const R rf = K.ljc2 * f;
-          const R t = -m * piv;
-          for (int d = 0; d < D; ++d) adv[d].add(t * gx[d] + rf * dxr[d]);
-          continue;
-        }
-        R dr = dv[0] * gx[0];
-        for (int d = 1; d < D; ++d) dr = dr + dv[d] * gx[d];
-        R psi = (R)0;
-        if (K.has_delta) {
-          R dxg = dxr[0] * gx[0];
-          for (int d = 1; d < D; ++d) dxg = dxg + dxr[d] * gx[d];
-          psi = K.psi_coef * (ri - rj) * dxg / r2r;
-        }
-        if (Q.pfn == GR_PFN_FLUID) {
-          R pj;
-          if (Q.pk == PK_DYNAMIC)
-            pj = pi;
-          else
-            pj = P.p[gj];
-          R dh;
-          if (K.sigma == 1)
-            dh = -(pi + pj) / (ri * rj);
-          else
-            dh = -(pi / (ri * ri) + pj / (rj * rj));
-          if (Q.pk == PK_TWO_SIDED) {
-            R ast = (R)0;
-            if (K.has_eps) ast = K.eps * (-(fabs(pi) + fabs(pj)) / (ri * rj));
-            const R t = dh + ast - piv;
-            for (int d = 0; d < D; ++d) adv[d].add(m * (t * gx[d]));
-          } else {
-            const R t = m * (dh - piv);
-            for (int d = 0; d < D; ++d) adv[d].add(t * gx[d]);
-          }
-          const R coef = (K.sigma == 1) ? ri / rj : (R)1;
-          adr.add(m * (dr * coef + psi / rj));
-        } else {  // CauchyFluidPfn
-          R sj[GR_MAXNS];
-          if (Q.pk == PK_DYNAMIC) {
-            for (int k = 0; k < NS; ++k) sj[k] = si[k];
-          } else {
-            for (int k = 0; k < NS; ++k) sj[k] = P.stress_s[k][gj];
-          }
-          const R ci = (R)1 / (ri * ri), cj = (R)1 / (rj * rj);
-          R hv[D];
-          if (D == 2) {
-            const int o = (NS == 3) ? 2 : 3;
-            hv[0] = (si[0] * gx[0] + si[o] * gx[1]) * ci + (sj[0] * gx[0] + sj[o] * gx[1]) * cj;
-            hv[1] = (si[o] * gx[0] + si[1] * gx[1]) * ci + (sj[o] * gx[0] + sj[1] * gx[1]) * cj;
-          } else {
-            hv[0] = (si[0] * gx[0] + si[3] * gx[1] + si[4] * gx[D - 1]) * ci +
-                    (sj[0] * gx[0] + sj[3] * gx[1] + sj[4] * gx[D - 1]) * cj;
-            hv[1] = (si[3] * gx[0] + si[1] * gx[1] + si[5] * gx[D - 1]) * ci +
-                    (sj[3] * gx[0] + sj[1] * gx[1] + sj[5] * gx[D - 1]) * cj;
-            hv[D - 1] = (si[4] * gx[0] + si[5] * gx[1] + si[2] * gx[D - 1]) * ci +
-                        (sj[4] * gx[0] + sj[5] * gx[1] + sj[2] * gx[D - 1]) * cj;
-          }
-          for (int d = 0; d < D; ++d) adv[d].add(m * (hv[d] - piv * gx[d]));
-          adr.add(m * (dr + psi / rj));
-        }
+        Gt.pair(P, Q, K, gj, dx, r2, da);
       }
     }
   }
-  if (CNT) {
-    // [0]: ordered self pairs (each unordered pair seen from both sides),
-    // [1]: coupled pairs seen from the gathering side.
-    unsigned long long a = npairs_self, o2 = npairs_other;
-    for (int o = 32; o > 0; o >>= 1) {
-      a += __shfl_xor(a, o, 64);
-      o2 += __shfl_xor(o2, o, 64);
-    }
-    if ((threadIdx.x & 63) == 0) {
-      if (a) atomicAdd(&pair_count[0], a);
-      if (o2) atomicAdd(&pair_count[1], o2);
-    }
-  }
-  if (!live) return;
-  if (FAM == FAM_STRAIN) {
-    for (int k = 0; k < NS; ++k) {
-      const R v = asr[k].get();
-      P.sr[k][gi] = T.assign ? v : P.sr[k][gi] + v;
-    }
-    if (T.write_vor)
-      for (int k = 0; k < T.nv; ++k) {
-        const R v = avo[k].get();
-        P.vor[k][gi] = T.assign ? v : P.vor[k][gi] + v;
-      }
-    return;
-  }
-  for (int d = 0; d < D; ++d) {
-    const R v = adv[d].get();
-    P.dvdt[d][gi] = T.assign ? (R)S.sv[d] + v : P.dvdt[d][gi] + v;
-  }
-  {
-    const R v = adr.get();
-    P.drhodt[gi] = T.assign ? (R)S.srho + v : P.drhodt[gi] + v;
-  }
+  count_pairs<CNT>(pair_count, ns, no);
+  if (live) Gt.store(P, S, T, gi);
 }
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
 
 template <int D, typename R, int FAM>
 static void launch_one(hipStream_t s, const Pool<R> &P, const SweepArgs &a, bool cnt) {
+  if (a.lt) {
+    if (cnt)
+      k_sweep_list<D, R, FAM, true><<<a.nblk, GR_BLOCK, 0, s>>>(
+          P, a.sd, a.n, a.tasks, a.srcs, a.grids, a.carr, a.cells, a.blk, a.nblk, a.pair_count,
+          a.lt, a.lcnt, a.lidx);
+    else
+      k_sweep_list<D, R, FAM, false><<<a.nblk, GR_BLOCK, 0, s>>>(
+          P, a.sd, a.n, a.tasks, a.srcs, a.grids, a.carr, a.cells, a.blk, a.nblk, a.pair_count,
+          a.lt, a.lcnt, a.lidx);
+    return;
+  }
   if (cnt)
     k_sweep<D, R, FAM, true><<<a.nblk, GR_BLOCK, 0, s>>>(P, a.sd, a.n, a.tasks, a.srcs, a.grids,
-                                                          a.carr, a.cells, a.blk, a.pair_count);
+                                                          a.carr, a.cells, a.blk, a.nblk,
+                                                          a.pair_count);
   else
     k_sweep<D, R, FAM, false><<<a.nblk, GR_BLOCK, 0, s>>>(P, a.sd, a.n, a.tasks, a.srcs, a.grids,
-                                                           a.carr, a.cells, a.blk, a.pair_count);
+                                                           a.carr, a.cells, a.blk, a.nblk,
+                                                           a.pair_count);
 }
 
 template <typename R>
@@ -378,5 +648,18 @@ template void launch_sweep<float>(hipStream_t, const Pool<float> &, const SweepA
                                   bool);
 template void launch_sweep<double>(hipStream_t, const Pool<double> &, const SweepArgs &, int, int,
                                    bool);
+
+template <typename R>
+void launch_nlist(hipStream_t s, const Pool<R> &P, const NlistArgs &a, int D) {
+  if (a.nblk <= 0) return;
+  if (D == 2)
+    k_nlist<2, R><<<a.nblk, GR_BLOCK, 0, s>>>(P, a.sd, a.n, a.lt, a.srcs, a.grids, a.carr,
+                                              a.cells, a.blk, a.nblk, a.lcnt, a.lidx);
+  else
+    k_nlist<3, R><<<a.nblk, GR_BLOCK, 0, s>>>(P, a.sd, a.n, a.lt, a.srcs, a.grids, a.carr,
+                                              a.cells, a.blk, a.nblk, a.lcnt, a.lidx);
+}
+template void launch_nlist<float>(hipStream_t, const Pool<float> &, const NlistArgs &, int);
+template void launch_nlist<double>(hipStream_t, const Pool<double> &, const NlistArgs &, int);
 
 }  // namespace gr

@@ -229,12 +229,15 @@ int gr_get_cell_start(gr_ctx *ctx, int inter, int which, int64_t *out,
  * every step of the following gr_step/gr_step_many calls (no extra
  * synchronisation inside a call); gr_get_timing returns the mean ms per step
  * of phases 0 sort, 1 ghosts, 2 grid, 3 half-step, 4 updaters + ghost
- * copies, 5 sweeps, 6 full step, 7 whole step.
+ * copies, 5 sweeps, 6 full step, 7 whole step, 8 neighbour search.
  * mode bit 1: count pairs with r < 2h in the sweeps; gr_get_pair_count
- * returns unique pairs per step (self pairs once, coupled pairs once). */
+ * returns unique pairs per step (self pairs once, coupled pairs once),
+ * gr_get_pair_stats the ordered self pairs (each unordered pair from both
+ * sides) and the coupled pairs seen from the gathering side. */
 int gr_set_timing(gr_ctx *ctx, int mode);
 int gr_get_timing(gr_ctx *ctx, double *ms_out, int nphases);
 int gr_get_pair_count(gr_ctx *ctx, int integ, int64_t *pairs_out);
+int gr_get_pair_stats(gr_ctx *ctx, int64_t *self_ordered, int64_t *coupled);
 /* Device-to-device copy of `bytes` bytes by a plain streaming kernel, `reps`
  * times (calibration of HBM traffic counters; no counterpart in the
  * reference). */

@@ -451,3 +451,30 @@ def test_graph_mode_equals_eager():
         res.append({f: getattr(cfg["fluid"], f).copy() for f in ("x", "v", "rho", "p")})
     for f in res[0]:
         assert np.array_equal(res[0][f], res[1][f]), f
+
+
+LIST_CASES = [
+    ("ep_column_2d", M.configs.ep_column_2d, dict(dx=0.01, width=0.2, height=0.1), 2, "f32"),
+    ("dambreak_3d", M.configs.dambreak_3d, dict(dx=0.04, water=(0.24, 0.2, 0.24),
+                                                tank=(0.48, 0.2, 0.32)), 3, "f32"),
+    ("dambreak_2d_f64", M.configs.dambreak_2d, dict(dx=0.04), 2, "f64"),
+]
+
+
+@pytest.mark.parametrize("name,builder,kw,D,prec", LIST_CASES, ids=[c[0] for c in LIST_CASES])
+def test_neighbour_lists_equal_cell_walk(monkeypatch, name, builder, kw, D, prec):
+    """Steps through the neighbour lists, through the direct cell walk, and
+    with lists so short that most targets overflow and fall back to the walk
+    give bit-identical states: the lists keep the partners in the walk's
+    order and the distance test is the same FP64 expression."""
+    res = {}
+    for mode, env in (("list", {}), ("walk", {"GRASPH_NO_NLIST": "1"}),
+                      ("overflow", {"GRASPH_NLIST_MAXNB": "6"})):
+        for k in ("GRASPH_NO_NLIST", "GRASPH_NLIST_MAXNB"):
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        res[mode] = _run_steps(builder, kw, D, prec, 8, "dev")
+    for mode in ("walk", "overflow"):
+        for key, ref in res["list"].items():
+            assert np.array_equal(res[mode][key], ref, equal_nan=True), f"{mode} {key}"
