@@ -184,8 +184,6 @@ struct gr_ctx {
   int allreal_nblk = 0;
   int64_t allreal_cap = -1;
   size_t allreal_nsys = 0;
-  DBuf xs_pool[2], xs_gxs;
-  int64_t xs_cap = -1;
   std::vector<char> id_written;
   bool graph_mode = false, timing = false, count_pairs = false;
   hipEvent_t ev[9] = {};
@@ -194,6 +192,31 @@ struct gr_ctx {
   long long last_pairs[2] = {0, 0};
   int last_pairs_integ_key = -1;
 };
+
+namespace {
+// Position component pointers of pool family w and of the ghost scratch.
+XP xp_pool(gr_ctx *c, int w) {
+  XP r;
+  for (int d = 0; d < 3; ++d) r.p[d] = c->x[w].as<double>() + d * c->pool_cap;
+  return r;
+}
+XPW xpw_pool(gr_ctx *c, int w) {
+  XPW r;
+  for (int d = 0; d < 3; ++d) r.p[d] = c->x[w].as<double>() + d * c->pool_cap;
+  return r;
+}
+XP xp_gxs(gr_ctx *c) {
+  XP r;
+  for (int d = 0; d < 3; ++d) r.p[d] = c->gxs.as<double>() + d * c->pool_cap;
+  return r;
+}
+XPW xpw_gxs(gr_ctx *c) {
+  XPW r;
+  for (int d = 0; d < 3; ++d) r.p[d] = c->gxs.as<double>() + d * c->pool_cap;
+  return r;
+}
+
+}  // namespace
 
 namespace {
 
@@ -700,11 +723,7 @@ void host_bbox(gr_ctx *c, const std::vector<int> &systems, std::vector<double> &
   db.upload(blk, c->stream);
   launch_minmax_reset(c->stream, dl.as<int>(), (int)list.size(), c->d_kmin.as<int>(),
                       c->d_kmax.as<int>(), c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>());
-  std::vector<double *> xs(3);
-  for (int d = 0; d < 3; ++d) xs[d] = c->x[c->cur].as<double>() + d * c->pool_cap;
-  DBuf dxs;
-  dxs.upload(xs, c->stream);
-  launch_key_minmax(c->stream, dxs.as<const double *>(), c->d_sd.as<SysDesc>(), c->d_n.as<int>(),
+  launch_key_minmax(c->stream, xp_pool(c, c->cur), c->d_sd.as<SysDesc>(), c->d_n.as<int>(),
                     c->d_st.as<StepSys>(), db.as<Blk>(), (int)blk.size(), c->D,
                     c->d_kmin.as<int>(), c->d_kmax.as<int>(), c->d_bbmin.as<long long>(),
                     c->d_bbmax.as<long long>(), 0);
@@ -1022,7 +1041,7 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
     L.cnt_off = cacc;
     lacc += (int64_t)L.cap * maxnb;
     cacc += (int64_t)L.cap * (L.src_end - L.src_begin + 1);
-    add_blocks(lbv, (int)ltv.size(), L.cap, GR_BLOCK);
+    add_blocks(lbv, (int)ltv.size(), L.cap, 64);  // one wavefront per workgroup
     ltask_of[kv.first] = (int)ltv.size();
     ltv.push_back(L);
   }
@@ -1154,32 +1173,11 @@ void all_real(gr_ctx *c, const Blk *&blk, int &nblk) {
 // Step phases
 // ---------------------------------------------------------------------------
 
-// Device arrays of the x-component pointers (pool A, pool B, ghost scratch).
-struct XsView {
-  DBuf *pool;
-  DBuf *gxs;
-};
-XsView xs_ptrs(gr_ctx *c) {
-  if (c->xs_cap != c->pool_cap) {
-    for (int w = 0; w < 2; ++w) {
-      std::vector<double *> v(3);
-      for (int d = 0; d < 3; ++d) v[d] = c->x[w].as<double>() + d * c->pool_cap;
-      c->xs_pool[w].upload(v, c->stream);
-    }
-    std::vector<double *> g(3);
-    for (int d = 0; d < 3; ++d) g[d] = c->gxs.as<double>() + d * c->pool_cap;
-    c->xs_gxs.upload(g, c->stream);
-    c->xs_cap = c->pool_cap;
-  }
-  return XsView{c->xs_pool, &c->xs_gxs};
-}
-
 template <typename R>
 void phase_sort_real(gr_ctx *c, Plan &P) {
   if (P.real_list.empty()) return;
   hipStream_t s = c->stream;
-  XsView X = xs_ptrs(c);
-  const double *const *xs = X.pool[c->cur].as<const double *>();
+  const XP xs = xp_pool(c, c->cur);
   launch_minmax_reset(s, P.d_real_list.as<int>(), (int)P.real_list.size(), c->d_kmin.as<int>(),
                       c->d_kmax.as<int>(), c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>());
   launch_key_minmax(s, xs, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), c->d_st.as<StepSys>(),
@@ -1208,19 +1206,18 @@ template <typename R>
 void phase_ghosts(gr_ctx *c, Plan &P, bool generate, bool sort) {
   if (P.ghost_list.empty()) return;
   hipStream_t s = c->stream;
-  XsView X = xs_ptrs(c);
-  const double *const *x = X.pool[c->cur].as<const double *>();
+  const XP x = xp_pool(c, c->cur);
   Pool<R> A = pool_view<R>(c, c->cur);
   if (generate) {
     launch_ghost_flags(s, x, c->d_sd.as<SysDesc>(), c->d_bnd.as<Bnd>(), P.blk_cand.as<Blk>(),
                        P.nblk_cand, P.d_cand_off.as<int>(), c->D, c->cand.as<int>());
     launch_scan(s, c->cand.as<int>(), c->cand_scan.as<int>(), c->scan_bsum.as<int>(), P.n_cand);
-    launch_ghost_scatter(s, x, X.gxs->as<double *const>(), c->gidx_s.as<int>(), c->gbnd_s.as<int>(),
+    launch_ghost_scatter(s, x, xpw_gxs(c), c->gidx_s.as<int>(), c->gbnd_s.as<int>(),
                          c->gnrm.as<int>(), c->d_sd.as<SysDesc>(), c->d_bnd.as<Bnd>(),
                          P.blk_cand.as<Blk>(), P.nblk_cand, P.d_cand_off.as<int>(),
                          c->cand_scan.as<int>(), c->D, c->d_n.as<int>());
   }
-  const double *const *gx = X.gxs->as<const double *>();
+  const XP gx = xp_gxs(c);
   launch_minmax_reset(s, P.d_ghost_list.as<int>(), (int)P.ghost_list.size(), c->d_kmin.as<int>(),
                       c->d_kmax.as<int>(), c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>());
   launch_key_minmax(s, gx, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), c->d_st.as<StepSys>(),
@@ -1241,7 +1238,7 @@ void phase_ghosts(gr_ctx *c, Plan &P, bool generate, bool sort) {
     launch_fill_i32(s, c->d_flags.as<int>(), 0, 1);
   }
   launch_permute_ghost(s, gx, c->gidx_s.as<int>(), c->gbnd_s.as<int>(),
-                       X.pool[c->cur].as<double *const>(), A.gidx, A.gbnd, c->d_sd.as<SysDesc>(),
+                       xpw_pool(c, c->cur), A.gidx, A.gbnd, c->d_sd.as<SysDesc>(),
                        c->d_n.as<int>(), P.blk_ghost.as<Blk>(), P.nblk_ghost,
                        P.d_bbase_ghost.as<int>(), c->vals.as<int>(), c->d_flags.as<int>(), c->D);
 }
@@ -1249,12 +1246,11 @@ void phase_ghosts(gr_ctx *c, Plan &P, bool generate, bool sort) {
 void phase_grids(gr_ctx *c, Plan &P) {
   if (P.inters.empty()) return;
   hipStream_t s = c->stream;
-  XsView X = xs_ptrs(c);
   launch_grid_params(s, P.d_inter_list.as<int>(), (int)P.inters.size(), c->d_inter_sys.as<int2>(),
                      c->d_cellsz.as<double>(), c->d_bbmin.as<long long>(),
                      c->d_bbmax.as<long long>(), c->d_n.as<int>(), c->d_grids.as<Grid>(),
                      c->d_inter_carr.as<int2>(), c->d_carr.as<CellArr>(), c->D, c->d_flags.as<int>());
-  launch_cells_build(s, X.pool[c->cur].as<const double *>(), c->cells.as<int>(),
+  launch_cells_build(s, xp_pool(c, c->cur), c->cells.as<int>(),
                      c->d_carr.as<CellArr>(), c->d_grids.as<Grid>(), c->d_cellsz.as<double>(),
                      c->d_sd.as<SysDesc>(), c->d_n.as<int>(), P.blk_cells.as<Blk>(), P.nblk_cells,
                      P.blk_mark.as<Blk>(), P.nblk_mark, P.carr_blocks.as<int2>(), P.ncarr,
@@ -1410,7 +1406,6 @@ void do_steps(gr_ctx *c, const std::vector<int> &integs, int nsteps) {
     const Blk *b;
     int nb;
     all_real(c, b, nb);  // build cached tables before any capture
-    xs_ptrs(c);
   }
   if (c->count_pairs) launch_fill_i32(c->stream, c->d_pairs.as<int>(), 0, 4);
   const int nst = (int)P.stages.size();
@@ -2093,8 +2088,7 @@ int gr_create_grid(gr_ctx *c, int inter) {
     dl.upload(syss, c->stream);
     launch_minmax_reset(c->stream, dl.as<int>(), (int)syss.size(), c->d_kmin.as<int>(), c->d_kmax.as<int>(),
                         c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>());
-    XsView X = xs_ptrs(c);
-    launch_key_minmax(c->stream, X.pool[c->cur].as<const double *>(), c->d_sd.as<SysDesc>(),
+    launch_key_minmax(c->stream, xp_pool(c, c->cur), c->d_sd.as<SysDesc>(),
                       c->d_n.as<int>(), c->d_st.as<StepSys>(), db.as<Blk>(), (int)blk.size(), c->D,
                       c->d_kmin.as<int>(), c->d_kmax.as<int>(), c->d_bbmin.as<long long>(),
                       c->d_bbmax.as<long long>(), 0);

@@ -77,6 +77,17 @@ struct CellArr {
   int32_t pad;
 };
 
+// Position component arrays, passed by value so that the loads of all
+// components issue together.
+struct XP {
+  const double *p[3];
+  __device__ __forceinline__ const double *operator[](int d) const { return p[d]; }
+};
+struct XPW {
+  double *p[3];
+  __device__ __forceinline__ double *operator[](int d) const { return p[d]; }
+};
+
 // One gather source of a target system (one interaction, one side).
 struct SrcDesc {
   int32_t pk, pfn, partner, grid;

@@ -64,8 +64,9 @@ __device__ inline int key_comp(double x, double cutoff) {
 
 // Block reduction of min/max of the integer cell key and of the position per
 // dimension for the block's system; one atomic per block per quantity.
-__global__ void k_key_minmax(const double *const *xs, const SysDesc *sd, const int *n_dev,
-                             const StepSys *st, const Blk *blk, int D, int *kmin, int *kmax,
+template <int D>
+__global__ void k_key_minmax(XP xs, const SysDesc *sd, const int *n_dev,
+                             const StepSys *st, const Blk *blk, int *kmin, int *kmax,
                              long long *bbmin, long long *bbmax, int with_keys) {
   __shared__ long long red[4][GR_BLOCK / 64][3];
   const Blk b = blk[blockIdx.x];
@@ -76,8 +77,12 @@ __global__ void k_key_minmax(const double *const *xs, const SysDesc *sd, const i
   const int64_t gi = sd[s].off + i;
   const double cutoff = st[s].sort_cutoff;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double xl[D];
+#pragma unroll
+  for (int d = 0; d < D; ++d) xl[d] = live ? xs[d][gi] : 0.0;
+#pragma unroll
   for (int d = 0; d < D; ++d) {
-    double xv = live ? xs[d][gi] : 0.0;
+    double xv = xl[d];
     long long o = ord_d(xv);
     long long mn = live ? o : LLONG_MAX, mx = live ? o : LLONG_MIN;
     int k = (live && with_keys) ? key_comp(xv, cutoff) : 0;
@@ -156,9 +161,11 @@ __global__ void k_key_base(const int *list, int nlist, int D, int pad, const int
   }
 }
 
-__device__ inline unsigned lin_key(const double *const *xs, int64_t gi, int D, double cutoff,
-                                   const int *kmin, const unsigned *krange, int s) {
+template <int D>
+__device__ inline unsigned lin_key(XP xs, int64_t gi, double cutoff, const int *kmin,
+                                   const unsigned *krange, int s) {
   unsigned k = 0;
+#pragma unroll
   for (int d = 0; d < D; ++d) {
     int c = key_comp(xs[d][gi], cutoff) - kmin[s * 3 + d];
     k = k * krange[s * 3 + d] + (unsigned)c;
@@ -168,8 +175,9 @@ __device__ inline unsigned lin_key(const double *const *xs, int64_t gi, int D, d
 
 // Write (key, pool index) at each particle's compact position and flag any
 // key inversion inside a system (src/Sorting.jl:179-186).
-__global__ void k_key_write(const double *const *xs, const SysDesc *sd, const int *n_dev,
-                            const StepSys *st, const Blk *blk, const int *bbase, int D,
+template <int D>
+__global__ void k_key_write(XP xs, const SysDesc *sd, const int *n_dev,
+                            const StepSys *st, const Blk *blk, const int *bbase,
                             const int *kmin, const unsigned *krange, const unsigned *kbase,
                             unsigned *keys, int *vals, int *flags) {
   const Blk b = blk[blockIdx.x];
@@ -185,9 +193,9 @@ __global__ void k_key_write(const double *const *xs, const SysDesc *sd, const in
   if (i < n) {
     unsigned rng = 1;
     for (int d = 0; d < D; ++d) rng *= krange[s * 3 + d];
-    key = kbase[s] + lin_key(xs, gi, D, cutoff, kmin, krange, s);
+    key = kbase[s] + lin_key<D>(xs, gi, cutoff, kmin, krange, s);
     if (i + 1 < n) {
-      unsigned k2 = kbase[s] + lin_key(xs, gi + 1, D, cutoff, kmin, krange, s);
+      unsigned k2 = kbase[s] + lin_key<D>(xs, gi + 1, cutoff, kmin, krange, s);
       if (k2 < key) atomicOr(&flags[0], 1);
     }
     (void)rng;
@@ -352,8 +360,9 @@ __global__ void k_permute_real(Pool<R> A, Pool<R> B, const SysDesc *sd, const Bl
 
 // Qualification of candidate (boundary b, source particle i) of ghost g,
 // laid out boundary-major: abs(da) <= cutoff && da > 0.
-__global__ void k_ghost_flags(const double *const *x, const SysDesc *sd, const Bnd *bnd,
-                              const Blk *blk, const int *cand_off, int D, int *flags) {
+template <int D>
+__global__ void k_ghost_flags(XP x, const SysDesc *sd, const Bnd *bnd, const Blk *blk,
+                              const int *cand_off, int *flags) {
   const Blk b = blk[blockIdx.x];
   const int g = b.item;
   const int c = b.first + threadIdx.x;
@@ -364,6 +373,7 @@ __global__ void k_ghost_flags(const double *const *x, const SysDesc *sd, const B
   const Bnd B = bnd[G.bnd + bi];
   const int64_t gi = S.off + i;
   double da = (x[0][gi] - B.p[0]) * B.n[0];
+#pragma unroll
   for (int d = 1; d < D; ++d) da = da + (x[d][gi] - B.p[d]) * B.n[d];
   flags[cand_off[g] + c] = (fabs(da) <= G.gcut && da > 0.0) ? 1 : 0;
 }
@@ -371,9 +381,10 @@ __global__ void k_ghost_flags(const double *const *x, const SysDesc *sd, const B
 // Scatter qualifying candidates to generation order: mirrored position,
 // source index, boundary index, and the boundary whose normal slot k keeps
 // (normals are not permuted by the ghost sort, src/Sorting.jl:33-40).
-__global__ void k_ghost_scatter(const double *const *x, double *const *gx, int *gidx, int *gbnd,
-                                int *gnrm, const SysDesc *sd, const Bnd *bnd, const Blk *blk,
-                                const int *cand_off, const int *scan, int D, int *n_dev) {
+template <int D>
+__global__ void k_ghost_scatter(XP x, XPW gx, int *gidx, int *gbnd, int *gnrm, const SysDesc *sd,
+                                const Bnd *bnd, const Blk *blk, const int *cand_off,
+                                const int *scan, int *n_dev) {
   const Blk b = blk[blockIdx.x];
   const int g = b.item;
   const int c = b.first + threadIdx.x;
@@ -389,8 +400,10 @@ __global__ void k_ghost_scatter(const double *const *x, double *const *gx, int *
   const Bnd B = bnd[G.bnd + bi];
   const int64_t gi = S.off + i;
   double da = (x[0][gi] - B.p[0]) * B.n[0];
+#pragma unroll
   for (int d = 1; d < D; ++d) da = da + (x[d][gi] - B.p[d]) * B.n[d];
   const int64_t go = G.off + k;
+#pragma unroll
   for (int d = 0; d < D; ++d) gx[d][go] = x[d][gi] - (2.0 * da) * B.n[d];
   gidx[go] = i;
   gbnd[go] = bi;
@@ -399,10 +412,10 @@ __global__ void k_ghost_scatter(const double *const *x, double *const *gx, int *
 
 // Ghost sort permutation: generation scratch → pool (x, idx_original,
 // idx_boundary).
-__global__ void k_permute_ghost(const double *const *gxs, const int *gidx_s, const int *gbnd_s,
-                                double *const *x, int *gidx, int *gbnd, const SysDesc *sd,
-                                const int *n_dev, const Blk *blk, const int *bbase,
-                                const int *vals, const int *flags, int D) {
+template <int D>
+__global__ void k_permute_ghost(XP gxs, const int *gidx_s, const int *gbnd_s, XPW x, int *gidx,
+                                int *gbnd, const SysDesc *sd, const int *n_dev, const Blk *blk,
+                                const int *bbase, const int *vals, const int *flags) {
   const Blk b = blk[blockIdx.x];
   const int g = b.item;
   const int k = b.first + threadIdx.x;
@@ -411,6 +424,7 @@ __global__ void k_permute_ghost(const double *const *gxs, const int *gidx_s, con
   const int64_t dst = G.off + k;
   int64_t src = dst;
   if (flags[0] != 0) src = vals[bbase[g] + k];
+#pragma unroll
   for (int d = 0; d < D; ++d) x[d][dst] = gxs[d][src];
   gidx[dst] = gidx_s[src];
   gbnd[dst] = gbnd_s[src];
@@ -637,9 +651,11 @@ __global__ void k_grid_params(const int *inter_list, int n, const int2 *inter_sy
   grids[it] = G;
 }
 
-__device__ inline int cell_flat0(const double *const *x, int64_t gi, const Grid &G, double cell,
-                                 int D) {  // _cell_1idx − 1 (:197-206)
+template <int D>
+__device__ inline int cell_flat0(XP x, int64_t gi, const Grid &G,
+                                 double cell) {  // _cell_1idx − 1 (:197-206)
   long long flat = 0, stride = 1;
+#pragma unroll
   for (int d = D - 1; d >= 0; --d) {
     long long cd = (long long)floor((x[d][gi] - G.ming[d]) / cell);
     flat += cd * stride;
@@ -662,20 +678,24 @@ __global__ void k_cells_clear(int *cells, const CellArr *carr, const Grid *grids
 
 // First particle of each run of equal cells (forward pass :219-226); the
 // last run of a cell wins, as the serial loop's last write does.
-__global__ void k_cells_mark(const double *const *x, int *cells, const CellArr *carr,
-                             const Grid *grids, const double *cellsz, const SysDesc *sd,
-                             const int *n_dev, const Blk *blk, int D) {
+template <int D>
+__global__ void k_cells_mark(XP x, int *cells, const CellArr *carr, const Grid *grids,
+                             const double *cellsz, const SysDesc *sd, const int *n_dev,
+                             const Blk *blk) {
   const Blk b = blk[blockIdx.x];
   const CellArr C = carr[b.item];
   const int i = b.first + threadIdx.x;
   const int n = n_dev[C.sys];
   const Grid G = grids[C.grid];
-  if (i >= n || G.ncells <= 0) return;
+  if (G.ncells <= 0) return;  // uniform over the block
+  const bool live = i < n;
   const double cell = cellsz[C.grid];
   const int64_t gi = sd[C.sys].off + i;
-  const int c = cell_flat0(x, gi, G, cell, D);
-  const int prev = i > 0 ? cell_flat0(x, gi - 1, G, cell, D) : -1;
-  if (c != prev && c >= 0 && c < G.ncells) atomicMax(&cells[C.off + c], i + 1);
+  const int c = live ? cell_flat0<D>(x, gi, G, cell) : -1;
+  // the cell of particle i − 1 is the previous lane's (lane 0 computes it)
+  int prev = __shfl_up(c, 1, 64);
+  if ((threadIdx.x & 63) == 0) prev = (live && i > 0) ? cell_flat0<D>(x, gi - 1, G, cell) : -1;
+  if (live && c != prev && c >= 0 && c < G.ncells) atomicMax(&cells[C.off + c], i + 1);
 }
 
 // Backward pass (:231-235): every empty cell takes the first non-empty start
@@ -716,13 +736,28 @@ __global__ void k_fill_local(int *cells, const CellArr *carr, const Grid *grids,
 }
 
 // carry[blk] = first non-zero start in the blocks to the right of blk
-// (blocks of one cell array are consecutive in the table).
+// (blocks of one cell array are consecutive in the table).  One workgroup
+// per cell array: chunk aggregates, a suffix scan over the chunks, then each
+// chunk's carries right to left.
 __global__ void k_fill_carry(const int2 *carr_blocks, int ncarr, const int *bagg, int *carry) {
-  int q = blockIdx.x * blockDim.x + threadIdx.x;
-  if (q >= ncarr) return;
-  const int2 r = carr_blocks[q];  // [first block, end block)
-  int c = 0;
-  for (int k = r.y - 1; k >= r.x; --k) {
+  __shared__ int s[GR_BLOCK];
+  const int2 r = carr_blocks[blockIdx.x];  // [first block, end block)
+  const int t = threadIdx.x;
+  const int per = (r.y - r.x + GR_BLOCK - 1) / GR_BLOCK;
+  const int k0 = r.x + t * per;
+  const int k1 = k0 + per < r.y ? k0 + per : r.y;
+  int agg = 0;
+  for (int k = k1 - 1; k >= k0; --k) agg = first_nz(bagg[k], agg);
+  s[t] = agg;
+  __syncthreads();
+  for (int o = 1; o < GR_BLOCK; o <<= 1) {
+    const int v = (t + o < GR_BLOCK) ? s[t + o] : 0;
+    __syncthreads();
+    s[t] = first_nz(s[t], v);
+    __syncthreads();
+  }
+  int c = (t + 1 < GR_BLOCK) ? s[t + 1] : 0;
+  for (int k = k1 - 1; k >= k0; --k) {
     carry[k] = c;
     c = first_nz(bagg[k], c);
   }
@@ -1039,12 +1074,16 @@ __global__ void k_fill_i32(int *p, int v, long long n) {
 
 static inline unsigned nb_of(long long n, int per) { return (unsigned)((n + per - 1) / per); }
 
-void launch_key_minmax(hipStream_t s, const double *const *xs, const SysDesc *sd, const int *n,
-                       const StepSys *st, const Blk *blk, int nblk, int D, int *kmin, int *kmax,
-                       long long *bbmin, long long *bbmax, int with_keys) {
-  if (nblk > 0)
-    k_key_minmax<<<nblk, GR_BLOCK, 0, s>>>(xs, sd, n, st, blk, D, kmin, kmax, bbmin, bbmax,
-                                           with_keys);
+void launch_key_minmax(hipStream_t s, XP xs, const SysDesc *sd, const int *n, const StepSys *st,
+                       const Blk *blk, int nblk, int D, int *kmin, int *kmax, long long *bbmin,
+                       long long *bbmax, int with_keys) {
+  if (nblk <= 0) return;
+  if (D == 2)
+    k_key_minmax<2><<<nblk, GR_BLOCK, 0, s>>>(xs, sd, n, st, blk, kmin, kmax, bbmin, bbmax,
+                                              with_keys);
+  else
+    k_key_minmax<3><<<nblk, GR_BLOCK, 0, s>>>(xs, sd, n, st, blk, kmin, kmax, bbmin, bbmax,
+                                              with_keys);
 }
 void launch_minmax_reset(hipStream_t s, const int *list, int nlist, int *kmin, int *kmax,
                          long long *bbmin, long long *bbmax) {
@@ -1057,13 +1096,17 @@ void launch_key_base(hipStream_t s, const int *list, int nlist, int D, int pad, 
                      int *flags) {
   k_key_base<<<1, 64, 0, s>>>(list, nlist, D, pad, n, kmin, kmax, kbase, krange, flags);
 }
-void launch_key_write(hipStream_t s, const double *const *xs, const SysDesc *sd, const int *n,
-                      const StepSys *st, const Blk *blk, int nblk, const int *bbase, int D,
-                      const int *kmin, const unsigned *krange, const unsigned *kbase,
-                      unsigned *keys, int *vals, int *flags) {
-  if (nblk > 0)
-    k_key_write<<<nblk, GR_BLOCK, 0, s>>>(xs, sd, n, st, blk, bbase, D, kmin, krange, kbase, keys,
-                                          vals, flags);
+void launch_key_write(hipStream_t s, XP xs, const SysDesc *sd, const int *n, const StepSys *st,
+                      const Blk *blk, int nblk, const int *bbase, int D, const int *kmin,
+                      const unsigned *krange, const unsigned *kbase, unsigned *keys, int *vals,
+                      int *flags) {
+  if (nblk <= 0) return;
+  if (D == 2)
+    k_key_write<2><<<nblk, GR_BLOCK, 0, s>>>(xs, sd, n, st, blk, bbase, kmin, krange, kbase, keys,
+                                             vals, flags);
+  else
+    k_key_write<3><<<nblk, GR_BLOCK, 0, s>>>(xs, sd, n, st, blk, bbase, kmin, krange, kbase, keys,
+                                             vals, flags);
 }
 void launch_radix_sort(hipStream_t s, unsigned *keys, int *vals, unsigned *keys2, int *vals2,
                        unsigned *hist, int N, const int *flags) {
@@ -1103,9 +1146,13 @@ template void launch_permute_real<double>(hipStream_t, const Pool<double> &,
                                           const Pool<double> &, const SysDesc *, const Blk *,
                                           int, const int *, const int *, const int *, int);
 
-void launch_ghost_flags(hipStream_t s, const double *const *x, const SysDesc *sd, const Bnd *bnd,
-                        const Blk *blk, int nblk, const int *cand_off, int D, int *flags) {
-  if (nblk > 0) k_ghost_flags<<<nblk, GR_BLOCK, 0, s>>>(x, sd, bnd, blk, cand_off, D, flags);
+void launch_ghost_flags(hipStream_t s, XP x, const SysDesc *sd, const Bnd *bnd, const Blk *blk,
+                        int nblk, const int *cand_off, int D, int *flags) {
+  if (nblk <= 0) return;
+  if (D == 2)
+    k_ghost_flags<2><<<nblk, GR_BLOCK, 0, s>>>(x, sd, bnd, blk, cand_off, flags);
+  else
+    k_ghost_flags<3><<<nblk, GR_BLOCK, 0, s>>>(x, sd, bnd, blk, cand_off, flags);
 }
 void launch_scan(hipStream_t s, const int *in, int *out, int *bsum, int N) {
   const int nb = (int)nb_of(N > 0 ? N : 1, SCAN_TILE);
@@ -1114,20 +1161,27 @@ void launch_scan(hipStream_t s, const int *in, int *out, int *bsum, int N) {
   k_scan_add<<<nb, 256, 0, s>>>(out, bsum, N);
 }
 int scan_block_count(int N) { return (int)nb_of(N > 0 ? N : 1, SCAN_TILE); }
-void launch_ghost_scatter(hipStream_t s, const double *const *x, double *const *gx, int *gidx,
-                          int *gbnd, int *gnrm, const SysDesc *sd, const Bnd *bnd, const Blk *blk,
-                          int nblk, const int *cand_off, const int *scan, int D, int *n) {
-  if (nblk > 0)
-    k_ghost_scatter<<<nblk, GR_BLOCK, 0, s>>>(x, gx, gidx, gbnd, gnrm, sd, bnd, blk, cand_off,
-                                              scan, D, n);
+void launch_ghost_scatter(hipStream_t s, XP x, XPW gx, int *gidx, int *gbnd, int *gnrm,
+                          const SysDesc *sd, const Bnd *bnd, const Blk *blk, int nblk,
+                          const int *cand_off, const int *scan, int D, int *n) {
+  if (nblk <= 0) return;
+  if (D == 2)
+    k_ghost_scatter<2><<<nblk, GR_BLOCK, 0, s>>>(x, gx, gidx, gbnd, gnrm, sd, bnd, blk, cand_off,
+                                                 scan, n);
+  else
+    k_ghost_scatter<3><<<nblk, GR_BLOCK, 0, s>>>(x, gx, gidx, gbnd, gnrm, sd, bnd, blk, cand_off,
+                                                 scan, n);
 }
-void launch_permute_ghost(hipStream_t s, const double *const *gxs, const int *gidx_s,
-                          const int *gbnd_s, double *const *x, int *gidx, int *gbnd,
-                          const SysDesc *sd, const int *n, const Blk *blk, int nblk,
-                          const int *bbase, const int *vals, const int *flags, int D) {
-  if (nblk > 0)
-    k_permute_ghost<<<nblk, GR_BLOCK, 0, s>>>(gxs, gidx_s, gbnd_s, x, gidx, gbnd, sd, n, blk,
-                                              bbase, vals, flags, D);
+void launch_permute_ghost(hipStream_t s, XP gxs, const int *gidx_s, const int *gbnd_s, XPW x,
+                          int *gidx, int *gbnd, const SysDesc *sd, const int *n, const Blk *blk,
+                          int nblk, const int *bbase, const int *vals, const int *flags, int D) {
+  if (nblk <= 0) return;
+  if (D == 2)
+    k_permute_ghost<2><<<nblk, GR_BLOCK, 0, s>>>(gxs, gidx_s, gbnd_s, x, gidx, gbnd, sd, n, blk,
+                                                 bbase, vals, flags);
+  else
+    k_permute_ghost<3><<<nblk, GR_BLOCK, 0, s>>>(gxs, gidx_s, gbnd_s, x, gidx, gbnd, sd, n, blk,
+                                                 bbase, vals, flags);
 }
 template <typename R>
 void launch_ghost_kin(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const Bnd *bnd,
@@ -1163,17 +1217,21 @@ void launch_grid_params(hipStream_t s, const int *inter_list, int n, const int2 
     k_grid_params<<<nb_of(n, 64), 64, 0, s>>>(inter_list, n, inter_sys, cellsz, bbmin, bbmax,
                                               n_dev, grids, inter_carr, carr, D, flags);
 }
-void launch_cells_build(hipStream_t s, const double *const *x, int *cells, const CellArr *carr,
+void launch_cells_build(hipStream_t s, XP x, int *cells, const CellArr *carr,
                         const Grid *grids, const double *cellsz, const SysDesc *sd,
                         const int *n_dev, const Blk *cblk, int ncblk, const Blk *mblk,
                         int nmblk, const int2 *carr_blocks, int ncarr, int *bagg, int *carry,
                         int D) {
   if (ncblk > 0) k_cells_clear<<<ncblk, GR_BLOCK, 0, s>>>(cells, carr, grids, n_dev, cblk);
-  if (nmblk > 0)
-    k_cells_mark<<<nmblk, GR_BLOCK, 0, s>>>(x, cells, carr, grids, cellsz, sd, n_dev, mblk, D);
+  if (nmblk > 0) {
+    if (D == 2)
+      k_cells_mark<2><<<nmblk, GR_BLOCK, 0, s>>>(x, cells, carr, grids, cellsz, sd, n_dev, mblk);
+    else
+      k_cells_mark<3><<<nmblk, GR_BLOCK, 0, s>>>(x, cells, carr, grids, cellsz, sd, n_dev, mblk);
+  }
   if (ncblk > 0) {
     k_fill_local<<<ncblk, GR_BLOCK, 0, s>>>(cells, carr, grids, cblk, bagg);
-    k_fill_carry<<<nb_of(ncarr, 64), 64, 0, s>>>(carr_blocks, ncarr, bagg, carry);
+    if (ncarr > 0) k_fill_carry<<<ncarr, GR_BLOCK, 0, s>>>(carr_blocks, ncarr, bagg, carry);
     k_fill_apply<<<ncblk, GR_BLOCK, 0, s>>>(cells, carr, grids, cblk, carry);
   }
 }

@@ -36,7 +36,7 @@ struct NlistArgs {
   int *lidx;
 };
 
-void launch_key_minmax(hipStream_t s, const double *const *xs, const SysDesc *sd, const int *n,
+void launch_key_minmax(hipStream_t s, XP xs, const SysDesc *sd, const int *n,
                        const StepSys *st, const Blk *blk, int nblk, int D, int *kmin, int *kmax,
                        long long *bbmin, long long *bbmax, int with_keys);
 void launch_minmax_reset(hipStream_t s, const int *list, int nlist, int *kmin, int *kmax,
@@ -44,7 +44,7 @@ void launch_minmax_reset(hipStream_t s, const int *list, int nlist, int *kmin, i
 void launch_key_base(hipStream_t s, const int *list, int nlist, int D, int pad, const int *n,
                      const int *kmin, const int *kmax, unsigned *kbase, unsigned *krange,
                      int *flags);
-void launch_key_write(hipStream_t s, const double *const *xs, const SysDesc *sd, const int *n,
+void launch_key_write(hipStream_t s, XP xs, const SysDesc *sd, const int *n,
                       const StepSys *st, const Blk *blk, int nblk, const int *bbase, int D,
                       const int *kmin, const unsigned *krange, const unsigned *kbase,
                       unsigned *keys, int *vals, int *flags);
@@ -55,15 +55,15 @@ template <typename R>
 void launch_permute_real(hipStream_t s, const Pool<R> &A, const Pool<R> &B, const SysDesc *sd,
                          const Blk *blk, int nblk, const int *bbase, const int *vals,
                          const int *flags, int D);
-void launch_ghost_flags(hipStream_t s, const double *const *x, const SysDesc *sd, const Bnd *bnd,
+void launch_ghost_flags(hipStream_t s, XP x, const SysDesc *sd, const Bnd *bnd,
                         const Blk *blk, int nblk, const int *cand_off, int D, int *flags);
 void launch_scan(hipStream_t s, const int *in, int *out, int *bsum, int N);
 int scan_block_count(int N);
-void launch_ghost_scatter(hipStream_t s, const double *const *x, double *const *gx, int *gidx,
+void launch_ghost_scatter(hipStream_t s, XP x, XPW gx, int *gidx,
                           int *gbnd, int *gnrm, const SysDesc *sd, const Bnd *bnd, const Blk *blk,
                           int nblk, const int *cand_off, const int *scan, int D, int *n);
-void launch_permute_ghost(hipStream_t s, const double *const *gxs, const int *gidx_s,
-                          const int *gbnd_s, double *const *x, int *gidx, int *gbnd,
+void launch_permute_ghost(hipStream_t s, XP gxs, const int *gidx_s,
+                          const int *gbnd_s, XPW x, int *gidx, int *gbnd,
                           const SysDesc *sd, const int *n, const Blk *blk, int nblk,
                           const int *bbase, const int *vals, const int *flags, int D);
 template <typename R>
@@ -76,7 +76,7 @@ void launch_grid_params(hipStream_t s, const int *inter_list, int n, const int2 
                         const double *cellsz, const long long *bbmin, const long long *bbmax,
                         const int *n_dev, Grid *grids, const int2 *inter_carr,
                         const CellArr *carr, int D, int *flags);
-void launch_cells_build(hipStream_t s, const double *const *x, int *cells, const CellArr *carr,
+void launch_cells_build(hipStream_t s, XP x, int *cells, const CellArr *carr,
                         const Grid *grids, const double *cellsz, const SysDesc *sd,
                         const int *n_dev, const Blk *cblk, int ncblk, const Blk *mblk,
                         int nmblk, const int2 *carr_blocks, int ncarr, int *bagg, int *carry,

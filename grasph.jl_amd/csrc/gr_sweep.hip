@@ -11,11 +11,10 @@
 // atomics; the result is independent of scheduling.
 //
 // Two passes per step:
-//  * k_nlist: neighbour search.  A wavefront walks the union of its lanes'
-//    candidate ranges with a wave-uniform index (candidate positions come
-//    through scalar loads, one per wavefront) and each lane keeps the
-//    candidates in its own CSR ranges with r² < (2h)², evaluated in FP64 in
-//    the reference's order, so the accepted pair set equals the reference's.
+//  * k_nlist: neighbour search.  A wavefront stages the candidates of each
+//    strip in LDS with coalesced loads and each lane keeps the candidates
+//    in its own CSR ranges with r² < (2h)², evaluated in FP64 in the
+//    reference's order, so the accepted pair set equals the reference's.
 //  * k_sweep_list: the functors over each lane's accepted partners, once per
 //    stage, so every lane does useful arithmetic on every iteration.
 // k_sweep walks the cells directly (fine-grained sweep! calls, and targets
@@ -86,6 +85,24 @@ __device__ inline R dwdq(int kk, R C, R q) {
   return ((-C * (R)10) * q) * (t * t * t);
 }
 
+// Reciprocals.  FP32 pair math uses the hardware reciprocal and reciprocal
+// square root (1 ulp) and reuses each reciprocal within a pair; FP64 keeps
+// correctly rounded division and square root.
+template <typename R>
+struct Fast;
+template <>
+struct Fast<float> {
+  static constexpr bool on = true;
+  __device__ static inline float rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+  __device__ static inline float rsq(float x) { return __builtin_amdgcn_rsqf(x); }
+};
+template <>
+struct Fast<double> {
+  static constexpr bool on = false;
+  __device__ static inline double rcp(double x) { return 1.0 / x; }
+  __device__ static inline double rsq(double x) { return 1.0 / sqrt(x); }
+};
+
 template <typename R>
 __device__ inline R jmin0(R v) {  // Julia min(v, 0)
   return v < (R)0 ? v : (isnan(v) ? v : (R)0);
@@ -103,10 +120,11 @@ __device__ inline int xcd_block(int nblk) {
 // Per-source constants in the arithmetic type.
 template <typename R>
 struct SrcK {
-  R h, norm, hf, alpha, beta, mc, h2, m_q, psi_coef, eps, ljcut, ljc2;
+  R h, ih, norm, hf, alpha, beta, mc, h2, m_q, psi_coef, eps, ljcut, ljc2;
   int sigma, has_delta, has_eps;
   __device__ inline void load(const SrcDesc &Q) {
     h = (R)Q.h;
+    ih = (R)(1.0 / Q.h);
     norm = (R)Q.norm;
     hf = (R)Q.hf;
     alpha = (R)Q.alpha;
@@ -128,7 +146,7 @@ struct SrcK {
 template <int D, typename R, int FAM>
 struct Gather {
   double xi[D];
-  R vi[D], ri, pi;
+  R vi[D], ri, pi, iri;
   R si[GR_MAXNS];
   int NS;
   Acc<R> adv[D], adr;
@@ -141,6 +159,7 @@ struct Gather {
       vi[d] = P.vs[d][gi];
     }
     ri = P.rhos[gi];
+    iri = Fast<R>::rcp(ri);
     pi = 0;
     NS = T.ns;
     if (FAM == FAM_FORCE) {
@@ -163,11 +182,18 @@ struct Gather {
   // and r² as evaluated by _pair_self!/_pair_coupled! (src/Interaction.jl:546-553).
   __device__ __forceinline__ void pair(const Pool<R> P, const SrcDesc &Q, const SrcK<R> &K,
                                        int64_t gj, const double *dx, double r2, double da) {
+    constexpr bool FA = Fast<R>::on;
     R dxr[D], gx[D];
     const R r2r = (R)r2;
-    const R r = sqrt(r2r);
-    const R qq = r / K.h;
-    const R g = dwdq<R>(Q.kk, K.norm, qq) / (r * K.h);
+    R r, g, ir = 0;
+    if (FA) {
+      ir = Fast<R>::rsq(r2r);
+      r = r2r * ir;
+      g = dwdq<R>(Q.kk, K.norm, r * K.ih) * (ir * K.ih);
+    } else {
+      r = sqrt(r2r);
+      g = dwdq<R>(Q.kk, K.norm, r / K.h) / (r * K.h);
+    }
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       dxr[d] = (R)dx[d];
@@ -195,11 +221,12 @@ struct Gather {
       for (int d = 0; d < D; ++d) vj[d] = P.vs[d][gj];
       rj = P.rhos[gj];
     }
+    const R irj = (FA && Q.pk == PK_DYNAMIC) ? iri : Fast<R>::rcp(rj);
     if (FAM == FAM_STRAIN) {  // strain_rate_tensor / vorticity_tensor
       R dv[D];
 #pragma unroll
       for (int d = 0; d < D; ++d) dv[d] = vj[d] - vi[d];
-      const R f = m / rj;
+      const R f = FA ? m * irj : m / rj;
       if (D == 2) {
         asr[0].add(dv[0] * gx[0] * f);
         asr[1].add(dv[1] * gx[1] * f);
@@ -233,11 +260,13 @@ struct Gather {
 #pragma unroll
     for (int d = 1; d < D; ++d) vr = vr + dxr[d] * dv[d];
     vr = jmin0(vr);
-    const R muv = K.hf * vr / (r2r + K.h2);
-    const R piv = (K.beta * muv - K.alpha * K.mc) * muv / ((R)0.5 * ri * rj);
+    const R muv = FA ? K.hf * vr * Fast<R>::rcp(r2r + K.h2) : K.hf * vr / (r2r + K.h2);
+    const R piv = FA ? (K.beta * muv - K.alpha * K.mc) * muv * ((R)2 * iri * irj)
+                     : (K.beta * muv - K.alpha * K.mc) * muv / ((R)0.5 * ri * rj);
     if (Q.pk == PK_STATIC) {  // lennard_jones + artificial viscosity
-      const R lr = K.ljcut / r;
-      const R f = (r < K.ljcut ? (R)1 : (R)0) * (pow_i(lr, 12) - pow_i(lr, 6)) / r2r;
+      const R lr = FA ? K.ljcut * ir : K.ljcut / r;
+      const R lj = (r < K.ljcut ? (R)1 : (R)0) * (pow_i(lr, 12) - pow_i(lr, 6));
+      const R f = FA ? lj * (ir * ir) : lj / r2r;
       const R rf = K.ljc2 * f;
       const R t = -m * piv;
 #pragma unroll
@@ -252,18 +281,27 @@ struct Gather {
       R dxg = dxr[0] * gx[0];
 #pragma unroll
       for (int d = 1; d < D; ++d) dxg = dxg + dxr[d] * gx[d];
-      psi = K.psi_coef * (ri - rj) * dxg / r2r;
+      psi = FA ? K.psi_coef * (ri - rj) * dxg * (ir * ir) : K.psi_coef * (ri - rj) * dxg / r2r;
     }
     if (Q.pfn == GR_PFN_FLUID) {
       const R pj = (Q.pk == PK_DYNAMIC) ? pi : P.p[gj];
       R dh;
-      if (K.sigma == 1)
-        dh = -(pi + pj) / (ri * rj);
-      else
-        dh = -(pi / (ri * ri) + pj / (rj * rj));
+      if (FA) {
+        if (K.sigma == 1)
+          dh = -(pi + pj) * (iri * irj);
+        else
+          dh = -(pi * (iri * iri) + pj * (irj * irj));
+      } else {
+        if (K.sigma == 1)
+          dh = -(pi + pj) / (ri * rj);
+        else
+          dh = -(pi / (ri * ri) + pj / (rj * rj));
+      }
       if (Q.pk == PK_TWO_SIDED) {
         R ast = (R)0;
-        if (K.has_eps) ast = K.eps * (-(fabs(pi) + fabs(pj)) / (ri * rj));
+        if (K.has_eps)
+          ast = FA ? K.eps * (-(fabs(pi) + fabs(pj)) * (iri * irj))
+                   : K.eps * (-(fabs(pi) + fabs(pj)) / (ri * rj));
         const R t = dh + ast - piv;
 #pragma unroll
         for (int d = 0; d < D; ++d) adv[d].add(m * (t * gx[d]));
@@ -272,8 +310,8 @@ struct Gather {
 #pragma unroll
         for (int d = 0; d < D; ++d) adv[d].add(t * gx[d]);
       }
-      const R coef = (K.sigma == 1) ? ri / rj : (R)1;
-      adr.add(m * (dr * coef + psi / rj));
+      const R coef = (K.sigma == 1) ? (FA ? ri * irj : ri / rj) : (R)1;
+      adr.add(m * (dr * coef + (FA ? psi * irj : psi / rj)));
     } else {  // cauchy_stress_force
       R sj[GR_MAXNS];
       if (Q.pk == PK_DYNAMIC) {
@@ -283,7 +321,7 @@ struct Gather {
 #pragma unroll
         for (int k = 0; k < GR_MAXNS; ++k) sj[k] = k < NS ? P.stress_s[k][gj] : (R)0;
       }
-      const R ci = (R)1 / (ri * ri), cj = (R)1 / (rj * rj);
+      const R ci = FA ? iri * iri : (R)1 / (ri * ri), cj = FA ? irj * irj : (R)1 / (rj * rj);
       R hv[D];
       if (D == 2) {
         R a0, a1;
@@ -306,7 +344,7 @@ struct Gather {
       }
 #pragma unroll
       for (int d = 0; d < D; ++d) adv[d].add(m * (hv[d] - piv * gx[d]));
-      adr.add(m * (dr + psi / rj));
+      adr.add(m * (dr + (FA ? psi * irj : psi / rj)));
     }
   }
 
@@ -461,16 +499,26 @@ __device__ inline int wave_max_i(int v) {
 // k-th partner (source-local index) of target i, lcnt[L.cnt_off + q·L.cap + i]
 // the number from source q.  Counts above L.maxnb mean "overflow" (the
 // physics pass then walks the cells directly).
+//
+// One wavefront per workgroup.  For each strip, lanes whose candidate ranges
+// start within 2·wmax of the lowest pending start form a group (targets in
+// one cell column); the group's union of ranges is staged in LDS, NL_WIN
+// candidates at a time, with coalesced loads, and every lane walks its own
+// range from LDS in ascending order, so it sees its candidates in the cell
+// walk's order and tests each once with the same FP64 expression.
+#define NL_WIN 128
 template <int D, typename R>
-__global__ void __launch_bounds__(GR_BLOCK)
+__global__ void __launch_bounds__(64)
     k_nlist(Pool<R> P, const SysDesc *__restrict__ sd, const int *__restrict__ n_dev,
             const LTask *__restrict__ lt, const SrcDesc *__restrict__ srcs,
             const Grid *__restrict__ grids, const CellArr *__restrict__ carr,
             const int *__restrict__ cells, const Blk *__restrict__ blk, int nblk, int *lcnt,
             int *lidx) {
+  __shared__ double sx[D][NL_WIN];
   const Blk b = blk[xcd_block(nblk)];
   const LTask L = lt[b.item];
-  const int i = b.first + threadIdx.x;
+  const int lane = threadIdx.x;
+  const int i = b.first + lane;
   const SysDesc S = sd[L.sys];
   const bool live = i < n_dev[L.sys];
   const int64_t gi = S.off + (live ? i : 0);
@@ -486,7 +534,8 @@ __global__ void __launch_bounds__(GR_BLOCK)
     if (G.ncells > 0) {
       const int *cs = cells + carr[Q.carr].off;
       const int64_t poff = sd[Q.partner].off;
-      const bool self = Q.pk == PK_SELF;
+      const int self_j = Q.pk == PK_SELF ? i : -1;
+      const double c2 = Q.cutoff2;
       int c[D];
       target_cell<D>(xi, G, Q.cell, c);
       for (int st = 0; st < ((D == 3) ? 9 : 3); ++st) {
@@ -496,55 +545,46 @@ __global__ void __launch_bounds__(GR_BLOCK)
           j0 = cs[base] - 1;
           j1 = cs[base + 3] - 1;
         }
-        const int lo = __builtin_amdgcn_readfirstlane(wave_min_i(j0 < j1 ? j0 : INT_MAX));
-        const int hi = __builtin_amdgcn_readfirstlane(wave_max_i(j0 < j1 ? j1 : INT_MIN));
         const int wmax = __builtin_amdgcn_readfirstlane(wave_max_i(j1 - j0));
-        if (hi <= lo) continue;
-        if (hi - lo <= 3 * wmax) {
-          // wave-uniform walk of the union of the lanes' ranges
-          for (int j = lo; j < hi; ++j) {
-            const int64_t gj = poff + j;
-            double xj[D];
+        bool pend = j0 < j1;
+        while (true) {
+          const int lo = __builtin_amdgcn_readfirstlane(wave_min_i(pend ? j0 : INT_MAX));
+          if (lo == INT_MAX) break;
+          const bool mem = pend && j0 - lo <= 2 * wmax;
+          const int hi = __builtin_amdgcn_readfirstlane(wave_max_i(mem ? j1 : INT_MIN));
+          for (int w0 = lo; w0 < hi; w0 += NL_WIN) {
+            const int w1 = hi - w0 < NL_WIN ? hi : w0 + NL_WIN;
+            __syncthreads();  // the previous window has been read
+            for (int t = lane; t < w1 - w0; t += 64) {
 #pragma unroll
-            for (int d = 0; d < D; ++d) xj[d] = P.x[d][gj];
-            if (j >= j0 && j < j1 && !(self && j == i)) {
+              for (int d = 0; d < D; ++d) sx[d][t] = P.x[d][poff + w0 + t];
+            }
+            __syncthreads();
+            const int a = mem ? (j0 > w0 ? j0 : w0) : 0;
+            const int e = mem ? (j1 < w1 ? j1 : w1) : 0;
+            for (int j = a; j < e; ++j) {
+              const int t = j - w0;
               double dx[D];
 #pragma unroll
-              for (int d = 0; d < D; ++d) dx[d] = xi[d] - xj[d];
+              for (int d = 0; d < D; ++d) dx[d] = xi[d] - sx[d][t];
               double r2 = dx[0] * dx[0];
 #pragma unroll
               for (int d = 1; d < D; ++d) r2 = r2 + dx[d] * dx[d];
-              if (r2 < Q.cutoff2) {
+              if (r2 < c2 && j != self_j) {
                 if (cnt < L.maxnb) my[(int64_t)cnt * L.cap] = j;
                 ++cnt;
                 ++cq;
               }
             }
           }
-        } else {
-          // lanes in distant columns: each lane walks its own range
-          for (int j = j0; j < j1; ++j) {
-            if (self && j == i) continue;
-            const int64_t gj = poff + j;
-            double dx[D];
-#pragma unroll
-            for (int d = 0; d < D; ++d) dx[d] = xi[d] - P.x[d][gj];
-            double r2 = dx[0] * dx[0];
-#pragma unroll
-            for (int d = 1; d < D; ++d) r2 = r2 + dx[d] * dx[d];
-            if (r2 < Q.cutoff2) {
-              if (cnt < L.maxnb) my[(int64_t)cnt * L.cap] = j;
-              ++cnt;
-              ++cq;
-            }
-          }
+          if (mem) pend = false;
         }
       }
     }
     if (live) lcnt[L.cnt_off + (int64_t)(q - L.src_begin) * L.cap + i] = cq;
   }
-  if (live && cnt > L.maxnb) lcnt[L.cnt_off + (int64_t)(L.src_end - L.src_begin) * L.cap + i] = 1;
-  else if (live) lcnt[L.cnt_off + (int64_t)(L.src_end - L.src_begin) * L.cap + i] = 0;
+  if (live)
+    lcnt[L.cnt_off + (int64_t)(L.src_end - L.src_begin) * L.cap + i] = cnt > L.maxnb ? 1 : 0;
 }
 
 // Functors over the accepted partners of each target (one stage).
@@ -653,11 +693,11 @@ template <typename R>
 void launch_nlist(hipStream_t s, const Pool<R> &P, const NlistArgs &a, int D) {
   if (a.nblk <= 0) return;
   if (D == 2)
-    k_nlist<2, R><<<a.nblk, GR_BLOCK, 0, s>>>(P, a.sd, a.n, a.lt, a.srcs, a.grids, a.carr,
-                                              a.cells, a.blk, a.nblk, a.lcnt, a.lidx);
+    k_nlist<2, R><<<a.nblk, 64, 0, s>>>(P, a.sd, a.n, a.lt, a.srcs, a.grids, a.carr, a.cells,
+                                        a.blk, a.nblk, a.lcnt, a.lidx);
   else
-    k_nlist<3, R><<<a.nblk, GR_BLOCK, 0, s>>>(P, a.sd, a.n, a.lt, a.srcs, a.grids, a.carr,
-                                              a.cells, a.blk, a.nblk, a.lcnt, a.lidx);
+    k_nlist<3, R><<<a.nblk, 64, 0, s>>>(P, a.sd, a.n, a.lt, a.srcs, a.grids, a.carr, a.cells,
+                                        a.blk, a.nblk, a.lcnt, a.lidx);
 }
 template void launch_nlist<float>(hipStream_t, const Pool<float> &, const NlistArgs &, int);
 template void launch_nlist<double>(hipStream_t, const Pool<double> &, const NlistArgs &, int);
