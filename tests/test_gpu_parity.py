@@ -478,3 +478,26 @@ def test_neighbour_lists_equal_cell_walk(monkeypatch, name, builder, kw, D, prec
     for mode in ("walk", "overflow"):
         for key, ref in res["list"].items():
             assert np.array_equal(res[mode][key], ref, equal_nan=True), f"{mode} {key}"
+
+
+# ---------------------------------------------------------------------------
+# Golden fixtures (tests/golden/make_golden.py): the device in FP64 gives the
+# fixture's permutation, CSR and counts bit for bit and its one-step state to
+# rounding.
+# ---------------------------------------------------------------------------
+
+
+@pytest.mark.parametrize("D", [2, 3])
+def test_device_reproduces_golden(D):
+    import sys
+    from pathlib import Path
+    here = Path(__file__).resolve().parent
+    sys.path.insert(0, str(here / "golden"))
+    import make_golden as MG
+    with np.load(here / "golden" / f"golden_{D}d.npz", allow_pickle=False) as z:
+        g = {k: z[k] for k in z.files}
+    dev = MG.make(D, scene_factory=lambda nd: M.Session(nd, "f64").scene)
+    for k in ("perm", "mingridx", "ngridx", "cell_start", "counts"):
+        assert np.array_equal(dev[k], g[k]), k
+    for k in ("step_x", "step_v", "step_rho", "step_p", "step_dvdt", "step_drhodt"):
+        assert rel(dev[k], g[k]) < 1e-12, k

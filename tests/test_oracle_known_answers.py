@@ -447,6 +447,63 @@ def test_ep_hydrostatic_tension_is_nan():
 
 
 # ---------------------------------------------------------------------------
+# Visco-plastic Mohr-Coulomb updater (src/StateUpdaters.jl:144-189): closed
+# forms.  No reference test covers it; these pin the rate magnitude (factor 2
+# on the shear terms, which components are shear), the 2(c + p·tanφ) scale,
+# the pressure shift and the apex zeroing.
+# ---------------------------------------------------------------------------
+
+
+def _vpmc_call(D, ns, rho, sr, phi, coh, eos="tait"):
+    upd = M.TaitEOSUpdater(1000.0) if eos == "tait" else M.LinearEOSUpdater(1000.0)
+    ps = M.StressParticleSystem("v", 1, D, ns, 1.0, 10.0,
+                                state_updater=(M.ViscoPlasticMCStressUpdater(upd, phi, coh),))
+    ps.rho[0] = rho
+    ps.strain_rate[0] = sr
+    sc = oracle_scene(D)
+    sc.upload(ps)
+    sc.be.update_state(sc.system(ps), 1, 1e-4)
+    sc.download(ps)
+    return ps
+
+
+def _tait(rho, rho0=1000.0, c=10.0, gamma=7.0):
+    return rho0 * c * c / gamma * ((rho / rho0) ** 7 - 1.0)
+
+
+def test_vpmc_pure_shear_3d():
+    g, phi, coh = 3.0, 0.5, 20.0
+    ps = _vpmc_call(3, 6, 1010.0, (0, 0, 0, g, 0, 0), phi, coh)
+    p = _tait(1010.0)
+    assert ps.p[0] == pytest.approx(p, rel=1e-14)
+    tau = 2 * (coh + p * math.tan(phi)) * g / math.sqrt(2 * g * g)
+    assert np.allclose(ps.stress[0], (-p, -p, -p, tau, 0, 0), rtol=1e-14, atol=0)
+
+
+def test_vpmc_plane_rate_2d():
+    a, b, phi, coh = 2.0, 1.5, 0.6, 5.0
+    ps = _vpmc_call(2, 4, 1005.0, (a, -a, 0.0, b), phi, coh)
+    p = _tait(1005.0)
+    mag = math.sqrt(a * a + a * a + 2 * b * b)
+    sc = 2 * (coh + p * math.tan(phi)) / mag
+    expect = (sc * a - p, -sc * a - p, -p, sc * b)
+    assert np.allclose(ps.stress[0], expect, rtol=1e-14, atol=0)
+
+
+def test_vpmc_zero_rate_is_hydrostatic():
+    ps = _vpmc_call(3, 6, 1010.0, (0, 0, 0, 0, 0, 0), 0.5, 20.0, eos="linear")
+    p = 100.0 * 10.0  # c²(ρ − ρ0)
+    assert ps.p[0] == pytest.approx(p, rel=1e-15)
+    assert np.array_equal(ps.stress[0], np.array([-p, -p, -p, 0.0, 0.0, 0.0]))
+
+
+def test_vpmc_tension_beyond_apex_is_zeroed():
+    ps = _vpmc_call(3, 6, 990.0, (0, 0, 0, 1.0, 0, 0), 0.5, 0.0)
+    assert ps.p[0] < 0
+    assert np.all(ps.stress[0] == 0.0)
+
+
+# ---------------------------------------------------------------------------
 # Ghosts (test/test_ghost_particles.jl:112-193, test_ghost_copier.jl:186-305)
 # ---------------------------------------------------------------------------
 
