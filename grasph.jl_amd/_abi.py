@@ -86,6 +86,7 @@ _SIGS = {
 # entry points only the device library has
 _GPU_SIGS = {
     "integrator_set_dt": [_i, _d],
+    "integrator_release": [_i],
     "step_many": [_i, _ip, _i],
     "set_graph_mode": [_i],
     "set_timing": [_i],
@@ -273,6 +274,9 @@ class CBackend:
         return a
 
     # ---- device-only ------------------------------------------------------
+    def integrator_release(self, integ):
+        self.call("integrator_release", integ)
+
     def integrator_set_dt(self, integ, dt):
         self.call("integrator_set_dt", integ, dt)
 

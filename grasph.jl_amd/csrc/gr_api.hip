@@ -83,6 +83,7 @@ struct SysH {
   double gcut = 0;
   bool ex_p = false, ex_stress = false;
   std::vector<int2> copy;  // per stage: {copy_p, stress_mode}
+  bool dead = false;       // released with its integrator; segment reusable
 };
 
 struct InterH {
@@ -93,12 +94,44 @@ struct InterH {
   std::vector<gr_pfn_desc> pfns;
   int carr_b = -1, carr_a = -1;  // CellArr indices (b or self; a when coupled)
   bool grid_built = false;
+  bool dead = false;
 };
 
 struct IntegH {
   std::vector<int> sys, inter, ghost;
   double gamma = 0, h = 0, c = 0, dt = 0;
   int nst = 0;
+  bool dead = false;
+};
+
+// Free ranges [off, off + len) of a pool, kept sorted and coalesced.
+struct FreeList {
+  std::vector<std::pair<int64_t, int64_t>> seg;
+  void put(int64_t off, int64_t len) {
+    if (len <= 0) return;
+    seg.push_back({off, len});
+    std::sort(seg.begin(), seg.end());
+    std::vector<std::pair<int64_t, int64_t>> m;
+    for (auto &q : seg) {
+      if (!m.empty() && m.back().first + m.back().second == q.first)
+        m.back().second += q.second;
+      else
+        m.push_back(q);
+    }
+    seg.swap(m);
+  }
+  // First fit; -1 if no range is long enough.
+  int64_t take(int64_t len) {
+    for (size_t k = 0; k < seg.size(); ++k)
+      if (seg[k].second >= len) {
+        const int64_t off = seg[k].first;
+        seg[k].first += len;
+        seg[k].second -= len;
+        if (seg[k].second == 0) seg.erase(seg.begin() + (long)k);
+        return off;
+      }
+    return -1;
+  }
 };
 
 enum PK { P_SELF = PK_SELF, P_TWO = PK_TWO_SIDED, P_ONE = PK_ONE_SIDED, P_STATIC = PK_STATIC, P_DYN = PK_DYNAMIC };
@@ -156,6 +189,8 @@ struct gr_ctx {
   std::vector<Bnd> bnds;
   // pool
   int64_t pool_cap = 0;
+  int64_t pool_end = 0;  // end of the laid-out segments
+  FreeList free_pool, free_cells;
   bool dirty_layout = true;
   int nsmax = 4;
   int rsize = 4;  // sizeof(R)
@@ -164,6 +199,7 @@ struct gr_ctx {
       vor[2], strain[2], strainp[2];
   int cur = 0;
   DBuf vs, rhos, rho_h, gidx, gbnd, gnrm, gxs, gidx_s, gbnd_s;
+  DBuf ra, rb;  // partner records (RecA / RecB), rebuilt before every stage's sweeps
   // tables
   DBuf d_sd, d_n, d_st, d_bnd, d_kmin, d_kmax, d_bbmin, d_bbmax, d_kbase, d_krange, d_flags;
   DBuf d_grids, d_carr, d_cellsz, d_inter_sys, d_inter_carr, cells;
@@ -252,6 +288,8 @@ template <typename R>
 Pool<R> pool_view(gr_ctx *c, int w) {
   Pool<R> P;
   const int64_t cap = c->pool_cap;
+  P.ra = c->ra.as<RecA<R>>();
+  P.rb = c->rb.as<RecB<R>>();
   for (int d = 0; d < 3; ++d) {
     P.x[d] = c->x[w].as<double>() + d * cap;
     P.v[d] = c->v[w].as<double>() + d * cap;
@@ -296,12 +334,16 @@ void grow(gr_ctx *c, DBuf &b, int comps, size_t esz, int64_t old_cap, int64_t ne
 
 void commit(gr_ctx *c) {
   if (!c->dirty_layout) return;
-  int64_t off = 0;
+  // new systems take a released segment (first fit) or go at the end
   for (auto &s : c->sys) {
-    if (s.off < 0) s.off = off;
-    off = std::max(off, s.off + s.cap);
+    if (s.dead || s.off >= 0) continue;
+    s.off = c->free_pool.take(s.cap);
+    if (s.off < 0) {
+      s.off = c->pool_end;
+      c->pool_end += s.cap;
+    }
   }
-  const int64_t need_cap = std::max<int64_t>(off, 1);
+  const int64_t need_cap = std::max<int64_t>(c->pool_end, 1);
   if (need_cap > c->pool_cap) {
     const int64_t old = c->pool_cap;
     const int64_t nc = std::max(need_cap, old + old / 2);
@@ -330,13 +372,15 @@ void commit(gr_ctx *c) {
     grow(c, c->gxs, 3, 8, old, nc);
     grow(c, c->gidx_s, 1, 4, old, nc);
     grow(c, c->gbnd_s, 1, 4, old, nc);
+    grow(c, c->ra, 1, 32, old, nc);
+    grow(c, c->rb, 1, R == 4 ? 16 : 32, old, nc);
     c->pool_cap = nc;
   }
   c->plans.clear();  // block tables and captured graphs refer to the layout
   // id = 1..n for systems laid out for the first time (src/Particles.jl:92)
   c->id_written.resize(c->sys.size(), 0);
   for (size_t i = 0; i < c->sys.size(); ++i) {
-    if (c->id_written[i] || c->sys[i].kind == GR_SYS_GHOST) continue;
+    if (c->id_written[i] || c->sys[i].kind == GR_SYS_GHOST || c->sys[i].dead) continue;
     std::vector<int> ids(c->sys[i].n);
     for (int64_t k = 0; k < c->sys[i].n; ++k) ids[k] = (int)(k + 1);
     HIPCHK(hipMemcpyAsync(c->id[c->cur].as<int>() + c->sys[i].off, ids.data(), ids.size() * 4,
@@ -748,6 +792,17 @@ void host_bbox(gr_ctx *c, const std::vector<int> &systems, std::vector<double> &
     }
 }
 
+// Destroy every captured step graph (device tables or pools they point to
+// are about to move).
+void drop_graphs(gr_ctx *c) {
+  for (auto &pk : c->plans)
+    for (auto &gg : pk.second->graph)
+      if (gg) {
+        (void)hipGraphExecDestroy(gg);
+        gg = nullptr;
+      }
+}
+
 // Allocate cell arrays of the interactions that have none yet.
 void ensure_cells(gr_ctx *c, const std::vector<int> &inters, const std::vector<int> &union_sys) {
   bool any = false;
@@ -771,12 +826,15 @@ void ensure_cells(gr_ctx *c, const std::vector<int> &inters, const std::vector<i
     const int64_t cap = cell_capacity(c, in, mn.data(), mx.data());
     auto mk = [&](int sysid) {
       CellArr a;
-      a.off = c->cells_cap;
+      a.off = c->free_cells.take(cap);
+      if (a.off < 0) {
+        a.off = c->cells_cap;
+        c->cells_cap += cap;
+      }
       a.cap = (int32_t)cap;
       a.grid = it;
       a.sys = sysid;
       a.pad = 0;
-      c->cells_cap += cap;
       c->carr.push_back(a);
       return (int)c->carr.size() - 1;
     };
@@ -787,8 +845,11 @@ void ensure_cells(gr_ctx *c, const std::vector<int> &inters, const std::vector<i
       in.carr_a = mk(in.a);
     }
   }
-  // (re)allocate the cell pool; grids are rebuilt every step, so no copy
-  c->cells.alloc(std::max<int64_t>(c->cells_cap, 1) * 4);
+  // grow the cell pool (grids are rebuilt every step, so no copy); captured
+  // graphs of other plans hold the old table and pool pointers
+  drop_graphs(c);
+  if ((size_t)std::max<int64_t>(c->cells_cap, 1) * 4 > c->cells.bytes)
+    c->cells.alloc(std::max<int64_t>(c->cells_cap, 1) * 4);
   c->d_carr.upload(c->carr, c->stream);
   const int ni = (int)c->inter.size();
   std::vector<int2> isys(ni), icarr(ni);
@@ -1055,12 +1116,7 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
       c->lcnt.alloc(std::max<int64_t>(cacc, 1) * 4);
       c->lidx_cap = lacc;
       c->lcnt_cap = cacc;
-      for (auto &pk : c->plans)
-        for (auto &gg : pk.second->graph)
-          if (gg) {
-            (void)hipGraphExecDestroy(gg);
-            gg = nullptr;
-          }
+      drop_graphs(c);
     }
   }
   // materialise stage plans
@@ -1129,6 +1185,7 @@ Plan &get_plan(gr_ctx *c, const std::vector<int> &integs) {
   std::vector<int> reals, ghosts, inters, all;
   for (int ig : integs) {
     const IntegH &I = c->integ[ig];
+    need(!I.dead, GR_ERR_STATE, "integrator was released");
     for (int s : I.sys) reals.push_back(s);
     for (int g : I.ghost) ghosts.push_back(g);
     for (int t : I.inter) inters.push_back(t);
@@ -1158,7 +1215,8 @@ void all_real(gr_ctx *c, const Blk *&blk, int &nblk) {
   if (c->allreal_cap != c->pool_cap || c->allreal_nsys != c->sys.size()) {
     std::vector<Blk> v;
     for (int s = 0; s < (int)c->sys.size(); ++s)
-      if (c->sys[s].kind != GR_SYS_GHOST) add_blocks(v, s, c->sys[s].cap, GR_BLOCK);
+      if (c->sys[s].kind != GR_SYS_GHOST && !c->sys[s].dead)
+        add_blocks(v, s, c->sys[s].cap, GR_BLOCK);
     c->allreal_blk.release();
     c->allreal_blk.upload(v, c->stream);
     c->allreal_nblk = (int)v.size();
@@ -1267,6 +1325,12 @@ void phase_stage(gr_ctx *c, Plan &P, int k, bool cnt, hipEvent_t *mid) {
   launch_ghost_copy<R>(s, A, c->d_sd.as<SysDesc>(), c->d_bnd.as<Bnd>(), c->d_n.as<int>(),
                        S.blk_gcopy.as<Blk>(), S.nblk_gcopy, S.gcopy.as<int2>(), c->D);
   if (mid) HIPCHK(hipEventRecord(*mid, s));
+  if (S.fam[FAM_FORCE].nblk + S.fam[FAM_COUNT].nblk + S.fam[FAM_STRAIN].nblk > 0) {
+    launch_pack<R>(s, A, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), P.blk_real.as<Blk>(),
+                   P.nblk_real, c->D);
+    launch_pack<R>(s, A, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), P.blk_ghost.as<Blk>(),
+                   P.nblk_ghost, c->D);
+  }
   for (int fam : {FAM_FORCE, FAM_COUNT, FAM_STRAIN}) {
     FamPlan &F = S.fam[fam];
     if (F.nblk == 0) continue;
@@ -1440,6 +1504,7 @@ void do_steps(gr_ctx *c, const std::vector<int> &integs, int nsteps) {
 
 void validate_sys(gr_ctx *c, int s) {
   need(s >= 0 && s < (int)c->sys.size(), GR_ERR_ARGUMENT, "bad system handle");
+  need(!c->sys[s].dead, GR_ERR_STATE, "system was released");
 }
 
 template <typename R>
@@ -1481,6 +1546,20 @@ void refresh_shadows(gr_ctx *c, const std::vector<int> &systems) {
   b.upload(v, c->stream);
   k_refresh<R><<<(unsigned)v.size(), GR_BLOCK, 0, c->stream>>>(pool_view<R>(c, c->cur),
                                                               c->d_sd.as<SysDesc>(), b.as<Blk>(), c->D);
+  HIPCHK(hipStreamSynchronize(c->stream));
+}
+
+// Partner records of the listed systems (real and ghost) before a
+// fine-grained sweep.
+template <typename R>
+void pack_systems(gr_ctx *c, const std::vector<int> &systems) {
+  std::vector<Blk> v;
+  for (int s : systems) add_blocks(v, s, c->sys[s].cap, GR_BLOCK);
+  if (v.empty()) return;
+  DBuf b;
+  b.upload(v, c->stream);
+  launch_pack<R>(c->stream, pool_view<R>(c, c->cur), c->d_sd.as<SysDesc>(), c->d_n.as<int>(),
+                 b.as<Blk>(), (int)v.size(), c->D);
   HIPCHK(hipStreamSynchronize(c->stream));
 }
 
@@ -1956,6 +2035,34 @@ int gr_integrator_create(gr_ctx *c, int nsys, const int *systems, int ninter, co
   });
 }
 
+int gr_integrator_release(gr_ctx *c, int integ) {
+  return guard(c, [&] {
+    need(integ >= 0 && integ < (int)c->integ.size(), GR_ERR_ARGUMENT, "bad integrator");
+    IntegH &I = c->integ[integ];
+    need(!I.dead, GR_ERR_STATE, "integrator was released");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    drop_graphs(c);
+    c->plans.clear();
+    I.dead = true;
+    auto free_sys = [&](int s) {
+      SysH &S = c->sys[s];
+      if (S.dead) return;
+      S.dead = true;
+      if (S.off >= 0) c->free_pool.put(S.off, S.cap);
+    };
+    for (int s : I.sys) free_sys(s);
+    for (int g : I.ghost) free_sys(g);
+    for (int t : I.inter) {
+      InterH &in = c->inter[t];
+      if (in.dead) continue;
+      in.dead = true;
+      for (int ca : {in.carr_b, in.carr_a})
+        if (ca >= 0) c->free_cells.put(c->carr[ca].off, c->carr[ca].cap);
+    }
+    c->allreal_cap = -1;  // rebuild the all-real-systems table
+  });
+}
+
 int gr_integrator_dt(gr_ctx *c, int integ, double cfl, double *dt_out) {
   return guard(c, [&] {
     need(integ >= 0 && integ < (int)c->integ.size(), GR_ERR_ARGUMENT, "bad integrator");
@@ -2122,6 +2229,7 @@ int gr_sweep(gr_ctx *c, int inter, int stage) {
     by_prec(c, [&](auto *tag) {
       using R = std::remove_pointer_t<decltype(tag)>;
       refresh_shadows<R>(c, syss);
+      pack_systems<R>(c, syss);
     });
     const int pk = classify(c, in, f.kind);
     std::vector<Task> tv;

@@ -143,7 +143,44 @@ struct Blk {  // block table entry
 
 // Device view of the pool for one precision.
 template <typename R>
+struct Pool;
+
+// Partner records: what a pair functor reads of a partner particle, packed
+// per particle so that one pair costs three 16-byte loads (RecA: position and
+// the scalar fields, RecB: half-step velocity) instead of eight scattered
+// 4/8-byte gathers from separate arrays.  Written by k_pack before the
+// sweeps of each stage, from the arrays below.
+template <typename R>
+struct RecA;
+template <>
+struct __align__(16) RecA<float> {
+  double x[3];
+  float rho, p;
+};
+template <>
+struct __align__(16) RecA<double> {
+  double x[3];
+  double rho;
+};
+template <typename R>
+struct RecB;
+template <>
+struct __align__(16) RecB<float> {
+  float v[3];
+  float pad;
+};
+template <>
+struct __align__(16) RecB<double> {
+  double v[3];
+  double p;
+};
+__device__ inline float rec_pres(const RecA<float> &a, const RecB<float> &) { return a.p; }
+__device__ inline double rec_pres(const RecA<double> &, const RecB<double> &b) { return b.p; }
+
+template <typename R>
 struct Pool {
+  RecA<R> *ra;
+  RecB<R> *rb;
   double *x[3], *v[3], *rho, *rho_h;
   R *vs[3], *rhos, *dvdt[3], *drhodt, *p;
   double *stress[GR_MAXNS];

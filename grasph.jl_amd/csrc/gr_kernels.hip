@@ -62,62 +62,94 @@ __device__ inline int key_comp(double x, double cutoff) {
 // Keys: per-system key range and position bounding box
 // ---------------------------------------------------------------------------
 
-// Block reduction of min/max of the integer cell key and of the position per
-// dimension for the block's system; one atomic per block per quantity.
+// Min/max of the integer cell key and of the position per dimension, per
+// system.  Each workgroup reduces `per` consecutive entries of the block
+// table in registers and flushes (workgroup reduction + one atomic per
+// quantity) whenever the system changes and at the end, so the number of
+// global atomics is ~12 per `per` table entries.
 template <int D>
-__global__ void k_key_minmax(XP xs, const SysDesc *sd, const int *n_dev,
-                             const StepSys *st, const Blk *blk, int *kmin, int *kmax,
+__global__ void k_key_minmax(XP xs, const SysDesc *sd, const int *n_dev, const StepSys *st,
+                             const Blk *blk, int nblk, int per, int *kmin, int *kmax,
                              long long *bbmin, long long *bbmax, int with_keys) {
   __shared__ long long red[4][GR_BLOCK / 64][3];
-  const Blk b = blk[blockIdx.x];
-  const int s = b.item;
-  const int i = b.first + threadIdx.x;
-  const int n = n_dev[s];
-  const bool live = i < n;
-  const int64_t gi = sd[s].off + i;
-  const double cutoff = st[s].sort_cutoff;
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  double xl[D];
+  const int b0 = blockIdx.x * per;
+  const int b1 = b0 + per < nblk ? b0 + per : nblk;
+  long long mn[D], mx[D];
+  int kmn[D], kmx[D];
+  auto reset = [&]() {
 #pragma unroll
-  for (int d = 0; d < D; ++d) xl[d] = live ? xs[d][gi] : 0.0;
+    for (int d = 0; d < D; ++d) {
+      mn[d] = LLONG_MAX;
+      mx[d] = LLONG_MIN;
+      kmn[d] = INT_MAX;
+      kmx[d] = INT_MIN;
+    }
+  };
+  auto flush = [&](int s) {
 #pragma unroll
-  for (int d = 0; d < D; ++d) {
-    double xv = xl[d];
-    long long o = ord_d(xv);
-    long long mn = live ? o : LLONG_MAX, mx = live ? o : LLONG_MIN;
-    int k = (live && with_keys) ? key_comp(xv, cutoff) : 0;
-    int kmn = live ? k : INT_MAX, kmx = live ? k : INT_MIN;
-    mn = wave_min_ll(mn);
-    mx = wave_max_ll(mx);
-    kmn = wave_min_i(kmn);
-    kmx = wave_max_i(kmx);
-    if (lane == 0) {
-      red[0][w][d] = mn;
-      red[1][w][d] = mx;
-      red[2][w][d] = kmn;
-      red[3][w][d] = kmx;
+    for (int d = 0; d < D; ++d) {
+      const long long a = wave_min_ll(mn[d]), bq = wave_max_ll(mx[d]);
+      const int c = wave_min_i(kmn[d]), e = wave_max_i(kmx[d]);
+      if (lane == 0) {
+        red[0][w][d] = a;
+        red[1][w][d] = bq;
+        red[2][w][d] = c;
+        red[3][w][d] = e;
+      }
     }
-  }
-  __syncthreads();
-  if (threadIdx.x < D) {
-    const int d = threadIdx.x;
-    long long mn = LLONG_MAX, mx = LLONG_MIN;
-    int kmn = INT_MAX, kmx = INT_MIN;
-    for (int q = 0; q < GR_BLOCK / 64; ++q) {
-      mn = red[0][q][d] < mn ? red[0][q][d] : mn;
-      mx = red[1][q][d] > mx ? red[1][q][d] : mx;
-      kmn = (int)red[2][q][d] < kmn ? (int)red[2][q][d] : kmn;
-      kmx = (int)red[3][q][d] > kmx ? (int)red[3][q][d] : kmx;
+    __syncthreads();
+    if (threadIdx.x < D) {
+      const int d = threadIdx.x;
+      long long a = LLONG_MAX, bq = LLONG_MIN;
+      int c = INT_MAX, e = INT_MIN;
+      for (int q = 0; q < GR_BLOCK / 64; ++q) {
+        a = red[0][q][d] < a ? red[0][q][d] : a;
+        bq = red[1][q][d] > bq ? red[1][q][d] : bq;
+        c = (int)red[2][q][d] < c ? (int)red[2][q][d] : c;
+        e = (int)red[3][q][d] > e ? (int)red[3][q][d] : e;
+      }
+      if (a != LLONG_MAX) {
+        atomicMin(&bbmin[s * 3 + d], a);
+        atomicMax(&bbmax[s * 3 + d], bq);
+        if (with_keys) {
+          atomicMin(&kmin[s * 3 + d], c);
+          atomicMax(&kmax[s * 3 + d], e);
+        }
+      }
     }
-    if (mn != LLONG_MAX) {
-      atomicMin(&bbmin[s * 3 + d], mn);
-      atomicMax(&bbmax[s * 3 + d], mx);
-      if (with_keys) {
-        atomicMin(&kmin[s * 3 + d], kmn);
-        atomicMax(&kmax[s * 3 + d], kmx);
+    __syncthreads();
+  };
+  reset();
+  int cur = blk[b0].item;
+  for (int bb = b0; bb < b1; ++bb) {
+    const Blk b = blk[bb];
+    if (b.item != cur) {  // uniform over the workgroup
+      flush(cur);
+      reset();
+      cur = b.item;
+    }
+    const int i = b.first + threadIdx.x;
+    if (i < n_dev[cur]) {
+      const int64_t gi = sd[cur].off + i;
+      const double cutoff = st[cur].sort_cutoff;
+      double xl[D];
+#pragma unroll
+      for (int d = 0; d < D; ++d) xl[d] = xs[d][gi];
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const long long o = ord_d(xl[d]);
+        mn[d] = o < mn[d] ? o : mn[d];
+        mx[d] = o > mx[d] ? o : mx[d];
+        if (with_keys) {
+          const int k = key_comp(xl[d], cutoff);
+          kmn[d] = k < kmn[d] ? k : kmn[d];
+          kmx[d] = k > kmx[d] ? k : kmx[d];
+        }
       }
     }
   }
+  flush(cur);
 }
 
 // Reset the reduction slots of the listed systems.
@@ -1068,6 +1100,32 @@ __global__ void k_fill_i32(int *p, int v, long long n) {
   if (i < n) p[i] = v;
 }
 
+// Partner records of the live particles of the block's system (positions,
+// shadows of density and velocity, pressure), read by the pair sweeps.
+template <typename R, int D>
+__global__ void k_pack(Pool<R> P, const SysDesc *sd, const int *n_dev, const Blk *blk) {
+  const Blk b = blk[blockIdx.x];
+  const int i = b.first + threadIdx.x;
+  if (i >= n_dev[b.item]) return;
+  const int64_t g = sd[b.item].off + i;
+  RecA<R> a;
+  RecB<R> r;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    a.x[d] = d < D ? P.x[d][g] : 0.0;
+    r.v[d] = d < D ? P.vs[d][g] : (R)0;
+  }
+  a.rho = P.rhos[g];
+  if constexpr (sizeof(R) == 4) {
+    a.p = P.p[g];
+    r.pad = 0.0f;
+  } else {
+    r.p = P.p[g];
+  }
+  P.ra[g] = a;
+  P.rb[g] = r;
+}
+
 // ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
@@ -1078,12 +1136,14 @@ void launch_key_minmax(hipStream_t s, XP xs, const SysDesc *sd, const int *n, co
                        const Blk *blk, int nblk, int D, int *kmin, int *kmax, long long *bbmin,
                        long long *bbmax, int with_keys) {
   if (nblk <= 0) return;
+  const int per = 16;
+  const unsigned g = nb_of(nblk, per);
   if (D == 2)
-    k_key_minmax<2><<<nblk, GR_BLOCK, 0, s>>>(xs, sd, n, st, blk, kmin, kmax, bbmin, bbmax,
-                                              with_keys);
+    k_key_minmax<2><<<g, GR_BLOCK, 0, s>>>(xs, sd, n, st, blk, nblk, per, kmin, kmax, bbmin,
+                                           bbmax, with_keys);
   else
-    k_key_minmax<3><<<nblk, GR_BLOCK, 0, s>>>(xs, sd, n, st, blk, kmin, kmax, bbmin, bbmax,
-                                              with_keys);
+    k_key_minmax<3><<<g, GR_BLOCK, 0, s>>>(xs, sd, n, st, blk, nblk, per, kmin, kmax, bbmin,
+                                           bbmax, with_keys);
 }
 void launch_minmax_reset(hipStream_t s, const int *list, int nlist, int *kmin, int *kmax,
                          long long *bbmin, long long *bbmax) {
@@ -1270,6 +1330,20 @@ void launch_updaters(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const S
 INST(float)
 INST(double)
 #undef INST
+
+template <typename R>
+void launch_pack(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const int *n, const Blk *blk,
+                 int nblk, int D) {
+  if (nblk <= 0) return;
+  if (D == 2)
+    k_pack<R, 2><<<nblk, GR_BLOCK, 0, s>>>(P, sd, n, blk);
+  else
+    k_pack<R, 3><<<nblk, GR_BLOCK, 0, s>>>(P, sd, n, blk);
+}
+template void launch_pack<float>(hipStream_t, const Pool<float> &, const SysDesc *, const int *,
+                                 const Blk *, int, int);
+template void launch_pack<double>(hipStream_t, const Pool<double> &, const SysDesc *, const int *,
+                                  const Blk *, int, int);
 
 void launch_fill_i32(hipStream_t s, int *p, int v, long long n) {
   if (n > 0) k_fill_i32<<<nb_of(n, GR_BLOCK), GR_BLOCK, 0, s>>>(p, v, n);

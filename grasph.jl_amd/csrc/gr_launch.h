@@ -93,6 +93,9 @@ void launch_updaters(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const S
 template <typename R>
 void launch_sweep(hipStream_t s, const Pool<R> &P, const SweepArgs &a, int D, int fam, bool cnt);
 template <typename R>
+void launch_pack(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const int *n, const Blk *blk,
+                 int nblk, int D);
+template <typename R>
 void launch_nlist(hipStream_t s, const Pool<R> &P, const NlistArgs &a, int D);
 void launch_fill_i32(hipStream_t s, int *p, int v, long long n);
 

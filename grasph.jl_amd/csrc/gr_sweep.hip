@@ -181,7 +181,8 @@ struct Gather {
   // One accepted pair (target i, partner j at pool index gj), dx = x_i − x_j
   // and r² as evaluated by _pair_self!/_pair_coupled! (src/Interaction.jl:546-553).
   __device__ __forceinline__ void pair(const Pool<R> P, const SrcDesc &Q, const SrcK<R> &K,
-                                       int64_t gj, const double *dx, double r2, double da) {
+                                       int64_t gj, const RecA<R> &Aj, const double *dx, double r2,
+                                       double da) {
     constexpr bool FA = Fast<R>::on;
     R dxr[D], gx[D];
     const R r2r = (R)r2;
@@ -204,22 +205,25 @@ struct Gather {
       adr.add((R)1);
       return;
     }
-    R vj[D], rj;
+    R vj[D], rj, pj = (R)0;
     const R m = K.m_q;
     if (Q.pk == PK_DYNAMIC) {  // DynamicBoundarySystem: mirrored velocity
-      double db = (P.x[0][gj] - Q.wp[0]) * Q.wn[0];
+      double db = (Aj.x[0] - Q.wp[0]) * Q.wn[0];
 #pragma unroll
-      for (int d = 1; d < D; ++d) db = db + (P.x[d][gj] - Q.wp[d]) * Q.wn[d];
+      for (int d = 1; d < D; ++d) db = db + (Aj.x[d] - Q.wp[d]) * Q.wn[d];
       const double rat = fabs(db / da);
       const double mn = isnan(rat) ? rat : (rat < Q.wbeta ? rat : Q.wbeta);
       const R fac = (R)(-mn);
 #pragma unroll
       for (int d = 0; d < D; ++d) vj[d] = fac * vi[d];
       rj = ri;
+      pj = pi;
     } else {
+      const RecB<R> Bj = P.rb[gj];
 #pragma unroll
-      for (int d = 0; d < D; ++d) vj[d] = P.vs[d][gj];
-      rj = P.rhos[gj];
+      for (int d = 0; d < D; ++d) vj[d] = Bj.v[d];
+      rj = Aj.rho;
+      pj = rec_pres(Aj, Bj);
     }
     const R irj = (FA && Q.pk == PK_DYNAMIC) ? iri : Fast<R>::rcp(rj);
     if (FAM == FAM_STRAIN) {  // strain_rate_tensor / vorticity_tensor
@@ -284,7 +288,6 @@ struct Gather {
       psi = FA ? K.psi_coef * (ri - rj) * dxg * (ir * ir) : K.psi_coef * (ri - rj) * dxg / r2r;
     }
     if (Q.pfn == GR_PFN_FLUID) {
-      const R pj = (Q.pk == PK_DYNAMIC) ? pi : P.p[gj];
       R dh;
       if (FA) {
         if (K.sigma == 1)
@@ -421,9 +424,10 @@ __device__ void gather_direct(Gather<D, R, FAM> &Gt, const Pool<R> P, const SysD
       for (int j = j0; j < j1; ++j) {
         if (Q.pk == PK_SELF && j == i) continue;
         const int64_t gj = poff + j;
+        const RecA<R> Aj = P.ra[gj];
         double dx[D];
 #pragma unroll
-        for (int d = 0; d < D; ++d) dx[d] = Gt.xi[d] - P.x[d][gj];
+        for (int d = 0; d < D; ++d) dx[d] = Gt.xi[d] - Aj.x[d];
         double r2 = dx[0] * dx[0];
 #pragma unroll
         for (int d = 1; d < D; ++d) r2 = r2 + dx[d] * dx[d];
@@ -432,7 +436,7 @@ __device__ void gather_direct(Gather<D, R, FAM> &Gt, const Pool<R> P, const SysD
           ++ns;
         else
           ++no;
-        Gt.pair(P, Q, K, gj, dx, r2, da);
+        Gt.pair(P, Q, K, gj, Aj, dx, r2, da);
       }
     }
   }
@@ -624,9 +628,10 @@ __global__ void __launch_bounds__(GR_BLOCK)
       for (int k = 0; k < cq; ++k) {
         const int j = my[(int64_t)(off + k) * L.cap];
         const int64_t gj = poff + j;
+        const RecA<R> Aj = P.ra[gj];
         double dx[D];
 #pragma unroll
-        for (int d = 0; d < D; ++d) dx[d] = Gt.xi[d] - P.x[d][gj];
+        for (int d = 0; d < D; ++d) dx[d] = Gt.xi[d] - Aj.x[d];
         double r2 = dx[0] * dx[0];
 #pragma unroll
         for (int d = 1; d < D; ++d) r2 = r2 + dx[d] * dx[d];
@@ -636,7 +641,7 @@ __global__ void __launch_bounds__(GR_BLOCK)
           else
             ++no;
         }
-        Gt.pair(P, Q, K, gj, dx, r2, da);
+        Gt.pair(P, Q, K, gj, Aj, dx, r2, da);
       }
     }
   }

@@ -126,6 +126,24 @@ class DeviceScene:
         self._objs.append(lf)
         return h
 
+    def release(self, lf):
+        """Release an integrator and everything it owns on the context
+        (gr_integrator_release); its pool segments are reused by systems
+        registered later.  The host objects stay usable as plain arrays."""
+        h = self._integ.pop(id(lf))
+        self.be.integrator_release(h)
+        for ps in lf.systems:
+            inner = ps.inner if isinstance(ps, (M.StaticBoundarySystem,
+                                                M.DynamicBoundarySystem)) else ps
+            self._sys.pop(id(inner), None)
+        for ge in lf.ghosts:
+            self._ghost.pop(id(ge.ghost), None)
+        for si in lf.interactions:
+            self._inter.pop(id(si), None)
+        drop = {id(lf)} | {id(ps) for ps in lf.systems} | {id(si) for si in lf.interactions}
+        drop |= {id(ge) for ge in lf.ghosts} | {id(ge.ghost) for ge in lf.ghosts}
+        self._objs = [o for o in self._objs if id(o) not in drop]
+
     # ---- host <-> context -------------------------------------------------
     def upload(self, ps, fields=None):
         if isinstance(ps, (M.StaticBoundarySystem, M.DynamicBoundarySystem)):

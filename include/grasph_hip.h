@@ -185,6 +185,12 @@ int gr_interaction_create(gr_ctx *ctx, int kernel_kind, double h, int sys_a,
 int gr_integrator_create(gr_ctx *ctx, int nsys, const int *systems,
                          int ninter, const int *inters, int nghost,
                          const int *ghosts, double gamma, int *integ_out);
+/* Release an integrator with its systems, ghosts and interactions: their
+ * pool segments and cell arrays are reused by systems created later (no
+ * counterpart in the reference, where Julia's GC frees a finished
+ * simulation; a serving loop that admits and retires simulations needs it).
+ * Handles of released objects are rejected with GR_ERR_STATE. */
+int gr_integrator_release(gr_ctx *ctx, int integ);
 /* dt = CFL·h/c with h = min kernel h, c = max system c
  * (src/TimeIntegration.jl:63-64,598). */
 int gr_integrator_dt(gr_ctx *ctx, int integ, double cfl, double *dt_out);

@@ -41,6 +41,9 @@ def test_python_binding_covers_the_header():
     bound = {"gr_" + k for k in list(A._SIGS) + list(A._GPU_SIGS)}
     bound |= {"gr_ctx_create", "gr_ctx_destroy", "gr_last_error"}
     assert sorted(set(names) - bound) == []
+    # and each bound entry point has a method of the same name
+    methods = {n for n in list(A._SIGS) + list(A._GPU_SIGS) if callable(getattr(A.CBackend, n, None))}
+    assert sorted(set(A._SIGS) | set(A._GPU_SIGS)) == sorted(methods)
 
 
 def test_oracle_exports_every_declared_symbol():
@@ -56,6 +59,7 @@ def test_oracle_mirrors_the_device_api():
     dev = {n[3:] for n in declared(ROOT / "include" / "grasph_hip.h", "gr_")}
     orc = {n[4:] for n in declared(ROOT / "oracle" / "grasph_oracle.h", "orc_")}
     device_only = {"ctx_create", "ctx_destroy", "ctx_info", "last_error", "integrator_set_dt",
+                   "integrator_release",
                    "step_many", "set_graph_mode", "set_timing", "get_timing",
                    "get_pair_count", "get_pair_stats", "bench_copy"}
     assert sorted(dev - orc - device_only) == []

@@ -501,3 +501,47 @@ def test_device_reproduces_golden(D):
         assert np.array_equal(dev[k], g[k]), k
     for k in ("step_x", "step_v", "step_rho", "step_p", "step_dvdt", "step_drhodt"):
         assert rel(dev[k], g[k]) < 1e-12, k
+
+
+def test_release_reuses_segments_and_keeps_results():
+    """Integrators admitted and retired on one context (the serving pattern):
+    after A is released, C takes A's pool segments; B and C stepped together
+    (graph replay on) equal each stepped alone on a fresh context."""
+    def col(w, name):
+        return M.configs.ep_column_2d(dx=0.01, width=w, height=0.08, name=name)
+
+    def alone(w, name, nsteps):
+        cfg = col(w, name)
+        s = M.Session(2, "f32")
+        h = s.scene.integrator(cfg["integrator"])
+        s.scene.upload_all(cfg["integrator"])
+        s.backend.integrator_set_dt(h, s.backend.integrator_dt(h, cfg["cfl"]))
+        s.backend.step_many([h], nsteps)
+        s.scene.download_all(cfg["integrator"])
+        return cfg["fluid"]
+
+    s = M.Session(2, "f32")
+    s.backend.set_graph_mode(True)
+    cfgs = {k: col(w, k) for k, w in (("a", 0.12), ("b", 0.1), ("c", 0.12))}
+    hs = {}
+    for k in ("a", "b"):
+        lf = cfgs[k]["integrator"]
+        hs[k] = s.scene.integrator(lf)
+        s.scene.upload_all(lf)
+        s.backend.integrator_set_dt(hs[k], s.backend.integrator_dt(hs[k], cfgs[k]["cfl"]))
+    s.backend.step_many([hs["a"], hs["b"]], 5)
+    s.scene.release(cfgs["a"]["integrator"])
+    lf = cfgs["c"]["integrator"]
+    hs["c"] = s.scene.integrator(lf)
+    s.scene.upload_all(lf)
+    s.backend.integrator_set_dt(hs["c"], s.backend.integrator_dt(hs["c"], cfgs["c"]["cfl"]))
+    s.backend.step_many([hs["c"]], 5)        # c catches up with b
+    s.backend.step_many([hs["b"], hs["c"]], 5)
+    ref_b = alone(0.1, "b", 10)
+    ref_c = alone(0.12, "c", 10)
+    for k, ref in (("b", ref_b), ("c", ref_c)):
+        s.scene.download_all(cfgs[k]["integrator"])
+        for f in ("x", "v", "rho", "stress"):
+            assert np.array_equal(getattr(cfgs[k]["fluid"], f), getattr(ref, f)), (k, f)
+    with pytest.raises(RuntimeError):
+        s.backend.step_many([hs["a"]], 1)
