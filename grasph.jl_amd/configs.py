@@ -181,3 +181,49 @@ def ensemble_columns(n_sims=256, dx=0.0025, seed=7, aspect=(0.5, 4.0), width=(0.
         sims.append(ep_column_2d(dx=dx, width=w, height=hgt, rho0=r0, phi_deg=ph,
                                  name=f"col{k}"))
     return sims
+
+
+def granular_column_3d(dx=0.005, width=0.2, depth=0.05, height=0.1, rho0=1850.0, c=20.0,
+                       phi_deg=19.8, alpha=0.1, beta=0.1, h_fac=1.2, cfl=0.1, box_x=None,
+                       name="soil3d"):
+    """3D granular column (GranularColumnCollapse3D.jl): StressParticleSystem
+    NS = 6 with (ZeroFieldUpdater(:strain_rate),
+    ViscoPlasticMCStressUpdater(LinearEOSUpdater(ρ0), φ, 0)), cubic spline
+    h = 1.2dx, stages (StrainRatePfn, CauchyFluidPfn(α, β, h)); a 3-layer
+    dynamic floor sheet (β = 3) under a box 4× the column width, and ghost
+    mirrors (cutoff 3h) on the left, front and back walls and the two
+    vertical corners, copying stress at stage 2.  z is vertical."""
+    phi = phi_deg * math.pi / 180.0
+    h = h_fac * dx
+    nfx, nfy, nfz = (max(1, int(math.floor(v / dx + 1e-9))) for v in (width, depth, height))
+    mass = rho0 * dx ** 3
+    soil = M.StressParticleSystem(
+        name, nfx * nfy * nfz, 3, 6, mass, c, source_v=[0.0, 0.0, -G],
+        state_updater=(M.ZeroFieldUpdater("strain_rate"),
+                       M.ViscoPlasticMCStressUpdater(M.LinearEOSUpdater(rho0), phi, 0.0)))
+    soil.x[:] = _lattice(nfx, nfy, dx, nfz)
+    soil.rho[:] = rho0
+    nbx = int(math.floor((4.0 * width if box_x is None else box_x) / dx + 1e-9))
+    nby = nfy
+    i, j, m = np.meshgrid(np.arange(1, nbx + 4), np.arange(1, nby + 7), np.arange(1, 4),
+                          indexing="ij")
+    bot = np.stack([(i - 3.5) * dx, (j - 2.5) * dx, -(m - 0.5) * dx], -1).reshape(-1, 3)
+    bottom, bottom_w = _boundary(f"{name}_bottom", bot, mass, c, (0.0, 0.0, 1.0), (0.0, 0.0, 0.0))
+    bottom.rho[:] = rho0
+    y_max = nfy * dx
+    r2 = math.sqrt(2.0)
+    ghost = M.GhostParticleSystem(soil, None, M.GhostCopier("stress"), name=f"{name}_ghost")
+    entry = M.GhostEntry(ghost, 3.0 * h,
+                         ((1.0, 0.0, 0.0), (0.0, 0.0, 0.0)),
+                         ((0.0, 1.0, 0.0), (0.0, 0.0, 0.0)),
+                         ((0.0, -1.0, 0.0), (0.0, y_max, 0.0)),
+                         ((1.0 / r2, -1.0 / r2, 0.0), (0.0, y_max, 0.0)),
+                         ((1.0 / r2, 1.0 / r2, 0.0), (0.0, 0.0, 0.0)))
+    kern = M.CubicSplineKernel(h, ndims=3)
+    pfns = (M.StrainRatePfn(), M.CauchyFluidPfn(alpha, beta, h))
+    ints = [M.SystemInteraction(kern, pfns, soil),
+            M.SystemInteraction(kern, pfns, soil, bottom_w),
+            M.SystemInteraction(kern, pfns, soil, ghost)]
+    lf = M.LeapFrogTimeIntegrator([soil, bottom], ints, ghosts=(entry,))
+    return {"integrator": lf, "fluid": soil, "cfl": cfl, "name": "granular_column_3d",
+            "n_particles": soil.n + bottom.n, "fluid_n": soil.n, "boundary_n": bottom.n}

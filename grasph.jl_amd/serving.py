@@ -1,0 +1,278 @@
+"""Continuous-batching simulation server and the plain serial baseline.
+
+Jobs (a scene builder, a step count, an arrival time) are served by one
+device context per spatial dimension, each driven by its own host thread so
+2D and 3D work overlaps on the GPU (ctypes releases the GIL inside the
+library calls).  A context loop:
+
+  1. admits every queued job that has arrived and whose estimated HBM
+     footprint fits the shared budget: builds the scene, registers it on the
+     context, uploads it;
+  2. advances all running jobs of the context together, `quantum` steps at a
+     time (gr_step_many: one launch per phase over the packed pool, each job
+     with its own dt), a quantum ending early at the next job completion;
+  3. retires finished jobs: downloads their state if asked, releases them
+     (gr_integrator_release, so later jobs reuse their pool segments) and
+     returns their footprint to the budget.
+
+`run_plain` serves the same jobs one at a time, each alone on the GPU, in
+arrival order: the "plain single-sim kernel run serially" baseline.
+
+A job's result does not depend on what it was batched with: every particle's
+sums use only its own simulation's particles, in the same order.  The
+device tests check this bit for bit.
+"""
+from __future__ import annotations
+
+import math
+import threading
+import time
+from dataclasses import dataclass, field
+from typing import Callable, Optional
+
+import numpy as np
+
+from .integration import Session
+
+
+@dataclass
+class Job:
+    jid: int
+    ndims: int
+    build: Callable[[], dict]
+    steps: int
+    arrival: float = 0.0
+    n_particles: int = 0
+    bytes: int = 0
+    admitted: float = math.nan
+    finished: float = math.nan
+    done_steps: int = 0
+    result: Optional[dict] = None
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def latency(self):
+        return self.finished - self.arrival
+
+
+def footprint_bytes(cfg, precision="f32", maxnb=None):
+    """Upper estimate of the device bytes a scene holds while resident: two
+    pool families of every per-particle array, the shared shadows and
+    partner records, the neighbour lists and the ghost capacity."""
+    lf = cfg["integrator"]
+    D = lf.systems[0].ndims
+    r = 4 if precision == "f32" else 8
+    maxnb = maxnb or (192 if D == 3 else 96)
+    per = 0
+    n_real = 0
+    for ps in lf.systems:
+        ns = getattr(ps, "ns", 0) or 0
+        fam = 8 * D * 2 + 8 + r * D + r + r + 4 + ns * (8 + r + r + 16) + 3 * r
+        per += ps.n * (2 * fam + r * D + r + 8 + 12 + 8 * D + 48)
+        n_real += ps.n
+    lists = n_real * (maxnb * 4 + 8 * 4)
+    ghosts = 0
+    for ge in lf.ghosts:
+        src = ge.ghost.source
+        ghosts += len(ge.normals) * src.n * (2 * (8 * D * 2 + 8 + 8 * r) + 64)
+    return int(per + lists + ghosts + (1 << 20))
+
+
+class _Budget:
+    def __init__(self, total):
+        self.total = float(total)
+        self.used = 0.0
+        self.lock = threading.Lock()
+
+    def try_take(self, b):
+        with self.lock:
+            if self.used + b <= self.total:
+                self.used += b
+                return True
+            return False
+
+    def give(self, b):
+        with self.lock:
+            self.used -= b
+
+
+class Server:
+    """Continuous batching over one context per spatial dimension."""
+
+    def __init__(self, precision="f32", hbm_budget=200e9, quantum=50, graph=False,
+                 keep_results=False, session_factory=None):
+        self.precision = precision
+        self._make_session = session_factory or (lambda D: Session(D, precision))
+        self.budget = _Budget(hbm_budget)
+        self.quantum = int(quantum)
+        self.graph = bool(graph)
+        self.keep_results = keep_results
+        self.sessions = {}
+        self.errors = []
+
+    def _session(self, D):
+        if D not in self.sessions:
+            s = self._make_session(D)
+            s.backend.set_graph_mode(self.graph)
+            self.sessions[D] = s
+        return self.sessions[D]
+
+    def _loop(self, D, jobs, t0):
+        try:
+            s = self._session(D)
+            be, sc = s.backend, s.scene
+            queue = sorted(jobs, key=lambda j: (j.arrival, j.jid))
+            running = []  # (job, cfg, handle)
+            while queue or running:
+                now = time.perf_counter() - t0
+                # admit arrived jobs in order while they fit
+                while queue and queue[0].arrival <= now:
+                    job = queue[0]
+                    if not job.bytes:
+                        cfg = job.build()
+                        job.meta["cfg"] = cfg
+                        job.n_particles = cfg["n_particles"]
+                        job.bytes = footprint_bytes(cfg, self.precision)
+                    if job.bytes > self.budget.total:  # can never fit: reject
+                        queue.pop(0)
+                        job.meta.pop("cfg", None)
+                        job.meta["rejected"] = True
+                        continue
+                    if not self.budget.try_take(job.bytes):
+                        if not running:
+                            time.sleep(0.001)  # the other context will free memory
+                        break
+                    queue.pop(0)
+                    cfg = job.meta.pop("cfg", None) or job.build()
+                    lf = cfg["integrator"]
+                    h = sc.integrator(lf)
+                    sc.upload_all(lf)
+                    be.integrator_set_dt(h, be.integrator_dt(h, cfg["cfl"]))
+                    job.admitted = time.perf_counter() - t0
+                    running.append((job, cfg, h))
+                if not running:
+                    if queue:
+                        wait = queue[0].arrival - (time.perf_counter() - t0)
+                        if wait > 0:
+                            time.sleep(min(wait, 0.05))
+                    continue
+                q = min(self.quantum, min(j.steps - j.done_steps for j, _, _ in running))
+                be.step_many([h for _, _, h in running], q)
+                keep = []
+                for job, cfg, h in running:
+                    job.done_steps += q
+                    if job.done_steps < job.steps:
+                        keep.append((job, cfg, h))
+                        continue
+                    lf = cfg["integrator"]
+                    if self.keep_results:
+                        sc.download_all(lf)
+                        job.result = {f: getattr(cfg["fluid"], f).copy()
+                                      for f in cfg["fluid"].field_names()}
+                    sc.release(lf)
+                    job.finished = time.perf_counter() - t0
+                    self.budget.give(job.bytes)
+                running = keep
+        except Exception as e:  # surfaced by run()
+            self.errors.append(e)
+
+    def run(self, jobs):
+        """Serve `jobs` (arrival times in seconds from now); returns a report."""
+        t0 = time.perf_counter()
+        by_dim = {}
+        for j in jobs:
+            by_dim.setdefault(j.ndims, []).append(j)
+        for D in by_dim:  # create contexts before the clock matters
+            self._session(D)
+        t0 = time.perf_counter()
+        threads = [threading.Thread(target=self._loop, args=(D, js, t0)) for D, js in by_dim.items()]
+        for t in threads:
+            t.start()
+        for t in threads:
+            t.join()
+        if self.errors:
+            raise self.errors[0]
+        return report(jobs, time.perf_counter() - t0)
+
+
+def run_plain(jobs, precision="f32", keep_results=False, session_factory=None):
+    """Each job alone on the GPU, one after another in arrival order (a job
+    starts when it has arrived and the previous one has finished)."""
+    make = session_factory or (lambda D: Session(D, precision))
+    sessions = {}
+    t0 = time.perf_counter()
+    for D in sorted({j.ndims for j in jobs}):
+        sessions[D] = make(D)
+    t0 = time.perf_counter()
+    for job in sorted(jobs, key=lambda j: (j.arrival, j.jid)):
+        wait = job.arrival - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        s = sessions[job.ndims]
+        cfg = job.build()
+        job.n_particles = cfg["n_particles"]
+        lf = cfg["integrator"]
+        h = s.scene.integrator(lf)
+        s.scene.upload_all(lf)
+        s.backend.integrator_set_dt(h, s.backend.integrator_dt(h, cfg["cfl"]))
+        job.admitted = time.perf_counter() - t0
+        s.backend.step_many([h], job.steps)
+        job.done_steps = job.steps
+        if keep_results:
+            s.scene.download_all(lf)
+            job.result = {f: getattr(cfg["fluid"], f).copy() for f in cfg["fluid"].field_names()}
+        s.scene.release(lf)
+        job.finished = time.perf_counter() - t0
+    return report(jobs, time.perf_counter() - t0)
+
+
+def report(jobs, makespan):
+    done = [j for j in jobs if not j.meta.get("rejected")]
+    lat = np.array([j.latency for j in done]) if done else np.array([math.nan])
+    psteps = sum(j.n_particles * j.steps for j in done)
+    return {"jobs": len(done), "rejected": len(jobs) - len(done), "makespan_s": makespan,
+            "jobs_per_h": len(done) / makespan * 3600.0,
+            "latency_p50_s": float(np.percentile(lat, 50)),
+            "latency_p99_s": float(np.percentile(lat, 99)),
+            "particle_steps_per_s": psteps / makespan}
+
+
+# ---------------------------------------------------------------------------
+# Served workload: Poisson arrivals of 2D and 3D granular column collapses
+# ---------------------------------------------------------------------------
+
+
+def column_jobs(n_jobs, rate=20.0, frac_3d=0.2, steps=(500, 20000), step_scale=1.0, seed=11,
+                dx2=0.0025, dx3=0.005, n3=(50_000, 500_000), size_scale=1.0):
+    """Job stream: Poisson arrivals at `rate` jobs/s; 2D elasto-plastic
+    columns (aspect U[0.5, 4], width U[0.1, 0.4] m, φ U[20°, 40°],
+    ρ U[1500, 2000], dx = 2.5 mm) and, with probability `frac_3d`, 3D
+    visco-plastic columns (target N log-uniform in `n3`, dx = 5 mm,
+    φ U[25°, 40°]); step counts log-uniform in `steps`.  `step_scale` and
+    `size_scale` shrink a sample for short runs (stated with the results)."""
+    from . import configs as CF
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    jobs = []
+    for k in range(n_jobs):
+        t += rng.exponential(1.0 / rate)
+        nsteps = int(round(math.exp(rng.uniform(math.log(steps[0]), math.log(steps[1])))
+                           * step_scale))
+        nsteps = max(nsteps, 1)
+        if rng.uniform() < frac_3d:
+            n = math.exp(rng.uniform(math.log(n3[0]), math.log(n3[1]))) * size_scale
+            a = rng.uniform(0.5, 1.0)
+            w = (2.0 * n * dx3 ** 3 / a) ** (1.0 / 3.0)
+            ph = rng.uniform(25.0, 40.0)
+            build = (lambda w=w, a=a, ph=ph, k=k: CF.granular_column_3d(
+                dx=dx3, width=w, depth=w / 2, height=a * w, phi_deg=ph, name=f"job{k}"))
+            jobs.append(Job(k, 3, build, nsteps, t, meta={"kind": "3d"}))
+        else:
+            a = rng.uniform(0.5, 4.0)
+            w = rng.uniform(0.1, 0.4) * math.sqrt(size_scale)
+            ph = rng.uniform(20.0, 40.0)
+            r0 = rng.uniform(1500.0, 2000.0)
+            build = (lambda w=w, a=a, ph=ph, r0=r0, k=k: CF.ep_column_2d(
+                dx=dx2, width=w, height=a * w, rho0=r0, phi_deg=ph, name=f"job{k}"))
+            jobs.append(Job(k, 2, build, nsteps, t, meta={"kind": "2d"}))
+    return jobs

@@ -375,6 +375,7 @@ STEP_CASES = [
     ("ep_column_2d", M.configs.ep_column_2d, dict(dx=0.01, width=0.2, height=0.1), 2, 20),
     ("dambreak_3d", M.configs.dambreak_3d, dict(dx=0.04, water=(0.24, 0.2, 0.24),
                                                 tank=(0.48, 0.2, 0.32)), 3, 10),
+    ("granular_column_3d", M.configs.granular_column_3d, dict(dx=0.01), 3, 20),
 ]
 
 
@@ -458,6 +459,7 @@ LIST_CASES = [
     ("dambreak_3d", M.configs.dambreak_3d, dict(dx=0.04, water=(0.24, 0.2, 0.24),
                                                 tank=(0.48, 0.2, 0.32)), 3, "f32"),
     ("dambreak_2d_f64", M.configs.dambreak_2d, dict(dx=0.04), 2, "f64"),
+    ("granular_column_3d", M.configs.granular_column_3d, dict(dx=0.01), 3, "f32"),
 ]
 
 
