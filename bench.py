@@ -44,7 +44,60 @@ def parse():
     ap.add_argument("--graph", type=int, default=1, help="replay the step as a hipGraph")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also run the plain copy kernel on this many MiB (counter calibration)")
+    ap.add_argument("--ensemble", type=int, default=0,
+                    help="also step an ensemble of this many 2D columns in shared launches "
+                         "and the same columns one by one")
+    ap.add_argument("--ensemble-steps", type=int, default=50)
+    ap.add_argument("--serve", type=int, default=0,
+                    help="also serve this many column-collapse jobs (continuous batching) "
+                         "and the same jobs serially")
+    ap.add_argument("--serve-step-scale", type=float, default=0.02,
+                    help="step counts of the served sample are scaled by this factor")
+    ap.add_argument("--serve-size-scale", type=float, default=0.25,
+                    help="particle counts of the served sample are scaled by this factor")
     return ap.parse_args()
+
+
+def ensemble_leg(n_sims, nsteps, precision):
+    """`n_sims` 2D granular columns (ensemble parameters) advanced together
+    (gr_step_many, one launch per phase) vs one at a time."""
+    g = __graft_entry__.package()
+    sims = g.configs.ensemble_columns(n_sims=n_sims)
+    npart = sum(c["n_particles"] for c in sims)
+    s = g.Session(2, precision)
+    s.backend.set_graph_mode(True)
+    hs = []
+    for cfg in sims:
+        h = s.scene.integrator(cfg["integrator"])
+        s.scene.upload_all(cfg["integrator"])
+        s.backend.integrator_set_dt(h, s.backend.integrator_dt(h, cfg["cfl"]))
+        hs.append(h)
+    s.backend.step_many(hs, 2)  # plan, capture
+    t0 = time.perf_counter()
+    s.backend.step_many(hs, nsteps)
+    batched = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for h in hs:
+        s.backend.step_many([h], nsteps)
+    serial = time.perf_counter() - t0
+    return {"sims": n_sims, "particles": npart, "steps": nsteps,
+            "batched_s": batched, "serial_s": serial,
+            "sim_steps_per_s_batched": n_sims * nsteps / batched,
+            "sim_steps_per_s_serial": n_sims * nsteps / serial,
+            "particle_steps_per_s_batched": npart * nsteps / batched,
+            "speedup": serial / batched}
+
+
+def serve_leg(n_jobs, step_scale, size_scale, precision):
+    g = __graft_entry__.package()
+    S = g.serving
+    mk = lambda: S.column_jobs(n_jobs, step_scale=step_scale, size_scale=size_scale)  # noqa: E731
+    served = S.Server(precision=precision, quantum=50, graph=True).run(mk())
+    plain = S.run_plain(mk(), precision=precision)
+    return {"sample": f"{n_jobs} jobs, Poisson 20/s, 80% 2D / 20% 3D columns, steps x{step_scale}, "
+                      f"particles x{size_scale}",
+            "served": served, "plain": plain,
+            "jobs_per_h_ratio": served["jobs_per_h"] / plain["jobs_per_h"]}
 
 
 def sweep_bytes(cfg, grids):
@@ -162,6 +215,9 @@ def main():
     achieved = bytes_sweep / (sweep_ms * 1e-3) / 1e9
     traffic = load_traffic()
     cpu = cpu_baseline(g.configs.dambreak_3d, kw, args.cpu_steps, pairs)
+    ens = ensemble_leg(args.ensemble, args.ensemble_steps, args.precision) if args.ensemble else None
+    srv = (serve_leg(args.serve, args.serve_step_scale, args.serve_size_scale, args.precision)
+           if args.serve else None)
     total_pairs = pairs * args.steps
     value = total_pairs / el
     res = {
@@ -201,6 +257,10 @@ def main():
         "ms_per_step_graph": el_graph / args.steps * 1e3,
         "device": info,
     }
+    if ens:
+        res["ensemble"] = ens
+    if srv:
+        res["serving"] = srv
     print(json.dumps(res))
 
 

@@ -18,3 +18,4 @@ from .integration import (  # noqa: F401
     Session, create_grid, sort_particles, sweep, time_integrate, update_state)
 from . import configs  # noqa: F401
 from . import serving  # noqa: F401
+from . import snapshot  # noqa: F401

@@ -566,7 +566,34 @@ __global__ void __launch_bounds__(64)
             __syncthreads();
             const int a = mem ? (j0 > w0 ? j0 : w0) : 0;
             const int e = mem ? (j1 < w1 ? j1 : w1) : 0;
-            for (int j = a; j < e; ++j) {
+            // two candidates per iteration (independent chains), kept in order
+            int j = a;
+            for (; j + 1 < e; j += 2) {
+              const int t = j - w0;
+              double da[D], db[D];
+#pragma unroll
+              for (int d = 0; d < D; ++d) {
+                da[d] = xi[d] - sx[d][t];
+                db[d] = xi[d] - sx[d][t + 1];
+              }
+              double ra = da[0] * da[0], rb = db[0] * db[0];
+#pragma unroll
+              for (int d = 1; d < D; ++d) {
+                ra = ra + da[d] * da[d];
+                rb = rb + db[d] * db[d];
+              }
+              if (ra < c2 && j != self_j) {
+                if (cnt < L.maxnb) my[(int64_t)cnt * L.cap] = j;
+                ++cnt;
+                ++cq;
+              }
+              if (rb < c2 && j + 1 != self_j) {
+                if (cnt < L.maxnb) my[(int64_t)cnt * L.cap] = j + 1;
+                ++cnt;
+                ++cq;
+              }
+            }
+            if (j < e) {
               const int t = j - w0;
               double dx[D];
 #pragma unroll

@@ -9,7 +9,6 @@ device against the same host arrays.
 """
 from __future__ import annotations
 
-import os
 import time
 import warnings
 
@@ -17,6 +16,7 @@ import numpy as np
 
 from . import _abi as A
 from . import model as M
+from . import snapshot as SN
 from .scene import DeviceScene
 
 _PREC = {"f32": A.GR_PREC_F32, "f64": A.GR_PREC_F64}
@@ -58,26 +58,16 @@ def _print_summary(ps, step, t):
     print(f"[{ps.name}] step {step} (t = {t:.6g})\n" + "\n".join(rows))
 
 
-def _save_snapshot(lf, path, step, dt):
-    """Snapshot of every system at `path` (.npz, one group prefix per system;
-    stands in for the HDF5 writer of src/TimeIntegration.jl:520-549)."""
-    d = os.path.dirname(path)
-    if d:
-        os.makedirs(d, exist_ok=True)
-    out = {"step": np.int64(step), "sim_time": np.float64(step * dt)}
-    for ps in lf.systems:
-        for f in ps.field_names() + ["id"]:
-            out[f"{ps.name}/{f}"] = getattr(ps, f)
-    np.savez(path, **out)
-
-
 def time_integrate(integrator, num_timesteps, print_interval_step, save_interval_step, CFL,
                    output_prefix, *, step_offset=0, output_width=None, print_timer=True,
                    session=None, precision="f32"):
     """Run `num_timesteps` leapfrog steps on the device.
 
-    dt = CFL·h/c (src/TimeIntegration.jl:598).  Returns a dict of timings
-    (seconds of wall time around device work, and steps/s).
+    dt = CFL·h/c (src/TimeIntegration.jl:598).  Every `save_interval_step`
+    global steps an HDF5 snapshot in the reference's layout is written to
+    "<output_prefix>_<padded step>.h5" (src/TimeIntegration.jl:520-549).
+    Returns a dict of timings (seconds of wall time around device work, and
+    steps/s).
     """
     lf = integrator
     nst = len(lf.interactions[0].pfns)
@@ -116,7 +106,8 @@ def time_integrate(integrator, num_timesteps, print_interval_step, save_interval
                 for ps in lf.systems:
                     _print_summary(ps, g, g * dt)
             if saving:
-                _save_snapshot(lf, f"{output_prefix}_{str(g).zfill(width)}.npz", g, dt)
+                SN.save_snapshot(SN.snapshot_path(output_prefix, g, width), lf.systems,
+                                 lf.ghosts, g, dt)
     sc.download_all(lf)
     el = time.perf_counter() - t0
     res = {"seconds": el, "steps": num_timesteps, "dt": dt,

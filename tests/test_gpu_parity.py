@@ -547,3 +547,36 @@ def test_release_reuses_segments_and_keeps_results():
             assert np.array_equal(getattr(cfgs[k]["fluid"], f), getattr(ref, f)), (k, f)
     with pytest.raises(RuntimeError):
         s.backend.step_many([hs["a"]], 1)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_device_snapshot_restart_continues_bit_identically(tmp_path, prec):
+    """Device: 12 steps straight vs 7 steps, HDF5 snapshot in the reference
+    layout (with dvdt/drhodt and ghosts), restart into fresh objects on a new
+    context, 5 more steps (src/Driver.jl:79-96)."""
+    SN = M.snapshot
+
+    def cfg():
+        return M.configs.ep_column_2d(dx=0.01, width=0.1, height=0.06)
+
+    def run(c, nsteps):
+        lf = c["integrator"]
+        s = M.Session(2, prec)
+        h = s.scene.integrator(lf)
+        s.scene.upload_all(lf)
+        dt = s.backend.integrator_dt(h, c["cfl"])
+        s.backend.step(h, nsteps, dt)
+        s.scene.download_all(lf)
+        return dt
+
+    ref = cfg()
+    run(ref, 12)
+    a = cfg()
+    dt = run(a, 7)
+    path = str(tmp_path / "col_07.h5")
+    SN.save_snapshot(path, a["integrator"].systems, a["integrator"].ghosts, 7, dt)
+    b = cfg()
+    assert SN.load_restart(path, b["integrator"].systems) == 7
+    run(b, 5)
+    for f in ["x"] + SN.sim_field_names(ref["fluid"]):
+        assert np.array_equal(getattr(b["fluid"], f), getattr(ref["fluid"], f)), f
