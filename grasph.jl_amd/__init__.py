@@ -9,10 +9,11 @@ from . import _abi  # noqa: F401
 from .model import (  # noqa: F401
     BasicParticleSystem, CauchyFluidPfn, CubicSplineKernel, DynamicBoundarySystem,
     ElastoPlasticParticleSystem, ElastoPlasticStressUpdater, FluidParticleSystem, FluidPfn,
-    GhostCopier, GhostEntry, GhostParticleSystem, HouseholderReflect, LeapFrogTimeIntegrator,
-    LinearEOSUpdater, NeighborCounterPfn, StaticBoundarySystem, StrainRatePfn,
-    StrainRateVorticityPfn, StressParticleSystem, SystemInteraction, TaitEOSUpdater,
-    ViscoPlasticMCStressUpdater, WenlandC2Kernel, ZeroFieldUpdater)
+    GhostCopier, GhostEntry, GhostParticleSystem, HouseholderReflect, InterpolateFieldFn,
+    LeapFrogTimeIntegrator, LinearEOSUpdater, NeighborCounterPfn, PrescribedVelocityUpdater,
+    StaticBoundarySystem, StrainRatePfn, StrainRateVorticityPfn, StressParticleSystem,
+    SystemInteraction, TaitEOSUpdater, ViscoPlasticMCStressUpdater, VirtualNormUpdater,
+    VirtualParticleSystem, WenlandC2Kernel, ZeroFieldUpdater)
 from .scene import DeviceScene  # noqa: F401
 from .integration import (  # noqa: F401
     Session, create_grid, sort_particles, sweep, time_integrate, update_state)

@@ -24,22 +24,24 @@ GR_SYS_BASIC, GR_SYS_FLUID, GR_SYS_STRESS, GR_SYS_EP, GR_SYS_GHOST = range(5)
 # ---- fields -------------------------------------------------------------------
 (GR_F_X, GR_F_V, GR_F_RHO, GR_F_DVDT, GR_F_DRHODT, GR_F_P, GR_F_STRESS,
  GR_F_STRAIN_RATE, GR_F_VORTICITY, GR_F_STRAIN, GR_F_STRAIN_P,
- GR_F_V_ADJUSTMENT, GR_F_NORMALS) = range(13)
+ GR_F_V_ADJUSTMENT, GR_F_NORMALS, GR_F_W_SUM) = range(14)
 GR_I_ID, GR_I_IDX_ORIGINAL, GR_I_IDX_BOUNDARY = range(3)
 FIELD_IDS = {
     "x": GR_F_X, "v": GR_F_V, "rho": GR_F_RHO, "dvdt": GR_F_DVDT,
     "drhodt": GR_F_DRHODT, "p": GR_F_P, "stress": GR_F_STRESS,
     "strain_rate": GR_F_STRAIN_RATE, "vorticity": GR_F_VORTICITY,
     "strain": GR_F_STRAIN, "strain_p": GR_F_STRAIN_P,
-    "v_adjustment": GR_F_V_ADJUSTMENT, "normals": GR_F_NORMALS,
+    "v_adjustment": GR_F_V_ADJUSTMENT, "normals": GR_F_NORMALS, "w_sum": GR_F_W_SUM,
 }
 INDEX_IDS = {"id": GR_I_ID, "idx_original": GR_I_IDX_ORIGINAL,
              "idx_boundary": GR_I_IDX_BOUNDARY}
 # ---- kernels / functors / wrappers / updaters / copier modes ------------------
 GR_KERNEL_CUBIC, GR_KERNEL_WENDLAND_C2 = 0, 1
-GR_PFN_NONE, GR_PFN_FLUID, GR_PFN_CAUCHY, GR_PFN_SR, GR_PFN_SRV, GR_PFN_COUNT = range(6)
+(GR_PFN_NONE, GR_PFN_FLUID, GR_PFN_CAUCHY, GR_PFN_SR, GR_PFN_SRV, GR_PFN_COUNT,
+ GR_PFN_INTERP) = range(7)
 GR_WRAP_NONE, GR_WRAP_STATIC, GR_WRAP_DYNAMIC = range(3)
-GR_UPD_NOTHING, GR_UPD_ZERO, GR_UPD_TAIT, GR_UPD_LINEAR, GR_UPD_EP, GR_UPD_VPMC = range(6)
+(GR_UPD_NOTHING, GR_UPD_ZERO, GR_UPD_TAIT, GR_UPD_LINEAR, GR_UPD_EP, GR_UPD_VPMC, GR_UPD_VNORM,
+ GR_UPD_PRESCRIBED_V) = range(8)
 GR_COPY_PLAIN, GR_COPY_HOUSEHOLDER = 0, 1
 
 
@@ -67,6 +69,7 @@ _SIGS = {
     "system_download_index": [_i, _i, _i64p, _i64],
     "system_get_n": [_i, _i64p],
     "ghost_create": [_i, _d, _i, _dp, _dp, _ip],
+    "virtual_create": [_i, _i, _ip, _dp],
     "ghost_set_copier": [_i, _i, _i, _ip, _ip],
     "interaction_create": [_i, _d, _i, _i, _i, _dp, _dp, _d, _d, _i,
                            C.POINTER(gr_pfn_desc), _ip],
@@ -197,6 +200,11 @@ class CBackend:
         out = C.c_int64()
         self.call("system_get_n", sys, C.byref(out))
         return out.value
+
+    def virtual_create(self, sys, zero_fields, prescribed_v):
+        z = np.ascontiguousarray(zero_fields, dtype=np.int32)
+        pv = np.ascontiguousarray(prescribed_v, dtype=np.float64)
+        self.call("virtual_create", sys, len(z), _iptr(z), _dptr(pv))
 
     def ghost_create(self, src, cutoff, normals, points):
         nn = np.ascontiguousarray(normals, dtype=np.float64).reshape(-1)

@@ -227,3 +227,55 @@ def granular_column_3d(dx=0.005, width=0.2, depth=0.05, height=0.1, rho0=1850.0,
     lf = M.LeapFrogTimeIntegrator([soil, bottom], ints, ghosts=(entry,))
     return {"integrator": lf, "fluid": soil, "cfl": cfl, "name": "granular_column_3d",
             "n_particles": soil.n + bottom.n, "fluid_n": soil.n, "boundary_n": bottom.n}
+
+
+def ep_column_virtual_2d(dx=0.0025, width=0.2, height=0.1, rho0=1850.0, E=0.84e6, nu=0.3,
+                         phi_deg=19.8, psi=0.0, cohesion=0.0, alpha=0.1, beta=0.1, h_fac=1.5,
+                         cfl=0.1, name="soil"):
+    """EP column collapse with VirtualParticleSystem boundaries
+    (EP_ColumnCollapse2.jl): four stages per step, (InterpolateFieldFn(:v,
+    :rho), StrainRateVorticityPfn, InterpolateFieldFn(:stress; no w_sum),
+    CauchyFluidPfn) against a no-slip virtual floor (v_mult (−1, −1)) and a
+    free-slip virtual left wall (v_mult (−1, 1)), 3 layers each; soil
+    updaters (nothing, Zero(:strain_rate, :vorticity), Drucker–Prager,
+    nothing); initial stress from the stage-3 updater at rest (zero)."""
+    phi = phi_deg * math.pi / 180.0
+    c = math.sqrt(E * (1 - nu) / (rho0 * (1 + nu) * (1 - 2 * nu)))
+    h = h_fac * dx
+    nfx, nfy = _round(width / dx), _round(height / dx)
+    nbx = _round(4.0 * width / dx)
+    mass = rho0 * dx * dx
+    soil = M.ElastoPlasticParticleSystem(
+        name, nfx * nfy, 2, 4, mass, c, source_v=[0.0, -G],
+        state_updater=(None, M.ZeroFieldUpdater("strain_rate", "vorticity"),
+                       M.ElastoPlasticStressUpdater(E, nu, phi, psi, cohesion), None))
+    soil.x[:] = _lattice(nfx, nfy, dx)
+    soil.rho[:] = rho0
+    i, j = np.meshgrid(np.arange(1, nbx + 4), np.arange(1, 4), indexing="ij")
+    bot_src = M.StressParticleSystem(f"{name}_bottom_boundary", i.size, 2, 4, mass, c)
+    bot_src.x[:] = np.stack([(i - 3.5) * dx, -(j - 0.5) * dx], -1).reshape(-1, 2)
+    bot_src.rho[:] = rho0
+    jj, col = np.meshgrid(np.arange(nfy), np.arange(1, 4), indexing="ij")
+    left_src = M.StressParticleSystem(f"{name}_left_boundary", jj.size, 2, 4, mass, c)
+    left_src.x[:] = np.stack([-(col - 0.5) * dx, (jj + 0.5) * dx], -1).reshape(-1, 2)
+    left_src.rho[:] = rho0
+    bottom = M.VirtualParticleSystem(
+        bot_src, f"{name}_bottom_virt", bot_src.n, 2, mass, c, zero_fields=("v", "rho", "stress"),
+        state_updater=(None, M.VirtualNormUpdater((-1.0, -1.0), "v", "rho"), None,
+                       M.VirtualNormUpdater((-1.0, -1.0), "stress")))
+    left = M.VirtualParticleSystem(
+        left_src, f"{name}_left_virt", left_src.n, 2, mass, c, zero_fields=("v", "rho", "stress"),
+        state_updater=(None, M.VirtualNormUpdater((-1.0, 1.0), "v", "rho"), None,
+                       M.VirtualNormUpdater((-1.0, 1.0), "stress")))
+    kern = M.CubicSplineKernel(h, ndims=2)
+    sr = M.StrainRateVorticityPfn()
+    kin = M.CauchyFluidPfn(alpha, beta, h)
+    iv = M.InterpolateFieldFn("v", "rho", accumulate_wsum=True)
+    ist = M.InterpolateFieldFn("stress", accumulate_wsum=False)
+    ints = [M.SystemInteraction(kern, (None, sr, None, kin), soil),
+            M.SystemInteraction(kern, (iv, sr, ist, kin), soil, bottom),
+            M.SystemInteraction(kern, (iv, sr, ist, kin), soil, left)]
+    lf = M.LeapFrogTimeIntegrator([soil], ints, virtual_systems=(bottom, left))
+    return {"integrator": lf, "fluid": soil, "cfl": cfl, "name": "ep_column_virtual_2d",
+            "n_particles": soil.n + bot_src.n + left_src.n, "fluid_n": soil.n,
+            "virtual": (bottom, left)}

@@ -107,7 +107,7 @@ def time_integrate(integrator, num_timesteps, print_interval_step, save_interval
                     _print_summary(ps, g, g * dt)
             if saving:
                 SN.save_snapshot(SN.snapshot_path(output_prefix, g, width), lf.systems,
-                                 lf.ghosts, g, dt)
+                                 lf.ghosts, g, dt, lf.virtual_systems)
     sc.download_all(lf)
     el = time.perf_counter() - t0
     res = {"seconds": el, "steps": num_timesteps, "dt": dt,

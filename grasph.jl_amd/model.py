@@ -131,8 +131,37 @@ class ViscoPlasticMCStressUpdater:
         return A.GR_UPD_VPMC, [float(kind), prm[0], gamma, self.friction_angle, self.cohesion]
 
 
+class VirtualNormUpdater:
+    """VirtualNormUpdater(v_mult, :f1, :f2, ...) (src/StateUpdaters.jl:58-84):
+    divides each field by w_sum (zero where w_sum is 0), then v ← v·v_mult +
+    prescribed_v.  Fields: v, rho, p, stress."""
+
+    def __init__(self, v_mult, *fields):
+        for f in fields:
+            if f not in ("v", "rho", "p", "stress"):
+                raise ValueError(f"VirtualNormUpdater field :{f} has no device implementation")
+        self.v_mult = np.asarray(v_mult, dtype=np.float64).ravel()
+        self.fields = tuple(fields)
+
+    def encode(self):
+        mask = 0
+        for f in self.fields:
+            mask |= 1 << A.FIELD_IDS[f]
+        vm = list(self.v_mult) + [0.0] * (3 - len(self.v_mult))
+        return A.GR_UPD_VNORM, vm[:3] + [float(mask)]
+
+
+class PrescribedVelocityUpdater:
+    """PrescribedVelocityUpdater() (src/StateUpdaters.jl:86-92): v ← v +
+    prescribed_v of the virtual system."""
+
+    def encode(self):
+        return A.GR_UPD_PRESCRIBED_V, []
+
+
 _UPDATER_TYPES = (ZeroFieldUpdater, TaitEOSUpdater, LinearEOSUpdater,
-                  ElastoPlasticStressUpdater, ViscoPlasticMCStressUpdater)
+                  ElastoPlasticStressUpdater, ViscoPlasticMCStressUpdater, VirtualNormUpdater,
+                  PrescribedVelocityUpdater)
 
 
 def _normalise_updaters(state_updater):
@@ -246,6 +275,48 @@ class ElastoPlasticParticleSystem(StressParticleSystem):
 
     def field_names(self):
         return super().field_names() + ["vorticity", "strain", "strain_p"]
+
+
+class VirtualParticleSystem:
+    """VirtualParticleSystem(source, name, n, ndims, mass, c; state_updater,
+    zero_fields, prescribed_v) (src/Particles.jl:476-511).
+
+    Every field is the source system's (getproperty forwarding, :513-518);
+    the virtual system adds w_sum, its own state updaters, the fields zeroed
+    before each step's stage loop and a prescribed velocity."""
+
+    def __init__(self, source, name, n, ndims, mass, c, state_updater=(), zero_fields=(),
+                 prescribed_v=None):
+        if not isinstance(source, _ParticleSystem):
+            raise ValueError("source must be a particle system")
+        if int(ndims) != source.ndims:
+            raise ValueError(f"ndims={ndims} does not match source ndims={source.ndims}")
+        if int(n) != source.n:
+            raise ValueError(f"n={n} does not match source n={source.n}")
+        for f in zero_fields:
+            if f not in A.FIELD_IDS or f == "x" or not hasattr(source, f):
+                raise ValueError(f"zero_fields: :{f} is not a field of the source")
+        self.__dict__.update(
+            source=source, name=str(name), state_updater=_normalise_updaters(state_updater),
+            zero_fields=tuple(zero_fields), w_sum=np.zeros(source.n),
+            prescribed_v=(np.zeros(source.ndims) if prescribed_v is None
+                          else np.asarray(prescribed_v, dtype=np.float64).reshape(source.ndims)))
+
+    def __getattr__(self, k):
+        return getattr(self.__dict__["source"], k)
+
+    def __setattr__(self, k, v):
+        if k in self.__dict__:
+            self.__dict__[k] = v
+        else:
+            setattr(self.__dict__["source"], k, v)
+
+    @property
+    def kind(self):
+        return self.source.kind
+
+    def field_names(self):
+        return self.source.field_names()
 
 
 # ---------------------------------------------------------------------------
@@ -439,7 +510,29 @@ class NeighborCounterPfn:
         return (A.GR_PFN_COUNT, 2, 0.0, 0.0, 0.0, math.nan, math.nan)
 
 
-_PFN_TYPES = (FluidPfn, CauchyFluidPfn, StrainRatePfn, StrainRateVorticityPfn, NeighborCounterPfn)
+class InterpolateFieldFn:
+    """InterpolateFieldFn(:f1, :f2, ...; accumulate_wsum=true)
+    (src/PairwiseFunctors.jl:405-434): the virtual side of a (real, virtual)
+    interaction accumulates f += (m/ρ)_real · W · f_real (and w_sum).
+    Fields: v, rho, p, stress."""
+
+    def __init__(self, *fields, accumulate_wsum=True):
+        for f in fields:
+            if f not in ("v", "rho", "p", "stress"):
+                raise ValueError(f"InterpolateFieldFn field :{f} has no device implementation")
+        self.fields = tuple(fields)
+        self.accumulate_wsum = bool(accumulate_wsum)
+
+    def encode(self):
+        mask = 0
+        for f in self.fields:
+            mask |= 1 << A.FIELD_IDS[f]
+        return (A.GR_PFN_INTERP, mask, 1.0 if self.accumulate_wsum else 0.0, 0.0, 0.0,
+                math.nan, math.nan)
+
+
+_PFN_TYPES = (FluidPfn, CauchyFluidPfn, StrainRatePfn, StrainRateVorticityPfn, NeighborCounterPfn,
+              InterpolateFieldFn)
 
 
 def encode_pfn(p):
@@ -482,12 +575,16 @@ class SystemInteraction:
 
 
 class LeapFrogTimeIntegrator:
-    """LeapFrogTimeIntegrator(systems, interactions; ghosts=(), Γ=0)."""
+    """LeapFrogTimeIntegrator(systems, interactions; ghosts=(),
+    virtual_systems=(), Γ=0)."""
 
-    def __init__(self, systems, interactions, ghosts=(), gamma=0.0):
+    def __init__(self, systems, interactions, ghosts=(), gamma=0.0, virtual_systems=()):
         sys = (systems,) if isinstance(systems, _ParticleSystem) else tuple(systems)
         ints = (interactions,) if isinstance(interactions, SystemInteraction) else tuple(interactions)
         gsts = (ghosts,) if isinstance(ghosts, GhostEntry) else tuple(ghosts)
+        vsys = ((virtual_systems,) if isinstance(virtual_systems, VirtualParticleSystem)
+                else tuple(virtual_systems))
+        self.virtual_systems = vsys
         if not sys:
             raise ValueError("systems must not be empty")
         if not ints:

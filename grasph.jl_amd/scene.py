@@ -36,13 +36,18 @@ class DeviceScene:
         self._ghost = {}
         self._inter = {}
         self._integ = {}
+        self._virt = {}
         self._objs = []  # keep the mirrored objects alive (ids stay valid)
 
     # ---- registration -------------------------------------------------
     def system(self, ps):
-        """Handle of a real particle system (boundary wrappers → inner)."""
+        """Handle of a real particle system (boundary wrappers → inner; a
+        VirtualParticleSystem → its source, registered with the virtual
+        system's updaters, zero fields and prescribed velocity)."""
         if isinstance(ps, (M.StaticBoundarySystem, M.DynamicBoundarySystem)):
             ps = ps.inner
+        if isinstance(ps, M.VirtualParticleSystem):
+            return self._virtual(ps)
         k = id(ps)
         if k in self._sys:
             return self._sys[k]
@@ -64,6 +69,33 @@ class DeviceScene:
                     self.be.system_add_updater(h, stage, kind, prm)
         self._sys[k] = h
         self._objs.append(ps)
+        return h
+
+    def _virtual(self, vps):
+        src = vps.source
+        k = id(src)
+        if k in self._sys:
+            if self._virt.get(k) is not vps:
+                raise ValueError(f"system {src.name} is already registered as a plain system")
+            return self._sys[k]
+        if src.ndims != self.ndims:
+            raise ValueError("system dimension does not match the context")
+        h = self.be.system_create(src.kind, src.n, src.ns, src.mass, src.c, src.source_v,
+                                  src.source_rho)
+        self.be.virtual_create(h, [A.FIELD_IDS[f] for f in vps.zero_fields], vps.prescribed_v)
+        for stage, entry in enumerate(vps.state_updater, start=1):
+            items = entry if isinstance(entry, tuple) else (entry,)
+            if not items:
+                self.be.system_add_updater(h, stage, A.GR_UPD_NOTHING, [])
+            for u in items:
+                if u is None:
+                    self.be.system_add_updater(h, stage, A.GR_UPD_NOTHING, [])
+                else:
+                    kind, prm = u.encode()
+                    self.be.system_add_updater(h, stage, kind, prm)
+        self._sys[k] = h
+        self._virt[k] = vps
+        self._objs.extend([vps, src])
         return h
 
     def ghost(self, entry):
@@ -119,6 +151,7 @@ class DeviceScene:
         if k in self._integ:
             return self._integ[k]
         sys = [self.system(ps) for ps in lf.systems]
+        sys += [self.system(v) for v in getattr(lf, "virtual_systems", ())]
         gh = [self.ghost(ge) for ge in lf.ghosts]
         its = [self.interaction(si) for si in lf.interactions]
         h = self.be.integrator_create(sys, its, gh, lf.gamma)
@@ -136,6 +169,9 @@ class DeviceScene:
             inner = ps.inner if isinstance(ps, (M.StaticBoundarySystem,
                                                 M.DynamicBoundarySystem)) else ps
             self._sys.pop(id(inner), None)
+        for v in getattr(lf, "virtual_systems", ()):
+            self._sys.pop(id(v.source), None)
+            self._virt.pop(id(v.source), None)
         for ge in lf.ghosts:
             self._ghost.pop(id(ge.ghost), None)
         for si in lf.interactions:
@@ -148,6 +184,9 @@ class DeviceScene:
     def upload(self, ps, fields=None):
         if isinstance(ps, (M.StaticBoundarySystem, M.DynamicBoundarySystem)):
             ps = ps.inner
+        if isinstance(ps, M.VirtualParticleSystem):
+            self._virtual(ps)
+            ps = ps.source
         if isinstance(ps, M.GhostParticleSystem):
             h = self.ghost_handle(ps)
             names = fields or ["x", "v", "rho"] + list(ps.extras)
@@ -163,6 +202,10 @@ class DeviceScene:
     def download(self, ps, fields=None):
         if isinstance(ps, (M.StaticBoundarySystem, M.DynamicBoundarySystem)):
             ps = ps.inner
+        if isinstance(ps, M.VirtualParticleSystem):
+            h = self._virtual(ps)
+            ps.w_sum[:] = self.be.system_download(h, A.GR_F_W_SUM, ps.n)
+            ps = ps.source
         if isinstance(ps, M.GhostParticleSystem):
             h = self.ghost_handle(ps)
             n = self.be.system_get_n(h)
@@ -184,11 +227,11 @@ class DeviceScene:
 
     def upload_all(self, lf):
         self.integrator(lf)
-        for ps in lf.systems:
+        for ps in list(lf.systems) + list(getattr(lf, "virtual_systems", ())):
             self.upload(ps)
 
     def download_all(self, lf):
-        for ps in lf.systems:
+        for ps in list(lf.systems) + list(getattr(lf, "virtual_systems", ())):
             self.download(ps)
         for ge in lf.ghosts:
             self.download(ge.ghost)

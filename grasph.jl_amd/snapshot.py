@@ -203,7 +203,11 @@ def sim_field_names(ps):
 
 
 def _inner(ps):
-    return ps.inner if isinstance(ps, (M.StaticBoundarySystem, M.DynamicBoundarySystem)) else ps
+    if isinstance(ps, (M.StaticBoundarySystem, M.DynamicBoundarySystem)):
+        return ps.inner
+    if isinstance(ps, M.VirtualParticleSystem):  # fields forwarded to the source
+        return ps.source
+    return ps
 
 
 def write_group(loc, ps):
@@ -268,9 +272,10 @@ def snapshot_path(prefix, step, width):
     return f"{prefix}_{str(step).rjust(width, '0')}.h5"
 
 
-def save_snapshot(path, systems, ghosts, step, dt):
+def save_snapshot(path, systems, ghosts, step, dt, virtual_systems=()):
     """The body of _maybe_save! (src/TimeIntegration.jl:526-547): file attrs
-    step and sim_time, one group per system, then one per ghost system."""
+    step and sim_time, one group per system, then one per ghost system, then
+    one per virtual system (named after the virtual system)."""
     d = os.path.dirname(str(path))
     if d:
         Path(d).mkdir(parents=True, exist_ok=True)
@@ -279,10 +284,11 @@ def save_snapshot(path, systems, ghosts, step, dt):
     try:
         _attr_write(f, "step", int(step))
         _attr_write(f, "sim_time", float(step * dt))
-        for ps in list(systems) + [ge.ghost for ge in ghosts]:
+        for ps in list(systems) + [ge.ghost for ge in ghosts] + list(virtual_systems):
+            name = ps.name
             ps = _inner(ps)
-            g = _ok(h.H5Gcreate2(f, _bytes(ps.name), _P_DEFAULT, _P_DEFAULT, _P_DEFAULT),
-                    f"group {ps.name}")
+            g = _ok(h.H5Gcreate2(f, _bytes(name), _P_DEFAULT, _P_DEFAULT, _P_DEFAULT),
+                    f"group {name}")
             try:
                 write_group(g, ps)
             finally:
@@ -299,9 +305,10 @@ def load_restart(path, systems):
     try:
         step = _attr_read(f, "step", int)
         for ps in systems:
+            name = ps.name
             ps = _inner(ps)
-            if _exists(f, ps.name):
-                g = _ok(h.H5Gopen2(f, _bytes(ps.name), _P_DEFAULT), f"group {ps.name}")
+            if _exists(f, name):
+                g = _ok(h.H5Gopen2(f, _bytes(name), _P_DEFAULT), f"group {name}")
                 try:
                     read_group(g, ps)
                 finally:

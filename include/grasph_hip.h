@@ -61,7 +61,8 @@ enum {
   GR_F_P = 5, GR_F_STRESS = 6, GR_F_STRAIN_RATE = 7, GR_F_VORTICITY = 8,
   GR_F_STRAIN = 9, GR_F_STRAIN_P = 10, GR_F_V_ADJUSTMENT = 11,
   GR_F_NORMALS = 12,
-  GR_NUM_FIELDS = 13
+  GR_F_W_SUM = 13,  /* VirtualParticleSystem w_sum (download only) */
+  GR_NUM_FIELDS = 14
 };
 /* ---- per-particle index fields (int64, 1-based) ----------------------- */
 enum { GR_I_ID = 0, GR_I_IDX_ORIGINAL = 1, GR_I_IDX_BOUNDARY = 2 };
@@ -76,9 +77,14 @@ enum {
   GR_PFN_CAUCHY = 2,  /* CauchyFluidPfn :252                               */
   GR_PFN_SR = 3,      /* StrainRatePfn :18                                 */
   GR_PFN_SRV = 4,     /* StrainRateVorticityPfn :78                        */
-  GR_PFN_COUNT = 5    /* neighbour counter: +1 to dvdt[1] and drhodt per pair,
+  GR_PFN_COUNT = 5,   /* neighbour counter: +1 to dvdt[1] and drhodt per pair,
                          both sides (the counting functors of
                          test/test_interaction_sweep.jl:14-30)              */
+  GR_PFN_INTERP = 6   /* InterpolateFieldFn :405-434 between a real system
+                         and a virtual one: the virtual side accumulates
+                         f += (m/ρ)·W·f_real; `sigma` holds the field bit
+                         mask (1<<GR_F_V | 1<<GR_F_RHO | 1<<GR_F_P |
+                         1<<GR_F_STRESS), `alpha` != 0 accumulates w_sum  */
 };
 
 /* ---- partner wrappers (src/BoundaryParticles.jl:38,68) ---------------- */
@@ -93,9 +99,13 @@ enum {
   GR_UPD_LINEAR = 3,   /* LinearEOSUpdater :131; params = {rho0}           */
   GR_UPD_EP = 4,       /* ElastoPlasticStressUpdater :202;
                           params = {E, nu, phi, psi, cohesion}             */
-  GR_UPD_VPMC = 5      /* ViscoPlasticMCStressUpdater :144;
+  GR_UPD_VPMC = 5,     /* ViscoPlasticMCStressUpdater :144;
                           params = {eos_kind(GR_UPD_TAIT|LINEAR), rho0,
                                     gamma, phi, cohesion}                  */
+  GR_UPD_VNORM = 6,    /* VirtualNormUpdater :58-77 (virtual systems);
+                          params = {v_mult[0], v_mult[1], v_mult[2],
+                                    field bit mask}                        */
+  GR_UPD_PRESCRIBED_V = 7  /* PrescribedVelocityUpdater :88-92; no params */
 };
 
 /* ---- ghost copier modes (src/GhostParticles.jl:26,53) ------------------ */
@@ -152,6 +162,17 @@ int gr_system_download_index(gr_ctx *ctx, int sys, int field, int64_t *data,
 /* Current particle count (ghost systems: the count after the last
  * generate_ghosts; reference `ghost.n`, src/GhostParticles.jl:228). */
 int gr_system_get_n(gr_ctx *ctx, int sys, int64_t *n_out);
+
+/* Turn system `sys` into the source of a VirtualParticleSystem
+ * (src/Particles.jl:488-511): it is sorted every step but not integrated,
+ * `zero_fields` (GR_F_* ids) and its w_sum are zeroed before each step's
+ * stage loop, updaters registered on it are the virtual system's (with
+ * GR_UPD_VNORM / GR_UPD_PRESCRIBED_V), its positions advance by
+ * prescribed_v·dt (ndims doubles, may be NULL for zero), and as an
+ * interaction partner it is one-sided (src/PairwiseFunctors.jl:286,97,193).
+ * Pass it in gr_integrator_create's `systems`; it is excluded from dt's c. */
+int gr_virtual_create(gr_ctx *ctx, int sys, int nzero, const int *zero_fields,
+                      const double *prescribed_v);
 
 /* ---- ghosts --------------------------------------------------------------
  * GhostParticleSystem(ps, copiers...) + GhostEntry(ghost, cutoff,

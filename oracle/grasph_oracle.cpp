@@ -54,6 +54,11 @@ struct Sys {
   std::vector<double> normals;
   bool ex_p = false, ex_stress = false;
   std::vector<std::vector<std::pair<int, int>>> copiers;  // per stage
+  // VirtualParticleSystem over this system's arrays (src/Particles.jl:476-511)
+  bool virt = false;
+  std::vector<double> wsum;
+  int64_t zero_mask = 0;
+  double pv[3] = {0, 0, 0};
 };
 
 struct Inter {
@@ -70,7 +75,7 @@ struct Inter {
 };
 
 struct Integ {
-  std::vector<int> sys, inter, ghost;
+  std::vector<int> sys, inter, ghost, virt;
   double gamma = 0, h = 0, c = 0;
   int nst = 0;
 };
@@ -347,7 +352,7 @@ PartnerKind classify(const orc_scene *sc, const Inter &in, int pfn_kind) {
   }
   const Sys &b = sc->sys[in.b];
   const Sys &a = sc->sys[in.a];
-  if (b.kind == GR_SYS_GHOST) return P_ONE_SIDED;
+  if (b.kind == GR_SYS_GHOST || b.virt || a.virt) return P_ONE_SIDED;
   switch (pfn_kind) {
     case GR_PFN_FLUID:
       return (a.kind == GR_SYS_FLUID && b.kind == GR_SYS_FLUID) ? P_TWO_SIDED
@@ -373,7 +378,24 @@ struct PairCtx {
   Sys *Bw;       // writable partner (two-sided / self), else null
   const double *wn, *wp;
   double beta, ljcut;
+  int interp_side = 0;  // InterpolateFieldFn: 1 = into system_b, 2 = into system_a
+  Sys *Vw = nullptr;    // the virtual system written by InterpolateFieldFn
 };
+
+// f_v[t] += kw · f_r[s] for the fields of `mask` (InterpolateFieldFn
+// _interp_fields_ab!/_ba!, src/PairwiseFunctors.jl:411-422).
+void interp_fields(int mask, Sys &V, int64_t t, const Sys &Rl, int64_t s, double kw) {
+  const int D = V.D;
+  if (mask & (1 << GR_F_V))
+    for (int d = 0; d < D; ++d) V.v[t * D + d] = V.v[t * D + d] + kw * Rl.v[s * D + d];
+  if (mask & (1 << GR_F_RHO)) V.rho[t] = V.rho[t] + kw * Rl.rho[s];
+  if ((mask & (1 << GR_F_P)) && !V.p.empty() && !Rl.p.empty()) V.p[t] = V.p[t] + kw * Rl.p[s];
+  if ((mask & (1 << GR_F_STRESS)) && !V.stress.empty() && !Rl.stress.empty()) {
+    const int NS = V.NS;
+    for (int k = 0; k < NS; ++k)
+      V.stress[t * NS + k] = V.stress[t * NS + k] + kw * Rl.stress[s * NS + k];
+  }
+}
 
 // One accepted pair: the functor call of _pair_self!/_pair_coupled!
 // (src/Interaction.jl:554,729) with the dispatch chosen by classify().
@@ -388,6 +410,19 @@ void apply_pair(const PairCtx &P, int64_t i, int64_t j, const double *dx,
   const double *vj = &B.v[j * D];
   double dv[3], tmp[6];
   switch (f.kind) {
+    case GR_PFN_INTERP: {  // :424-434
+      const int mask = f.sigma;
+      if (P.interp_side == 1) {
+        const double kw = w * (P.mass_a / A.rho[i]);
+        interp_fields(mask, *P.Vw, j, A, i, kw);
+        if (f.alpha != 0.0) P.Vw->wsum[j] += kw;
+      } else if (P.interp_side == 2) {
+        const double kw = w * (P.B.mass / B.rho[j]);
+        interp_fields(mask, *P.Vw, i, B, j, kw);
+        if (f.alpha != 0.0) P.Vw->wsum[i] += kw;
+      }
+      return;
+    }
     case GR_PFN_COUNT: {
       P.Aw->dvdt[i * D] += 1.0;
       P.Aw->drhodt[i] += 1.0;
@@ -815,6 +850,19 @@ int sweep(orc_scene *sc, Inter &in, int stage) {
     P.B = make_view(sc, in.b);
     P.Bw = (P.pk == P_TWO_SIDED) ? &sc->sys[in.b] : nullptr;
   }
+  if (f.kind == GR_PFN_INTERP) {
+    if (in.b >= 0 && sc->sys[in.b].virt && in.wrap == GR_WRAP_NONE) {
+      P.interp_side = 1;
+      P.Vw = &sc->sys[in.b];
+    } else if (in.b >= 0 && A.virt && !sc->sys[in.b].virt && in.wrap == GR_WRAP_NONE &&
+               sc->sys[in.b].kind != GR_SYS_GHOST) {
+      P.interp_side = 2;
+      P.Vw = &A;
+    } else {
+      return fail(sc, GR_ERR_ARGUMENT,
+                  "InterpolateFieldFn needs a real and a virtual particle system");
+    }
+  }
   const double cutoff = in.cell;
   const double cutoff_sq = cutoff * cutoff;
   const int D = in.D;
@@ -1055,24 +1103,28 @@ void vpmc_update(Sys &s, const Upd &u, int64_t i) {
   }
 }
 
+void zero_fields(Sys &s, int64_t mask) {
+  if (mask & (1 << GR_F_V)) std::fill(s.v.begin(), s.v.end(), 0.0);
+  if (mask & (1 << GR_F_RHO)) std::fill(s.rho.begin(), s.rho.end(), 0.0);
+  if (mask & (1 << GR_F_STRAIN_RATE)) std::fill(s.sr.begin(), s.sr.end(), 0.0);
+  if (mask & (1 << GR_F_VORTICITY)) std::fill(s.vor.begin(), s.vor.end(), 0.0);
+  if (mask & (1 << GR_F_DVDT)) std::fill(s.dvdt.begin(), s.dvdt.end(), 0.0);
+  if (mask & (1 << GR_F_DRHODT)) std::fill(s.drhodt.begin(), s.drhodt.end(), 0.0);
+  if (mask & (1 << GR_F_P)) std::fill(s.p.begin(), s.p.end(), 0.0);
+  if (mask & (1 << GR_F_STRESS)) std::fill(s.stress.begin(), s.stress.end(), 0.0);
+  if (mask & (1 << GR_F_STRAIN)) std::fill(s.strain.begin(), s.strain.end(), 0.0);
+  if (mask & (1 << GR_F_STRAIN_P)) std::fill(s.strainp.begin(), s.strainp.end(), 0.0);
+}
+
 int update_state(orc_scene *sc, Sys &s, int stage, double dt) {
   if (stage < 1 || stage > (int)s.upd.size()) return GR_OK;
   for (const Upd &u : s.upd[stage - 1]) {
     switch (u.kind) {
       case GR_UPD_NOTHING:
         break;
-      case GR_UPD_ZERO: {  // :30-42
-        int64_t mask = (int64_t)u.prm[0];
-        if (mask & (1 << GR_F_STRAIN_RATE)) std::fill(s.sr.begin(), s.sr.end(), 0.0);
-        if (mask & (1 << GR_F_VORTICITY)) std::fill(s.vor.begin(), s.vor.end(), 0.0);
-        if (mask & (1 << GR_F_DVDT)) std::fill(s.dvdt.begin(), s.dvdt.end(), 0.0);
-        if (mask & (1 << GR_F_DRHODT)) std::fill(s.drhodt.begin(), s.drhodt.end(), 0.0);
-        if (mask & (1 << GR_F_P)) std::fill(s.p.begin(), s.p.end(), 0.0);
-        if (mask & (1 << GR_F_STRESS)) std::fill(s.stress.begin(), s.stress.end(), 0.0);
-        if (mask & (1 << GR_F_STRAIN)) std::fill(s.strain.begin(), s.strain.end(), 0.0);
-        if (mask & (1 << GR_F_STRAIN_P)) std::fill(s.strainp.begin(), s.strainp.end(), 0.0);
+      case GR_UPD_ZERO:  // :30-42
+        zero_fields(s, (int64_t)u.prm[0]);
         break;
-      }
       case GR_UPD_TAIT:
       case GR_UPD_LINEAR:
 #pragma omp parallel for
@@ -1086,6 +1138,33 @@ int update_state(orc_scene *sc, Sys &s, int stage, double dt) {
 #pragma omp parallel for
         for (int64_t i = 0; i < s.n; ++i) vpmc_update(s, u, i);
         break;
+      case GR_UPD_VNORM: {  // VirtualNormUpdater :74-84
+        if (!s.virt) return fail(sc, GR_ERR_ARGUMENT, "VirtualNormUpdater needs a virtual system");
+        const int64_t mask = (int64_t)u.prm[3];
+        const int D = s.D, NS = s.NS;
+        for (int64_t i = 0; i < s.n; ++i) {
+          const double w = s.wsum[i];
+          const bool z = w == 0.0;
+          if (mask & (1 << GR_F_V))
+            for (int d = 0; d < D; ++d) {
+              const double val = z ? 0.0 : s.v[i * D + d] / w;
+              s.v[i * D + d] = val * u.prm[d] + s.pv[d];
+            }
+          if (mask & (1 << GR_F_RHO)) s.rho[i] = z ? 0.0 : s.rho[i] / w;
+          if ((mask & (1 << GR_F_P)) && !s.p.empty()) s.p[i] = z ? 0.0 : s.p[i] / w;
+          if ((mask & (1 << GR_F_STRESS)) && !s.stress.empty())
+            for (int k = 0; k < NS; ++k)
+              s.stress[i * NS + k] = z ? 0.0 : s.stress[i * NS + k] / w;
+        }
+        break;
+      }
+      case GR_UPD_PRESCRIBED_V: {  // :88-92
+        if (!s.virt) return fail(sc, GR_ERR_ARGUMENT, "PrescribedVelocityUpdater needs a virtual system");
+        const int D = s.D;
+        for (int64_t i = 0; i < s.n; ++i)
+          for (int d = 0; d < D; ++d) s.v[i * D + d] = s.v[i * D + d] + s.pv[d];
+        break;
+      }
       default:
         return fail(sc, GR_ERR_ARGUMENT, "unknown updater kind");
     }
@@ -1213,6 +1292,7 @@ int step(orc_scene *sc, Integ &ig, double dt) {
   }
   for (int g : ig.ghost) generate_ghosts(sc, sc->sys[g]);
   for (int g : ig.ghost) sort_system(sc, sc->sys[g], cutoff);
+  for (int v : ig.virt) sort_system(sc, sc->sys[v], cutoff);  // _prepare_grids! :373-375
   for (int it : ig.inter) {
     int rc = create_grid(sc, sc->inter[it]);
     if (rc) return rc;
@@ -1230,6 +1310,11 @@ int step(orc_scene *sc, Integ &ig, double dt) {
     }
   }
   for (int g : ig.ghost) ghost_kinematics(sc, sc->sys[g]);
+  for (int v : ig.virt) {  // auto_zero_virtual! (src/Particles.jl:521-528)
+    Sys &s = sc->sys[v];
+    std::fill(s.wsum.begin(), s.wsum.end(), 0.0);
+    zero_fields(s, s.zero_mask);
+  }
   for (int stage = 1; stage <= ig.nst; ++stage) {  // _sweep_all_stages! :401-419
     for (int id : ig.sys) {
       Sys &s = sc->sys[id];
@@ -1238,6 +1323,12 @@ int step(orc_scene *sc, Integ &ig, double dt) {
       if (rc) return rc;
     }
     for (int g : ig.ghost) update_ghost(sc, sc->sys[g], stage);
+    for (int v : ig.virt) {
+      Sys &s = sc->sys[v];
+      if ((int)s.upd.size() != ig.nst) continue;
+      int rc = update_state(sc, s, stage, dt);
+      if (rc) return rc;
+    }
     for (int it : ig.inter) {
       int rc = sweep(sc, sc->inter[it], stage);
       if (rc) return rc;
@@ -1262,6 +1353,12 @@ int step(orc_scene *sc, Integ &ig, double dt) {
       for (int d = 0; d < D; ++d) s.x[i * D + d] = s.x[i * D + d] + dt * s.v[i * D + d];
     }
   }
+  for (int v : ig.virt) {  // _update_virtual_positions! :389-397
+    Sys &s = sc->sys[v];
+    const int D = s.D;
+    for (int64_t i = 0; i < s.n; ++i)
+      for (int d = 0; d < D; ++d) s.x[i * D + d] = s.x[i * D + d] + s.pv[d] * dt;
+  }
   return GR_OK;
 }
 
@@ -1279,6 +1376,7 @@ int ncomp(const orc_scene *sc, const Sys &s, int field) {
     case GR_F_RHO:
     case GR_F_DRHODT:
     case GR_F_P:
+    case GR_F_W_SUM:
       return 1;
     case GR_F_STRESS:
     case GR_F_STRAIN_RATE:
@@ -1307,6 +1405,7 @@ std::vector<double> *field_ptr(Sys &s, int field) {
     case GR_F_STRAIN_P: return s.kind == GR_SYS_EP ? &s.strainp : nullptr;
     case GR_F_V_ADJUSTMENT: return s.kind == GR_SYS_GHOST ? nullptr : &s.vadj;
     case GR_F_NORMALS: return s.kind == GR_SYS_GHOST ? &s.normals : nullptr;
+    case GR_F_W_SUM: return s.virt ? &s.wsum : nullptr;
   }
   return nullptr;
 }
@@ -1382,6 +1481,25 @@ int orc_system_create(orc_scene *s, int kind, int64_t n, int ns, double mass,
   return GR_OK;
 }
 
+int orc_virtual_create(orc_scene *s, int sys, int nzero, const int *zero_fields,
+                       const double *prescribed_v) {
+  if (sys < 0 || sys >= (int)s->sys.size()) return fail(s, GR_ERR_ARGUMENT, "bad system");
+  Sys &y = s->sys[sys];
+  if (y.kind == GR_SYS_GHOST || y.virt) return fail(s, GR_ERR_ARGUMENT, "not a plain particle system");
+  y.virt = true;
+  y.wsum.assign(y.n, 0.0);
+  y.zero_mask = 0;
+  for (int k = 0; k < nzero; ++k) {
+    if (zero_fields[k] < 0 || zero_fields[k] >= GR_NUM_FIELDS || !field_ptr(y, zero_fields[k]) ||
+        zero_fields[k] == GR_F_X)
+      return fail(s, GR_ERR_ARGUMENT, "zero_fields: field not present on the virtual system");
+    y.zero_mask |= (int64_t)1 << zero_fields[k];
+  }
+  for (int d = 0; d < y.D; ++d) y.pv[d] = prescribed_v ? prescribed_v[d] : 0.0;
+  y.upd.clear();  // the virtual system's own updaters follow
+  return GR_OK;
+}
+
 int orc_system_add_updater(orc_scene *s, int sys, int stage, int kind,
                            const double *params, int nparams) {
   if (sys < 0 || sys >= (int)s->sys.size() || stage < 1) return fail(s, GR_ERR_ARGUMENT, "bad system/stage");
@@ -1391,7 +1509,8 @@ int orc_system_add_updater(orc_scene *s, int sys, int stage, int kind,
     return fail(s, GR_ERR_ASSERTION,
                 "ElastoPlasticStressUpdater requires NS=4 (plane-strain Voigt). Got NS=" + std::to_string(y.NS) + ".");
   if ((kind == GR_UPD_EP && y.kind != GR_SYS_EP) || (kind == GR_UPD_VPMC && y.kind < GR_SYS_STRESS) ||
-      ((kind == GR_UPD_TAIT || kind == GR_UPD_LINEAR) && y.kind < GR_SYS_FLUID))
+      ((kind == GR_UPD_TAIT || kind == GR_UPD_LINEAR) && y.kind < GR_SYS_FLUID) ||
+      ((kind == GR_UPD_VNORM || kind == GR_UPD_PRESCRIBED_V) && !y.virt))
     return fail(s, GR_ERR_ARGUMENT, "updater needs fields the system does not have");
   if ((int)y.upd.size() < stage) y.upd.resize(stage);
   Upd u;
@@ -1557,7 +1676,11 @@ int orc_integrator_create(orc_scene *s, int nsys, const int *systems, int ninter
   if (nsys < 1) return fail(s, GR_ERR_ARGUMENT, "systems must not be empty");
   if (ninter < 1) return fail(s, GR_ERR_ARGUMENT, "interactions must not be empty");
   Integ ig;
-  ig.sys.assign(systems, systems + nsys);
+  for (int k = 0; k < nsys; ++k) {
+    if (systems[k] < 0 || systems[k] >= (int)s->sys.size()) return fail(s, GR_ERR_ARGUMENT, "bad system");
+    (s->sys[systems[k]].virt ? ig.virt : ig.sys).push_back(systems[k]);
+  }
+  if (ig.sys.empty()) return fail(s, GR_ERR_ARGUMENT, "systems must not be empty");
   ig.inter.assign(inters, inters + ninter);
   if (nghost > 0) ig.ghost.assign(ghosts, ghosts + nghost);
   ig.gamma = gamma;

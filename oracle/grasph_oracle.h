@@ -43,6 +43,8 @@ int orc_system_download_index(orc_scene *s, int sys, int field, int64_t *data,
                               int64_t count);
 int orc_system_get_n(orc_scene *s, int sys, int64_t *n_out);
 
+int orc_virtual_create(orc_scene *s, int sys, int nzero, const int *zero_fields,
+                       const double *prescribed_v);
 int orc_ghost_create(orc_scene *s, int source_sys, double cutoff, int nb,
                      const double *normals, const double *points,
                      int *ghost_out);
