@@ -84,6 +84,10 @@ struct SysH {
   bool ex_p = false, ex_stress = false;
   std::vector<int2> copy;  // per stage: {copy_p, stress_mode}
   bool dead = false;       // released with its integrator; segment reusable
+  // source of a VirtualParticleSystem (src/Particles.jl:476-511)
+  bool virt = false;
+  long long zero_mask = 0;
+  double pv[3] = {0, 0, 0};
 };
 
 struct InterH {
@@ -98,7 +102,7 @@ struct InterH {
 };
 
 struct IntegH {
-  std::vector<int> sys, inter, ghost;
+  std::vector<int> sys, inter, ghost, virt;
   double gamma = 0, h = 0, c = 0, dt = 0;
   int nst = 0;
   bool dead = false;
@@ -145,7 +149,7 @@ struct StagePlan {
   int nblk_upd = 0;
   DBuf blk_gcopy, gcopy;
   int nblk_gcopy = 0;
-  FamPlan fam[3];
+  FamPlan fam[4];
 };
 
 struct Plan {
@@ -168,6 +172,9 @@ struct Plan {
   // neighbour lists
   DBuf ltasks, lsrcs, blk_list;
   int nblk_list = 0;
+  // integrated (non-virtual) real systems; virtual systems
+  DBuf blk_int, blk_virt;
+  int nblk_int = 0, nblk_virt = 0;
   bool do_sort = true, do_ghosts = true, do_grid = true, do_step = true;
   // graph capture per A/B parity
   hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -200,6 +207,7 @@ struct gr_ctx {
   int cur = 0;
   DBuf vs, rhos, rho_h, gidx, gbnd, gnrm, gxs, gidx_s, gbnd_s;
   DBuf ra, rb;  // partner records (RecA / RecB), rebuilt before every stage's sweeps
+  DBuf wsum;    // w_sum of virtual systems
   // tables
   DBuf d_sd, d_n, d_st, d_bnd, d_kmin, d_kmax, d_bbmin, d_bbmax, d_kbase, d_krange, d_flags;
   DBuf d_grids, d_carr, d_cellsz, d_inter_sys, d_inter_carr, cells;
@@ -290,6 +298,7 @@ Pool<R> pool_view(gr_ctx *c, int w) {
   const int64_t cap = c->pool_cap;
   P.ra = c->ra.as<RecA<R>>();
   P.rb = c->rb.as<RecB<R>>();
+  P.wsum = c->wsum.as<double>();
   for (int d = 0; d < 3; ++d) {
     P.x[d] = c->x[w].as<double>() + d * cap;
     P.v[d] = c->v[w].as<double>() + d * cap;
@@ -373,6 +382,7 @@ void commit(gr_ctx *c) {
     grow(c, c->gidx_s, 1, 4, old, nc);
     grow(c, c->gbnd_s, 1, 4, old, nc);
     grow(c, c->ra, 1, 32, old, nc);
+    grow(c, c->wsum, 1, 8, old, nc);
     grow(c, c->rb, 1, R == 4 ? 16 : 32, old, nc);
     c->pool_cap = nc;
   }
@@ -407,8 +417,13 @@ void commit(gr_ctx *c) {
     d.gcut = s.gcut;
     d.mass = s.mass;
     d.c = s.c;
-    for (int k = 0; k < 3; ++k) d.sv[k] = s.sv[k];
+    for (int k = 0; k < 3; ++k) {
+      d.sv[k] = s.sv[k];
+      d.pv[k] = s.pv[k];
+    }
     d.srho = s.srho;
+    d.virt = s.virt ? 1 : 0;
+    d.zero_mask = s.zero_mask;
     nn[i] = s.kind == GR_SYS_GHOST ? 0 : (int)s.n;
   }
   // keep device ghost counts across commits
@@ -492,6 +507,7 @@ int ncomp(gr_ctx *c, const SysH &s, int field) {
     case GR_F_RHO:
     case GR_F_DRHODT:
     case GR_F_P:
+    case GR_F_W_SUM:
       return 1;
     case GR_F_STRESS:
     case GR_F_STRAIN_RATE:
@@ -536,6 +552,8 @@ bool has_field(const SysH &s, int field) {
     case GR_F_STRAIN:
     case GR_F_STRAIN_P:
       return s.kind == GR_SYS_EP;
+    case GR_F_W_SUM:
+      return s.virt;
   }
   return false;
 }
@@ -568,6 +586,7 @@ FieldRef field_ref(gr_ctx *c, int field, bool shadow_only) {
     case GR_F_VORTICITY: add(c->vor[w], 3, R); break;
     case GR_F_STRAIN: add(c->strain[w], c->nsmax, 8); break;
     case GR_F_STRAIN_P: add(c->strainp[w], c->nsmax, 8); break;
+    case GR_F_W_SUM: add(c->wsum, 1, 8); break;
   }
   return f;
 }
@@ -639,7 +658,7 @@ int classify(gr_ctx *c, const InterH &in, int pfn) {
   if (in.wrap == GR_WRAP_DYNAMIC) return pfn == GR_PFN_COUNT ? P_ONE : P_DYN;
   const SysH &b = c->sys[in.b];
   const SysH &a = c->sys[in.a];
-  if (b.kind == GR_SYS_GHOST) return P_ONE;
+  if (b.kind == GR_SYS_GHOST || b.virt || a.virt) return P_ONE;
   switch (pfn) {
     case GR_PFN_FLUID:
       return (a.kind == GR_SYS_FLUID && b.kind == GR_SYS_FLUID) ? P_TWO : P_ONE;
@@ -655,6 +674,7 @@ int family_of(int pfn) {
   if (pfn == GR_PFN_FLUID || pfn == GR_PFN_CAUCHY) return FAM_FORCE;
   if (pfn == GR_PFN_SR || pfn == GR_PFN_SRV) return FAM_STRAIN;
   if (pfn == GR_PFN_COUNT) return FAM_COUNT;
+  if (pfn == GR_PFN_INTERP) return FAM_INTERP;
   return -1;
 }
 
@@ -731,7 +751,7 @@ void set_pfn(SrcDesc &q, const gr_pfn_desc &f, int pk, gr_ctx *c, const InterH &
   // CauchyFluidPfn with a ghost / virtual partner uses the partner's c in
   // the artificial viscosity (src/PairwiseFunctors.jl:293)
   if (f.kind == GR_PFN_CAUCHY && pk == P_ONE && in.b >= 0 &&
-      c->sys[in.b].kind == GR_SYS_GHOST)
+      (c->sys[in.b].kind == GR_SYS_GHOST || c->sys[in.b].virt))
     q.c_q_piv = src_c(c, in.b);
 }
 
@@ -955,8 +975,10 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
   const int ns = (int)c->sys.size();
   std::vector<std::vector<std::vector<UpdDesc>>> upd(ns);
   std::vector<int> sys_integ(ns, -1);
-  for (int ig : P.integs)
+  for (int ig : P.integs) {
     for (int s : c->integ[ig].sys) sys_integ[s] = ig;
+    for (int v : c->integ[ig].virt) sys_integ[v] = ig;
+  }
   for (int ig : P.integs) {
     const IntegH &I = c->integ[ig];
     for (int s : I.sys) {
@@ -983,6 +1005,19 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
         }
       }
     }
+    for (int v : I.virt) {  // sorted, updated per stage, never integrated
+      SysH &S = c->sys[v];
+      StepSys &T = c->stepsys[v];
+      memset(&T, 0, sizeof(T));
+      T.dt = I.dt;
+      T.half = I.dt / 2;
+      T.sort_cutoff = 2.0 * I.h;
+      upd[v].assign(I.nst, {});
+      if ((int)S.upd.size() == I.nst)  // _sweep_all_stages! :411-414
+        for (int k = 0; k < I.nst; ++k)
+          for (const UpdDesc &u : S.upd[k])
+            if (u.kind != GR_UPD_NOTHING) upd[v][k].push_back(u);
+    }
     for (int g : I.ghost) {
       StepSys &T = c->stepsys[g];
       memset(&T, 0, sizeof(T));
@@ -995,7 +1030,7 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
     Task t;
     std::vector<SrcDesc> src;
   };
-  std::vector<std::map<int, TaskB>> tasks(nst * 3);  // [stage*3+fam] target → task
+  std::vector<std::map<int, TaskB>> tasks(nst * 4);  // [stage*4+fam] target → task
   for (int k = 0; k < nst; ++k)
     for (int ig : P.integs)
       for (int it : c->integ[ig].inter) {
@@ -1006,7 +1041,7 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
         if (fam < 0) continue;
         const int pk = classify(c, in, f.kind);
         auto add = [&](int target, bool b_side) {
-          auto &m = tasks[k * 3 + fam];
+          auto &m = tasks[k * 4 + fam];
           TaskB &tb = m[target];
           if (tb.src.empty()) {
             memset(&tb.t, 0, sizeof(Task));
@@ -1018,8 +1053,20 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
           set_pfn(q, f, pk, c, in);
           if (f.kind == GR_PFN_CAUCHY) tb.t.has_stress = 1;
           if (f.kind == GR_PFN_SRV) tb.t.write_vor = 1;
+          if (f.kind == GR_PFN_INTERP) tb.t.imask |= f.sigma | (f.alpha != 0.0 ? 1 << 30 : 0);
           tb.src.push_back(q);
         };
+        if (f.kind == GR_PFN_INTERP) {  // the virtual side gathers (:424-434)
+          need(in.b >= 0 && in.wrap == GR_WRAP_NONE &&
+                   (c->sys[in.b].virt != c->sys[in.a].virt) &&
+                   c->sys[in.b].kind != GR_SYS_GHOST,
+               GR_ERR_ARGUMENT, "InterpolateFieldFn needs a real and a virtual particle system");
+          if (c->sys[in.b].virt)
+            add(in.b, true);
+          else
+            add(in.a, false);
+          continue;
+        }
         add(in.a, false);
         if (pk == P_TWO && in.b >= 0) add(in.b, true);
       }
@@ -1029,7 +1076,7 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
     int first = -1, firstfam = -1;
     for (int k = 0; k < nst && first < 0; ++k)
       for (int fam : {FAM_FORCE, FAM_COUNT})
-        if (tasks[k * 3 + fam].count(s)) {
+        if (tasks[k * 4 + fam].count(s)) {
           first = k;
           firstfam = fam;
           break;
@@ -1047,7 +1094,7 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
   }
   // assignment: STRAIN group absorbs a stage's ZeroFieldUpdater
   for (int k = 0; k < nst; ++k)
-    for (auto &kv : tasks[k * 3 + FAM_STRAIN]) {
+    for (auto &kv : tasks[k * 4 + FAM_STRAIN]) {
       const int s = kv.first;
       if (k >= (int)upd[s].size()) continue;
       auto &L = upd[s][k];
@@ -1069,8 +1116,8 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
   // neighbour lists: one per target, over the union of its sources
   std::map<int, std::vector<SrcDesc>> tl;  // target → list sources
   for (int k = 0; k < nst; ++k)
-    for (int fam = 0; fam < 3; ++fam)
-      for (auto &kv : tasks[k * 3 + fam]) {
+    for (int fam = 0; fam < 4; ++fam)
+      for (auto &kv : tasks[k * 4 + fam]) {
         auto &L = tl[kv.first];
         for (auto &q : kv.second.src) {
           int idx = -1;
@@ -1148,11 +1195,11 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
     sp->gcopy.upload(gc, c->stream);
     sp->blk_gcopy.upload(gb, c->stream);
     sp->nblk_gcopy = (int)gb.size();
-    for (int fam = 0; fam < 3; ++fam) {
+    for (int fam = 0; fam < 4; ++fam) {
       std::vector<Task> tv;
       std::vector<SrcDesc> sv;
       std::vector<Blk> bv;
-      for (auto &kv : tasks[k * 3 + fam]) {
+      for (auto &kv : tasks[k * 4 + fam]) {
         Task t = kv.second.t;
         t.ltask = c->use_lists ? ltask_of[kv.first] : -1;
         t.src_begin = (int)sv.size();
@@ -1171,6 +1218,7 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
   P.stepsys.clear();
   for (int ig : P.integs) {
     for (int s : c->integ[ig].sys) P.stepsys.push_back({s, c->stepsys[s]});
+    for (int v : c->integ[ig].virt) P.stepsys.push_back({v, c->stepsys[v]});
     for (int g : c->integ[ig].ghost) P.stepsys.push_back({g, c->stepsys[g]});
   }
   upload_stepsys(c);
@@ -1187,6 +1235,7 @@ Plan &get_plan(gr_ctx *c, const std::vector<int> &integs) {
     const IntegH &I = c->integ[ig];
     need(!I.dead, GR_ERR_STATE, "integrator was released");
     for (int s : I.sys) reals.push_back(s);
+    for (int v : I.virt) reals.push_back(v);  // sorted with the real systems
     for (int g : I.ghost) ghosts.push_back(g);
     for (int t : I.inter) inters.push_back(t);
   }
@@ -1200,6 +1249,17 @@ Plan &get_plan(gr_ctx *c, const std::vector<int> &integs) {
   }
   ensure_cells(c, inters, reals);
   plan_sort_real(c, *P, reals);
+  {
+    std::vector<Blk> bi, bv;
+    for (int ig : integs) {
+      for (int s : c->integ[ig].sys) add_blocks(bi, s, c->sys[s].cap, GR_BLOCK);
+      for (int v : c->integ[ig].virt) add_blocks(bv, v, c->sys[v].cap, GR_BLOCK);
+    }
+    P->blk_int.upload(bi, c->stream);
+    P->blk_virt.upload(bv, c->stream);
+    P->nblk_int = (int)bi.size();
+    P->nblk_virt = (int)bv.size();
+  }
   plan_ghosts(c, *P, ghosts);
   plan_grids(c, *P, inters);
   plan_stages(c, *P, false);
@@ -1325,15 +1385,24 @@ void phase_stage(gr_ctx *c, Plan &P, int k, bool cnt, hipEvent_t *mid) {
   launch_ghost_copy<R>(s, A, c->d_sd.as<SysDesc>(), c->d_bnd.as<Bnd>(), c->d_n.as<int>(),
                        S.blk_gcopy.as<Blk>(), S.nblk_gcopy, S.gcopy.as<int2>(), c->D);
   if (mid) HIPCHK(hipEventRecord(*mid, s));
-  if (S.fam[FAM_FORCE].nblk + S.fam[FAM_COUNT].nblk + S.fam[FAM_STRAIN].nblk > 0) {
+  auto pack = [&]() {
     launch_pack<R>(s, A, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), P.blk_real.as<Blk>(),
                    P.nblk_real, c->D);
     launch_pack<R>(s, A, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), P.blk_ghost.as<Blk>(),
                    P.nblk_ghost, c->D);
-  }
-  for (int fam : {FAM_FORCE, FAM_COUNT, FAM_STRAIN}) {
+  };
+  const int nother = S.fam[FAM_FORCE].nblk + S.fam[FAM_COUNT].nblk + S.fam[FAM_STRAIN].nblk;
+  if (nother + S.fam[FAM_INTERP].nblk > 0) pack();
+  // interpolation into virtual systems runs first; the functors of the same
+  // stage then see the interpolated fields
+  for (int fam : {FAM_INTERP, FAM_FORCE, FAM_COUNT, FAM_STRAIN}) {
     FamPlan &F = S.fam[fam];
     if (F.nblk == 0) continue;
+    if (fam == FAM_FORCE && S.fam[FAM_INTERP].nblk > 0) pack();
+    if (fam == FAM_COUNT && S.fam[FAM_INTERP].nblk > 0 && S.fam[FAM_FORCE].nblk == 0) pack();
+    if (fam == FAM_STRAIN && S.fam[FAM_INTERP].nblk > 0 && S.fam[FAM_FORCE].nblk == 0 &&
+        S.fam[FAM_COUNT].nblk == 0)
+      pack();
     SweepArgs a;
     a.sd = c->d_sd.as<SysDesc>();
     a.n = c->d_n.as<int>();
@@ -1399,18 +1468,22 @@ void run_step(gr_ctx *c, Plan &P, bool cnt, std::vector<hipEvent_t> *ev) {
     launch_nlist<R>(s, A, a, c->D);
   }
   rec(4);
-  launch_halfstep<R>(s, A, c->d_sd.as<SysDesc>(), c->d_st.as<StepSys>(), P.blk_real.as<Blk>(),
-                     P.nblk_real, c->D);
+  launch_halfstep<R>(s, A, c->d_sd.as<SysDesc>(), c->d_st.as<StepSys>(), P.blk_int.as<Blk>(),
+                     P.nblk_int, c->D);
   launch_ghost_kin<R>(s, A, c->d_sd.as<SysDesc>(), c->d_bnd.as<Bnd>(), c->d_n.as<int>(),
                       P.blk_ghost.as<Blk>(), P.nblk_ghost, c->D);
+  launch_virtual<R>(s, A, c->d_sd.as<SysDesc>(), c->d_st.as<StepSys>(), c->d_n.as<int>(),
+                    P.blk_virt.as<Blk>(), P.nblk_virt, 0, c->D);
   rec(5);
   const int nst = (int)P.stages.size();
   for (int k = 0; k < nst; ++k) {
     phase_stage<R>(c, P, k, cnt, ev ? &(*ev)[6 + 2 * k] : nullptr);
     rec(7 + 2 * k);
   }
-  launch_fullstep<R>(s, A, c->d_sd.as<SysDesc>(), c->d_st.as<StepSys>(), P.blk_real.as<Blk>(),
-                     P.nblk_real, c->D);
+  launch_fullstep<R>(s, A, c->d_sd.as<SysDesc>(), c->d_st.as<StepSys>(), P.blk_int.as<Blk>(),
+                     P.nblk_int, c->D);
+  launch_virtual<R>(s, A, c->d_sd.as<SysDesc>(), c->d_st.as<StepSys>(), c->d_n.as<int>(),
+                    P.blk_virt.as<Blk>(), P.nblk_virt, 1, c->D);
   rec(6 + 2 * nst);
 }
 
@@ -1687,9 +1760,11 @@ int gr_system_add_updater(gr_ctx *c, int sys, int stage, int kind, const double 
            "ElastoPlasticStressUpdater requires NS=4 (plane-strain Voigt). Got NS=" +
                std::to_string(s.ns) + ".");
     need(!((kind == GR_UPD_EP && s.kind != GR_SYS_EP) || (kind == GR_UPD_VPMC && s.kind < GR_SYS_STRESS) ||
-           ((kind == GR_UPD_TAIT || kind == GR_UPD_LINEAR) && s.kind < GR_SYS_FLUID)),
+           ((kind == GR_UPD_TAIT || kind == GR_UPD_LINEAR) && s.kind < GR_SYS_FLUID) ||
+           ((kind == GR_UPD_VNORM || kind == GR_UPD_PRESCRIBED_V) && !s.virt)),
          GR_ERR_ARGUMENT, "updater needs fields the system does not have");
-    need(kind >= GR_UPD_NOTHING && kind <= GR_UPD_VPMC, GR_ERR_ARGUMENT, "unknown updater kind");
+    need(kind >= GR_UPD_NOTHING && kind <= GR_UPD_PRESCRIBED_V, GR_ERR_ARGUMENT,
+         "unknown updater kind");
     if ((int)s.upd.size() < stage) s.upd.resize(stage);
     UpdDesc u;
     memset(&u, 0, sizeof(u));
@@ -1725,6 +1800,12 @@ int gr_system_add_updater(gr_ctx *c, int sys, int stage, int kind, const double 
         u.prm[4] = prm[4];
         need((int)prm[0] == GR_UPD_TAIT || (int)prm[0] == GR_UPD_LINEAR, GR_ERR_ARGUMENT,
              "ViscoPlasticMCStressUpdater needs a Tait or Linear EOS");
+        break;
+      case GR_UPD_VNORM:  // v_mult, field mask
+        u.prm[0] = prm[0];
+        u.prm[1] = prm[1];
+        u.prm[2] = prm[2];
+        u.mask = (long long)prm[3];
         break;
     }
     s.upd[stage - 1].push_back(u);
@@ -1872,6 +1953,28 @@ int gr_system_get_n(gr_ctx *c, int sys, int64_t *n_out) {
   });
 }
 
+int gr_virtual_create(gr_ctx *c, int sys, int nzero, const int *zero_fields,
+                      const double *prescribed_v) {
+  return guard(c, [&] {
+    validate_sys(c, sys);
+    SysH &s = c->sys[sys];
+    need(s.kind != GR_SYS_GHOST && !s.virt, GR_ERR_ARGUMENT, "not a plain particle system");
+    long long m = 0;
+    for (int k = 0; k < nzero; ++k) {
+      const int f = zero_fields[k];
+      need(f >= 0 && f < GR_NUM_FIELDS && f != GR_F_X && f != GR_F_W_SUM && has_field(s, f),
+           GR_ERR_ARGUMENT, "zero_fields: field not present on the virtual system");
+      m |= 1LL << f;
+    }
+    s.virt = true;
+    s.zero_mask = m;
+    for (int d = 0; d < 3; ++d) s.pv[d] = (prescribed_v && d < c->D) ? prescribed_v[d] : 0.0;
+    s.upd.clear();  // the virtual system's own updaters follow
+    c->dirty_layout = true;
+    c->plans.clear();
+  });
+}
+
 int gr_ghost_create(gr_ctx *c, int source_sys, double cutoff, int nb, const double *normals,
                     const double *points, int *ghost_out) {
   return guard(c, [&] {
@@ -2000,13 +2103,15 @@ int gr_integrator_create(gr_ctx *c, int nsys, const int *systems, int ninter, co
     need(nsys >= 1, GR_ERR_ARGUMENT, "systems must not be empty");
     need(ninter >= 1, GR_ERR_ARGUMENT, "interactions must not be empty");
     IntegH I;
-    I.sys.assign(systems, systems + nsys);
+    for (int k = 0; k < nsys; ++k) {
+      validate_sys(c, systems[k]);
+      need(c->sys[systems[k]].kind != GR_SYS_GHOST, GR_ERR_ARGUMENT,
+           "ghost systems go in `ghosts`");
+      (c->sys[systems[k]].virt ? I.virt : I.sys).push_back(systems[k]);
+    }
+    need(!I.sys.empty(), GR_ERR_ARGUMENT, "systems must not be empty");
     I.inter.assign(inters, inters + ninter);
     if (nghost > 0) I.ghost.assign(ghosts, ghosts + nghost);
-    for (int s : I.sys) {
-      validate_sys(c, s);
-      need(c->sys[s].kind != GR_SYS_GHOST, GR_ERR_ARGUMENT, "ghost systems go in `ghosts`");
-    }
     for (int g : I.ghost) {
       validate_sys(c, g);
       need(c->sys[g].kind == GR_SYS_GHOST, GR_ERR_ARGUMENT, "not a ghost system");
@@ -2016,6 +2121,7 @@ int gr_integrator_create(gr_ctx *c, int nsys, const int *systems, int ninter, co
       const InterH &in = c->inter[t];
       auto owned = [&](int s) {
         return std::find(I.sys.begin(), I.sys.end(), s) != I.sys.end() ||
+               std::find(I.virt.begin(), I.virt.end(), s) != I.virt.end() ||
                std::find(I.ghost.begin(), I.ghost.end(), s) != I.ghost.end();
       };
       need(owned(in.a) && (in.b < 0 || owned(in.b)), GR_ERR_ARGUMENT,
@@ -2051,6 +2157,7 @@ int gr_integrator_release(gr_ctx *c, int integ) {
       if (S.off >= 0) c->free_pool.put(S.off, S.cap);
     };
     for (int s : I.sys) free_sys(s);
+    for (int v : I.virt) free_sys(v);
     for (int g : I.ghost) free_sys(g);
     for (int t : I.inter) {
       InterH &in = c->inter[t];
@@ -2245,6 +2352,7 @@ int gr_sweep(gr_ctx *c, int inter, int stage) {
       t.write_vor = f.kind == GR_PFN_SRV;
       t.assign = 0;
       t.ltask = -1;
+      if (f.kind == GR_PFN_INTERP) t.imask = f.sigma | (f.alpha != 0.0 ? 1 << 30 : 0);
       SrcDesc q = make_src(c, inter, target, b_side, pk);
       set_pfn(q, f, pk, c, in);
       t.src_begin = (int)sv.size();
@@ -2253,8 +2361,18 @@ int gr_sweep(gr_ctx *c, int inter, int stage) {
       add_blocks(bv, (int)tv.size(), c->sys[target].cap, GR_BLOCK);
       tv.push_back(t);
     };
-    add(in.a, false);
-    if (pk == P_TWO && in.b >= 0) add(in.b, true);
+    if (f.kind == GR_PFN_INTERP) {
+      need(in.b >= 0 && in.wrap == GR_WRAP_NONE && (c->sys[in.b].virt != c->sys[in.a].virt) &&
+               c->sys[in.b].kind != GR_SYS_GHOST,
+           GR_ERR_ARGUMENT, "InterpolateFieldFn needs a real and a virtual particle system");
+      if (c->sys[in.b].virt)
+        add(in.b, true);
+      else
+        add(in.a, false);
+    } else {
+      add(in.a, false);
+      if (pk == P_TWO && in.b >= 0) add(in.b, true);
+    }
     DBuf dt, ds, db;
     dt.upload(tv, c->stream);
     ds.upload(sv, c->stream);

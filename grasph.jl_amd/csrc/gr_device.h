@@ -27,7 +27,7 @@ enum PartnerKind : int32_t {
 };
 
 // Functor families with a kernel each.
-enum Family : int32_t { FAM_FORCE = 0, FAM_STRAIN = 1, FAM_COUNT = 2 };
+enum Family : int32_t { FAM_FORCE = 0, FAM_STRAIN = 1, FAM_COUNT = 2, FAM_INTERP = 3 };
 
 struct SysDesc {
   int64_t off;     // pool offset of local index 0
@@ -37,10 +37,12 @@ struct SysDesc {
   int32_t src;     // ghost: source system; else -1
   int32_t nb;      // ghost: boundaries
   int32_t bnd;     // ghost: first boundary in the boundary table
-  int32_t pad0;
+  int32_t virt;    // source of a VirtualParticleSystem
   double gcut;     // ghost cutoff
   double mass, c;  // ghosts carry their source's
   double sv[3], srho;
+  double pv[3];            // virtual: prescribed velocity
+  long long zero_mask;     // virtual: fields zeroed before the stage loop
 };
 
 // Per-(system) parameters of one step of a batch.
@@ -113,7 +115,7 @@ struct Task {
   int32_t has_stress;
   int32_t write_vor;  // an SRV source writes vorticity
   int32_t ltask;      // neighbour-list task of the target (-1: walk cells)
-  int32_t pad;
+  int32_t imask;      // FAM_INTERP: union of the sources' field masks; bit 30: w_sum
 };
 
 // Neighbour list of one target system: sources [src_begin, src_end) of the
@@ -181,6 +183,7 @@ template <typename R>
 struct Pool {
   RecA<R> *ra;
   RecB<R> *rb;
+  double *wsum;  // virtual systems: interpolation weight sums
   double *x[3], *v[3], *rho, *rho_h;
   R *vs[3], *rhos, *dvdt[3], *drhodt, *p;
   double *stress[GR_MAXNS];

@@ -1085,10 +1085,110 @@ __global__ void k_updaters(Pool<R> P, const SysDesc *sd, const StepSys *st, cons
       case GR_UPD_VPMC:
         vpmc_update(P, g, S.ns, u.prm[0], u.prm[1], u.prm[2], u.prm[3], u.prm[4], S.c);
         break;
+      case GR_UPD_VNORM: {  // VirtualNormUpdater (src/StateUpdaters.jl:74-84)
+        const long long m = u.mask;
+        const double w = P.wsum[g];
+        const bool z = w == 0.0;
+        if (m & (1 << GR_F_V)) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            if (d >= D) continue;
+            const double val = z ? 0.0 : P.v[d][g] / w;
+            const double nv = val * u.prm[d] + S.pv[d];
+            P.v[d][g] = nv;
+            P.vs[d][g] = (R)nv;
+          }
+        }
+        if (m & (1 << GR_F_RHO)) {
+          const double val = z ? 0.0 : P.rho[g] / w;
+          P.rho[g] = val;
+          P.rho_h[g] = val;
+          P.rhos[g] = (R)val;
+        }
+        if (m & (1 << GR_F_P)) P.p[g] = (R)(z ? 0.0 : (double)P.p[g] / w);
+        if (m & (1 << GR_F_STRESS)) {
+#pragma unroll
+          for (int k = 0; k < GR_MAXNS; ++k) {
+            if (k >= S.ns) continue;
+            const double val = z ? 0.0 : P.stress[k][g] / w;
+            P.stress[k][g] = val;
+            P.stress_s[k][g] = (R)val;
+          }
+        }
+        break;
+      }
+      case GR_UPD_PRESCRIBED_V: {  // PrescribedVelocityUpdater (:88-92)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          if (d >= D) continue;
+          const double nv = P.v[d][g] + S.pv[d];
+          P.v[d][g] = nv;
+          P.vs[d][g] = (R)nv;
+        }
+        break;
+      }
       default:
         break;
     }
   }
+}
+
+// Virtual systems: mode 0 refreshes the shadows of velocity and density
+// after the sort, then clears w_sum and the zero fields before the stage
+// loop (auto_zero_virtual!, src/Particles.jl:521-528); mode 1 advances the
+// positions by prescribed_v·dt (_update_virtual_positions!,
+// src/TimeIntegration.jl:389-397).
+template <typename R, int D>
+__global__ void k_virtual(Pool<R> P, const SysDesc *sd, const StepSys *st, const int *n_dev,
+                          const Blk *blk, int mode) {
+  const Blk b = blk[blockIdx.x];
+  const int s = b.item;
+  const int i = b.first + threadIdx.x;
+  if (i >= n_dev[s]) return;
+  const SysDesc S = sd[s];
+  const int64_t g = S.off + i;
+  if (mode == 1) {
+    const double dt = st[s].dt;
+#pragma unroll
+    for (int d = 0; d < D; ++d) P.x[d][g] = P.x[d][g] + S.pv[d] * dt;
+    return;
+  }
+  // the sort moved the masters; the single-copy shadows follow them
+#pragma unroll
+  for (int d = 0; d < D; ++d) P.vs[d][g] = (R)P.v[d][g];
+  P.rhos[g] = (R)P.rho[g];
+  P.rho_h[g] = P.rho[g];
+  P.wsum[g] = 0.0;
+  const long long m = S.zero_mask;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    if (m & (1 << GR_F_V)) {
+      P.v[d][g] = 0.0;
+      P.vs[d][g] = (R)0;
+    }
+    if (m & (1 << GR_F_DVDT)) P.dvdt[d][g] = (R)0;
+  }
+  if (m & (1 << GR_F_RHO)) {
+    P.rho[g] = 0.0;
+    P.rho_h[g] = 0.0;
+    P.rhos[g] = (R)0;
+  }
+  if (m & (1 << GR_F_DRHODT)) P.drhodt[g] = (R)0;
+  if (m & (1 << GR_F_P)) P.p[g] = (R)0;
+#pragma unroll
+  for (int k = 0; k < GR_MAXNS; ++k) {
+    if (k >= S.ns) continue;
+    if (m & (1 << GR_F_STRESS)) {
+      P.stress[k][g] = 0.0;
+      P.stress_s[k][g] = (R)0;
+    }
+    if (m & (1 << GR_F_STRAIN_RATE)) P.sr[k][g] = (R)0;
+    if (m & (1 << GR_F_STRAIN)) P.strain[k][g] = 0.0;
+    if (m & (1 << GR_F_STRAIN_P)) P.strainp[k][g] = 0.0;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; ++k)
+    if ((m & (1 << GR_F_VORTICITY)) && k < S.nv) P.vor[k][g] = (R)0;
 }
 
 // ---------------------------------------------------------------------------
@@ -1330,6 +1430,20 @@ void launch_updaters(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const S
 INST(float)
 INST(double)
 #undef INST
+
+template <typename R>
+void launch_virtual(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
+                    const int *n, const Blk *blk, int nblk, int mode, int D) {
+  if (nblk <= 0) return;
+  if (D == 2)
+    k_virtual<R, 2><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, n, blk, mode);
+  else
+    k_virtual<R, 3><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, n, blk, mode);
+}
+template void launch_virtual<float>(hipStream_t, const Pool<float> &, const SysDesc *,
+                                    const StepSys *, const int *, const Blk *, int, int, int);
+template void launch_virtual<double>(hipStream_t, const Pool<double> &, const SysDesc *,
+                                     const StepSys *, const int *, const Blk *, int, int, int);
 
 template <typename R>
 void launch_pack(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const int *n, const Blk *blk,

@@ -93,6 +93,9 @@ void launch_updaters(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const S
 template <typename R>
 void launch_sweep(hipStream_t s, const Pool<R> &P, const SweepArgs &a, int D, int fam, bool cnt);
 template <typename R>
+void launch_virtual(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
+                    const int *n, const Blk *blk, int nblk, int mode, int D);
+template <typename R>
 void launch_pack(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const int *n, const Blk *blk,
                  int nblk, int D);
 template <typename R>

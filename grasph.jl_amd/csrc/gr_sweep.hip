@@ -68,6 +68,7 @@ __device__ inline double pow_i(double x, int n) {
 }
 __device__ inline float pow_i(float x, int n) {
   float x2 = x * x, x3 = x2 * x, x6 = x3 * x3;
+  if (n == 4) return x2 * x2;
   return n == 12 ? x6 * x6 : x6;
 }
 
@@ -102,6 +103,20 @@ struct Fast<double> {
   __device__ static inline double rcp(double x) { return 1.0 / x; }
   __device__ static inline double rsq(double x) { return 1.0 / sqrt(x); }
 };
+
+// kernel_w (src/Kernels.jl:70-72 cubic, :142-144 Wendland C2).
+template <typename R>
+__device__ inline R wq(int kk, R C, R q) {
+  if (kk == GR_KERNEL_CUBIC) {
+    R t2 = (R)2 - q, t1 = (R)1 - q;
+    t2 = t2 > (R)0 ? t2 : (R)0;
+    t1 = t1 > (R)0 ? t1 : (R)0;
+    return C * ((R)0.25 * (t2 * t2 * t2) - t1 * t1 * t1);
+  }
+  R t = (R)2 - q;
+  t = t > (R)0 ? t : (R)0;
+  return (C * pow_i(t, 4)) * ((R)2 * q + (R)1);
+}
 
 template <typename R>
 __device__ inline R jmin0(R v) {  // Julia min(v, 0)
@@ -203,6 +218,26 @@ struct Gather {
     if (FAM == FAM_COUNT) {
       adv[0].add((R)1);
       adr.add((R)1);
+      return;
+    }
+    if (FAM == FAM_INTERP) {  // InterpolateFieldFn (src/PairwiseFunctors.jl:424-434)
+      const R w = wq<R>(Q.kk, K.norm, FA ? r * K.ih : r / K.h);
+      const RecB<R> Bj = P.rb[gj];
+      const R rj = Aj.rho;
+      const R kw = FA ? w * (K.m_q * Fast<R>::rcp(rj)) : w * (K.m_q / rj);
+      const int m = Q.sigma;
+      if (m & (1 << GR_F_V)) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) adv[d].add(kw * Bj.v[d]);
+      }
+      if (m & (1 << GR_F_RHO)) adr.add(kw * rj);
+      if (m & (1 << GR_F_P)) avo[0].add(kw * rec_pres(Aj, Bj));
+      if (m & (1 << GR_F_STRESS)) {
+#pragma unroll
+        for (int k = 0; k < GR_MAXNS; ++k)
+          if (k < NS) asr[k].add(kw * P.stress_s[k][gj]);
+      }
+      if (K.alpha != (R)0) avo[1].add(kw);
       return;
     }
     R vj[D], rj, pj = (R)0;
@@ -352,6 +387,35 @@ struct Gather {
   }
 
   __device__ inline void store(const Pool<R> P, const SysDesc &S, const Task &T, int64_t gi) {
+    if (FAM == FAM_INTERP) {  // += into the virtual system's fields (and shadows)
+      const int m = T.imask;
+      if (m & (1 << GR_F_V)) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          const double nv = P.v[d][gi] + (double)adv[d].get();
+          P.v[d][gi] = nv;
+          P.vs[d][gi] = (R)nv;
+        }
+      }
+      if (m & (1 << GR_F_RHO)) {
+        const double nv = P.rho[gi] + (double)adr.get();
+        P.rho[gi] = nv;
+        P.rho_h[gi] = nv;
+        P.rhos[gi] = (R)nv;
+      }
+      if (m & (1 << GR_F_P)) P.p[gi] = (R)((double)P.p[gi] + (double)avo[0].get());
+      if (m & (1 << GR_F_STRESS)) {
+#pragma unroll
+        for (int k = 0; k < GR_MAXNS; ++k)
+          if (k < NS) {
+            const double nv = P.stress[k][gi] + (double)asr[k].get();
+            P.stress[k][gi] = nv;
+            P.stress_s[k][gi] = (R)nv;
+          }
+      }
+      if (m & (1 << 30)) P.wsum[gi] += (double)avo[1].get();
+      return;
+    }
     if (FAM == FAM_STRAIN) {
 #pragma unroll
       for (int k = 0; k < GR_MAXNS; ++k)
@@ -709,10 +773,12 @@ void launch_sweep(hipStream_t s, const Pool<R> &P, const SweepArgs &a, int D, in
   if (D == 2) {
     if (fam == FAM_FORCE) launch_one<2, R, FAM_FORCE>(s, P, a, cnt);
     else if (fam == FAM_STRAIN) launch_one<2, R, FAM_STRAIN>(s, P, a, cnt);
+    else if (fam == FAM_INTERP) launch_one<2, R, FAM_INTERP>(s, P, a, cnt);
     else launch_one<2, R, FAM_COUNT>(s, P, a, cnt);
   } else {
     if (fam == FAM_FORCE) launch_one<3, R, FAM_FORCE>(s, P, a, cnt);
     else if (fam == FAM_STRAIN) launch_one<3, R, FAM_STRAIN>(s, P, a, cnt);
+    else if (fam == FAM_INTERP) launch_one<3, R, FAM_INTERP>(s, P, a, cnt);
     else launch_one<3, R, FAM_COUNT>(s, P, a, cnt);
   }
 }

@@ -376,6 +376,8 @@ STEP_CASES = [
     ("dambreak_3d", M.configs.dambreak_3d, dict(dx=0.04, water=(0.24, 0.2, 0.24),
                                                 tank=(0.48, 0.2, 0.32)), 3, 10),
     ("granular_column_3d", M.configs.granular_column_3d, dict(dx=0.01), 3, 20),
+    ("ep_column_virtual_2d", M.configs.ep_column_virtual_2d,
+     dict(dx=0.01, width=0.1, height=0.06), 2, 20),
 ]
 
 
@@ -460,6 +462,8 @@ LIST_CASES = [
                                                 tank=(0.48, 0.2, 0.32)), 3, "f32"),
     ("dambreak_2d_f64", M.configs.dambreak_2d, dict(dx=0.04), 2, "f64"),
     ("granular_column_3d", M.configs.granular_column_3d, dict(dx=0.01), 3, "f32"),
+    ("ep_column_virtual_2d", M.configs.ep_column_virtual_2d,
+     dict(dx=0.01, width=0.1, height=0.06), 2, "f32"),
 ]
 
 
@@ -580,3 +584,28 @@ def test_device_snapshot_restart_continues_bit_identically(tmp_path, prec):
     run(b, 5)
     for f in ["x"] + SN.sim_field_names(ref["fluid"]):
         assert np.array_equal(getattr(b["fluid"], f), getattr(ref["fluid"], f)), f
+
+
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 2e-6)])
+def test_virtual_interpolation_matches_oracle(prec, tol):
+    """The small virtual scene of the oracle tests (interpolation, zero
+    weight, normalisation with v_mult and prescribed velocity, accumulation
+    of fields outside zero_fields, prescribed motion) on the device."""
+    import test_oracle_virtual as TV
+    res = {}
+    for make in ("dev", "orc"):
+        real, vps, lf, vx = TV.scene(zero_fields=("v", "rho"), vnorm=("v", "rho"),
+                                     pv=(0.2, -0.1), shuffle_virtual=True,
+                                     interp2=M.InterpolateFieldFn("v", accumulate_wsum=False))
+        sc = M.Session(2, prec).scene if make == "dev" else oracle_scene(2)
+        h = sc.integrator(lf)
+        sc.upload_all(lf)
+        sc.be.step(h, 3, 1e-4)
+        sc.download_all(lf)
+        res[make] = {f: getattr(vps, f).copy() for f in ("x", "v", "rho", "p", "w_sum", "id")}
+        res[make].update({"real_" + f: getattr(real, f).copy() for f in ("x", "v", "rho")})
+    assert np.array_equal(res["dev"]["id"], res["orc"]["id"])
+    for k, ref in res["orc"].items():
+        if k == "id":
+            continue
+        assert rel(res["dev"][k], ref) <= tol, (k, rel(res["dev"][k], ref))
