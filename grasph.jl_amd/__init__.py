@@ -10,7 +10,7 @@ from .model import (  # noqa: F401
     BasicParticleSystem, CauchyFluidPfn, CubicSplineKernel, DynamicBoundarySystem,
     ElastoPlasticParticleSystem, ElastoPlasticStressUpdater, FluidParticleSystem, FluidPfn,
     GhostCopier, GhostEntry, GhostParticleSystem, HouseholderReflect, InterpolateFieldFn,
-    LeapFrogTimeIntegrator, LinearEOSUpdater, NeighborCounterPfn, PrescribedVelocityUpdater,
+    LeapFrogTimeIntegrator, LinearEOSUpdater, RK4TimeIntegrator, NeighborCounterPfn, PrescribedVelocityUpdater,
     StaticBoundarySystem, StrainRatePfn, StrainRateVorticityPfn, StressParticleSystem,
     SystemInteraction, TaitEOSUpdater, ViscoPlasticMCStressUpdater, VirtualNormUpdater,
     VirtualParticleSystem, WenlandC2Kernel, ZeroFieldUpdater)

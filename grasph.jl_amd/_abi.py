@@ -43,6 +43,7 @@ GR_WRAP_NONE, GR_WRAP_STATIC, GR_WRAP_DYNAMIC = range(3)
 (GR_UPD_NOTHING, GR_UPD_ZERO, GR_UPD_TAIT, GR_UPD_LINEAR, GR_UPD_EP, GR_UPD_VPMC, GR_UPD_VNORM,
  GR_UPD_PRESCRIBED_V) = range(8)
 GR_COPY_PLAIN, GR_COPY_HOUSEHOLDER = 0, 1
+GR_SCHEME_LEAPFROG, GR_SCHEME_RK4 = 0, 1
 
 
 class gr_pfn_desc(C.Structure):
@@ -74,6 +75,7 @@ _SIGS = {
     "interaction_create": [_i, _d, _i, _i, _i, _dp, _dp, _d, _d, _i,
                            C.POINTER(gr_pfn_desc), _ip],
     "integrator_create": [_i, _ip, _i, _ip, _i, _ip, _d, _ip],
+    "integrator_set_scheme": [_i, _i],
     "integrator_dt": [_i, _d, _dp],
     "step": [_i, _i, _d],
     "sort": [_i, _d],
@@ -239,6 +241,9 @@ class CBackend:
         self.call("integrator_create", len(systems), _iptr(s), len(inters), _iptr(i),
                   len(ghosts), _iptr(g), gamma, C.byref(out))
         return out.value
+
+    def integrator_set_scheme(self, integ, scheme):
+        self.call("integrator_set_scheme", integ, scheme)
 
     def integrator_dt(self, integ, cfl):
         out = C.c_double()

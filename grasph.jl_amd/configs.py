@@ -279,3 +279,60 @@ def ep_column_virtual_2d(dx=0.0025, width=0.2, height=0.1, rho0=1850.0, E=0.84e6
     return {"integrator": lf, "fluid": soil, "cfl": cfl, "name": "ep_column_virtual_2d",
             "n_particles": soil.n + bot_src.n + left_src.n, "fluid_n": soil.n,
             "virtual": (bottom, left)}
+
+
+def bubble_2d(dx=0.05, R=1.0, variant="bubble", cfl=1.5, rk4=True):
+    """Rising bubble (bubble.jl / bubble3.jl, after Colagrossi & Landrini
+    2003): a 6R × 10R column of heavy fluid X with a light fluid Y (half the
+    density) filling x² + y² < R, hydrostatic densities, Wendland C2
+    h = 1.2dx, ghost walls on all four sides and corners copying p (cutoff
+    3h), RK4.  variant "bubble": FluidPfn(0.01, 0, h) everywhere, Tait γ = 7
+    for both fluids.  variant "bubble3": FluidPfn with S = 1, surface tension
+    ε = 0.1 between the phases, γ = 1.4 for Y."""
+    height, width = 10.0 * R, 6.0 * R
+    y_min, x_min = -2.0 * R, -3.0 * R
+    y_max, x_max = y_min + height, x_min + width
+    h = 1.2 * dx
+    rho_x, rho_y = 1000.0, 500.0
+    g = 9.81
+    c_x = math.sqrt(800.0 * g * R)
+    c_y = 40.0 * math.sqrt(g * R)
+    alpha, beta = 0.01, 0.0
+    nx, ny = int(math.floor(width / dx)), int(math.floor(height / dx))
+    i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    x = x_min + (i.ravel() + 0.5) * dx
+    y = y_min + (j.ravel() + 0.5) * dx
+    inb = x * x + y * y < R
+    gam_y = 1.4 if variant == "bubble3" else 7.0
+    fx = M.FluidParticleSystem("fluid X", int((~inb).sum()), 2, dx * dx * rho_x, c_x,
+                               source_v=[0.0, -g], state_updater=M.TaitEOSUpdater(rho_x))
+    fy = M.FluidParticleSystem("fluid Y", int(inb.sum()), 2, dx * dx * rho_y, c_y,
+                               source_v=[0.0, -g],
+                               state_updater=M.TaitEOSUpdater(rho_y, gamma=gam_y))
+    fx.x[:] = np.stack([x[~inb], y[~inb]], 1)
+    fy.x[:] = np.stack([x[inb], y[inb]], 1)
+    px = (y_min + height - fx.x[:, 1]) * rho_x * g
+    fx.rho[:] = (px * 7.0 / (c_x * c_x * rho_x) + 1.0) ** (1.0 / 7.0) * rho_x
+    py = (y_min + height - fy.x[:, 1]) * rho_x * g
+    fy.rho[:] = (py * gam_y / (c_y * c_y * rho_y) + 1.0) ** (1.0 / gam_y) * rho_y
+    ghost = M.GhostParticleSystem(fx, M.GhostCopier("p"), name="ghost[fluid_X]")
+    r2 = math.sqrt(2.0)
+    entry = M.GhostEntry(ghost, 3.0 * h,
+                         ((1.0, 0.0), (x_min, 0.0)), ((-1.0, 0.0), (x_max, 0.0)),
+                         ((0.0, 1.0), (0.0, y_min)), ((0.0, -1.0), (0.0, y_max)),
+                         ((1.0 / r2, 1.0 / r2), (x_min, y_min)),
+                         ((-1.0 / r2, 1.0 / r2), (x_max, y_min)),
+                         ((1.0 / r2, -1.0 / r2), (x_min, y_max)),
+                         ((-1.0 / r2, -1.0 / r2), (x_max, y_max)))
+    kern = M.WenlandC2Kernel(h, ndims=2)
+    if variant == "bubble3":
+        pf = M.FluidPfn(alpha, beta, h, sigma=1)
+        pxy = M.FluidPfn(alpha, beta, h, sigma=1, epsilon=0.1)
+    else:
+        pf = pxy = M.FluidPfn(alpha, beta, h)
+    ints = [M.SystemInteraction(kern, pf, fx), M.SystemInteraction(kern, pf, fy),
+            M.SystemInteraction(kern, pxy, fy, fx), M.SystemInteraction(kern, pf, fx, ghost)]
+    cls = M.RK4TimeIntegrator if rk4 else M.LeapFrogTimeIntegrator
+    lf = cls([fx, fy], ints, ghosts=(entry,))
+    return {"integrator": lf, "fluid": fx, "light": fy, "cfl": cfl, "name": f"{variant}_2d",
+            "n_particles": fx.n + fy.n, "fluid_n": fx.n}

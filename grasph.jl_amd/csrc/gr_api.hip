@@ -106,6 +106,7 @@ struct IntegH {
   double gamma = 0, h = 0, c = 0, dt = 0;
   int nst = 0;
   bool dead = false;
+  int scheme = GR_SCHEME_LEAPFROG;
 };
 
 // Free ranges [off, off + len) of a pool, kept sorted and coalesced.
@@ -172,6 +173,7 @@ struct Plan {
   // neighbour lists
   DBuf ltasks, lsrcs, blk_list;
   int nblk_list = 0;
+  int scheme = GR_SCHEME_LEAPFROG;
   // integrated (non-virtual) real systems; virtual systems
   DBuf blk_int, blk_virt;
   int nblk_int = 0, nblk_virt = 0;
@@ -208,6 +210,7 @@ struct gr_ctx {
   DBuf vs, rhos, rho_h, gidx, gbnd, gnrm, gxs, gidx_s, gbnd_s;
   DBuf ra, rb;  // partner records (RecA / RecB), rebuilt before every stage's sweeps
   DBuf wsum;    // w_sum of virtual systems
+  DBuf acc;     // RK4 accumulators (4 components)
   // tables
   DBuf d_sd, d_n, d_st, d_bnd, d_kmin, d_kmax, d_bbmin, d_bbmax, d_kbase, d_krange, d_flags;
   DBuf d_grids, d_carr, d_cellsz, d_inter_sys, d_inter_carr, cells;
@@ -299,6 +302,7 @@ Pool<R> pool_view(gr_ctx *c, int w) {
   P.ra = c->ra.as<RecA<R>>();
   P.rb = c->rb.as<RecB<R>>();
   P.wsum = c->wsum.as<double>();
+  for (int k = 0; k < 4; ++k) P.acc[k] = c->acc.as<double>() + k * cap;
   for (int d = 0; d < 3; ++d) {
     P.x[d] = c->x[w].as<double>() + d * cap;
     P.v[d] = c->v[w].as<double>() + d * cap;
@@ -383,6 +387,7 @@ void commit(gr_ctx *c) {
     grow(c, c->gbnd_s, 1, 4, old, nc);
     grow(c, c->ra, 1, 32, old, nc);
     grow(c, c->wsum, 1, 8, old, nc);
+    grow(c, c->acc, 4, 8, old, nc);
     grow(c, c->rb, 1, R == 4 ? 16 : 32, old, nc);
     c->pool_cap = nc;
   }
@@ -1248,6 +1253,10 @@ Plan &get_plan(gr_ctx *c, const std::vector<int> &integs) {
          "a system belongs to more than one integrator of the batch");
   }
   ensure_cells(c, inters, reals);
+  P->scheme = c->integ[integs[0]].scheme;
+  for (int ig : integs)
+    need(c->integ[ig].scheme == P->scheme, GR_ERR_ARGUMENT,
+         "integrators advanced together must use the same time integration scheme");
   plan_sort_real(c, *P, reals);
   {
     std::vector<Blk> bi, bv;
@@ -1468,22 +1477,35 @@ void run_step(gr_ctx *c, Plan &P, bool cnt, std::vector<hipEvent_t> *ev) {
     launch_nlist<R>(s, A, a, c->D);
   }
   rec(4);
-  launch_halfstep<R>(s, A, c->d_sd.as<SysDesc>(), c->d_st.as<StepSys>(), P.blk_int.as<Blk>(),
-                     P.nblk_int, c->D);
-  launch_ghost_kin<R>(s, A, c->d_sd.as<SysDesc>(), c->d_bnd.as<Bnd>(), c->d_n.as<int>(),
-                      P.blk_ghost.as<Blk>(), P.nblk_ghost, c->D);
-  launch_virtual<R>(s, A, c->d_sd.as<SysDesc>(), c->d_st.as<StepSys>(), c->d_n.as<int>(),
-                    P.blk_virt.as<Blk>(), P.nblk_virt, 0, c->D);
-  rec(5);
   const int nst = (int)P.stages.size();
-  for (int k = 0; k < nst; ++k) {
-    phase_stage<R>(c, P, k, cnt, ev ? &(*ev)[6 + 2 * k] : nullptr);
-    rec(7 + 2 * k);
+  const SysDesc *sd = c->d_sd.as<SysDesc>();
+  const StepSys *st = c->d_st.as<StepSys>();
+  // the stage loop of one pass (_sweep_all_stages!) after the stage state
+  auto pass = [&]() {
+    launch_ghost_kin<R>(s, A, sd, c->d_bnd.as<Bnd>(), c->d_n.as<int>(), P.blk_ghost.as<Blk>(),
+                        P.nblk_ghost, c->D);
+    launch_virtual<R>(s, A, sd, st, c->d_n.as<int>(), P.blk_virt.as<Blk>(), P.nblk_virt, 0, c->D);
+    rec(5);
+    for (int k = 0; k < nst; ++k) {
+      phase_stage<R>(c, P, k, cnt, ev ? &(*ev)[6 + 2 * k] : nullptr);
+      rec(7 + 2 * k);
+    }
+  };
+  if (P.scheme == GR_SCHEME_RK4) {  // src/TimeIntegration.jl:789-815
+    const double adv[4] = {0.0, 0.5, 0.5, 1.0};
+    const double wts[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+    for (int r = 0; r < 4; ++r) {
+      launch_halfstep<R>(s, A, sd, st, P.blk_int.as<Blk>(), P.nblk_int, c->D, adv[r], r == 0);
+      pass();
+      launch_rk4_acc<R>(s, A, sd, st, P.blk_int.as<Blk>(), P.nblk_int, c->D, wts[r], r == 0);
+    }
+    launch_rk4_apply<R>(s, A, sd, st, P.blk_int.as<Blk>(), P.nblk_int, c->D);
+  } else {
+    launch_halfstep<R>(s, A, sd, st, P.blk_int.as<Blk>(), P.nblk_int, c->D, 0.5, 0);
+    pass();
+    launch_fullstep<R>(s, A, sd, st, P.blk_int.as<Blk>(), P.nblk_int, c->D);
   }
-  launch_fullstep<R>(s, A, c->d_sd.as<SysDesc>(), c->d_st.as<StepSys>(), P.blk_int.as<Blk>(),
-                     P.nblk_int, c->D);
-  launch_virtual<R>(s, A, c->d_sd.as<SysDesc>(), c->d_st.as<StepSys>(), c->d_n.as<int>(),
-                    P.blk_virt.as<Blk>(), P.nblk_virt, 1, c->D);
+  launch_virtual<R>(s, A, sd, st, c->d_n.as<int>(), P.blk_virt.as<Blk>(), P.nblk_virt, 1, c->D);
   rec(6 + 2 * nst);
 }
 
@@ -2167,6 +2189,20 @@ int gr_integrator_release(gr_ctx *c, int integ) {
         if (ca >= 0) c->free_cells.put(c->carr[ca].off, c->carr[ca].cap);
     }
     c->allreal_cap = -1;  // rebuild the all-real-systems table
+  });
+}
+
+int gr_integrator_set_scheme(gr_ctx *c, int integ, int scheme) {
+  return guard(c, [&] {
+    need(integ >= 0 && integ < (int)c->integ.size(), GR_ERR_ARGUMENT, "bad integrator");
+    need(!c->integ[integ].dead, GR_ERR_STATE, "integrator was released");
+    need(scheme == GR_SCHEME_LEAPFROG || scheme == GR_SCHEME_RK4, GR_ERR_ARGUMENT,
+         "unknown time integration scheme");
+    if (c->integ[integ].scheme != scheme) {
+      c->integ[integ].scheme = scheme;
+      drop_graphs(c);
+      c->plans.clear();
+    }
   });
 }
 

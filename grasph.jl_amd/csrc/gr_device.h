@@ -183,7 +183,8 @@ template <typename R>
 struct Pool {
   RecA<R> *ra;
   RecB<R> *rb;
-  double *wsum;  // virtual systems: interpolation weight sums
+  double *wsum;    // virtual systems: interpolation weight sums
+  double *acc[4];  // RK4 accumulators of dvdt (D components) and drhodt
   double *x[3], *v[3], *rho, *rho_h;
   R *vs[3], *rhos, *dvdt[3], *drhodt, *p;
   double *stress[GR_MAXNS];

@@ -864,7 +864,40 @@ __device__ inline double eos_p(int kind, double rho, double rho0, double gamma, 
 // (v and ρ keep their start-of-step values for the full step), the
 // FP64 half-step density, the stage-1 EOS when fused, and the source reset.
 template <typename R, int D>
-__global__ void k_halfstep(Pool<R> P, const SysDesc *sd, const StepSys *st, const Blk *blk) {
+__global__ void k_halfstep(Pool<R> P, const SysDesc *sd, const StepSys *st, const Blk *blk,
+                           double mult, int keep) {
+  const Blk b = blk[blockIdx.x];
+  const int s = b.item;
+  const int i = b.first + threadIdx.x;
+  const SysDesc S = sd[s];
+  if (i >= S.cap) return;
+  const StepSys T = st[s];
+  const int64_t g = S.off + i;
+  // stage state q = q0 + (mult·dt)·dqdt: the leapfrog half step (mult ½) and
+  // the RK4 stage advances (_rk4_advance_all!, src/TimeIntegration.jl:340-348;
+  // keep: the first RK4 stage, q = q0)
+  const double cf = mult * T.dt;
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    double vh = keep ? P.v[d][g] : P.v[d][g] + cf * (double)P.dvdt[d][g];
+    P.vs[d][g] = (R)vh;
+  }
+  const double rh = keep ? P.rho[g] : fma(cf, (double)P.drhodt[g], P.rho[g]);
+  P.rho_h[g] = rh;
+  P.rhos[g] = (R)rh;
+  if (T.fuse_eos) P.p[g] = (R)eos_p(T.fuse_eos, rh, T.eos_rho0, T.eos_gamma, S.c);
+  if (T.reset_force) {
+#pragma unroll
+    for (int d = 0; d < D; ++d) P.dvdt[d][g] = (R)S.sv[d];
+    P.drhodt[g] = (R)S.srho;
+  }
+}
+
+// RK4 (src/TimeIntegration.jl:350-362,801-811): damping of the stage rates,
+// then acc += w·dqdt (vectors unfused, the density by the fused axpy).
+template <typename R, int D>
+__global__ void k_rk4_acc(Pool<R> P, const SysDesc *sd, const StepSys *st, const Blk *blk,
+                          double w, int first) {
   const Blk b = blk[blockIdx.x];
   const int s = b.item;
   const int i = b.first + threadIdx.x;
@@ -874,18 +907,34 @@ __global__ void k_halfstep(Pool<R> P, const SysDesc *sd, const StepSys *st, cons
   const int64_t g = S.off + i;
 #pragma unroll
   for (int d = 0; d < D; ++d) {
-    double vh = P.v[d][g] + T.half * (double)P.dvdt[d][g];
-    P.vs[d][g] = (R)vh;
+    double a = (double)P.dvdt[d][g];
+    if (T.damping) {
+      a = a - T.gdt * (double)P.vs[d][g];
+      P.dvdt[d][g] = (R)a;
+      a = (double)P.dvdt[d][g];
+    }
+    P.acc[d][g] = (first ? 0.0 : P.acc[d][g]) + w * a;
   }
-  const double rh = fma(T.half, (double)P.drhodt[g], P.rho[g]);
-  P.rho_h[g] = rh;
-  P.rhos[g] = (R)rh;
-  if (T.fuse_eos) P.p[g] = (R)eos_p(T.fuse_eos, rh, T.eos_rho0, T.eos_gamma, S.c);
-  if (T.reset_force) {
+  P.acc[3][g] = fma(w, (double)P.drhodt[g], first ? 0.0 : P.acc[3][g]);
+}
+
+// q = q0 + dt·acc, then x += dt·v (_apply_acc_all!, _update_positions_all!).
+template <typename R, int D>
+__global__ void k_rk4_apply(Pool<R> P, const SysDesc *sd, const StepSys *st, const Blk *blk) {
+  const Blk b = blk[blockIdx.x];
+  const int s = b.item;
+  const int i = b.first + threadIdx.x;
+  const SysDesc S = sd[s];
+  if (i >= S.cap) return;
+  const StepSys T = st[s];
+  const int64_t g = S.off + i;
 #pragma unroll
-    for (int d = 0; d < D; ++d) P.dvdt[d][g] = (R)S.sv[d];
-    P.drhodt[g] = (R)S.srho;
+  for (int d = 0; d < D; ++d) {
+    const double vn = P.v[d][g] + T.dt * P.acc[d][g];
+    P.v[d][g] = vn;
+    P.x[d][g] = P.x[d][g] + T.dt * vn;
   }
+  P.rho[g] = fma(T.dt, P.acc[3][g], P.rho[g]);
 }
 
 // _apply_damping_all! (:251-271), _fullstep_q_all! (:227-232),
@@ -1398,12 +1447,30 @@ void launch_cells_build(hipStream_t s, XP x, int *cells, const CellArr *carr,
 
 template <typename R>
 void launch_halfstep(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
-                     const Blk *blk, int nblk, int D) {
+                     const Blk *blk, int nblk, int D, double mult, int keep) {
   if (nblk <= 0) return;
   if (D == 2)
-    k_halfstep<R, 2><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, blk);
+    k_halfstep<R, 2><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, blk, mult, keep);
   else
-    k_halfstep<R, 3><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, blk);
+    k_halfstep<R, 3><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, blk, mult, keep);
+}
+template <typename R>
+void launch_rk4_acc(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
+                    const Blk *blk, int nblk, int D, double w, int first) {
+  if (nblk <= 0) return;
+  if (D == 2)
+    k_rk4_acc<R, 2><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, blk, w, first);
+  else
+    k_rk4_acc<R, 3><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, blk, w, first);
+}
+template <typename R>
+void launch_rk4_apply(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
+                      const Blk *blk, int nblk, int D) {
+  if (nblk <= 0) return;
+  if (D == 2)
+    k_rk4_apply<R, 2><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, blk);
+  else
+    k_rk4_apply<R, 3><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, blk);
 }
 template <typename R>
 void launch_fullstep(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
@@ -1421,7 +1488,11 @@ void launch_updaters(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const S
 }
 #define INST(R)                                                                                \
   template void launch_halfstep<R>(hipStream_t, const Pool<R> &, const SysDesc *,              \
-                                   const StepSys *, const Blk *, int, int);                    \
+                                   const StepSys *, const Blk *, int, int, double, int);       \
+  template void launch_rk4_acc<R>(hipStream_t, const Pool<R> &, const SysDesc *,               \
+                                  const StepSys *, const Blk *, int, int, double, int);        \
+  template void launch_rk4_apply<R>(hipStream_t, const Pool<R> &, const SysDesc *,             \
+                                    const StepSys *, const Blk *, int, int);                   \
   template void launch_fullstep<R>(hipStream_t, const Pool<R> &, const SysDesc *,              \
                                    const StepSys *, const Blk *, int, int);                    \
   template void launch_updaters<R>(hipStream_t, const Pool<R> &, const SysDesc *,              \

@@ -83,7 +83,13 @@ void launch_cells_build(hipStream_t s, XP x, int *cells, const CellArr *carr,
                         int D);
 template <typename R>
 void launch_halfstep(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
-                     const Blk *blk, int nblk, int D);
+                     const Blk *blk, int nblk, int D, double mult, int keep);
+template <typename R>
+void launch_rk4_acc(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
+                    const Blk *blk, int nblk, int D, double w, int first);
+template <typename R>
+void launch_rk4_apply(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
+                      const Blk *blk, int nblk, int D);
 template <typename R>
 void launch_fullstep(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
                      const Blk *blk, int nblk, int D);

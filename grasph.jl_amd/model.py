@@ -578,6 +578,8 @@ class LeapFrogTimeIntegrator:
     """LeapFrogTimeIntegrator(systems, interactions; ghosts=(),
     virtual_systems=(), Γ=0)."""
 
+    scheme = A.GR_SCHEME_LEAPFROG
+
     def __init__(self, systems, interactions, ghosts=(), gamma=0.0, virtual_systems=()):
         sys = (systems,) if isinstance(systems, _ParticleSystem) else tuple(systems)
         ints = (interactions,) if isinstance(interactions, SystemInteraction) else tuple(interactions)
@@ -596,3 +598,12 @@ class LeapFrogTimeIntegrator:
         self.c = max(ps.c for ps in sys)
         self.h = min(si.kernel.h for si in ints)
         self.ndims = sys[0].ndims
+
+
+class RK4TimeIntegrator(LeapFrogTimeIntegrator):
+    """RK4TimeIntegrator(systems, interactions; ghosts=(), virtual_systems=(),
+    Γ=0) (src/TimeIntegration.jl:84-124): same construction as the leapfrog;
+    time_integrate! runs four stage passes per step over one grid
+    (:712-838)."""
+
+    scheme = A.GR_SCHEME_RK4

@@ -155,6 +155,8 @@ class DeviceScene:
         gh = [self.ghost(ge) for ge in lf.ghosts]
         its = [self.interaction(si) for si in lf.interactions]
         h = self.be.integrator_create(sys, its, gh, lf.gamma)
+        if getattr(lf, "scheme", A.GR_SCHEME_LEAPFROG) != A.GR_SCHEME_LEAPFROG:
+            self.be.integrator_set_scheme(h, lf.scheme)
         self._integ[k] = h
         self._objs.append(lf)
         return h

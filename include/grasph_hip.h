@@ -212,6 +212,12 @@ int gr_integrator_create(gr_ctx *ctx, int nsys, const int *systems,
  * simulation; a serving loop that admits and retires simulations needs it).
  * Handles of released objects are rejected with GR_ERR_STATE. */
 int gr_integrator_release(gr_ctx *ctx, int integ);
+/* Time integration scheme of an integrator: GR_SCHEME_LEAPFROG (default,
+ * LeapFrogTimeIntegrator) or GR_SCHEME_RK4 (RK4TimeIntegrator,
+ * src/TimeIntegration.jl:84-124,712-838: one grid per step, four stage
+ * passes). */
+enum { GR_SCHEME_LEAPFROG = 0, GR_SCHEME_RK4 = 1 };
+int gr_integrator_set_scheme(gr_ctx *ctx, int integ, int scheme);
 /* dt = CFL·h/c with h = min kernel h, c = max system c
  * (src/TimeIntegration.jl:63-64,598). */
 int gr_integrator_dt(gr_ctx *ctx, int integ, double cfl, double *dt_out);

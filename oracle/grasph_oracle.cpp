@@ -78,6 +78,7 @@ struct Integ {
   std::vector<int> sys, inter, ghost, virt;
   double gamma = 0, h = 0, c = 0;
   int nst = 0;
+  int scheme = GR_SCHEME_LEAPFROG;
 };
 
 }  // namespace
@@ -1282,7 +1283,125 @@ void update_ghost(orc_scene *sc, Sys &g, int stage) {  // :112-129, :261-278
 // Leapfrog step (src/TimeIntegration.jl:645-693)
 // ---------------------------------------------------------------------------
 
+// _sweep_all_stages! (src/TimeIntegration.jl:401-419).
+int sweep_all_stages(orc_scene *sc, Integ &ig, double dt) {
+  for (int v : ig.virt) {  // auto_zero_virtual! (src/Particles.jl:521-528)
+    Sys &s = sc->sys[v];
+    std::fill(s.wsum.begin(), s.wsum.end(), 0.0);
+    zero_fields(s, s.zero_mask);
+  }
+  for (int stage = 1; stage <= ig.nst; ++stage) {
+    for (int id : ig.sys) {
+      Sys &s = sc->sys[id];
+      if ((int)s.upd.size() != ig.nst) continue;
+      int rc = update_state(sc, s, stage, dt);
+      if (rc) return rc;
+    }
+    for (int g : ig.ghost) update_ghost(sc, sc->sys[g], stage);
+    for (int v : ig.virt) {
+      Sys &s = sc->sys[v];
+      if ((int)s.upd.size() != ig.nst) continue;
+      int rc = update_state(sc, s, stage, dt);
+      if (rc) return rc;
+    }
+    for (int it : ig.inter) {
+      int rc = sweep(sc, sc->inter[it], stage);
+      if (rc) return rc;
+    }
+  }
+  return GR_OK;
+}
+
+void apply_damping(orc_scene *sc, Integ &ig, double dt) {  // _apply_damping_all! :251-271
+  if (ig.gamma == 0.0) return;
+  const double gdt = ig.gamma / dt;
+  for (int id : ig.sys) {
+    Sys &s = sc->sys[id];
+    for (int64_t i = 0; i < s.n * s.D; ++i) s.dvdt[i] = s.dvdt[i] - gdt * s.v[i];
+  }
+}
+
+void reset_dqdt(Sys &s) {  // _reset_dqdt_pairs! :150-154
+  const int D = s.D;
+  for (int64_t i = 0; i < s.n; ++i) {
+    for (int d = 0; d < D; ++d) s.dvdt[i * D + d] = s.sv[d];
+    s.drhodt[i] = s.srho;
+  }
+}
+
+void advance_virtual_positions(orc_scene *sc, Integ &ig, double dt) {  // :389-397
+  for (int v : ig.virt) {
+    Sys &s = sc->sys[v];
+    const int D = s.D;
+    for (int64_t i = 0; i < s.n; ++i)
+      for (int d = 0; d < D; ++d) s.x[i * D + d] = s.x[i * D + d] + s.pv[d] * dt;
+  }
+}
+
+// time_integrate!(::RK4TimeIntegrator) per-step body (src/TimeIntegration.jl:
+// 773-819): one grid per step, four stage passes from q0 with the advance
+// coefficients (0, ½, ½, 1)·dt, weights (1/6, 1/3, 1/3, 1/6), q = q0 + dt·acc.
+// Scalars use the fused axpy (@fastmath, src/Utils.jl:19-29), vectors do not.
+int step_rk4(orc_scene *sc, Integ &ig, double dt) {
+  const double cutoff = 2.0 * ig.h;
+  for (int id : ig.sys) sort_system(sc, sc->sys[id], cutoff);
+  const size_t ns = ig.sys.size();
+  std::vector<std::vector<double>> v0(ns), r0(ns), av(ns), ar(ns);
+  for (size_t k = 0; k < ns; ++k) {
+    const Sys &s = sc->sys[ig.sys[k]];
+    v0[k] = s.v;
+    r0[k] = s.rho;
+    av[k].assign(s.v.size(), 0.0);
+    ar[k].assign(s.rho.size(), 0.0);
+  }
+  for (int g : ig.ghost) generate_ghosts(sc, sc->sys[g]);
+  for (int g : ig.ghost) sort_system(sc, sc->sys[g], cutoff);
+  for (int v : ig.virt) sort_system(sc, sc->sys[v], cutoff);
+  for (int it : ig.inter) {
+    int rc = create_grid(sc, sc->inter[it]);
+    if (rc) return rc;
+  }
+  const double adv[4] = {0.0, 0.5, 0.5, 1.0};
+  const double wts[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+  for (int r = 0; r < 4; ++r) {
+    for (size_t k = 0; k < ns; ++k) {
+      Sys &s = sc->sys[ig.sys[k]];
+      const int D = s.D;
+      if (r > 0) {  // _rk4_advance_all!: q = q0 + (adv·dt)·dqdt
+        const double cf = adv[r] * dt;
+        for (int64_t i = 0; i < s.n; ++i) {
+          for (int d = 0; d < D; ++d)
+            s.v[i * D + d] = v0[k][i * D + d] + cf * s.dvdt[i * D + d];
+          s.rho[i] = std::fma(cf, s.drhodt[i], r0[k][i]);
+        }
+      }
+      reset_dqdt(s);
+    }
+    for (int g : ig.ghost) ghost_kinematics(sc, sc->sys[g]);
+    int rc = sweep_all_stages(sc, ig, dt);
+    if (rc) return rc;
+    apply_damping(sc, ig, dt);
+    for (size_t k = 0; k < ns; ++k) {  // _acc_dqdt_all!: acc += w·dqdt
+      Sys &s = sc->sys[ig.sys[k]];
+      for (size_t i = 0; i < av[k].size(); ++i) av[k][i] = av[k][i] + wts[r] * s.dvdt[i];
+      for (size_t i = 0; i < ar[k].size(); ++i) ar[k][i] = std::fma(wts[r], s.drhodt[i], ar[k][i]);
+    }
+  }
+  for (size_t k = 0; k < ns; ++k) {  // _apply_acc_all!, _update_positions_all!
+    Sys &s = sc->sys[ig.sys[k]];
+    const int D = s.D;
+    for (int64_t i = 0; i < s.n; ++i) {
+      for (int d = 0; d < D; ++d) s.v[i * D + d] = v0[k][i * D + d] + dt * av[k][i * D + d];
+      s.rho[i] = std::fma(dt, ar[k][i], r0[k][i]);
+      for (int d = 0; d < D; ++d) s.x[i * D + d] = s.x[i * D + d] + dt * s.v[i * D + d];
+    }
+  }
+  advance_virtual_positions(sc, ig, dt);
+  return GR_OK;
+}
+
 int step(orc_scene *sc, Integ &ig, double dt) {
+  if (ig.scheme == GR_SCHEME_RK4) return step_rk4(sc, ig, dt);
   const double cutoff = 2.0 * ig.h;  // sort_cutoff :612
   for (int id : ig.sys) sort_system(sc, sc->sys[id], cutoff);
   std::vector<std::vector<double>> v0(ig.sys.size()), r0(ig.sys.size());
@@ -1694,6 +1813,14 @@ int orc_integrator_create(orc_scene *s, int nsys, const int *systems, int ninter
       return fail(s, GR_ERR_ASSERTION, "All interactions must have the same number of stages (pfns length)");
   s->integ.push_back(std::move(ig));
   *integ_out = (int)s->integ.size() - 1;
+  return GR_OK;
+}
+
+int orc_integrator_set_scheme(orc_scene *s, int integ, int scheme) {
+  if (integ < 0 || integ >= (int)s->integ.size()) return fail(s, GR_ERR_ARGUMENT, "bad integrator");
+  if (scheme != GR_SCHEME_LEAPFROG && scheme != GR_SCHEME_RK4)
+    return fail(s, GR_ERR_ARGUMENT, "unknown time integration scheme");
+  s->integ[integ].scheme = scheme;
   return GR_OK;
 }
 

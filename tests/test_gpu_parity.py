@@ -353,6 +353,13 @@ def test_ghosts(D):
 # ---------------------------------------------------------------------------
 
 
+def _as_rk4(cfg):
+    lf = cfg["integrator"]
+    cfg["integrator"] = M.RK4TimeIntegrator(lf.systems, lf.interactions, ghosts=lf.ghosts,
+                                            virtual_systems=lf.virtual_systems, gamma=lf.gamma)
+    return cfg
+
+
 def _run_steps(builder, kw, D, prec, nsteps, make):
     cfg = builder(**kw)
     lf = cfg["integrator"]
@@ -378,6 +385,10 @@ STEP_CASES = [
     ("granular_column_3d", M.configs.granular_column_3d, dict(dx=0.01), 3, 20),
     ("ep_column_virtual_2d", M.configs.ep_column_virtual_2d,
      dict(dx=0.01, width=0.1, height=0.06), 2, 20),
+    ("bubble_2d_rk4", M.configs.bubble_2d, dict(dx=0.2), 2, 10),
+    ("bubble3_2d_rk4", M.configs.bubble_2d, dict(dx=0.2, variant="bubble3"), 2, 10),
+    ("ep_column_2d_rk4", lambda **kw: _as_rk4(M.configs.ep_column_2d(**kw)),
+     dict(dx=0.01, width=0.2, height=0.1), 2, 10),
 ]
 
 
