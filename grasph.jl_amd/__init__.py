@@ -13,7 +13,7 @@ from .model import (  # noqa: F401
     LeapFrogTimeIntegrator, LinearEOSUpdater, RK4TimeIntegrator, NeighborCounterPfn, PrescribedVelocityUpdater,
     StaticBoundarySystem, StrainRatePfn, StrainRateVorticityPfn, StressParticleSystem,
     SystemInteraction, TaitEOSUpdater, ViscoPlasticMCStressUpdater, VirtualNormUpdater,
-    VirtualParticleSystem, WenlandC2Kernel, ZeroFieldUpdater)
+    VirtualParticleSystem, WenlandC2Kernel, XSPHPfn, ZeroFieldUpdater)
 from .scene import DeviceScene  # noqa: F401
 from .integration import (  # noqa: F401
     Session, create_grid, sort_particles, sweep, time_integrate, update_state)

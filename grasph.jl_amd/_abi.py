@@ -38,7 +38,7 @@ INDEX_IDS = {"id": GR_I_ID, "idx_original": GR_I_IDX_ORIGINAL,
 # ---- kernels / functors / wrappers / updaters / copier modes ------------------
 GR_KERNEL_CUBIC, GR_KERNEL_WENDLAND_C2 = 0, 1
 (GR_PFN_NONE, GR_PFN_FLUID, GR_PFN_CAUCHY, GR_PFN_SR, GR_PFN_SRV, GR_PFN_COUNT,
- GR_PFN_INTERP) = range(7)
+ GR_PFN_INTERP, GR_PFN_XSPH) = range(8)
 GR_WRAP_NONE, GR_WRAP_STATIC, GR_WRAP_DYNAMIC = range(3)
 (GR_UPD_NOTHING, GR_UPD_ZERO, GR_UPD_TAIT, GR_UPD_LINEAR, GR_UPD_EP, GR_UPD_VPMC, GR_UPD_VNORM,
  GR_UPD_PRESCRIBED_V) = range(8)
@@ -75,6 +75,7 @@ _SIGS = {
     "interaction_create": [_i, _d, _i, _i, _i, _dp, _dp, _d, _d, _i,
                            C.POINTER(gr_pfn_desc), _ip],
     "integrator_create": [_i, _ip, _i, _ip, _i, _ip, _d, _ip],
+    "interaction_set_vadjust": [_i, _d],
     "integrator_set_scheme": [_i, _i],
     "integrator_dt": [_i, _d, _dp],
     "step": [_i, _i, _d],
@@ -241,6 +242,9 @@ class CBackend:
         self.call("integrator_create", len(systems), _iptr(s), len(inters), _iptr(i),
                   len(ghosts), _iptr(g), gamma, C.byref(out))
         return out.value
+
+    def interaction_set_vadjust(self, inter, epsilon):
+        self.call("interaction_set_vadjust", inter, float(epsilon))
 
     def integrator_set_scheme(self, integ, scheme):
         self.call("integrator_set_scheme", integ, scheme)

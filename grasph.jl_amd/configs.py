@@ -281,14 +281,15 @@ def ep_column_virtual_2d(dx=0.0025, width=0.2, height=0.1, rho0=1850.0, E=0.84e6
             "virtual": (bottom, left)}
 
 
-def bubble_2d(dx=0.05, R=1.0, variant="bubble", cfl=1.5, rk4=True):
+def bubble_2d(dx=0.05, R=1.0, variant="bubble", cfl=1.5, rk4=True, xsph=None):
     """Rising bubble (bubble.jl / bubble3.jl, after Colagrossi & Landrini
     2003): a 6R × 10R column of heavy fluid X with a light fluid Y (half the
     density) filling x² + y² < R, hydrostatic densities, Wendland C2
     h = 1.2dx, ghost walls on all four sides and corners copying p (cutoff
     3h), RK4.  variant "bubble": FluidPfn(0.01, 0, h) everywhere, Tait γ = 7
     for both fluids.  variant "bubble3": FluidPfn with S = 1, surface tension
-    ε = 0.1 between the phases, γ = 1.4 for Y."""
+    ε = 0.1 between the phases, γ = 1.4 for Y, and XSPHPfn(0.5) on the X, Y
+    and X–ghost interactions (xsph overrides ε_XSPH; 0 switches it off)."""
     height, width = 10.0 * R, 6.0 * R
     y_min, x_min = -2.0 * R, -3.0 * R
     y_max, x_max = y_min + height, x_min + width
@@ -330,8 +331,13 @@ def bubble_2d(dx=0.05, R=1.0, variant="bubble", cfl=1.5, rk4=True):
         pxy = M.FluidPfn(alpha, beta, h, sigma=1, epsilon=0.1)
     else:
         pf = pxy = M.FluidPfn(alpha, beta, h)
-    ints = [M.SystemInteraction(kern, pf, fx), M.SystemInteraction(kern, pf, fy),
-            M.SystemInteraction(kern, pxy, fy, fx), M.SystemInteraction(kern, pf, fx, ghost)]
+    if xsph is None:
+        xsph = 0.5 if variant == "bubble3" else 0.0
+    va = M.XSPHPfn(xsph) if xsph else None
+    ints = [M.SystemInteraction(kern, pf, fx, velocity_adjust_pairwise_fn=va),
+            M.SystemInteraction(kern, pf, fy, velocity_adjust_pairwise_fn=va),
+            M.SystemInteraction(kern, pxy, fy, fx),
+            M.SystemInteraction(kern, pf, fx, ghost, velocity_adjust_pairwise_fn=va)]
     cls = M.RK4TimeIntegrator if rk4 else M.LeapFrogTimeIntegrator
     lf = cls([fx, fy], ints, ghosts=(entry,))
     return {"integrator": lf, "fluid": fx, "light": fy, "cfl": cfl, "name": f"{variant}_2d",

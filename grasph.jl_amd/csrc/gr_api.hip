@@ -97,6 +97,8 @@ struct InterH {
   double wn[3] = {0, 0, 0}, wp[3] = {0, 0, 0}, beta = 0, ljcut = 0;
   std::vector<gr_pfn_desc> pfns;
   int carr_b = -1, carr_a = -1;  // CellArr indices (b or self; a when coupled)
+  bool has_xsph = false;  // velocity_adjust_pairwise_fn = XSPHPfn(xsph_eps)
+  double xsph_eps = 0;
   bool grid_built = false;
   bool dead = false;
 };
@@ -150,7 +152,7 @@ struct StagePlan {
   int nblk_upd = 0;
   DBuf blk_gcopy, gcopy;
   int nblk_gcopy = 0;
-  FamPlan fam[4];
+  FamPlan fam[NFAM];
 };
 
 struct Plan {
@@ -174,6 +176,11 @@ struct Plan {
   DBuf ltasks, lsrcs, blk_list;
   int nblk_list = 0;
   int scheme = GR_SCHEME_LEAPFROG;
+  // velocity adjustment after the full step (XSPH sweep tasks; vadj_live:
+  // the correction runs, also with no XSPH sweep when v_adjustment was set)
+  FamPlan xsph;
+  std::vector<int2> xsph_alias;  // (ghost, source n) of ghost-target XSPH tasks
+  int vadj_live = 0;
   // integrated (non-virtual) real systems; virtual systems
   DBuf blk_int, blk_virt;
   int nblk_int = 0, nblk_virt = 0;
@@ -211,6 +218,8 @@ struct gr_ctx {
   DBuf ra, rb;  // partner records (RecA / RecB), rebuilt before every stage's sweeps
   DBuf wsum;    // w_sum of virtual systems
   DBuf acc;     // RK4 accumulators (4 components)
+  DBuf vadj[2];  // v_adjustment (XSPH), swapped with the sort
+  bool vadj_any = false;  // some v_adjustment may be nonzero: permute and correct it
   // tables
   DBuf d_sd, d_n, d_st, d_bnd, d_kmin, d_kmax, d_bbmin, d_bbmax, d_kbase, d_krange, d_flags;
   DBuf d_grids, d_carr, d_cellsz, d_inter_sys, d_inter_carr, cells;
@@ -303,6 +312,7 @@ Pool<R> pool_view(gr_ctx *c, int w) {
   P.rb = c->rb.as<RecB<R>>();
   P.wsum = c->wsum.as<double>();
   for (int k = 0; k < 4; ++k) P.acc[k] = c->acc.as<double>() + k * cap;
+  for (int d = 0; d < 3; ++d) P.vadj[d] = c->vadj[w].as<double>() + d * cap;
   for (int d = 0; d < 3; ++d) {
     P.x[d] = c->x[w].as<double>() + d * cap;
     P.v[d] = c->v[w].as<double>() + d * cap;
@@ -348,12 +358,15 @@ void grow(gr_ctx *c, DBuf &b, int comps, size_t esz, int64_t old_cap, int64_t ne
 void commit(gr_ctx *c) {
   if (!c->dirty_layout) return;
   // new systems take a released segment (first fit) or go at the end
+  std::vector<std::pair<int64_t, int64_t>> fresh;
   for (auto &s : c->sys) {
     if (s.dead || s.off >= 0) continue;
     s.off = c->free_pool.take(s.cap);
     if (s.off < 0) {
       s.off = c->pool_end;
       c->pool_end += s.cap;
+    } else {
+      fresh.push_back({s.off, s.cap});  // a reused segment keeps old contents
     }
   }
   const int64_t need_cap = std::max<int64_t>(c->pool_end, 1);
@@ -375,6 +388,7 @@ void commit(gr_ctx *c) {
       grow(c, c->vor[w], 3, R, old, nc);
       grow(c, c->strain[w], c->nsmax, 8, old, nc);
       grow(c, c->strainp[w], c->nsmax, 8, old, nc);
+      grow(c, c->vadj[w], 3, 8, old, nc);
     }
     grow(c, c->vs, 3, R, old, nc);
     grow(c, c->rhos, 1, R, old, nc);
@@ -391,6 +405,12 @@ void commit(gr_ctx *c) {
     grow(c, c->rb, 1, R == 4 ? 16 : 32, old, nc);
     c->pool_cap = nc;
   }
+  // v_adjustment starts at zero (src/Particles.jl:82) in both families
+  for (auto &f : fresh)
+    for (int w = 0; w < 2; ++w)
+      for (int d = 0; d < 3; ++d)
+        HIPCHK(hipMemsetAsync(c->vadj[w].as<double>() + d * c->pool_cap + f.first, 0,
+                              (size_t)f.second * 8, c->stream));
   c->plans.clear();  // block tables and captured graphs refer to the layout
   // id = 1..n for systems laid out for the first time (src/Particles.jl:92)
   c->id_written.resize(c->sys.size(), 0);
@@ -592,6 +612,7 @@ FieldRef field_ref(gr_ctx *c, int field, bool shadow_only) {
     case GR_F_STRAIN: add(c->strain[w], c->nsmax, 8); break;
     case GR_F_STRAIN_P: add(c->strainp[w], c->nsmax, 8); break;
     case GR_F_W_SUM: add(c->wsum, 1, 8); break;
+    case GR_F_V_ADJUSTMENT: add(c->vadj[w], c->D, 8); break;
   }
   return f;
 }
@@ -680,6 +701,7 @@ int family_of(int pfn) {
   if (pfn == GR_PFN_SR || pfn == GR_PFN_SRV) return FAM_STRAIN;
   if (pfn == GR_PFN_COUNT) return FAM_COUNT;
   if (pfn == GR_PFN_INTERP) return FAM_INTERP;
+  if (pfn == GR_PFN_XSPH) return FAM_XSPH;
   return -1;
 }
 
@@ -1035,24 +1057,43 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
     Task t;
     std::vector<SrcDesc> src;
   };
-  std::vector<std::map<int, TaskB>> tasks(nst * 4);  // [stage*4+fam] target → task
-  for (int k = 0; k < nst; ++k)
+  // [stage·NFAM + fam] target → task; stage slot nst holds the XSPH sweep
+  std::vector<std::map<int, TaskB>> tasks((nst + 1) * NFAM);
+  for (int k = 0; k <= nst; ++k)
     for (int ig : P.integs)
       for (int it : c->integ[ig].inter) {
         const InterH &in = c->inter[it];
-        if (k >= (int)in.pfns.size()) continue;
-        const gr_pfn_desc &f = in.pfns[k];
+        gr_pfn_desc xf;
+        const gr_pfn_desc *fp = nullptr;
+        if (k == nst) {  // adjust_v! (src/Interaction.jl:309): the generic two-sided method
+          if (!in.has_xsph) continue;
+          memset(&xf, 0, sizeof(xf));
+          xf.kind = GR_PFN_XSPH;
+          xf.alpha = in.xsph_eps;
+          fp = &xf;
+        } else {
+          if (k >= (int)in.pfns.size()) continue;
+          fp = &in.pfns[k];
+        }
+        const gr_pfn_desc &f = *fp;
         const int fam = family_of(f.kind);
         if (fam < 0) continue;
-        const int pk = classify(c, in, f.kind);
+        const int pk = f.kind == GR_PFN_XSPH ? (in.b < 0 ? P_SELF : P_TWO) : classify(c, in, f.kind);
         auto add = [&](int target, bool b_side) {
-          auto &m = tasks[k * 4 + fam];
+          auto &m = tasks[k * NFAM + fam];
           TaskB &tb = m[target];
           if (tb.src.empty()) {
             memset(&tb.t, 0, sizeof(Task));
             tb.t.sys = target;
             tb.t.ns = c->sys[target].ns;
             tb.t.nv = c->sys[target].nv;
+            tb.t.alias = -1;
+            if (f.kind == GR_PFN_XSPH && c->sys[target].kind == GR_SYS_GHOST) {
+              const SysH &src = c->sys[c->sys[target].src];
+              tb.t.alias = c->sys[target].src;
+              tb.t.alias_cap = (int32_t)src.n;
+              tb.t.alias_off = src.off;
+            }
           }
           SrcDesc q = make_src(c, it, target, b_side, pk);
           set_pfn(q, f, pk, c, in);
@@ -1081,7 +1122,7 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
     int first = -1, firstfam = -1;
     for (int k = 0; k < nst && first < 0; ++k)
       for (int fam : {FAM_FORCE, FAM_COUNT})
-        if (tasks[k * 4 + fam].count(s)) {
+        if (tasks[k * NFAM + fam].count(s)) {
           first = k;
           firstfam = fam;
           break;
@@ -1094,12 +1135,12 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
           zeroed = true;
     if (!zeroed) {
       c->stepsys[s].reset_force = 0;
-      tasks[first * 3 + firstfam][s].t.assign = 1;
+      tasks[first * NFAM + firstfam][s].t.assign = 1;
     }
   }
   // assignment: STRAIN group absorbs a stage's ZeroFieldUpdater
   for (int k = 0; k < nst; ++k)
-    for (auto &kv : tasks[k * 4 + FAM_STRAIN]) {
+    for (auto &kv : tasks[k * NFAM + FAM_STRAIN]) {
       const int s = kv.first;
       if (k >= (int)upd[s].size()) continue;
       auto &L = upd[s][k];
@@ -1120,9 +1161,10 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
     }
   // neighbour lists: one per target, over the union of its sources
   std::map<int, std::vector<SrcDesc>> tl;  // target → list sources
-  for (int k = 0; k < nst; ++k)
-    for (int fam = 0; fam < 4; ++fam)
-      for (auto &kv : tasks[k * 4 + fam]) {
+  for (int k = 0; k <= nst; ++k)
+    for (int fam = 0; fam < NFAM; ++fam)
+      for (auto &kv : tasks[k * NFAM + fam]) {
+        if (kv.second.t.alias >= 0) continue;  // ghost targets walk the cells
         auto &L = tl[kv.first];
         for (auto &q : kv.second.src) {
           int idx = -1;
@@ -1200,11 +1242,11 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
     sp->gcopy.upload(gc, c->stream);
     sp->blk_gcopy.upload(gb, c->stream);
     sp->nblk_gcopy = (int)gb.size();
-    for (int fam = 0; fam < 4; ++fam) {
+    for (int fam = 0; fam < NFAM; ++fam) {
       std::vector<Task> tv;
       std::vector<SrcDesc> sv;
       std::vector<Blk> bv;
-      for (auto &kv : tasks[k * 4 + fam]) {
+      for (auto &kv : tasks[k * NFAM + fam]) {
         Task t = kv.second.t;
         t.ltask = c->use_lists ? ltask_of[kv.first] : -1;
         t.src_begin = (int)sv.size();
@@ -1219,6 +1261,26 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
       sp->fam[fam].nblk = (int)bv.size();
     }
     P.stages.push_back(std::move(sp));
+  }
+  {
+    std::vector<Task> tv;
+    std::vector<SrcDesc> sv;
+    std::vector<Blk> bv;
+    for (auto &kv : tasks[nst * NFAM + FAM_XSPH]) {
+      Task t = kv.second.t;
+      t.ltask = (c->use_lists && t.alias < 0) ? ltask_of[kv.first] : -1;
+      if (t.alias >= 0) P.xsph_alias.push_back(make_int2(t.sys, t.alias_cap));
+      t.src_begin = (int)sv.size();
+      for (auto &q : kv.second.src) sv.push_back(q);
+      t.src_end = (int)sv.size();
+      add_blocks(bv, (int)tv.size(), c->sys[t.sys].cap, GR_BLOCK);
+      tv.push_back(t);
+    }
+    P.xsph.tasks.upload(tv, c->stream);
+    P.xsph.srcs.upload(sv, c->stream);
+    P.xsph.blk.upload(bv, c->stream);
+    P.xsph.nblk = (int)bv.size();
+    P.vadj_live = c->vadj_any ? 1 : 0;
   }
   P.stepsys.clear();
   for (int ig : P.integs) {
@@ -1324,7 +1386,8 @@ void phase_sort_real(gr_ctx *c, Plan &P) {
   all_real(c, arb, arn);
   Pool<R> A = pool_view<R>(c, c->cur), B = pool_view<R>(c, 1 - c->cur);
   launch_permute_real<R>(s, A, B, c->d_sd.as<SysDesc>(), arb, arn,
-                         P.d_bbase_real.as<int>(), c->vals.as<int>(), c->d_flags.as<int>(), c->D);
+                         P.d_bbase_real.as<int>(), c->vals.as<int>(), c->d_flags.as<int>(),
+                         c->vadj_any ? 1 : 0, c->D);
   c->cur = 1 - c->cur;
 }
 
@@ -1432,6 +1495,43 @@ void phase_stage(gr_ctx *c, Plan &P, int k, bool cnt, hipEvent_t *mid) {
   }
 }
 
+// _xsph_correction! (src/TimeIntegration.jl:423-436) between the full step
+// and the position update, which it performs.
+template <typename R>
+void phase_xsph(gr_ctx *c, Plan &P) {
+  hipStream_t s = c->stream;
+  Pool<R> A = pool_view<R>(c, c->cur);
+  const SysDesc *sd = c->d_sd.as<SysDesc>();
+  const StepSys *st = c->d_st.as<StepSys>();
+  launch_xsph<R>(s, A, sd, st, c->d_n.as<int>(), P.blk_int.as<Blk>(), P.nblk_int, c->D, 0);
+  if (P.xsph.nblk > 0) {
+    launch_xsph<R>(s, A, sd, st, c->d_n.as<int>(), P.blk_virt.as<Blk>(), P.nblk_virt, c->D, 1);
+    launch_pack<R>(s, A, sd, c->d_n.as<int>(), P.blk_real.as<Blk>(), P.nblk_real, c->D);
+    if (!P.xsph_alias.empty())
+      launch_pack<R>(s, A, sd, c->d_n.as<int>(), P.blk_ghost.as<Blk>(), P.nblk_ghost, c->D);
+    SweepArgs a;
+    a.sd = sd;
+    a.n = c->d_n.as<int>();
+    a.tasks = P.xsph.tasks.as<Task>();
+    a.srcs = P.xsph.srcs.as<SrcDesc>();
+    a.grids = c->d_grids.as<Grid>();
+    a.carr = c->d_carr.as<CellArr>();
+    a.cells = c->cells.as<int>();
+    a.blk = P.xsph.blk.as<Blk>();
+    a.nblk = P.xsph.nblk;
+    a.pair_count = c->d_pairs.as<unsigned long long>();
+    if (P.nblk_list > 0) {
+      a.lt = P.ltasks.as<LTask>();
+      a.lcnt = c->lcnt.as<int>();
+      a.lidx = c->lidx.as<int>();
+    }
+    launch_sweep<R>(s, A, a, c->D, FAM_XSPH, false);
+    for (const int2 &g : P.xsph_alias)
+      launch_alias_check(s, c->d_n.as<int>(), g.x, g.y, c->d_flags.as<int>());
+  }
+  launch_xsph<R>(s, A, sd, st, c->d_n.as<int>(), P.blk_int.as<Blk>(), P.nblk_int, c->D, 2);
+}
+
 // Phase events of one step (timing mode): 0 start, 1 sorted, 2 ghosts,
 // 3 grids, 4 half-step, then per stage (updaters end, sweeps end), last
 // full-step end.  Recorded without synchronisation; read after the call.
@@ -1499,12 +1599,13 @@ void run_step(gr_ctx *c, Plan &P, bool cnt, std::vector<hipEvent_t> *ev) {
       pass();
       launch_rk4_acc<R>(s, A, sd, st, P.blk_int.as<Blk>(), P.nblk_int, c->D, wts[r], r == 0);
     }
-    launch_rk4_apply<R>(s, A, sd, st, P.blk_int.as<Blk>(), P.nblk_int, c->D);
+    launch_rk4_apply<R>(s, A, sd, st, P.blk_int.as<Blk>(), P.nblk_int, c->D, !P.vadj_live);
   } else {
     launch_halfstep<R>(s, A, sd, st, P.blk_int.as<Blk>(), P.nblk_int, c->D, 0.5, 0);
     pass();
-    launch_fullstep<R>(s, A, sd, st, P.blk_int.as<Blk>(), P.nblk_int, c->D);
+    launch_fullstep<R>(s, A, sd, st, P.blk_int.as<Blk>(), P.nblk_int, c->D, !P.vadj_live);
   }
+  if (P.vadj_live) phase_xsph<R>(c, P);
   launch_virtual<R>(s, A, sd, st, c->d_n.as<int>(), P.blk_virt.as<Blk>(), P.nblk_virt, 1, c->D);
   rec(6 + 2 * nst);
 }
@@ -1591,6 +1692,10 @@ void do_steps(gr_ctx *c, const std::vector<int> &integs, int nsteps) {
     int z[4] = {0, 0, 0, 0};
     HIPCHK(hipMemcpyAsync(c->d_flags.p, z, sizeof(z), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    if (flags[1] & 4)
+      throw GrError(GR_ERR_ARGUMENT,
+                    "XSPHPfn with a ghost partner: more ghosts than source particles "
+                    "(ps_b.v_adjustment[j] would index past the source's array)");
     throw GrError(GR_ERR_CAPACITY,
                   (flags[1] & 2) ? "cell grid outgrew its capacity (particles left the planned domain)"
                                  : "cell-key range exceeds 32 bits");
@@ -1845,7 +1950,14 @@ int gr_system_upload(gr_ctx *c, int sys, int field, const double *data, int64_t 
     const int nc = ncomp(c, s, field);
     const int64_t n = live_n(c, sys);
     need(count == n * nc, GR_ERR_ARGUMENT, "count mismatch");
-    if (field == GR_F_V_ADJUSTMENT) return;  // not stored: no XSPH functor on the device
+    if (field == GR_F_V_ADJUSTMENT && !c->vadj_any) {
+      bool nz = false;
+      for (int64_t i = 0; i < count; ++i) nz = nz || data[i] != 0.0;
+      if (nz) {  // from now on v_adjustment follows the sort and the correction runs
+        c->vadj_any = true;
+        c->plans.clear();
+      }
+    }
     const int64_t off = s.off;
     const size_t R = c->rsize;
     const int64_t cap = c->pool_cap;
@@ -1898,10 +2010,6 @@ int gr_system_download(gr_ctx *c, int sys, int field, double *data, int64_t coun
     need(count == n * nc, GR_ERR_ARGUMENT, "count mismatch");
     const int64_t off = s.off, cap = c->pool_cap;
     const size_t R = c->rsize;
-    if (field == GR_F_V_ADJUSTMENT) {
-      std::fill(data, data + count, 0.0);
-      return;
-    }
     if (s.kind == GR_SYS_GHOST) {
       switch (field) {
         case GR_F_X:
@@ -2092,7 +2200,9 @@ int gr_interaction_create(gr_ctx *c, int kernel_kind, double h, int sys_a, int p
     const SysH &A = c->sys[sys_a];
     for (const gr_pfn_desc &f : in.pfns) {
       if (f.kind == GR_PFN_NONE) continue;
-      need(f.kind >= GR_PFN_FLUID && f.kind <= GR_PFN_COUNT, GR_ERR_ARGUMENT, "unknown functor kind");
+      need(f.kind >= GR_PFN_FLUID && f.kind <= GR_PFN_XSPH, GR_ERR_ARGUMENT, "unknown functor kind");
+      need(f.kind != GR_PFN_XSPH, GR_ERR_ARGUMENT,
+           "XSPHPfn runs as the interaction's velocity_adjust_pairwise_fn");
       if (f.kind == GR_PFN_FLUID)
         need(f.sigma == 1 || f.sigma == 2, GR_ERR_ARGUMENT, "FluidPfn sigma must be 1 or 2");
       need(!(f.kind == GR_PFN_FLUID && A.kind < GR_SYS_FLUID), GR_ERR_ARGUMENT, "FluidPfn needs p on system_a");
@@ -2116,6 +2226,28 @@ int gr_interaction_create(gr_ctx *c, int kernel_kind, double h, int sys_a, int p
     }
     c->inter.push_back(in);
     *inter_out = (int)c->inter.size() - 1;
+  });
+}
+
+int gr_interaction_set_vadjust(gr_ctx *c, int inter, double epsilon) {
+  return guard(c, [&] {
+    need(inter >= 0 && inter < (int)c->inter.size() && !c->inter[inter].dead, GR_ERR_ARGUMENT,
+         "bad interaction");
+    InterH &in = c->inter[inter];
+    if (std::isnan(epsilon)) {  // velocity_adjust_pairwise_fn = nothing
+      if (in.has_xsph) c->plans.clear();
+      in.has_xsph = false;
+      return;
+    }
+    // XSPHPfn's generic method writes ps_b.v_adjustment[j]: for a ghost
+    // partner that is the source system's entry at the ghost's index
+    // (src/GhostParticles.jl:226-236), which this layout does not hold
+    need(in.b < 0 || c->sys[in.b].kind != GR_SYS_GHOST, GR_ERR_ARGUMENT,
+         "XSPHPfn with a ghost partner writes the source's v_adjustment at ghost indices");
+    in.has_xsph = true;
+    in.xsph_eps = epsilon;
+    c->vadj_any = true;
+    c->plans.clear();  // tasks, lists and captured graphs change
   });
 }
 
@@ -2388,6 +2520,7 @@ int gr_sweep(gr_ctx *c, int inter, int stage) {
       t.write_vor = f.kind == GR_PFN_SRV;
       t.assign = 0;
       t.ltask = -1;
+      t.alias = -1;
       if (f.kind == GR_PFN_INTERP) t.imask = f.sigma | (f.alpha != 0.0 ? 1 << 30 : 0);
       SrcDesc q = make_src(c, inter, target, b_side, pk);
       set_pfn(q, f, pk, c, in);

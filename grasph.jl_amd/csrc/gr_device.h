@@ -27,7 +27,14 @@ enum PartnerKind : int32_t {
 };
 
 // Functor families with a kernel each.
-enum Family : int32_t { FAM_FORCE = 0, FAM_STRAIN = 1, FAM_COUNT = 2, FAM_INTERP = 3 };
+enum Family : int32_t {
+  FAM_FORCE = 0,
+  FAM_STRAIN = 1,
+  FAM_COUNT = 2,
+  FAM_INTERP = 3,
+  FAM_XSPH = 4,  // velocity adjustment (XSPHPfn), once per step after the full step
+  NFAM = 5
+};
 
 struct SysDesc {
   int64_t off;     // pool offset of local index 0
@@ -116,6 +123,12 @@ struct Task {
   int32_t write_vor;  // an SRV source writes vorticity
   int32_t ltask;      // neighbour-list task of the target (-1: walk cells)
   int32_t imask;      // FAM_INTERP: union of the sources' field masks; bit 30: w_sum
+  // FAM_XSPH with a ghost target: ghost.v_adjustment is the source system's
+  // array (src/GhostParticles.jl:226-236), so the sum of ghost j goes to the
+  // source's entry j (pool index alias_off + j, j < alias_cap); -1: own entry
+  int32_t alias;
+  int32_t alias_cap;
+  int64_t alias_off;
 };
 
 // Neighbour list of one target system: sources [src_begin, src_end) of the
@@ -185,6 +198,7 @@ struct Pool {
   RecB<R> *rb;
   double *wsum;    // virtual systems: interpolation weight sums
   double *acc[4];  // RK4 accumulators of dvdt (D components) and drhodt
+  double *vadj[3];  // v_adjustment (XSPH), swapped with the sort
   double *x[3], *v[3], *rho, *rho_h;
   R *vs[3], *rhos, *dvdt[3], *drhodt, *p;
   double *stress[GR_MAXNS];

@@ -344,7 +344,7 @@ __global__ void k_radix_scatter(const unsigned *kin, const int *vin, unsigned *k
 
 template <typename R, int D>
 __global__ void k_permute_real(Pool<R> A, Pool<R> B, const SysDesc *sd, const Blk *blk,
-                               const int *bbase, const int *vals, const int *flags) {
+                               const int *bbase, const int *vals, const int *flags, int vadj) {
   const Blk b = blk[blockIdx.x];
   const int s = b.item;
   const int i = b.first + threadIdx.x;
@@ -358,6 +358,7 @@ __global__ void k_permute_real(Pool<R> A, Pool<R> B, const SysDesc *sd, const Bl
     B.x[d][dst] = A.x[d][src];
     B.v[d][dst] = A.v[d][src];
     B.dvdt[d][dst] = A.dvdt[d][src];
+    if (vadj) B.vadj[d][dst] = A.vadj[d][src];
   }
   B.rho[dst] = A.rho[src];
   B.drhodt[dst] = A.drhodt[src];
@@ -920,7 +921,8 @@ __global__ void k_rk4_acc(Pool<R> P, const SysDesc *sd, const StepSys *st, const
 
 // q = q0 + dt·acc, then x += dt·v (_apply_acc_all!, _update_positions_all!).
 template <typename R, int D>
-__global__ void k_rk4_apply(Pool<R> P, const SysDesc *sd, const StepSys *st, const Blk *blk) {
+__global__ void k_rk4_apply(Pool<R> P, const SysDesc *sd, const StepSys *st, const Blk *blk,
+                            int pos) {
   const Blk b = blk[blockIdx.x];
   const int s = b.item;
   const int i = b.first + threadIdx.x;
@@ -932,7 +934,7 @@ __global__ void k_rk4_apply(Pool<R> P, const SysDesc *sd, const StepSys *st, con
   for (int d = 0; d < D; ++d) {
     const double vn = P.v[d][g] + T.dt * P.acc[d][g];
     P.v[d][g] = vn;
-    P.x[d][g] = P.x[d][g] + T.dt * vn;
+    if (pos) P.x[d][g] = P.x[d][g] + T.dt * vn;
   }
   P.rho[g] = fma(T.dt, P.acc[3][g], P.rho[g]);
 }
@@ -941,7 +943,8 @@ __global__ void k_rk4_apply(Pool<R> P, const SysDesc *sd, const StepSys *st, con
 // _update_positions_all! (:235-240).  v and x use separate products and sums
 // (SVector arithmetic is outside @fastmath); ρ uses the fused axpy.
 template <typename R, int D>
-__global__ void k_fullstep(Pool<R> P, const SysDesc *sd, const StepSys *st, const Blk *blk) {
+__global__ void k_fullstep(Pool<R> P, const SysDesc *sd, const StepSys *st, const Blk *blk,
+                           int pos) {
   const Blk b = blk[blockIdx.x];
   const int s = b.item;
   const int i = b.first + threadIdx.x;
@@ -959,9 +962,46 @@ __global__ void k_fullstep(Pool<R> P, const SysDesc *sd, const StepSys *st, cons
     }
     double vn = P.v[d][g] + T.dt * a;
     P.v[d][g] = vn;
-    P.x[d][g] = P.x[d][g] + T.dt * vn;
+    if (pos) P.x[d][g] = P.x[d][g] + T.dt * vn;
   }
   P.rho[g] = fma(T.dt, (double)P.drhodt[g], P.rho[g]);
+}
+
+// _xsph_correction! (src/TimeIntegration.jl:423-436) around the XSPH sweep.
+// mode 0 (integrated systems): v ← v + (−1)·v_adjustment, v_adjustment ← 0,
+//   and the sweep's view of v and ρ (the shadows) from the masters;
+// mode 1 (virtual systems): shadows from the masters only;
+// mode 2 (integrated systems): v ← v + 1·v_adjustment, then the position
+//   update x ← x + dt·v (_update_positions_all!, :235-240).
+template <typename R, int D>
+__global__ void k_xsph(Pool<R> P, const SysDesc *sd, const StepSys *st, const int *n_dev,
+                       const Blk *blk, int mode) {
+  const Blk b = blk[blockIdx.x];
+  const int s = b.item;
+  const int i = b.first + threadIdx.x;
+  if (i >= n_dev[s]) return;
+  const int64_t g = sd[s].off + i;
+  if (mode == 2) {
+    const double dt = st[s].dt;
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      const double vn = P.v[d][g] + 1.0 * P.vadj[d][g];
+      P.v[d][g] = vn;
+      P.x[d][g] = P.x[d][g] + dt * vn;
+    }
+    return;
+  }
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    double vn = P.v[d][g];
+    if (mode == 0) {
+      vn = vn + (-1.0) * P.vadj[d][g];
+      P.v[d][g] = vn;
+      P.vadj[d][g] = 0.0;
+    }
+    P.vs[d][g] = (R)vn;
+  }
+  P.rhos[g] = (R)P.rho[g];
 }
 
 // ---------------------------------------------------------------------------
@@ -1341,19 +1381,19 @@ int radix_hist_entries(int N) { return 256 * (int)nb_of(N > 0 ? N : 1, RADIX_TIL
 template <typename R>
 void launch_permute_real(hipStream_t s, const Pool<R> &A, const Pool<R> &B, const SysDesc *sd,
                          const Blk *blk, int nblk, const int *bbase, const int *vals,
-                         const int *flags, int D) {
+                         const int *flags, int vadj, int D) {
   if (nblk <= 0) return;
   if (D == 2)
-    k_permute_real<R, 2><<<nblk, GR_BLOCK, 0, s>>>(A, B, sd, blk, bbase, vals, flags);
+    k_permute_real<R, 2><<<nblk, GR_BLOCK, 0, s>>>(A, B, sd, blk, bbase, vals, flags, vadj);
   else
-    k_permute_real<R, 3><<<nblk, GR_BLOCK, 0, s>>>(A, B, sd, blk, bbase, vals, flags);
+    k_permute_real<R, 3><<<nblk, GR_BLOCK, 0, s>>>(A, B, sd, blk, bbase, vals, flags, vadj);
 }
 template void launch_permute_real<float>(hipStream_t, const Pool<float> &, const Pool<float> &,
                                          const SysDesc *, const Blk *, int, const int *,
-                                         const int *, const int *, int);
+                                         const int *, const int *, int, int);
 template void launch_permute_real<double>(hipStream_t, const Pool<double> &,
                                           const Pool<double> &, const SysDesc *, const Blk *,
-                                          int, const int *, const int *, const int *, int);
+                                          int, const int *, const int *, const int *, int, int);
 
 void launch_ghost_flags(hipStream_t s, XP x, const SysDesc *sd, const Bnd *bnd, const Blk *blk,
                         int nblk, const int *cand_off, int D, int *flags) {
@@ -1465,21 +1505,30 @@ void launch_rk4_acc(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const St
 }
 template <typename R>
 void launch_rk4_apply(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
-                      const Blk *blk, int nblk, int D) {
+                      const Blk *blk, int nblk, int D, int pos) {
   if (nblk <= 0) return;
   if (D == 2)
-    k_rk4_apply<R, 2><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, blk);
+    k_rk4_apply<R, 2><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, blk, pos);
   else
-    k_rk4_apply<R, 3><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, blk);
+    k_rk4_apply<R, 3><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, blk, pos);
 }
 template <typename R>
 void launch_fullstep(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
-                     const Blk *blk, int nblk, int D) {
+                     const Blk *blk, int nblk, int D, int pos) {
   if (nblk <= 0) return;
   if (D == 2)
-    k_fullstep<R, 2><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, blk);
+    k_fullstep<R, 2><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, blk, pos);
   else
-    k_fullstep<R, 3><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, blk);
+    k_fullstep<R, 3><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, blk, pos);
+}
+template <typename R>
+void launch_xsph(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
+                 const int *n, const Blk *blk, int nblk, int D, int mode) {
+  if (nblk <= 0) return;
+  if (D == 2)
+    k_xsph<R, 2><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, n, blk, mode);
+  else
+    k_xsph<R, 3><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, n, blk, mode);
 }
 template <typename R>
 void launch_updaters(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
@@ -1492,9 +1541,11 @@ void launch_updaters(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const S
   template void launch_rk4_acc<R>(hipStream_t, const Pool<R> &, const SysDesc *,               \
                                   const StepSys *, const Blk *, int, int, double, int);        \
   template void launch_rk4_apply<R>(hipStream_t, const Pool<R> &, const SysDesc *,             \
-                                    const StepSys *, const Blk *, int, int);                   \
+                                    const StepSys *, const Blk *, int, int, int);              \
   template void launch_fullstep<R>(hipStream_t, const Pool<R> &, const SysDesc *,              \
-                                   const StepSys *, const Blk *, int, int);                    \
+                                   const StepSys *, const Blk *, int, int, int);               \
+  template void launch_xsph<R>(hipStream_t, const Pool<R> &, const SysDesc *, const StepSys *, \
+                               const int *, const Blk *, int, int, int);                       \
   template void launch_updaters<R>(hipStream_t, const Pool<R> &, const SysDesc *,              \
                                    const StepSys *, const Blk *, int, const int2 *,            \
                                    const UpdDesc *, int);
@@ -1529,6 +1580,15 @@ template void launch_pack<float>(hipStream_t, const Pool<float> &, const SysDesc
                                  const Blk *, int, int);
 template void launch_pack<double>(hipStream_t, const Pool<double> &, const SysDesc *, const int *,
                                   const Blk *, int, int);
+
+// A ghost-target XSPH task stores into its source's v_adjustment at the
+// ghost's index: flag (bit 4 of flags[1]) a ghost count above the source's.
+__global__ void k_alias_check(const int *n_dev, int ghost, int src_n, int *flags) {
+  if (threadIdx.x == 0 && n_dev[ghost] > src_n) atomicOr(&flags[1], 4);
+}
+void launch_alias_check(hipStream_t s, const int *n, int ghost, int src_n, int *flags) {
+  k_alias_check<<<1, 64, 0, s>>>(n, ghost, src_n, flags);
+}
 
 void launch_fill_i32(hipStream_t s, int *p, int v, long long n) {
   if (n > 0) k_fill_i32<<<nb_of(n, GR_BLOCK), GR_BLOCK, 0, s>>>(p, v, n);

@@ -54,7 +54,7 @@ int radix_hist_entries(int N);
 template <typename R>
 void launch_permute_real(hipStream_t s, const Pool<R> &A, const Pool<R> &B, const SysDesc *sd,
                          const Blk *blk, int nblk, const int *bbase, const int *vals,
-                         const int *flags, int D);
+                         const int *flags, int vadj, int D);
 void launch_ghost_flags(hipStream_t s, XP x, const SysDesc *sd, const Bnd *bnd,
                         const Blk *blk, int nblk, const int *cand_off, int D, int *flags);
 void launch_scan(hipStream_t s, const int *in, int *out, int *bsum, int N);
@@ -89,10 +89,13 @@ void launch_rk4_acc(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const St
                     const Blk *blk, int nblk, int D, double w, int first);
 template <typename R>
 void launch_rk4_apply(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
-                      const Blk *blk, int nblk, int D);
+                      const Blk *blk, int nblk, int D, int pos);
 template <typename R>
 void launch_fullstep(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
-                     const Blk *blk, int nblk, int D);
+                     const Blk *blk, int nblk, int D, int pos);
+template <typename R>
+void launch_xsph(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
+                 const int *n, const Blk *blk, int nblk, int D, int mode);
 template <typename R>
 void launch_updaters(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
                      const Blk *blk, int nblk, const int2 *urange, const UpdDesc *ud, int D);
@@ -107,5 +110,6 @@ void launch_pack(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const int *
 template <typename R>
 void launch_nlist(hipStream_t s, const Pool<R> &P, const NlistArgs &a, int D);
 void launch_fill_i32(hipStream_t s, int *p, int v, long long n);
+void launch_alias_check(hipStream_t s, const int *n, int ghost, int src_n, int *flags);
 
 }  // namespace gr

@@ -240,6 +240,22 @@ struct Gather {
       if (K.alpha != (R)0) avo[1].add(kw);
       return;
     }
+    if (FAM == FAM_XSPH) {  // XSPHPfn (src/PairwiseFunctors.jl:364-390)
+      // du = −ε·(v_i − v_j)·w/(ρ_i + ρ_j) (src/PairwisePhysics.jl:268-270);
+      // the target gains du·m_partner, which is the reference's update of
+      // either particle of the pair (du flips sign exactly under i <-> j)
+      const R w = wq<R>(Q.kk, K.norm, FA ? r * K.ih : r / K.h);
+      const RecB<R> Bj = P.rb[gj];
+      const R rs = ri + Aj.rho;
+      const R ne = -K.alpha;
+#pragma unroll
+      for (int d = 0; d < D; ++d) {
+        const R t = (ne * (vi[d] - Bj.v[d])) * w;
+        const R du = FA ? t * Fast<R>::rcp(rs) : t / rs;
+        adv[d].add(du * K.m_q);
+      }
+      return;
+    }
     R vj[D], rj, pj = (R)0;
     const R m = K.m_q;
     if (Q.pk == PK_DYNAMIC) {  // DynamicBoundarySystem: mirrored velocity
@@ -414,6 +430,17 @@ struct Gather {
           }
       }
       if (m & (1 << 30)) P.wsum[gi] += (double)avo[1].get();
+      return;
+    }
+    if (FAM == FAM_XSPH) {
+      int64_t gs = gi;
+      if (T.alias >= 0) {
+        const int64_t j = gi - S.off;
+        if (j >= T.alias_cap) return;  // reported by k_alias_check
+        gs = T.alias_off + j;
+      }
+#pragma unroll
+      for (int d = 0; d < D; ++d) P.vadj[d][gs] = P.vadj[d][gs] + (double)adv[d].get();
       return;
     }
     if (FAM == FAM_STRAIN) {
@@ -693,7 +720,8 @@ __global__ void __launch_bounds__(GR_BLOCK)
                  const int *__restrict__ lcnt, const int *__restrict__ lidx) {
   const Blk b = blk[xcd_block(nblk)];
   const Task T = tasks[b.item];
-  const LTask L = lt[T.ltask];
+  // a task without a list (ltask < 0) walks the cells
+  const LTask L = lt[T.ltask >= 0 ? T.ltask : 0];
   const int i = b.first + threadIdx.x;
   const SysDesc S = sd[T.sys];
   const bool live = i < n_dev[T.sys];
@@ -702,7 +730,8 @@ __global__ void __launch_bounds__(GR_BLOCK)
   Gt.load(P, S, T, gi);
   unsigned long long ns = 0, no = 0;
   const int nls = L.src_end - L.src_begin;
-  const bool overflow = live && lcnt[L.cnt_off + (int64_t)nls * L.cap + i] != 0;
+  const bool overflow =
+      live && (T.ltask < 0 || lcnt[L.cnt_off + (int64_t)nls * L.cap + i] != 0);
   if (overflow) {
     gather_direct<D, R, FAM>(Gt, P, sd, T, srcs, grids, carr, cells, i, ns, no);
   } else if (live) {
@@ -774,11 +803,13 @@ void launch_sweep(hipStream_t s, const Pool<R> &P, const SweepArgs &a, int D, in
     if (fam == FAM_FORCE) launch_one<2, R, FAM_FORCE>(s, P, a, cnt);
     else if (fam == FAM_STRAIN) launch_one<2, R, FAM_STRAIN>(s, P, a, cnt);
     else if (fam == FAM_INTERP) launch_one<2, R, FAM_INTERP>(s, P, a, cnt);
+    else if (fam == FAM_XSPH) launch_one<2, R, FAM_XSPH>(s, P, a, cnt);
     else launch_one<2, R, FAM_COUNT>(s, P, a, cnt);
   } else {
     if (fam == FAM_FORCE) launch_one<3, R, FAM_FORCE>(s, P, a, cnt);
     else if (fam == FAM_STRAIN) launch_one<3, R, FAM_STRAIN>(s, P, a, cnt);
     else if (fam == FAM_INTERP) launch_one<3, R, FAM_INTERP>(s, P, a, cnt);
+    else if (fam == FAM_XSPH) launch_one<3, R, FAM_XSPH>(s, P, a, cnt);
     else launch_one<3, R, FAM_COUNT>(s, P, a, cnt);
   }
 }

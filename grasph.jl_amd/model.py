@@ -216,7 +216,7 @@ class _ParticleSystem:
     # fields moved between host and device, in the order of _particle_arrays
     # (src/Sorting.jl:15-26)
     def field_names(self):
-        return ["x", "v", "rho", "dvdt", "drhodt"]
+        return ["x", "v", "rho", "dvdt", "drhodt", "v_adjustment"]
 
 
 class BasicParticleSystem(_ParticleSystem):
@@ -548,10 +548,24 @@ def encode_pfn(p):
 # ---------------------------------------------------------------------------
 
 
-class SystemInteraction:
-    """SystemInteraction(kernel, pairwise_fn, system_a[, system_b])."""
+class XSPHPfn:
+    """XSPHPfn(epsilon) (src/PairwiseFunctors.jl:360-390): velocity-adjust
+    functor, v_adjustment += −ε·(v_i − v_j)·W/(ρ_i + ρ_j)·m (both sides)."""
 
-    def __init__(self, kernel, pairwise_fn, system_a, system_b=None):
+    def __init__(self, epsilon):
+        self.epsilon = float(epsilon)
+
+
+class SystemInteraction:
+    """SystemInteraction(kernel, pairwise_fn, system_a[, system_b];
+    velocity_adjust_pairwise_fn=nothing)."""
+
+    def __init__(self, kernel, pairwise_fn, system_a, system_b=None,
+                 velocity_adjust_pairwise_fn=None):
+        if velocity_adjust_pairwise_fn is not None and not isinstance(
+                velocity_adjust_pairwise_fn, XSPHPfn):
+            raise ValueError("velocity_adjust_pairwise_fn must be an XSPHPfn on the device path")
+        self.vadjust_pfn = velocity_adjust_pairwise_fn
         nd = system_a.ndims
         if kernel.ndims != nd:
             raise ValueError(f"kernel.ndims ({kernel.ndims}) must match system_a.ndims ({nd})")

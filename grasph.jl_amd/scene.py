@@ -142,6 +142,8 @@ class DeviceScene:
         pfns = [M.encode_pfn(p) for p in si.pfns]
         h = self.be.interaction_create(si.kernel.kind, si.kernel.h, a, partner, wrap, normal,
                                        point, beta, ljcut, pfns)
+        if getattr(si, "vadjust_pfn", None) is not None:
+            self.be.interaction_set_vadjust(h, si.vadjust_pfn.epsilon)
         self._inter[k] = h
         self._objs.append(si)
         return h

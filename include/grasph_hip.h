@@ -80,11 +80,13 @@ enum {
   GR_PFN_COUNT = 5,   /* neighbour counter: +1 to dvdt[1] and drhodt per pair,
                          both sides (the counting functors of
                          test/test_interaction_sweep.jl:14-30)              */
-  GR_PFN_INTERP = 6   /* InterpolateFieldFn :405-434 between a real system
+  GR_PFN_INTERP = 6,  /* InterpolateFieldFn :405-434 between a real system
                          and a virtual one: the virtual side accumulates
                          f += (m/ρ)·W·f_real; `sigma` holds the field bit
                          mask (1<<GR_F_V | 1<<GR_F_RHO | 1<<GR_F_P |
                          1<<GR_F_STRESS), `alpha` != 0 accumulates w_sum  */
+  GR_PFN_XSPH = 7     /* XSPHPfn :360-390, only as an interaction's
+                         velocity-adjust functor (gr_interaction_set_vadjust) */
 };
 
 /* ---- partner wrappers (src/BoundaryParticles.jl:38,68) ---------------- */
@@ -198,6 +200,15 @@ int gr_interaction_create(gr_ctx *ctx, int kernel_kind, double h, int sys_a,
                           const double *point, double beta, double lj_cutoff,
                           int n_stages, const gr_pfn_desc *pfns,
                           int *inter_out);
+
+/* Velocity-adjust functor of an interaction: XSPHPfn(epsilon)
+ * (SystemInteraction's velocity_adjust_pairwise_fn, src/Interaction.jl:61-66,
+ * applied by _xsph_correction!, src/TimeIntegration.jl:423-436, after the
+ * full step and before positions move).  NaN removes it.  A ghost partner is
+ * rejected: the reference writes through the ghost's forwarded v_adjustment
+ * into its source at the ghost's index (src/GhostParticles.jl:235), which is
+ * out of bounds once ghosts outnumber source particles. */
+int gr_interaction_set_vadjust(gr_ctx *ctx, int inter, double epsilon);
 
 /* ---- integrator ----------------------------------------------------------
  * LeapFrogTimeIntegrator(systems, interactions; ghosts, Γ)

@@ -72,6 +72,8 @@ struct Inter {
   std::vector<int64_t> cs, csa;  // 1-based values, Julia cs[c] at cs[c-1]
   double mina[3] = {0, 0, 0}, maxa[3] = {0, 0, 0};
   bool built = false;
+  double xsph_eps = 0.0;  // velocity_adjust_pairwise_fn = XSPHPfn(xsph_eps)
+  bool has_xsph = false;
 };
 
 struct Integ {
@@ -411,6 +413,18 @@ void apply_pair(const PairCtx &P, int64_t i, int64_t j, const double *dx,
   const double *vj = &B.v[j * D];
   double dv[3], tmp[6];
   switch (f.kind) {
+    case GR_PFN_XSPH: {  // XSPHPfn :364-390, xsph_veladjust (PairwisePhysics.jl:268-270)
+      const double eps = f.alpha;
+      const double ri = A.rho[i], rj = B.rho[j];
+      double du[3];
+      for (int d = 0; d < D; ++d) du[d] = (((-eps) * (vi[d] - vj[d])) * w) / (ri + rj);
+      const double mi = P.mass_a, mj = P.pk == P_SELF ? P.mass_a : P.B.mass;
+      for (int d = 0; d < D; ++d) {
+        P.Aw->vadj[i * D + d] += du[d] * mj;
+        P.Bw->vadj[j * D + d] -= du[d] * mi;
+      }
+      return;
+    }
     case GR_PFN_INTERP: {  // :424-434
       const int mask = f.sigma;
       if (P.interp_side == 1) {
@@ -825,9 +839,14 @@ int create_grid(orc_scene *sc, Inter &in) {
 // Sweeps (src/Interaction.jl:393-439, 477-541, 600-645, 660-715)
 // ---------------------------------------------------------------------------
 
+int sweep_fn(orc_scene *sc, Inter &in, const gr_pfn_desc &f);
+
 int sweep(orc_scene *sc, Inter &in, int stage) {
   if (stage < 1 || stage > (int)in.pfns.size()) return GR_OK;
-  const gr_pfn_desc &f = in.pfns[stage - 1];
+  return sweep_fn(sc, in, in.pfns[stage - 1]);
+}
+
+int sweep_fn(orc_scene *sc, Inter &in, const gr_pfn_desc &f) {
   if (f.kind == GR_PFN_NONE) return GR_OK;
   if (!in.built) return fail(sc, GR_ERR_STATE, "sweep!: create_grid! has not been called");
   PairCtx P;
@@ -850,6 +869,21 @@ int sweep(orc_scene *sc, Inter &in, int stage) {
   } else {
     P.B = make_view(sc, in.b);
     P.Bw = (P.pk == P_TWO_SIDED) ? &sc->sys[in.b] : nullptr;
+  }
+  if (f.kind == GR_PFN_XSPH) {  // generic two-sided method (also through wrappers)
+    P.pk = in.b < 0 ? P_SELF : P_TWO_SIDED;
+    P.Bw = in.b < 0 ? &A : &sc->sys[in.b];
+    if (in.b >= 0) P.B = make_view(sc, in.b);
+    if (in.b >= 0 && sc->sys[in.b].kind == GR_SYS_GHOST) {
+      // ghost.v_adjustment is forwarded to the source system's array
+      // (src/GhostParticles.jl:226-236), indexed with the ghost's own index j
+      Sys &src = sc->sys[sc->sys[in.b].src];
+      if (sc->sys[in.b].n > src.n)
+        return fail(sc, GR_ERR_ARGUMENT,
+                    "XSPHPfn with a ghost partner: more ghosts than source particles "
+                    "(ps_b.v_adjustment[j] would index past the source's array)");
+      P.Bw = &src;
+    }
   }
   if (f.kind == GR_PFN_INTERP) {
     if (in.b >= 0 && sc->sys[in.b].virt && in.wrap == GR_WRAP_NONE) {
@@ -1338,6 +1372,34 @@ void advance_virtual_positions(orc_scene *sc, Integ &ig, double dt) {  // :389-3
   }
 }
 
+// _xsph_correction! (src/TimeIntegration.jl:423-436): v −= v_adjustment,
+// v_adjustment ← 0, adjust_v! of every interaction, v += v_adjustment.
+int xsph_correction(orc_scene *sc, Integ &ig) {
+  for (int id : ig.sys) {
+    Sys &s = sc->sys[id];
+    for (size_t k = 0; k < s.v.size(); ++k) s.v[k] = s.v[k] + (-1.0) * s.vadj[k];
+  }
+  for (int id : ig.sys) {
+    Sys &s = sc->sys[id];
+    std::fill(s.vadj.begin(), s.vadj.end(), 0.0);
+  }
+  for (int it : ig.inter) {
+    Inter &in = sc->inter[it];
+    if (!in.has_xsph) continue;
+    gr_pfn_desc f;
+    std::memset(&f, 0, sizeof(f));
+    f.kind = GR_PFN_XSPH;
+    f.alpha = in.xsph_eps;
+    int rc = sweep_fn(sc, in, f);
+    if (rc) return rc;
+  }
+  for (int id : ig.sys) {
+    Sys &s = sc->sys[id];
+    for (size_t k = 0; k < s.v.size(); ++k) s.v[k] = s.v[k] + 1.0 * s.vadj[k];
+  }
+  return GR_OK;
+}
+
 // time_integrate!(::RK4TimeIntegrator) per-step body (src/TimeIntegration.jl:
 // 773-819): one grid per step, four stage passes from q0 with the advance
 // coefficients (0, ½, ½, 1)·dt, weights (1/6, 1/3, 1/3, 1/6), q = q0 + dt·acc.
@@ -1387,14 +1449,19 @@ int step_rk4(orc_scene *sc, Integ &ig, double dt) {
       for (size_t i = 0; i < ar[k].size(); ++i) ar[k][i] = std::fma(wts[r], s.drhodt[i], ar[k][i]);
     }
   }
-  for (size_t k = 0; k < ns; ++k) {  // _apply_acc_all!, _update_positions_all!
+  for (size_t k = 0; k < ns; ++k) {  // _apply_acc_all!
     Sys &s = sc->sys[ig.sys[k]];
     const int D = s.D;
     for (int64_t i = 0; i < s.n; ++i) {
       for (int d = 0; d < D; ++d) s.v[i * D + d] = v0[k][i * D + d] + dt * av[k][i * D + d];
       s.rho[i] = std::fma(dt, ar[k][i], r0[k][i]);
-      for (int d = 0; d < D; ++d) s.x[i * D + d] = s.x[i * D + d] + dt * s.v[i * D + d];
     }
+  }
+  int rcx = xsph_correction(sc, ig);
+  if (rcx) return rcx;
+  for (int id : ig.sys) {  // _update_positions_all!
+    Sys &s = sc->sys[id];
+    for (size_t k = 0; k < s.x.size(); ++k) s.x[k] = s.x[k] + dt * s.v[k];
   }
   advance_virtual_positions(sc, ig, dt);
   return GR_OK;
@@ -1468,9 +1535,15 @@ int step(orc_scene *sc, Integ &ig, double dt) {
     for (int64_t i = 0; i < s.n; ++i) {
       for (int d = 0; d < D; ++d) s.v[i * D + d] = vv[i * D + d] + dt * s.dvdt[i * D + d];
       s.rho[i] = std::fma(dt, s.drhodt[i], rr[i]);
-      // _xsph_correction! :423-436 is an identity without an XSPH functor.
-      for (int d = 0; d < D; ++d) s.x[i * D + d] = s.x[i * D + d] + dt * s.v[i * D + d];
     }
+  }
+  {
+    int rc = xsph_correction(sc, ig);
+    if (rc) return rc;
+  }
+  for (int id : ig.sys) {  // _update_positions_all! :235-240
+    Sys &s = sc->sys[id];
+    for (size_t k = 0; k < s.x.size(); ++k) s.x[k] = s.x[k] + dt * s.v[k];
   }
   for (int v : ig.virt) {  // _update_virtual_positions! :389-397
     Sys &s = sc->sys[v];
@@ -1813,6 +1886,18 @@ int orc_integrator_create(orc_scene *s, int nsys, const int *systems, int ninter
       return fail(s, GR_ERR_ASSERTION, "All interactions must have the same number of stages (pfns length)");
   s->integ.push_back(std::move(ig));
   *integ_out = (int)s->integ.size() - 1;
+  return GR_OK;
+}
+
+int orc_interaction_set_vadjust(orc_scene *s, int inter, double epsilon) {
+  if (inter < 0 || inter >= (int)s->inter.size()) return fail(s, GR_ERR_ARGUMENT, "bad interaction");
+  Inter &in = s->inter[inter];
+  if (std::isnan(epsilon)) {
+    in.has_xsph = false;
+    return GR_OK;
+  }
+  in.has_xsph = true;
+  in.xsph_eps = epsilon;
   return GR_OK;
 }
 

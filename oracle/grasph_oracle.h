@@ -60,6 +60,7 @@ int orc_interaction_create(orc_scene *s, int kernel_kind, double h, int sys_a,
 int orc_integrator_create(orc_scene *s, int nsys, const int *systems,
                           int ninter, const int *inters, int nghost,
                           const int *ghosts, double gamma, int *integ_out);
+int orc_interaction_set_vadjust(orc_scene *s, int inter, double epsilon);
 int orc_integrator_set_scheme(orc_scene *s, int integ, int scheme);
 int orc_integrator_dt(orc_scene *s, int integ, double cfl, double *dt_out);
 int orc_step(orc_scene *s, int integ, int nsteps, double dt);

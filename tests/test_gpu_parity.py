@@ -377,6 +377,13 @@ def _run_steps(builder, kw, D, prec, nsteps, make):
     return out
 
 
+def _bubble3_xy(**kw):
+    """bubble3 with XSPH also on the two-sided Y–X interaction."""
+    cfg = M.configs.bubble_2d(**kw)
+    cfg["integrator"].interactions[2].vadjust_pfn = M.XSPHPfn(0.3)
+    return cfg
+
+
 STEP_CASES = [
     ("dambreak_2d", M.configs.dambreak_2d, dict(dx=0.04), 2, 20),
     ("ep_column_2d", M.configs.ep_column_2d, dict(dx=0.01, width=0.2, height=0.1), 2, 20),
@@ -387,6 +394,8 @@ STEP_CASES = [
      dict(dx=0.01, width=0.1, height=0.06), 2, 20),
     ("bubble_2d_rk4", M.configs.bubble_2d, dict(dx=0.2), 2, 10),
     ("bubble3_2d_rk4", M.configs.bubble_2d, dict(dx=0.2, variant="bubble3"), 2, 10),
+    ("bubble3_xy_xsph_2d_leapfrog", _bubble3_xy, dict(dx=0.2, variant="bubble3", rk4=False), 2,
+     10),
     ("ep_column_2d_rk4", lambda **kw: _as_rk4(M.configs.ep_column_2d(**kw)),
      dict(dx=0.01, width=0.2, height=0.1), 2, 10),
 ]
