@@ -16,7 +16,7 @@ from .model import (  # noqa: F401
     VirtualParticleSystem, WenlandC2Kernel, XSPHPfn, ZeroFieldUpdater)
 from .scene import DeviceScene  # noqa: F401
 from .integration import (  # noqa: F401
-    Session, create_grid, sort_particles, sweep, time_integrate, update_state)
+    Session, adjust_v, create_grid, sort_particles, sweep, time_integrate, update_state)
 from . import configs  # noqa: F401
 from . import serving  # noqa: F401
 from . import snapshot  # noqa: F401

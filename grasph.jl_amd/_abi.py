@@ -82,6 +82,7 @@ _SIGS = {
     "sort": [_i, _d],
     "create_grid": [_i],
     "sweep": [_i, _i],
+    "adjust_v": [_i],
     "update_state": [_i, _i, _d],
     "generate_ghosts": [_i],
     "update_ghost_kinematics": [_i],
@@ -265,6 +266,9 @@ class CBackend:
 
     def sweep(self, inter, stage):
         self.call("sweep", inter, stage)
+
+    def adjust_v(self, inter):
+        self.call("adjust_v", inter)
 
     def update_state(self, sys, stage, dt):
         self.call("update_state", sys, stage, dt)

@@ -2490,12 +2490,12 @@ int gr_create_grid(gr_ctx *c, int inter) {
   });
 }
 
-int gr_sweep(gr_ctx *c, int inter, int stage) {
-  return guard(c, [&] {
-    need(inter >= 0 && inter < (int)c->inter.size(), GR_ERR_ARGUMENT, "bad interaction");
+}  // extern "C"
+
+namespace {
+// One functor of one interaction over its pairs (sweep!, adjust_v!).
+void sweep_one(gr_ctx *c, int inter, const gr_pfn_desc f) {
     InterH &in = c->inter[inter];
-    if (stage < 1 || stage > (int)in.pfns.size()) return;
-    const gr_pfn_desc f = in.pfns[stage - 1];
     const int fam = family_of(f.kind);
     if (fam < 0) return;
     need(in.grid_built, GR_ERR_STATE, "sweep!: create_grid! has not been called");
@@ -2506,7 +2506,8 @@ int gr_sweep(gr_ctx *c, int inter, int stage) {
       refresh_shadows<R>(c, syss);
       pack_systems<R>(c, syss);
     });
-    const int pk = classify(c, in, f.kind);
+    const int pk =
+        f.kind == GR_PFN_XSPH ? (in.b < 0 ? P_SELF : P_TWO) : classify(c, in, f.kind);
     std::vector<Task> tv;
     std::vector<SrcDesc> sv;
     std::vector<Blk> bv;
@@ -2521,6 +2522,15 @@ int gr_sweep(gr_ctx *c, int inter, int stage) {
       t.assign = 0;
       t.ltask = -1;
       t.alias = -1;
+      if (f.kind == GR_PFN_XSPH && c->sys[target].kind == GR_SYS_GHOST) {
+        const SysH &src = c->sys[c->sys[target].src];
+        need(live_n(c, target) <= src.n, GR_ERR_ARGUMENT,
+             "XSPHPfn with a ghost partner: more ghosts than source particles "
+             "(ps_b.v_adjustment[j] would index past the source's array)");
+        t.alias = c->sys[target].src;
+        t.alias_cap = (int32_t)src.n;
+        t.alias_off = src.off;
+      }
       if (f.kind == GR_PFN_INTERP) t.imask = f.sigma | (f.alpha != 0.0 ? 1 << 30 : 0);
       SrcDesc q = make_src(c, inter, target, b_side, pk);
       set_pfn(q, f, pk, c, in);
@@ -2562,6 +2572,30 @@ int gr_sweep(gr_ctx *c, int inter, int stage) {
       launch_sweep<R>(c->stream, pool_view<R>(c, c->cur), a, c->D, fam, false);
     });
     HIPCHK(hipStreamSynchronize(c->stream));
+}
+}  // namespace
+
+extern "C" {
+
+int gr_sweep(gr_ctx *c, int inter, int stage) {
+  return guard(c, [&] {
+    need(inter >= 0 && inter < (int)c->inter.size(), GR_ERR_ARGUMENT, "bad interaction");
+    InterH &in = c->inter[inter];
+    if (stage < 1 || stage > (int)in.pfns.size()) return;
+    sweep_one(c, inter, in.pfns[stage - 1]);
+  });
+}
+
+int gr_adjust_v(gr_ctx *c, int inter) {
+  return guard(c, [&] {
+    need(inter >= 0 && inter < (int)c->inter.size(), GR_ERR_ARGUMENT, "bad interaction");
+    InterH &in = c->inter[inter];
+    if (!in.has_xsph) return;  // velocity_adjust_pairwise_fn = nothing
+    gr_pfn_desc f;
+    memset(&f, 0, sizeof(f));
+    f.kind = GR_PFN_XSPH;
+    f.alpha = in.xsph_eps;
+    sweep_one(c, inter, f);
   });
 }
 

@@ -159,6 +159,23 @@ def sweep(si, stage=1, session=None):
             s.scene.download(ps)
 
 
+def adjust_v(si, session=None):
+    """adjust_v!(si) (src/Interaction.jl:309) on the device: the interaction's
+    velocity_adjust_pairwise_fn (XSPHPfn) accumulated into v_adjustment.  With
+    a ghost partner the ghost's share lands in its source's array at the
+    ghost's index, as the reference's forwarded field does."""
+    s = _session_for(si, si.ndims, session)
+    systems = [si.system_a] + ([si.system_b] if si.system_b is not None else [])
+    real = [ps.source if isinstance(ps, M.GhostParticleSystem) else ps for ps in systems]
+    for ps in real:
+        s.scene.upload(ps)
+    h = s.scene.interaction(si)
+    s.backend.create_grid(h)
+    s.backend.adjust_v(h)
+    for ps in real:
+        s.scene.download(ps)
+
+
 def update_state(ps, stage=1, dt=0.0, session=None):
     """update_state!(ps, stage, dt) (src/Particles.jl:542)."""
     s = _session_for(ps, ps.ndims, session)

@@ -204,10 +204,12 @@ int gr_interaction_create(gr_ctx *ctx, int kernel_kind, double h, int sys_a,
 /* Velocity-adjust functor of an interaction: XSPHPfn(epsilon)
  * (SystemInteraction's velocity_adjust_pairwise_fn, src/Interaction.jl:61-66,
  * applied by _xsph_correction!, src/TimeIntegration.jl:423-436, after the
- * full step and before positions move).  NaN removes it.  A ghost partner is
- * rejected: the reference writes through the ghost's forwarded v_adjustment
- * into its source at the ghost's index (src/GhostParticles.jl:235), which is
- * out of bounds once ghosts outnumber source particles. */
+ * full step and before positions move).  NaN removes it.  With a ghost
+ * partner the reference writes through the ghost's forwarded v_adjustment,
+ * i.e. into the source system at the ghost's own index
+ * (src/GhostParticles.jl:226-236); this is reproduced, and a step in which
+ * the ghosts outnumber the source particles (an out-of-bounds write in the
+ * reference) fails with GR_ERR_ARGUMENT. */
 int gr_interaction_set_vadjust(gr_ctx *ctx, int inter, double epsilon);
 
 /* ---- integrator ----------------------------------------------------------
@@ -253,6 +255,10 @@ int gr_create_grid(gr_ctx *ctx, int inter);
 /* sweep!(si, stage) (src/Interaction.jl:292): accumulates into the target
  * fields (+=), as the reference does. */
 int gr_sweep(gr_ctx *ctx, int inter, int stage);
+/* adjust_v!(si) (src/Interaction.jl:309): the velocity-adjust functor
+ * (XSPHPfn) over the interaction's pairs, accumulating into v_adjustment;
+ * a no-op when the interaction has none. */
+int gr_adjust_v(gr_ctx *ctx, int inter);
 /* update_state!(ps, stage, dt) (src/Particles.jl:542). */
 int gr_update_state(gr_ctx *ctx, int sys, int stage, double dt);
 /* generate_ghosts!(ge) / update_ghost_kinematics!(ge) / update_ghost!(ge, stage)

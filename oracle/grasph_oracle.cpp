@@ -1930,6 +1930,17 @@ int orc_sort(orc_scene *s, int sys, double cutoff) {
 }
 int orc_create_grid(orc_scene *s, int inter) { return create_grid(s, s->inter[inter]); }
 int orc_sweep(orc_scene *s, int inter, int stage) { return sweep(s, s->inter[inter], stage); }
+// adjust_v!(si) (src/Interaction.jl:309)
+int orc_adjust_v(orc_scene *s, int inter) {
+  if (inter < 0 || inter >= (int)s->inter.size()) return fail(s, GR_ERR_ARGUMENT, "bad interaction");
+  Inter &in = s->inter[inter];
+  if (!in.has_xsph) return GR_OK;
+  gr_pfn_desc f;
+  std::memset(&f, 0, sizeof(f));
+  f.kind = GR_PFN_XSPH;
+  f.alpha = in.xsph_eps;
+  return sweep_fn(s, in, f);
+}
 int orc_update_state(orc_scene *s, int sys, int stage, double dt) {
   return update_state(s, s->sys[sys], stage, dt);
 }

@@ -68,6 +68,7 @@ int orc_step(orc_scene *s, int integ, int nsteps, double dt);
 int orc_sort(orc_scene *s, int sys, double cutoff);
 int orc_create_grid(orc_scene *s, int inter);
 int orc_sweep(orc_scene *s, int inter, int stage);
+int orc_adjust_v(orc_scene *s, int inter);
 int orc_update_state(orc_scene *s, int sys, int stage, double dt);
 int orc_generate_ghosts(orc_scene *s, int ghost);
 int orc_update_ghost_kinematics(orc_scene *s, int ghost);
