@@ -39,7 +39,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--dx", type=float, default=0.006)
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
-    ap.add_argument("--cpu-steps", type=int, default=8,
+    ap.add_argument("--cpu-steps", type=int, default=60,
                     help="timed oracle steps on the host (0 disables the CPU baseline)")
     ap.add_argument("--graph", type=int, default=1, help="replay the step as a hipGraph")
     ap.add_argument("--calibrate-copy", type=int, default=0,
@@ -51,7 +51,7 @@ def parse():
     ap.add_argument("--serve", type=int, default=0,
                     help="also serve this many column-collapse jobs (continuous batching) "
                          "and the same jobs serially")
-    ap.add_argument("--serve-step-scale", type=float, default=0.02,
+    ap.add_argument("--serve-step-scale", type=float, default=0.2,
                     help="step counts of the served sample are scaled by this factor")
     ap.add_argument("--serve-size-scale", type=float, default=0.25,
                     help="particle counts of the served sample are scaled by this factor")
@@ -93,7 +93,7 @@ def serve_leg(n_jobs, step_scale, size_scale, precision):
     S = g.serving
     mk = lambda: S.column_jobs(n_jobs, step_scale=step_scale, size_scale=size_scale)  # noqa: E731
     served = S.Server(precision=precision, quantum=50, graph=True).run(mk())
-    plain = S.run_plain(mk(), precision=precision)
+    plain = S.run_plain(mk(), precision=precision, graph=True)
     return {"sample": f"{n_jobs} jobs, Poisson 20/s, 80% 2D / 20% 3D columns, steps x{step_scale}, "
                       f"particles x{size_scale}",
             "served": served, "plain": plain,

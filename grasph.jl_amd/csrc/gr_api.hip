@@ -1182,7 +1182,8 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
   std::vector<Blk> lbv;
   std::map<int, int> ltask_of;
   int64_t lacc = 0, cacc = 0;
-  const int maxnb = c->list_maxnb > 0 ? c->list_maxnb : (c->D == 3 ? 192 : 96);
+  // a multiple of 4: lists are stored in 4-entry chunks
+  const int maxnb = ((c->list_maxnb > 0 ? c->list_maxnb : (c->D == 3 ? 192 : 96)) + 3) & ~3;
   for (auto &kv : tl) {
     LTask L;
     memset(&L, 0, sizeof(L));
@@ -2239,11 +2240,8 @@ int gr_interaction_set_vadjust(gr_ctx *c, int inter, double epsilon) {
       in.has_xsph = false;
       return;
     }
-    // XSPHPfn's generic method writes ps_b.v_adjustment[j]: for a ghost
-    // partner that is the source system's entry at the ghost's index
-    // (src/GhostParticles.jl:226-236), which this layout does not hold
-    need(in.b < 0 || c->sys[in.b].kind != GR_SYS_GHOST, GR_ERR_ARGUMENT,
-         "XSPHPfn with a ghost partner writes the source's v_adjustment at ghost indices");
+    // a ghost partner's share goes to its source's v_adjustment at the
+    // ghost's index (Task.alias), as the reference's forwarded field does
     in.has_xsph = true;
     in.xsph_eps = epsilon;
     c->vadj_any = true;

@@ -132,8 +132,9 @@ struct Task {
 };
 
 // Neighbour list of one target system: sources [src_begin, src_end) of the
-// list source table; partners at lidx[list_off + k·cap + i], counts at
-// lcnt[cnt_off + q·cap + i], overflow flag at lcnt[cnt_off + nsrc·cap + i].
+// list source table; partner k of target i at lidx[list_off + 4·((k/4)·cap
+// + i) + k%4] (4-entry chunks), counts at lcnt[cnt_off + q·cap + i],
+// overflow flag at lcnt[cnt_off + nsrc·cap + i].
 struct LTask {
   int32_t sys;
   int32_t src_begin, src_end;

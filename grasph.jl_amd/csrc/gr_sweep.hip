@@ -221,7 +221,12 @@ struct Gather {
       return;
     }
     if (FAM == FAM_INTERP) {  // InterpolateFieldFn (src/PairwiseFunctors.jl:424-434)
-      const R w = wq<R>(Q.kk, K.norm, FA ? r * K.ih : r / K.h);
+      // W from the FP64 distance in every mode: an accepted pair (r < 2h in
+      // FP64) must weigh W > 0 as in the reference, or a virtual particle
+      // whose partners all sit at the edge of the support would get
+      // w_sum = 0, zeroed fields and a zero density while still having
+      // partners in the force sweeps
+      const R w = (R)wq<double>(Q.kk, Q.norm, sqrt(r2) / Q.h);
       const RecB<R> Bj = P.rb[gj];
       const R rj = Aj.rho;
       const R kw = FA ? w * (K.m_q * Fast<R>::rcp(rj)) : w * (K.m_q / rj);
@@ -590,9 +595,12 @@ __device__ inline int wave_max_i(int v) {
 }
 
 // Accepted partners of every target of a list task, per source, in the
-// order the sources are listed: lidx[L.list_off + k·L.cap + i] holds the
-// k-th partner (source-local index) of target i, lcnt[L.cnt_off + q·L.cap + i]
-// the number from source q.  Counts above L.maxnb mean "overflow" (the
+// order the sources are listed.  The k-th partner (source-local index) of
+// target i sits in chunk k/4 of the target, element k%4: chunks of 4 ints
+// (16 B), chunk row c of a task at lidx[L.list_off + 4·(c·L.cap + i)], so a
+// lane writes whole 16-byte chunks and a wavefront reads one chunk row as a
+// contiguous 1 KiB.  lcnt[L.cnt_off + q·L.cap + i] holds the number from
+// source q.  Counts above L.maxnb (a multiple of 4) mean "overflow" (the
 // physics pass then walks the cells directly).
 //
 // One wavefront per workgroup.  For each strip, lanes whose candidate ranges
@@ -620,8 +628,20 @@ __global__ void __launch_bounds__(64)
   double xi[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) xi[d] = P.x[d][gi];
-  int *my = lidx + L.list_off + i;
+  int4 *ch = reinterpret_cast<int4 *>(lidx + L.list_off) + i;
   int cnt = 0;
+  int b0 = 0, b1 = 0, b2 = 0, b3 = 0;  // the chunk being filled
+  auto push = [&](int j) {
+    if (cnt < L.maxnb) {
+      const int sl = cnt & 3;
+      b0 = sl == 0 ? j : b0;
+      b1 = sl == 1 ? j : b1;
+      b2 = sl == 2 ? j : b2;
+      b3 = sl == 3 ? j : b3;
+      if (sl == 3) ch[(int64_t)(cnt >> 2) * L.cap] = make_int4(b0, b1, b2, b3);
+    }
+    ++cnt;
+  };
   for (int q = L.src_begin; q < L.src_end; ++q) {
     const SrcDesc Q = srcs[q];
     const Grid G = grids[Q.grid];
@@ -674,13 +694,11 @@ __global__ void __launch_bounds__(64)
                 rb = rb + db[d] * db[d];
               }
               if (ra < c2 && j != self_j) {
-                if (cnt < L.maxnb) my[(int64_t)cnt * L.cap] = j;
-                ++cnt;
+                push(j);
                 ++cq;
               }
               if (rb < c2 && j + 1 != self_j) {
-                if (cnt < L.maxnb) my[(int64_t)cnt * L.cap] = j + 1;
-                ++cnt;
+                push(j + 1);
                 ++cq;
               }
             }
@@ -693,8 +711,7 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
               for (int d = 1; d < D; ++d) r2 = r2 + dx[d] * dx[d];
               if (r2 < c2 && j != self_j) {
-                if (cnt < L.maxnb) my[(int64_t)cnt * L.cap] = j;
-                ++cnt;
+                push(j);
                 ++cq;
               }
             }
@@ -705,6 +722,8 @@ __global__ void __launch_bounds__(64)
     }
     if (live) lcnt[L.cnt_off + (int64_t)(q - L.src_begin) * L.cap + i] = cq;
   }
+  if (live && cnt <= L.maxnb && (cnt & 3) != 0)  // the last, partial chunk
+    ch[(int64_t)(cnt >> 2) * L.cap] = make_int4(b0, b1, b2, b3);
   if (live)
     lcnt[L.cnt_off + (int64_t)(L.src_end - L.src_begin) * L.cap + i] = cnt > L.maxnb ? 1 : 0;
 }
@@ -735,7 +754,9 @@ __global__ void __launch_bounds__(GR_BLOCK)
   if (overflow) {
     gather_direct<D, R, FAM>(Gt, P, sd, T, srcs, grids, carr, cells, i, ns, no);
   } else if (live) {
-    const int *my = lidx + L.list_off + i;
+    const int4 *ch = reinterpret_cast<const int4 *>(lidx + L.list_off) + i;
+    int4 cur = make_int4(0, 0, 0, 0);
+    int curc = -1;  // chunk held in cur
     for (int q = T.src_begin; q < T.src_end; ++q) {
       const SrcDesc Q = srcs[q];
       const int64_t poff = sd[Q.partner].off;
@@ -745,8 +766,13 @@ __global__ void __launch_bounds__(GR_BLOCK)
       for (int l = 0; l < Q.lsrc; ++l) off += lcnt[L.cnt_off + (int64_t)l * L.cap + i];
       const int cq = lcnt[L.cnt_off + (int64_t)Q.lsrc * L.cap + i];
       const double da = (Q.pk == PK_DYNAMIC) ? Gt.da_of(Q) : 0.0;
-      for (int k = 0; k < cq; ++k) {
-        const int j = my[(int64_t)(off + k) * L.cap];
+      for (int k = off; k < off + cq; ++k) {
+        if ((k >> 2) != curc) {
+          curc = k >> 2;
+          cur = ch[(int64_t)curc * L.cap];
+        }
+        const int sl = k & 3;
+        const int j = sl == 0 ? cur.x : sl == 1 ? cur.y : sl == 2 ? cur.z : cur.w;
         const int64_t gj = poff + j;
         const RecA<R> Aj = P.ra[gj];
         double dx[D];

@@ -195,14 +195,16 @@ class Server:
         return report(jobs, time.perf_counter() - t0)
 
 
-def run_plain(jobs, precision="f32", keep_results=False, session_factory=None):
+def run_plain(jobs, precision="f32", keep_results=False, session_factory=None, graph=False):
     """Each job alone on the GPU, one after another in arrival order (a job
-    starts when it has arrived and the previous one has finished)."""
+    starts when it has arrived and the previous one has finished); `graph`
+    replays each job's step as a captured graph, as the server can."""
     make = session_factory or (lambda D: Session(D, precision))
     sessions = {}
     t0 = time.perf_counter()
     for D in sorted({j.ndims for j in jobs}):
         sessions[D] = make(D)
+        sessions[D].backend.set_graph_mode(bool(graph))
     t0 = time.perf_counter()
     for job in sorted(jobs, key=lambda j: (j.arrival, j.jid)):
         wait = job.arrival - (time.perf_counter() - t0)
