@@ -10,7 +10,8 @@ from .model import (  # noqa: F401
     BasicParticleSystem, CauchyFluidPfn, CubicSplineKernel, DynamicBoundarySystem,
     ElastoPlasticParticleSystem, ElastoPlasticStressUpdater, FluidParticleSystem, FluidPfn,
     GhostCopier, GhostEntry, GhostParticleSystem, HouseholderReflect, InterpolateFieldFn,
-    LeapFrogTimeIntegrator, LinearEOSUpdater, RK4TimeIntegrator, NeighborCounterPfn, PrescribedVelocityUpdater,
+    LeapFrogTimeIntegrator, LinearEOSUpdater, RK4TimeIntegrator, NeighborCounterPfn,
+    NeighborCountFn, PrescribedVelocityUpdater, ProbeParticleSystem,
     StaticBoundarySystem, StrainRatePfn, StrainRateVorticityPfn, StressParticleSystem,
     SystemInteraction, TaitEOSUpdater, ViscoPlasticMCStressUpdater, VirtualNormUpdater,
     VirtualParticleSystem, WenlandC2Kernel, XSPHPfn, ZeroFieldUpdater)

@@ -24,7 +24,7 @@ GR_SYS_BASIC, GR_SYS_FLUID, GR_SYS_STRESS, GR_SYS_EP, GR_SYS_GHOST = range(5)
 # ---- fields -------------------------------------------------------------------
 (GR_F_X, GR_F_V, GR_F_RHO, GR_F_DVDT, GR_F_DRHODT, GR_F_P, GR_F_STRESS,
  GR_F_STRAIN_RATE, GR_F_VORTICITY, GR_F_STRAIN, GR_F_STRAIN_P,
- GR_F_V_ADJUSTMENT, GR_F_NORMALS, GR_F_W_SUM) = range(14)
+ GR_F_V_ADJUSTMENT, GR_F_NORMALS, GR_F_W_SUM, GR_F_COUNT) = range(15)
 GR_I_ID, GR_I_IDX_ORIGINAL, GR_I_IDX_BOUNDARY = range(3)
 FIELD_IDS = {
     "x": GR_F_X, "v": GR_F_V, "rho": GR_F_RHO, "dvdt": GR_F_DVDT,
@@ -38,7 +38,7 @@ INDEX_IDS = {"id": GR_I_ID, "idx_original": GR_I_IDX_ORIGINAL,
 # ---- kernels / functors / wrappers / updaters / copier modes ------------------
 GR_KERNEL_CUBIC, GR_KERNEL_WENDLAND_C2 = 0, 1
 (GR_PFN_NONE, GR_PFN_FLUID, GR_PFN_CAUCHY, GR_PFN_SR, GR_PFN_SRV, GR_PFN_COUNT,
- GR_PFN_INTERP, GR_PFN_XSPH) = range(8)
+ GR_PFN_INTERP, GR_PFN_XSPH, GR_PFN_NCOUNT) = range(9)
 GR_WRAP_NONE, GR_WRAP_STATIC, GR_WRAP_DYNAMIC = range(3)
 (GR_UPD_NOTHING, GR_UPD_ZERO, GR_UPD_TAIT, GR_UPD_LINEAR, GR_UPD_EP, GR_UPD_VPMC, GR_UPD_VNORM,
  GR_UPD_PRESCRIBED_V) = range(8)
@@ -71,6 +71,9 @@ _SIGS = {
     "system_get_n": [_i, _i64p],
     "ghost_create": [_i, _d, _i, _dp, _dp, _ip],
     "virtual_create": [_i, _i, _ip, _dp],
+    "probe_create": [_i, _dp, _i, _i, _ip],
+    "integrator_set_probes": [_i, _i, _ip, _i, _ip],
+    "measure_probes": [_i],
     "ghost_set_copier": [_i, _i, _i, _ip, _ip],
     "interaction_create": [_i, _d, _i, _i, _i, _dp, _dp, _d, _d, _i,
                            C.POINTER(gr_pfn_desc), _ip],
@@ -209,6 +212,19 @@ class CBackend:
         z = np.ascontiguousarray(zero_fields, dtype=np.int32)
         pv = np.ascontiguousarray(prescribed_v, dtype=np.float64)
         self.call("virtual_create", sys, len(z), _iptr(z), _dptr(pv))
+
+    def probe_create(self, sys, prescribed_v, mirror, zero_fields):
+        z = np.ascontiguousarray(zero_fields, dtype=np.int32)
+        pv = np.ascontiguousarray(prescribed_v, dtype=np.float64)
+        self.call("probe_create", sys, _dptr(pv), int(mirror), len(z), _iptr(z))
+
+    def integrator_set_probes(self, integ, probes, inters):
+        p = np.ascontiguousarray(probes, dtype=np.int32)
+        t = np.ascontiguousarray(inters, dtype=np.int32)
+        self.call("integrator_set_probes", integ, len(p), _iptr(p), len(t), _iptr(t))
+
+    def measure_probes(self, integ):
+        self.call("measure_probes", integ)
 
     def ghost_create(self, src, cutoff, normals, points):
         nn = np.ascontiguousarray(normals, dtype=np.float64).reshape(-1)

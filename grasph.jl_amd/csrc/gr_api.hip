@@ -88,6 +88,10 @@ struct SysH {
   bool virt = false;
   long long zero_mask = 0;
   double pv[3] = {0, 0, 0};
+  // ProbeParticleSystem (src/ProbeParticles.jl:22-89): virtual-like target
+  // measured by gr_measure_probes; mirror: mirror_target system or -1
+  bool probe = false;
+  int mirror = -1;
 };
 
 struct InterH {
@@ -104,7 +108,7 @@ struct InterH {
 };
 
 struct IntegH {
-  std::vector<int> sys, inter, ghost, virt;
+  std::vector<int> sys, inter, ghost, virt, probes, probe_inter;
   double gamma = 0, h = 0, c = 0, dt = 0;
   int nst = 0;
   bool dead = false;
@@ -179,11 +183,14 @@ struct Plan {
   // velocity adjustment after the full step (XSPH sweep tasks; vadj_live:
   // the correction runs, also with no XSPH sweep when v_adjustment was set)
   FamPlan xsph;
-  std::vector<int2> xsph_alias;  // (ghost, source n) of ghost-target XSPH tasks
+  // ghost-target XSPH tasks, one launch each after the others: they add into
+  // their source's v_adjustment, which the source's own task also writes
+  std::vector<std::unique_ptr<FamPlan>> xsph_al;
+  std::vector<int2> xsph_alias;  // (ghost, source n) of those tasks
   int vadj_live = 0;
   // integrated (non-virtual) real systems; virtual systems
-  DBuf blk_int, blk_virt;
-  int nblk_int = 0, nblk_virt = 0;
+  DBuf blk_int, blk_virt, blk_probe;
+  int nblk_int = 0, nblk_virt = 0, nblk_probe = 0;
   bool do_sort = true, do_ghosts = true, do_grid = true, do_step = true;
   // graph capture per A/B parity
   hipGraphExec_t graph[2] = {nullptr, nullptr};
@@ -217,6 +224,8 @@ struct gr_ctx {
   DBuf vs, rhos, rho_h, gidx, gbnd, gnrm, gxs, gidx_s, gbnd_s;
   DBuf ra, rb;  // partner records (RecA / RecB), rebuilt before every stage's sweeps
   DBuf wsum;    // w_sum of virtual systems
+  DBuf pcnt;    // neighbour counts of probe systems
+  DBuf probe_scr;  // scratch of the probes' sort by id
   DBuf acc;     // RK4 accumulators (4 components)
   DBuf vadj[2];  // v_adjustment (XSPH), swapped with the sort
   bool vadj_any = false;  // some v_adjustment may be nonzero: permute and correct it
@@ -311,6 +320,7 @@ Pool<R> pool_view(gr_ctx *c, int w) {
   P.ra = c->ra.as<RecA<R>>();
   P.rb = c->rb.as<RecB<R>>();
   P.wsum = c->wsum.as<double>();
+  P.pcnt = c->pcnt.as<double>();
   for (int k = 0; k < 4; ++k) P.acc[k] = c->acc.as<double>() + k * cap;
   for (int d = 0; d < 3; ++d) P.vadj[d] = c->vadj[w].as<double>() + d * cap;
   for (int d = 0; d < 3; ++d) {
@@ -401,6 +411,7 @@ void commit(gr_ctx *c) {
     grow(c, c->gbnd_s, 1, 4, old, nc);
     grow(c, c->ra, 1, 32, old, nc);
     grow(c, c->wsum, 1, 8, old, nc);
+    grow(c, c->pcnt, 1, 8, old, nc);
     grow(c, c->acc, 4, 8, old, nc);
     grow(c, c->rb, 1, R == 4 ? 16 : 32, old, nc);
     c->pool_cap = nc;
@@ -533,6 +544,7 @@ int ncomp(gr_ctx *c, const SysH &s, int field) {
     case GR_F_DRHODT:
     case GR_F_P:
     case GR_F_W_SUM:
+    case GR_F_COUNT:
       return 1;
     case GR_F_STRESS:
     case GR_F_STRAIN_RATE:
@@ -579,6 +591,8 @@ bool has_field(const SysH &s, int field) {
       return s.kind == GR_SYS_EP;
     case GR_F_W_SUM:
       return s.virt;
+    case GR_F_COUNT:
+      return s.probe;
   }
   return false;
 }
@@ -612,6 +626,7 @@ FieldRef field_ref(gr_ctx *c, int field, bool shadow_only) {
     case GR_F_STRAIN: add(c->strain[w], c->nsmax, 8); break;
     case GR_F_STRAIN_P: add(c->strainp[w], c->nsmax, 8); break;
     case GR_F_W_SUM: add(c->wsum, 1, 8); break;
+    case GR_F_COUNT: add(c->pcnt, 1, 8); break;
     case GR_F_V_ADJUSTMENT: add(c->vadj[w], c->D, 8); break;
   }
   return f;
@@ -702,6 +717,7 @@ int family_of(int pfn) {
   if (pfn == GR_PFN_COUNT) return FAM_COUNT;
   if (pfn == GR_PFN_INTERP) return FAM_INTERP;
   if (pfn == GR_PFN_XSPH) return FAM_XSPH;
+  if (pfn == GR_PFN_NCOUNT) return FAM_INTERP;
   return -1;
 }
 
@@ -1051,6 +1067,12 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
       T.sort_cutoff = 2.0 * I.h;
       T.dt = I.dt;
     }
+    for (int q : I.probes) {  // advanced by prescribed_v·dt every step
+      StepSys &T = c->stepsys[q];
+      memset(&T, 0, sizeof(T));
+      T.sort_cutoff = 2.0 * I.h;
+      T.dt = I.dt;
+    }
   }
   // gather tasks: (stage, family, target) → sources, in interaction order
   struct TaskB {
@@ -1100,8 +1122,16 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
           if (f.kind == GR_PFN_CAUCHY) tb.t.has_stress = 1;
           if (f.kind == GR_PFN_SRV) tb.t.write_vor = 1;
           if (f.kind == GR_PFN_INTERP) tb.t.imask |= f.sigma | (f.alpha != 0.0 ? 1 << 30 : 0);
+          if (f.kind == GR_PFN_NCOUNT) tb.t.imask |= 1 << GR_F_COUNT;
           tb.src.push_back(q);
         };
+        if (f.kind == GR_PFN_NCOUNT) {  // the probe side counts (:443-460)
+          need(in.b >= 0 && c->sys[in.b].probe && !c->sys[in.a].virt && in.wrap == GR_WRAP_NONE,
+               GR_ERR_ARGUMENT,
+               "NeighborCountFn needs a real system_a and a ProbeParticleSystem system_b");
+          add(in.b, true);
+          continue;
+        }
         if (f.kind == GR_PFN_INTERP) {  // the virtual side gathers (:424-434)
           need(in.b >= 0 && in.wrap == GR_WRAP_NONE &&
                    (c->sys[in.b].virt != c->sys[in.a].virt) &&
@@ -1264,23 +1294,32 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
     P.stages.push_back(std::move(sp));
   }
   {
-    std::vector<Task> tv;
-    std::vector<SrcDesc> sv;
-    std::vector<Blk> bv;
+    auto build = [&](FamPlan &F, bool alias, int only) {
+      std::vector<Task> tv;
+      std::vector<SrcDesc> sv;
+      std::vector<Blk> bv;
+      for (auto &kv : tasks[nst * NFAM + FAM_XSPH]) {
+        Task t = kv.second.t;
+        if ((t.alias >= 0) != alias || (alias && kv.first != only)) continue;
+        t.ltask = (c->use_lists && t.alias < 0) ? ltask_of[kv.first] : -1;
+        t.src_begin = (int)sv.size();
+        for (auto &q : kv.second.src) sv.push_back(q);
+        t.src_end = (int)sv.size();
+        add_blocks(bv, (int)tv.size(), c->sys[t.sys].cap, GR_BLOCK);
+        tv.push_back(t);
+      }
+      F.tasks.upload(tv, c->stream);
+      F.srcs.upload(sv, c->stream);
+      F.blk.upload(bv, c->stream);
+      F.nblk = (int)bv.size();
+    };
+    build(P.xsph, false, -1);
     for (auto &kv : tasks[nst * NFAM + FAM_XSPH]) {
-      Task t = kv.second.t;
-      t.ltask = (c->use_lists && t.alias < 0) ? ltask_of[kv.first] : -1;
-      if (t.alias >= 0) P.xsph_alias.push_back(make_int2(t.sys, t.alias_cap));
-      t.src_begin = (int)sv.size();
-      for (auto &q : kv.second.src) sv.push_back(q);
-      t.src_end = (int)sv.size();
-      add_blocks(bv, (int)tv.size(), c->sys[t.sys].cap, GR_BLOCK);
-      tv.push_back(t);
+      if (kv.second.t.alias < 0) continue;
+      P.xsph_al.push_back(std::make_unique<FamPlan>());
+      build(*P.xsph_al.back(), true, kv.first);
+      P.xsph_alias.push_back(make_int2(kv.second.t.sys, kv.second.t.alias_cap));
     }
-    P.xsph.tasks.upload(tv, c->stream);
-    P.xsph.srcs.upload(sv, c->stream);
-    P.xsph.blk.upload(bv, c->stream);
-    P.xsph.nblk = (int)bv.size();
     P.vadj_live = c->vadj_any ? 1 : 0;
   }
   P.stepsys.clear();
@@ -1288,6 +1327,7 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
     for (int s : c->integ[ig].sys) P.stepsys.push_back({s, c->stepsys[s]});
     for (int v : c->integ[ig].virt) P.stepsys.push_back({v, c->stepsys[v]});
     for (int g : c->integ[ig].ghost) P.stepsys.push_back({g, c->stepsys[g]});
+    for (int q : c->integ[ig].probes) P.stepsys.push_back({q, c->stepsys[q]});
   }
   upload_stepsys(c);
 }
@@ -1322,15 +1362,22 @@ Plan &get_plan(gr_ctx *c, const std::vector<int> &integs) {
          "integrators advanced together must use the same time integration scheme");
   plan_sort_real(c, *P, reals);
   {
-    std::vector<Blk> bi, bv;
+    std::vector<Blk> bi, bv, bp;
     for (int ig : integs) {
       for (int s : c->integ[ig].sys) add_blocks(bi, s, c->sys[s].cap, GR_BLOCK);
       for (int v : c->integ[ig].virt) add_blocks(bv, v, c->sys[v].cap, GR_BLOCK);
+      for (int q : c->integ[ig].probes) {
+        const SysH &S = c->sys[q];
+        if (S.pv[0] != 0.0 || S.pv[1] != 0.0 || S.pv[2] != 0.0)  // iszero(pv) skips
+          add_blocks(bp, q, S.cap, GR_BLOCK);
+      }
     }
     P->blk_int.upload(bi, c->stream);
     P->blk_virt.upload(bv, c->stream);
+    P->blk_probe.upload(bp, c->stream);
     P->nblk_int = (int)bi.size();
     P->nblk_virt = (int)bv.size();
+    P->nblk_probe = (int)bp.size();
   }
   plan_ghosts(c, *P, ghosts);
   plan_grids(c, *P, inters);
@@ -1505,7 +1552,7 @@ void phase_xsph(gr_ctx *c, Plan &P) {
   const SysDesc *sd = c->d_sd.as<SysDesc>();
   const StepSys *st = c->d_st.as<StepSys>();
   launch_xsph<R>(s, A, sd, st, c->d_n.as<int>(), P.blk_int.as<Blk>(), P.nblk_int, c->D, 0);
-  if (P.xsph.nblk > 0) {
+  if (P.xsph.nblk > 0 || !P.xsph_al.empty()) {
     launch_xsph<R>(s, A, sd, st, c->d_n.as<int>(), P.blk_virt.as<Blk>(), P.nblk_virt, c->D, 1);
     launch_pack<R>(s, A, sd, c->d_n.as<int>(), P.blk_real.as<Blk>(), P.nblk_real, c->D);
     if (!P.xsph_alias.empty())
@@ -1527,6 +1574,13 @@ void phase_xsph(gr_ctx *c, Plan &P) {
       a.lidx = c->lidx.as<int>();
     }
     launch_sweep<R>(s, A, a, c->D, FAM_XSPH, false);
+    for (auto &F : P.xsph_al) {
+      a.tasks = F->tasks.as<Task>();
+      a.srcs = F->srcs.as<SrcDesc>();
+      a.blk = F->blk.as<Blk>();
+      a.nblk = F->nblk;
+      launch_sweep<R>(s, A, a, c->D, FAM_XSPH, false);
+    }
     for (const int2 &g : P.xsph_alias)
       launch_alias_check(s, c->d_n.as<int>(), g.x, g.y, c->d_flags.as<int>());
   }
@@ -1608,6 +1662,7 @@ void run_step(gr_ctx *c, Plan &P, bool cnt, std::vector<hipEvent_t> *ev) {
   }
   if (P.vadj_live) phase_xsph<R>(c, P);
   launch_virtual<R>(s, A, sd, st, c->d_n.as<int>(), P.blk_virt.as<Blk>(), P.nblk_virt, 1, c->D);
+  launch_virtual<R>(s, A, sd, st, c->d_n.as<int>(), P.blk_probe.as<Blk>(), P.nblk_probe, 1, c->D);
   rec(6 + 2 * nst);
 }
 
@@ -2106,6 +2161,29 @@ int gr_virtual_create(gr_ctx *c, int sys, int nzero, const int *zero_fields,
   });
 }
 
+int gr_probe_create(gr_ctx *c, int sys, const double *prescribed_v, int mirror, int nzero,
+                    const int *zero_fields) {
+  return guard(c, [&] {
+    validate_sys(c, sys);
+    SysH &s = c->sys[sys];
+    need(s.kind != GR_SYS_GHOST && !s.virt, GR_ERR_ARGUMENT, "not a plain particle system");
+    if (mirror >= 0) {
+      validate_sys(c, mirror);
+      need(c->sys[mirror].kind != GR_SYS_GHOST && !c->sys[mirror].virt, GR_ERR_ARGUMENT,
+           "mirror_target must be a real particle system");
+      need(c->sys[mirror].n == s.n, GR_ERR_ARGUMENT,
+           "a mirroring probe has one slot per target particle");
+    }
+    s.probe = true;  // has_field(GR_F_COUNT) for the zero_fields check
+    const int rc = gr_virtual_create(c, sys, nzero, zero_fields, prescribed_v);
+    if (rc) {
+      s.probe = false;
+      throw GrError(rc, c->err);
+    }
+    s.mirror = mirror;
+  });
+}
+
 int gr_ghost_create(gr_ctx *c, int source_sys, double cutoff, int nb, const double *normals,
                     const double *points, int *ghost_out) {
   return guard(c, [&] {
@@ -2201,7 +2279,7 @@ int gr_interaction_create(gr_ctx *c, int kernel_kind, double h, int sys_a, int p
     const SysH &A = c->sys[sys_a];
     for (const gr_pfn_desc &f : in.pfns) {
       if (f.kind == GR_PFN_NONE) continue;
-      need(f.kind >= GR_PFN_FLUID && f.kind <= GR_PFN_XSPH, GR_ERR_ARGUMENT, "unknown functor kind");
+      need(f.kind >= GR_PFN_FLUID && f.kind <= GR_PFN_NCOUNT, GR_ERR_ARGUMENT, "unknown functor kind");
       need(f.kind != GR_PFN_XSPH, GR_ERR_ARGUMENT,
            "XSPHPfn runs as the interaction's velocity_adjust_pairwise_fn");
       if (f.kind == GR_PFN_FLUID)
@@ -2341,6 +2419,29 @@ int gr_integrator_dt(gr_ctx *c, int integ, double cfl, double *dt_out) {
     need(integ >= 0 && integ < (int)c->integ.size(), GR_ERR_ARGUMENT, "bad integrator");
     const IntegH &I = c->integ[integ];
     *dt_out = cfl * I.h / I.c;
+  });
+}
+
+int gr_integrator_set_probes(gr_ctx *c, int integ, int nprobes, const int *probes, int ninters,
+                             const int *inters) {
+  return guard(c, [&] {
+    need(integ >= 0 && integ < (int)c->integ.size() && !c->integ[integ].dead, GR_ERR_ARGUMENT,
+         "bad integrator");
+    std::vector<int> pr(probes, probes + nprobes), it(inters, inters + ninters);
+    for (int q : pr) {
+      validate_sys(c, q);
+      need(c->sys[q].probe, GR_ERR_ARGUMENT, "probes must be ProbeParticleSystems");
+    }
+    for (int t : it) {
+      need(t >= 0 && t < (int)c->inter.size() && !c->inter[t].dead, GR_ERR_ARGUMENT,
+           "bad interaction");
+      const InterH &in = c->inter[t];
+      need(in.b >= 0 && std::find(pr.begin(), pr.end(), in.b) != pr.end(), GR_ERR_ARGUMENT,
+           "a probe interaction's system_b must be one of the probes");
+    }
+    c->integ[integ].probes = pr;
+    c->integ[integ].probe_inter = it;
+    c->plans.clear();
   });
 }
 
@@ -2530,6 +2631,7 @@ void sweep_one(gr_ctx *c, int inter, const gr_pfn_desc f) {
         t.alias_off = src.off;
       }
       if (f.kind == GR_PFN_INTERP) t.imask = f.sigma | (f.alpha != 0.0 ? 1 << 30 : 0);
+      if (f.kind == GR_PFN_NCOUNT) t.imask = 1 << GR_F_COUNT;
       SrcDesc q = make_src(c, inter, target, b_side, pk);
       set_pfn(q, f, pk, c, in);
       t.src_begin = (int)sv.size();
@@ -2538,7 +2640,12 @@ void sweep_one(gr_ctx *c, int inter, const gr_pfn_desc f) {
       add_blocks(bv, (int)tv.size(), c->sys[target].cap, GR_BLOCK);
       tv.push_back(t);
     };
-    if (f.kind == GR_PFN_INTERP) {
+    if (f.kind == GR_PFN_NCOUNT) {
+      need(in.b >= 0 && c->sys[in.b].probe && !c->sys[in.a].virt && in.wrap == GR_WRAP_NONE,
+           GR_ERR_ARGUMENT,
+           "NeighborCountFn needs a real system_a and a ProbeParticleSystem system_b");
+      add(in.b, true);
+    } else if (f.kind == GR_PFN_INTERP) {
       need(in.b >= 0 && in.wrap == GR_WRAP_NONE && (c->sys[in.b].virt != c->sys[in.a].virt) &&
                c->sys[in.b].kind != GR_SYS_GHOST,
            GR_ERR_ARGUMENT, "InterpolateFieldFn needs a real and a virtual particle system");
@@ -2550,26 +2657,46 @@ void sweep_one(gr_ctx *c, int inter, const gr_pfn_desc f) {
       add(in.a, false);
       if (pk == P_TWO && in.b >= 0) add(in.b, true);
     }
-    DBuf dt, ds, db;
-    dt.upload(tv, c->stream);
+    // a ghost-target XSPH task adds into its source's entries, which the
+    // source's own task writes: it gets a launch of its own, after the rest
+    std::vector<std::vector<int>> groups(1);
+    for (int k = 0; k < (int)tv.size(); ++k) {
+      if (tv[k].alias >= 0)
+        groups.push_back({k});
+      else
+        groups[0].push_back(k);
+    }
+    (void)bv;
+    DBuf ds;
     ds.upload(sv, c->stream);
-    db.upload(bv, c->stream);
-    SweepArgs a;
-    a.sd = c->d_sd.as<SysDesc>();
-    a.n = c->d_n.as<int>();
-    a.tasks = dt.as<Task>();
-    a.srcs = ds.as<SrcDesc>();
-    a.grids = c->d_grids.as<Grid>();
-    a.carr = c->d_carr.as<CellArr>();
-    a.cells = c->cells.as<int>();
-    a.blk = db.as<Blk>();
-    a.nblk = (int)bv.size();
-    a.pair_count = c->d_pairs.as<unsigned long long>();
-    by_prec(c, [&](auto *tag) {
-      using R = std::remove_pointer_t<decltype(tag)>;
-      launch_sweep<R>(c->stream, pool_view<R>(c, c->cur), a, c->D, fam, false);
-    });
-    HIPCHK(hipStreamSynchronize(c->stream));
+    for (const auto &grp : groups) {
+      if (grp.empty()) continue;
+      std::vector<Task> tg;
+      std::vector<Blk> bg;
+      for (int k : grp) {
+        add_blocks(bg, (int)tg.size(), c->sys[tv[k].sys].cap, GR_BLOCK);
+        tg.push_back(tv[k]);
+      }
+      DBuf dt, db;
+      dt.upload(tg, c->stream);
+      db.upload(bg, c->stream);
+      SweepArgs a;
+      a.sd = c->d_sd.as<SysDesc>();
+      a.n = c->d_n.as<int>();
+      a.tasks = dt.as<Task>();
+      a.srcs = ds.as<SrcDesc>();
+      a.grids = c->d_grids.as<Grid>();
+      a.carr = c->d_carr.as<CellArr>();
+      a.cells = c->cells.as<int>();
+      a.blk = db.as<Blk>();
+      a.nblk = (int)bg.size();
+      a.pair_count = c->d_pairs.as<unsigned long long>();
+      by_prec(c, [&](auto *tag) {
+        using R = std::remove_pointer_t<decltype(tag)>;
+        launch_sweep<R>(c->stream, pool_view<R>(c, c->cur), a, c->D, fam, false);
+      });
+      HIPCHK(hipStreamSynchronize(c->stream));
+    }
 }
 }  // namespace
 
@@ -2626,6 +2753,67 @@ int gr_update_state(gr_ctx *c, int sys, int stage, double dt) {
       launch_updaters<R>(c->stream, pool_view<R>(c, c->cur), c->d_sd.as<SysDesc>(), c->d_st.as<StepSys>(),
                          db.as<Blk>(), (int)bv.size(), dr.as<int2>(), du.as<UpdDesc>(), c->D);
     });
+    HIPCHK(hipStreamSynchronize(c->stream));
+  });
+}
+
+int gr_measure_probes(gr_ctx *c, int integ) {
+  return guard(c, [&] {
+    need(integ >= 0 && integ < (int)c->integ.size() && !c->integ[integ].dead, GR_ERR_ARGUMENT,
+         "bad integrator");
+    const IntegH &I = c->integ[integ];
+    if (I.probes.empty()) return;
+    commit(c);
+    auto chk = [&](int rc) {
+      if (rc) throw GrError(rc, c->err);
+    };
+    const double cutoff = 2.0 * I.h;  // sort_cutoff (src/TimeIntegration.jl:612)
+    std::vector<int> sorted;
+    for (int it : I.probe_inter) {  // sources moved since the step's sort
+      const int a = c->inter[it].a;
+      if (std::find(sorted.begin(), sorted.end(), a) != sorted.end()) continue;
+      sorted.push_back(a);
+      chk(gr_sort(c, a, cutoff));
+    }
+    by_prec(c, [&](auto *tag) {
+      using R = std::remove_pointer_t<decltype(tag)>;
+      for (int q : I.probes)
+        if (c->sys[q].mirror >= 0)
+          launch_probe_mirror<R>(c->stream, pool_view<R>(c, c->cur), c->d_sd.as<SysDesc>(), q,
+                                 c->sys[q].mirror, c->sys[c->sys[q].mirror].n, c->D);
+    });
+    for (int q : I.probes) chk(gr_sort(c, q, cutoff));
+    for (int it : I.probe_inter) chk(gr_create_grid(c, it));
+    {  // auto_zero_probe! (src/ProbeParticles.jl:141-148): w_sum and the extras
+      std::vector<Blk> bp;
+      for (int q : I.probes) add_blocks(bp, q, c->sys[q].cap, GR_BLOCK);
+      DBuf db;
+      db.upload(bp, c->stream);
+      by_prec(c, [&](auto *tag) {
+        using R = std::remove_pointer_t<decltype(tag)>;
+        launch_virtual<R>(c->stream, pool_view<R>(c, c->cur), c->d_sd.as<SysDesc>(),
+                          c->d_st.as<StepSys>(), c->d_n.as<int>(), db.as<Blk>(),
+                          (int)bp.size(), 0, c->D);
+      });
+      HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    for (int it : I.probe_inter)
+      if (!c->inter[it].pfns.empty()) sweep_one(c, it, c->inter[it].pfns[0]);
+    for (int q : I.probes) chk(gr_update_state(c, q, 1, 0.0));
+    for (int q : I.probes) {  // _sort_probe_by_id!
+      const SysH &S = c->sys[q];
+      const int ncol = 2 * c->D + 4 + (S.kind >= GR_SYS_STRESS ? S.ns : 0);
+      const size_t bytes = (size_t)ncol * S.n * 8;
+      if (c->probe_scr.bytes < bytes) c->probe_scr.alloc(bytes);
+      by_prec(c, [&](auto *tag) {
+        using R = std::remove_pointer_t<decltype(tag)>;
+        const Pool<R> P = pool_view<R>(c, c->cur);
+        launch_probe_by_id<R>(c->stream, P, c->d_sd.as<SysDesc>(), q, S.n,
+                              c->probe_scr.as<double>(), 0, c->D);
+        launch_probe_by_id<R>(c->stream, P, c->d_sd.as<SysDesc>(), q, S.n,
+                              c->probe_scr.as<double>(), 1, c->D);
+      });
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
   });
 }

@@ -200,6 +200,7 @@ struct Pool {
   double *wsum;    // virtual systems: interpolation weight sums
   double *acc[4];  // RK4 accumulators of dvdt (D components) and drhodt
   double *vadj[3];  // v_adjustment (XSPH), swapped with the sort
+  double *pcnt;     // probe neighbour counts (NeighborCountFn)
   double *x[3], *v[3], *rho, *rho_h;
   R *vs[3], *rhos, *dvdt[3], *drhodt, *p;
   double *stress[GR_MAXNS];

@@ -1264,6 +1264,7 @@ __global__ void k_virtual(Pool<R> P, const SysDesc *sd, const StepSys *st, const
   }
   if (m & (1 << GR_F_DRHODT)) P.drhodt[g] = (R)0;
   if (m & (1 << GR_F_P)) P.p[g] = (R)0;
+  if (m & (1 << GR_F_COUNT)) P.pcnt[g] = 0.0;
 #pragma unroll
   for (int k = 0; k < GR_MAXNS; ++k) {
     if (k >= S.ns) continue;
@@ -1580,6 +1581,100 @@ template void launch_pack<float>(hipStream_t, const Pool<float> &, const SysDesc
                                  const Blk *, int, int);
 template void launch_pack<double>(hipStream_t, const Pool<double> &, const SysDesc *, const int *,
                                   const Blk *, int, int);
+
+// Probe slot id[i] − 1 takes the position of mirror particle i
+// (_measure_probes!, src/TimeIntegration.jl:469-479).
+template <typename R, int D>
+__global__ void k_probe_mirror(Pool<R> P, const SysDesc *sd, int probe, int src) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const SysDesc Sp = sd[probe], Ss = sd[src];
+  if (i >= Ss.cap) return;
+  const int q = P.id[Ss.off + i] - 1;
+  if (q < 0 || q >= Sp.cap) return;
+#pragma unroll
+  for (int d = 0; d < D; ++d) P.x[d][Sp.off + q] = P.x[d][Ss.off + i];
+}
+
+// _sort_probe_by_id! (src/ProbeParticles.jl:150-158): ids are 1..n, so the
+// sort is the scatter slot i -> id[i] − 1, done through a scratch copy:
+// mode 0 scatters the probe's fields into scr (ncol columns of n doubles),
+// mode 1 copies them back and refreshes the shadows.
+template <typename R, int D>
+__global__ void k_probe_by_id(Pool<R> P, const SysDesc *sd, int probe, double *scr, int mode) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const SysDesc S = sd[probe];
+  const int64_t n = S.cap;
+  if (i >= n) return;
+  const int64_t g = S.off + i;
+  const int ns = S.kind >= GR_SYS_STRESS ? S.ns : 0;
+  if (mode == 0) {
+    const int64_t q = (int64_t)P.id[g] - 1;
+    if (q < 0 || q >= n) return;
+    int col = 0;
+#pragma unroll
+    for (int d = 0; d < D; ++d) scr[(col++) * n + q] = P.x[d][g];
+#pragma unroll
+    for (int d = 0; d < D; ++d) scr[(col++) * n + q] = P.v[d][g];
+    scr[(col++) * n + q] = P.rho[g];
+    scr[(col++) * n + q] = P.wsum[g];
+    scr[(col++) * n + q] = P.pcnt[g];
+    scr[(col++) * n + q] = S.kind >= GR_SYS_FLUID ? (double)P.p[g] : 0.0;
+    for (int k = 0; k < ns; ++k) scr[(col++) * n + q] = P.stress[k][g];
+    return;
+  }
+  int col = 0;
+#pragma unroll
+  for (int d = 0; d < D; ++d) P.x[d][g] = scr[(col++) * n + i];
+#pragma unroll
+  for (int d = 0; d < D; ++d) {
+    const double v = scr[(col++) * n + i];
+    P.v[d][g] = v;
+    P.vs[d][g] = (R)v;
+  }
+  const double rho = scr[(col++) * n + i];
+  P.rho[g] = rho;
+  P.rho_h[g] = rho;
+  P.rhos[g] = (R)rho;
+  P.wsum[g] = scr[(col++) * n + i];
+  P.pcnt[g] = scr[(col++) * n + i];
+  const double p = scr[(col++) * n + i];
+  if (S.kind >= GR_SYS_FLUID) P.p[g] = (R)p;
+  for (int k = 0; k < ns; ++k) {
+    const double sg = scr[(col++) * n + i];
+    P.stress[k][g] = sg;
+    P.stress_s[k][g] = (R)sg;
+  }
+  P.id[g] = (int)(i + 1);
+}
+
+template <typename R>
+void launch_probe_mirror(hipStream_t s, const Pool<R> &P, const SysDesc *sd, int probe, int src,
+                         int64_t n_src, int D) {
+  if (n_src <= 0) return;
+  const unsigned nb = (unsigned)((n_src + GR_BLOCK - 1) / GR_BLOCK);
+  if (D == 2)
+    k_probe_mirror<R, 2><<<nb, GR_BLOCK, 0, s>>>(P, sd, probe, src);
+  else
+    k_probe_mirror<R, 3><<<nb, GR_BLOCK, 0, s>>>(P, sd, probe, src);
+}
+template <typename R>
+void launch_probe_by_id(hipStream_t s, const Pool<R> &P, const SysDesc *sd, int probe, int64_t n,
+                        double *scr, int mode, int D) {
+  if (n <= 0) return;
+  const unsigned nb = (unsigned)((n + GR_BLOCK - 1) / GR_BLOCK);
+  if (D == 2)
+    k_probe_by_id<R, 2><<<nb, GR_BLOCK, 0, s>>>(P, sd, probe, scr, mode);
+  else
+    k_probe_by_id<R, 3><<<nb, GR_BLOCK, 0, s>>>(P, sd, probe, scr, mode);
+}
+template void launch_probe_mirror<float>(hipStream_t, const Pool<float> &, const SysDesc *, int,
+                                         int, int64_t, int);
+template void launch_probe_mirror<double>(hipStream_t, const Pool<double> &, const SysDesc *, int,
+                                          int, int64_t, int);
+template void launch_probe_by_id<float>(hipStream_t, const Pool<float> &, const SysDesc *, int,
+                                        int64_t, double *, int, int);
+template void launch_probe_by_id<double>(hipStream_t, const Pool<double> &, const SysDesc *, int,
+                                         int64_t, double *, int, int);
 
 // A ghost-target XSPH task stores into its source's v_adjustment at the
 // ghost's index: flag (bit 4 of flags[1]) a ghost count above the source's.

@@ -111,5 +111,11 @@ template <typename R>
 void launch_nlist(hipStream_t s, const Pool<R> &P, const NlistArgs &a, int D);
 void launch_fill_i32(hipStream_t s, int *p, int v, long long n);
 void launch_alias_check(hipStream_t s, const int *n, int ghost, int src_n, int *flags);
+template <typename R>
+void launch_probe_mirror(hipStream_t s, const Pool<R> &P, const SysDesc *sd, int probe, int src,
+                         int64_t n_src, int D);
+template <typename R>
+void launch_probe_by_id(hipStream_t s, const Pool<R> &P, const SysDesc *sd, int probe, int64_t n,
+                        double *scr, int mode, int D);
 
 }  // namespace gr

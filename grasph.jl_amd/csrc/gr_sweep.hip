@@ -220,6 +220,10 @@ struct Gather {
       adr.add((R)1);
       return;
     }
+    if (FAM == FAM_INTERP && Q.pfn == GR_PFN_NCOUNT) {  // NeighborCountFn (:443-460)
+      avo[2].add((R)1);
+      return;
+    }
     if (FAM == FAM_INTERP) {  // InterpolateFieldFn (src/PairwiseFunctors.jl:424-434)
       // W from the FP64 distance in every mode: an accepted pair (r < 2h in
       // FP64) must weigh W > 0 as in the reference, or a virtual particle
@@ -435,6 +439,7 @@ struct Gather {
           }
       }
       if (m & (1 << 30)) P.wsum[gi] += (double)avo[1].get();
+      if (m & (1 << GR_F_COUNT)) P.pcnt[gi] += (double)avo[2].get();
       return;
     }
     if (FAM == FAM_XSPH) {

@@ -100,6 +100,8 @@ def time_integrate(integrator, num_timesteps, print_interval_step, save_interval
         g = step_offset + done
         printing = print_interval_step > 0 and g % print_interval_step == 0
         saving = output_prefix is not None and save_interval_step > 0 and g % save_interval_step == 0
+        if saving and getattr(lf, "probes", ()):
+            sc.measure_probes(lf)  # _maybe_save! (src/TimeIntegration.jl:525-526)
         if printing or saving:
             sc.download_all(lf)
             if printing:
@@ -107,7 +109,8 @@ def time_integrate(integrator, num_timesteps, print_interval_step, save_interval
                     _print_summary(ps, g, g * dt)
             if saving:
                 SN.save_snapshot(SN.snapshot_path(output_prefix, g, width), lf.systems,
-                                 lf.ghosts, g, dt, lf.virtual_systems)
+                                 lf.ghosts, g, dt, lf.virtual_systems,
+                                 getattr(lf, "probes", ()))
     sc.download_all(lf)
     el = time.perf_counter() - t0
     res = {"seconds": el, "steps": num_timesteps, "dt": dt,

@@ -531,8 +531,20 @@ class InterpolateFieldFn:
                 math.nan, math.nan)
 
 
+class NeighborCountFn:
+    """NeighborCountFn(:field) (src/PairwiseFunctors.jl:443-460): with a
+    (real, ProbeParticleSystem) interaction, probe.field[j] += 1 for every
+    source particle within the kernel support of probe j."""
+
+    def __init__(self, field):
+        self.field = str(field)
+
+    def encode(self):
+        return (A.GR_PFN_NCOUNT, 1 << A.GR_F_COUNT, 0.0, 0.0, 0.0, math.nan, math.nan)
+
+
 _PFN_TYPES = (FluidPfn, CauchyFluidPfn, StrainRatePfn, StrainRateVorticityPfn, NeighborCounterPfn,
-              InterpolateFieldFn)
+              InterpolateFieldFn, NeighborCountFn)
 
 
 def encode_pfn(p):
@@ -556,6 +568,81 @@ class XSPHPfn:
         self.epsilon = float(epsilon)
 
 
+# Fields a probe's extras can hold on the device: the interpolation targets
+# map to the system's own arrays, any other name is a NeighborCountFn count.
+PROBE_INTERP_FIELDS = ("v", "rho", "p", "stress")
+
+
+class ProbeParticleSystem:
+    """ProbeParticleSystem(name, positions | mirror_target; extras,
+    state_updater, prescribed_v, dtype) (src/ProbeParticles.jl:22-89).
+
+    A passive observer measured at save cadence: it owns x, id (1..n), w_sum
+    and the `extras` arrays (a dict name → array of length n); any other
+    attribute is forwarded to `mirror_target` when there is one.  Its
+    positions advance by prescribed_v·dt every step."""
+
+    def __init__(self, name, target, extras=None, state_updater=(), prescribed_v=None,
+                 dtype=np.float64):
+        extras = dict(extras or {})
+        if isinstance(target, _ParticleSystem):
+            mirror = target
+            x = np.array(target.x, dtype=np.float64, copy=True)
+        else:
+            mirror = None
+            pos = [np.asarray(p, dtype=dtype) for p in target]
+            if not pos:
+                raise ValueError("positions must not be empty")
+            x = np.stack(pos).astype(dtype)
+        n, nd = x.shape
+        ext = {}
+        for k, a in extras.items():
+            a = np.asarray(a)
+            if len(a) != n:
+                raise ValueError(f"extras.{k} has length {len(a)} but n={n}")
+            ext[k] = a
+        pv = np.zeros(nd, dtype=dtype) if prescribed_v is None else np.asarray(
+            prescribed_v, dtype=dtype).reshape(nd)
+        self.__dict__.update(
+            name=str(name), n=int(n), ndims=int(nd), x=x, id=np.arange(1, n + 1, dtype=np.int64),
+            w_sum=np.zeros(n, dtype=dtype), extras=ext,
+            state_updater=_normalise_updaters(state_updater), prescribed_v=pv,
+            mirror_target=mirror)
+
+    def __getattr__(self, k):  # getproperty (src/ProbeParticles.jl:95-104)
+        d = self.__dict__
+        if k in d.get("extras", {}):
+            return d["extras"][k]
+        if d.get("mirror_target") is not None:
+            return getattr(d["mirror_target"], k)
+        raise AttributeError(f"ProbeParticleSystem has no field {k}")
+
+    @property
+    def kind(self):
+        if "stress" in self.extras:
+            return A.GR_SYS_STRESS
+        if "p" in self.extras:
+            return A.GR_SYS_FLUID
+        return A.GR_SYS_BASIC
+
+    @property
+    def ns(self):
+        return int(self.extras["stress"].shape[1]) if "stress" in self.extras else 0
+
+    def count_field(self):
+        """The extras name stored in the device's count slot (None if none)."""
+        other = [k for k in self.extras if k not in PROBE_INTERP_FIELDS]
+        if len(other) > 1:
+            raise ValueError(f"probe {self.name}: the device holds one NeighborCountFn field, "
+                             f"got extras {other}")
+        return other[0] if other else None
+
+    def device_fields(self):
+        """(extras name, GR_F_* id) of every extras array."""
+        cf = self.count_field()
+        return [(k, A.GR_F_COUNT if k == cf else A.FIELD_IDS[k]) for k in self.extras]
+
+
 class SystemInteraction:
     """SystemInteraction(kernel, pairwise_fn, system_a[, system_b];
     velocity_adjust_pairwise_fn=nothing)."""
@@ -575,6 +662,15 @@ class SystemInteraction:
         self.pfns = pairwise_fn if isinstance(pairwise_fn, tuple) else (pairwise_fn,)
         for p in self.pfns:
             encode_pfn(p)
+            if isinstance(p, NeighborCountFn):
+                if not isinstance(system_b, ProbeParticleSystem):
+                    raise ValueError("NeighborCountFn needs a ProbeParticleSystem system_b")
+                if p.field not in system_b.extras or system_b.count_field() != p.field:
+                    raise ValueError(f"probe {system_b.name} has no count field :{p.field}")
+            if isinstance(p, InterpolateFieldFn) and isinstance(system_b, ProbeParticleSystem):
+                for f in p.fields:
+                    if f not in system_b.extras:
+                        raise ValueError(f"probe {system_b.name} has no extras field :{f}")
         self.system_a = system_a
         self.system_b = system_b
         self.ndims = nd
@@ -594,7 +690,12 @@ class LeapFrogTimeIntegrator:
 
     scheme = A.GR_SCHEME_LEAPFROG
 
-    def __init__(self, systems, interactions, ghosts=(), gamma=0.0, virtual_systems=()):
+    def __init__(self, systems, interactions, ghosts=(), gamma=0.0, virtual_systems=(),
+                 probes=(), probe_interactions=()):
+        self.probes = ((probes,) if isinstance(probes, ProbeParticleSystem) else tuple(probes))
+        self.probe_interactions = ((probe_interactions,)
+                                   if isinstance(probe_interactions, SystemInteraction)
+                                   else tuple(probe_interactions))
         sys = (systems,) if isinstance(systems, _ParticleSystem) else tuple(systems)
         ints = (interactions,) if isinstance(interactions, SystemInteraction) else tuple(interactions)
         gsts = (ghosts,) if isinstance(ghosts, GhostEntry) else tuple(ghosts)

@@ -48,6 +48,8 @@ class DeviceScene:
             ps = ps.inner
         if isinstance(ps, M.VirtualParticleSystem):
             return self._virtual(ps)
+        if isinstance(ps, M.ProbeParticleSystem):
+            return self._probe(ps)
         k = id(ps)
         if k in self._sys:
             return self._sys[k]
@@ -96,6 +98,32 @@ class DeviceScene:
         self._sys[k] = h
         self._virt[k] = vps
         self._objs.extend([vps, src])
+        return h
+
+    def _probe(self, pr):
+        k = id(pr)
+        if k in self._sys:
+            return self._sys[k]
+        if pr.ndims != self.ndims:
+            raise ValueError("probe dimension does not match the context")
+        mt = pr.mirror_target
+        mirror = self.system(mt) if mt is not None else -1
+        mass = mt.mass if mt is not None else 0.0
+        c = mt.c if mt is not None else 0.0
+        h = self.be.system_create(pr.kind, pr.n, pr.ns, mass, c, np.zeros(pr.ndims), 0.0)
+        self.be.probe_create(h, pr.prescribed_v, mirror, [f for _, f in pr.device_fields()])
+        for stage, entry in enumerate(pr.state_updater, start=1):
+            items = entry if isinstance(entry, tuple) else (entry,)
+            if not items:
+                self.be.system_add_updater(h, stage, A.GR_UPD_NOTHING, [])
+            for u in items:
+                if u is None:
+                    self.be.system_add_updater(h, stage, A.GR_UPD_NOTHING, [])
+                else:
+                    kind, prm = u.encode()
+                    self.be.system_add_updater(h, stage, kind, prm)
+        self._sys[k] = h
+        self._objs.append(pr)
         return h
 
     def ghost(self, entry):
@@ -159,6 +187,9 @@ class DeviceScene:
         h = self.be.integrator_create(sys, its, gh, lf.gamma)
         if getattr(lf, "scheme", A.GR_SCHEME_LEAPFROG) != A.GR_SCHEME_LEAPFROG:
             self.be.integrator_set_scheme(h, lf.scheme)
+        if getattr(lf, "probes", ()):
+            self.be.integrator_set_probes(h, [self._probe(p) for p in lf.probes],
+                                          [self.interaction(si) for si in lf.probe_interactions])
         self._integ[k] = h
         self._objs.append(lf)
         return h
@@ -184,6 +215,14 @@ class DeviceScene:
         drop |= {id(ge) for ge in lf.ghosts} | {id(ge.ghost) for ge in lf.ghosts}
         self._objs = [o for o in self._objs if id(o) not in drop]
 
+    def measure_probes(self, lf):
+        """_measure_probes! on the context, then the probes' arrays back to
+        the host (the sources were re-sorted: download them with the rest)."""
+        h = self.integrator(lf)
+        self.be.measure_probes(h)
+        for p in lf.probes:
+            self.download(p)
+
     # ---- host <-> context -------------------------------------------------
     def upload(self, ps, fields=None):
         if isinstance(ps, (M.StaticBoundarySystem, M.DynamicBoundarySystem)):
@@ -191,6 +230,13 @@ class DeviceScene:
         if isinstance(ps, M.VirtualParticleSystem):
             self._virtual(ps)
             ps = ps.source
+        if isinstance(ps, M.ProbeParticleSystem):
+            h = self._probe(ps)
+            self.be.system_upload(h, A.GR_F_X, ps.x)
+            for name, fid in ps.device_fields():
+                self.be.system_upload(h, fid, np.asarray(ps.extras[name], dtype=np.float64))
+            self.be.system_upload_index(h, A.GR_I_ID, ps.id)
+            return
         if isinstance(ps, M.GhostParticleSystem):
             h = self.ghost_handle(ps)
             names = fields or ["x", "v", "rho"] + list(ps.extras)
@@ -210,6 +256,16 @@ class DeviceScene:
             h = self._virtual(ps)
             ps.w_sum[:] = self.be.system_download(h, A.GR_F_W_SUM, ps.n)
             ps = ps.source
+        if isinstance(ps, M.ProbeParticleSystem):
+            h = self._probe(ps)
+            ps.x[...] = self.be.system_download(h, A.GR_F_X, ps.n * ps.ndims).reshape(ps.x.shape)
+            ps.w_sum[:] = self.be.system_download(h, A.GR_F_W_SUM, ps.n)
+            for name, fid in ps.device_fields():
+                arr = ps.extras[name]
+                comps = int(np.prod(arr.shape[1:])) if arr.ndim > 1 else 1
+                arr[...] = self.be.system_download(h, fid, ps.n * comps).reshape(arr.shape)
+            ps.id[:] = self.be.system_download_index(h, A.GR_I_ID, ps.n)
+            return
         if isinstance(ps, M.GhostParticleSystem):
             h = self.ghost_handle(ps)
             n = self.be.system_get_n(h)
@@ -231,11 +287,13 @@ class DeviceScene:
 
     def upload_all(self, lf):
         self.integrator(lf)
-        for ps in list(lf.systems) + list(getattr(lf, "virtual_systems", ())):
+        for ps in (list(lf.systems) + list(getattr(lf, "virtual_systems", ()))
+                   + list(getattr(lf, "probes", ()))):
             self.upload(ps)
 
     def download_all(self, lf):
-        for ps in list(lf.systems) + list(getattr(lf, "virtual_systems", ())):
+        for ps in (list(lf.systems) + list(getattr(lf, "virtual_systems", ()))
+                   + list(getattr(lf, "probes", ()))):
             self.download(ps)
         for ge in lf.ghosts:
             self.download(ge.ghost)

@@ -212,8 +212,18 @@ def _inner(ps):
 
 def write_group(loc, ps):
     """_write_to_group! (src/Particles.jl:721-735) / ghost write_h5
-    (src/GhostParticles.jl:434-459)."""
+    (src/GhostParticles.jl:434-459) / probe write_h5 (src/ProbeParticles.jl:
+    166-185: attrs n, ndims; x, id and the extras, no w_sum)."""
     ps = _inner(ps)
+    if isinstance(ps, M.ProbeParticleSystem):
+        _attr_write(loc, "n", ps.n)
+        _attr_write(loc, "ndims", ps.ndims)
+        if ps.n > 0:
+            _dset_write(loc, "x", ps.x)
+            _dset_write(loc, "id", ps.id)
+            for k, a in ps.extras.items():
+                _dset_write(loc, k, a)
+        return
     if isinstance(ps, M.GhostParticleSystem):
         _attr_write(loc, "n", ps.n)
         _attr_write(loc, "ndims", ps.ndims)
@@ -234,6 +244,17 @@ def write_group(loc, ps):
 def read_group(loc, ps):
     """_read_from_group! (src/Particles.jl:759-787)."""
     ps = _inner(ps)
+    if isinstance(ps, M.ProbeParticleSystem):  # read_h5! (src/ProbeParticles.jl:187-204)
+        n = int(_attr_read(loc, "n", int))
+        if n != ps.n:
+            raise ValueError(f"Saved probe count ({n}) does not match probe.n={ps.n}")
+        if _exists(loc, "x"):
+            ps.x[...] = _dset_read(loc, "x").reshape(ps.x.shape)
+        ps.id[:] = np.arange(1, ps.n + 1)
+        for k, a in ps.extras.items():
+            if _exists(loc, k):
+                a[...] = _dset_read(loc, k).reshape(a.shape)
+        return ps
     n = int(_attr_read(loc, "n", int))
     nd = int(_attr_read(loc, "ndims", int))
     if n != ps.n:
@@ -272,10 +293,10 @@ def snapshot_path(prefix, step, width):
     return f"{prefix}_{str(step).rjust(width, '0')}.h5"
 
 
-def save_snapshot(path, systems, ghosts, step, dt, virtual_systems=()):
+def save_snapshot(path, systems, ghosts, step, dt, virtual_systems=(), probes=()):
     """The body of _maybe_save! (src/TimeIntegration.jl:526-547): file attrs
-    step and sim_time, one group per system, then one per ghost system, then
-    one per virtual system (named after the virtual system)."""
+    step and sim_time, one group per system, then one per ghost system, one
+    per virtual system (named after the virtual system), one per probe."""
     d = os.path.dirname(str(path))
     if d:
         Path(d).mkdir(parents=True, exist_ok=True)
@@ -284,7 +305,8 @@ def save_snapshot(path, systems, ghosts, step, dt, virtual_systems=()):
     try:
         _attr_write(f, "step", int(step))
         _attr_write(f, "sim_time", float(step * dt))
-        for ps in list(systems) + [ge.ghost for ge in ghosts] + list(virtual_systems):
+        for ps in (list(systems) + [ge.ghost for ge in ghosts] + list(virtual_systems)
+                   + list(probes)):
             name = ps.name
             ps = _inner(ps)
             g = _ok(h.H5Gcreate2(f, _bytes(name), _P_DEFAULT, _P_DEFAULT, _P_DEFAULT),

@@ -61,8 +61,9 @@ enum {
   GR_F_P = 5, GR_F_STRESS = 6, GR_F_STRAIN_RATE = 7, GR_F_VORTICITY = 8,
   GR_F_STRAIN = 9, GR_F_STRAIN_P = 10, GR_F_V_ADJUSTMENT = 11,
   GR_F_NORMALS = 12,
-  GR_F_W_SUM = 13,  /* VirtualParticleSystem w_sum (download only) */
-  GR_NUM_FIELDS = 14
+  GR_F_W_SUM = 13,  /* VirtualParticleSystem / probe w_sum (download only) */
+  GR_F_COUNT = 14,  /* probe neighbour count (NeighborCountFn's target field) */
+  GR_NUM_FIELDS = 15
 };
 /* ---- per-particle index fields (int64, 1-based) ----------------------- */
 enum { GR_I_ID = 0, GR_I_IDX_ORIGINAL = 1, GR_I_IDX_BOUNDARY = 2 };
@@ -85,8 +86,10 @@ enum {
                          f += (m/ρ)·W·f_real; `sigma` holds the field bit
                          mask (1<<GR_F_V | 1<<GR_F_RHO | 1<<GR_F_P |
                          1<<GR_F_STRESS), `alpha` != 0 accumulates w_sum  */
-  GR_PFN_XSPH = 7     /* XSPHPfn :360-390, only as an interaction's
+  GR_PFN_XSPH = 7,    /* XSPHPfn :360-390, only as an interaction's
                          velocity-adjust functor (gr_interaction_set_vadjust) */
+  GR_PFN_NCOUNT = 8   /* NeighborCountFn :443-460: system_b (a probe) counts
+                         its system_a partners into GR_F_COUNT              */
 };
 
 /* ---- partner wrappers (src/BoundaryParticles.jl:38,68) ---------------- */
@@ -173,6 +176,16 @@ int gr_system_get_n(gr_ctx *ctx, int sys, int64_t *n_out);
  * prescribed_v·dt (ndims doubles, may be NULL for zero), and as an
  * interaction partner it is one-sided (src/PairwiseFunctors.jl:286,97,193).
  * Pass it in gr_integrator_create's `systems`; it is excluded from dt's c. */
+/* Turn system `sys` into a ProbeParticleSystem (src/ProbeParticles.jl:
+ * 22-89): a target of InterpolateFieldFn / NeighborCountFn interactions
+ * that is measured only by gr_measure_probes.  `zero_fields` are its
+ * extras (zeroed with w_sum before each measurement, auto_zero_probe!
+ * :141-148); GR_F_COUNT holds a NeighborCountFn field.  `mirror` >= 0: the
+ * real system whose positions the probe takes at each measurement
+ * (mirror_target; same n).  Positions advance by prescribed_v·dt every step
+ * (_advance_probe_positions!, src/TimeIntegration.jl:439-450). */
+int gr_probe_create(gr_ctx *ctx, int sys, const double *prescribed_v, int mirror,
+                    int nzero, const int *zero_fields);
 int gr_virtual_create(gr_ctx *ctx, int sys, int nzero, const int *zero_fields,
                       const double *prescribed_v);
 
@@ -234,6 +247,15 @@ int gr_integrator_set_scheme(gr_ctx *ctx, int integ, int scheme);
 /* dt = CFL·h/c with h = min kernel h, c = max system c
  * (src/TimeIntegration.jl:63-64,598). */
 int gr_integrator_dt(gr_ctx *ctx, int integ, double cfl, double *dt_out);
+/* probes / probe_interactions of an integrator (src/TimeIntegration.jl:53,
+ * 98): each probe interaction's system_b is one of the probes. */
+int gr_integrator_set_probes(gr_ctx *ctx, int integ, int nprobes, const int *probes,
+                             int ninters, const int *inters);
+/* _measure_probes! (src/TimeIntegration.jl:454-502): re-sort the probe
+ * interactions' sources, mirror probe positions, sort the probes by cell,
+ * build the grids, zero w_sum and the extras, sweep stage 1 of each probe
+ * interaction, run the probes' stage-1 updaters, sort the probes by id. */
+int gr_measure_probes(gr_ctx *ctx, int integ);
 /* Override the step size of one integrator (ensembles: per-sim dt). */
 int gr_integrator_set_dt(gr_ctx *ctx, int integ, double dt);
 /* Advance one integrator `nsteps` leapfrog steps

@@ -59,6 +59,11 @@ struct Sys {
   std::vector<double> wsum;
   int64_t zero_mask = 0;
   double pv[3] = {0, 0, 0};
+  // ProbeParticleSystem (src/ProbeParticles.jl:22-89): a virtual-like
+  // target measured at save cadence; cnt holds NeighborCountFn counts
+  bool probe = false;
+  int mirror = -1;
+  std::vector<double> cnt;
 };
 
 struct Inter {
@@ -77,7 +82,7 @@ struct Inter {
 };
 
 struct Integ {
-  std::vector<int> sys, inter, ghost, virt;
+  std::vector<int> sys, inter, ghost, virt, probes, probe_inter;
   double gamma = 0, h = 0, c = 0;
   int nst = 0;
   int scheme = GR_SCHEME_LEAPFROG;
@@ -438,6 +443,9 @@ void apply_pair(const PairCtx &P, int64_t i, int64_t j, const double *dx,
       }
       return;
     }
+    case GR_PFN_NCOUNT:  // NeighborCountFn (src/PairwiseFunctors.jl:443-460)
+      if (P.interp_side == 1) P.Vw->cnt[j] += 1.0;
+      return;
     case GR_PFN_COUNT: {
       P.Aw->dvdt[i * D] += 1.0;
       P.Aw->drhodt[i] += 1.0;
@@ -729,6 +737,10 @@ void sort_system(orc_scene *sc, Sys &s, double cutoff) {
     if (s.ex_stress) permute(s.stress, perm, sc->sys[s.src].NS);
     return;
   }
+  if (s.probe) {  // _particle_arrays(::ProbeParticleSystem) (src/ProbeParticles.jl:133-139)
+    permute(s.wsum, perm, 1);
+    permute(s.cnt, perm, 1);
+  }
   // _particle_arrays :15-26
   permute(s.x, perm, D);
   permute(s.id, perm, 1);
@@ -884,6 +896,13 @@ int sweep_fn(orc_scene *sc, Inter &in, const gr_pfn_desc &f) {
                     "(ps_b.v_adjustment[j] would index past the source's array)");
       P.Bw = &src;
     }
+  }
+  if (f.kind == GR_PFN_NCOUNT) {
+    if (!(in.b >= 0 && sc->sys[in.b].probe && !sc->sys[in.a].virt && in.wrap == GR_WRAP_NONE))
+      return fail(sc, GR_ERR_ARGUMENT,
+                  "NeighborCountFn needs a real system_a and a ProbeParticleSystem system_b");
+    P.interp_side = 1;
+    P.Vw = &sc->sys[in.b];
   }
   if (f.kind == GR_PFN_INTERP) {
     if (in.b >= 0 && sc->sys[in.b].virt && in.wrap == GR_WRAP_NONE) {
@@ -1149,6 +1168,7 @@ void zero_fields(Sys &s, int64_t mask) {
   if (mask & (1 << GR_F_STRESS)) std::fill(s.stress.begin(), s.stress.end(), 0.0);
   if (mask & (1 << GR_F_STRAIN)) std::fill(s.strain.begin(), s.strain.end(), 0.0);
   if (mask & (1 << GR_F_STRAIN_P)) std::fill(s.strainp.begin(), s.strainp.end(), 0.0);
+  if (mask & (1 << GR_F_COUNT)) std::fill(s.cnt.begin(), s.cnt.end(), 0.0);
 }
 
 int update_state(orc_scene *sc, Sys &s, int stage, double dt) {
@@ -1372,6 +1392,78 @@ void advance_virtual_positions(orc_scene *sc, Integ &ig, double dt) {  // :389-3
   }
 }
 
+// _advance_probe_positions! (src/TimeIntegration.jl:439-450): x += pv·dt
+// when the prescribed velocity is nonzero.
+void advance_probe_positions(orc_scene *sc, Integ &ig, double dt) {
+  for (int q : ig.probes) {
+    Sys &s = sc->sys[q];
+    const int D = s.D;
+    bool nz = false;
+    for (int d = 0; d < D; ++d) nz = nz || s.pv[d] != 0.0;
+    if (!nz) continue;
+    for (int64_t i = 0; i < s.n; ++i)
+      for (int d = 0; d < D; ++d) s.x[i * D + d] = s.x[i * D + d] + s.pv[d] * dt;
+  }
+}
+
+// _measure_probes! (src/TimeIntegration.jl:454-502): re-sort the probe
+// interactions' sources, mirror probe positions from their targets, sort the
+// probes by cell, build the grids, zero w_sum and the extras, sweep stage 1,
+// run the probes' updaters, sort the probes back by id.
+int measure_probes(orc_scene *sc, Integ &ig) {
+  const double cutoff = 2.0 * ig.h;
+  std::vector<int> sorted;
+  for (int it : ig.probe_inter) {
+    const int a = sc->inter[it].a;
+    if (std::find(sorted.begin(), sorted.end(), a) != sorted.end()) continue;
+    sorted.push_back(a);
+    sort_system(sc, sc->sys[a], cutoff);
+  }
+  for (int q : ig.probes) {
+    Sys &pr = sc->sys[q];
+    if (pr.mirror < 0) continue;
+    const Sys &m = sc->sys[pr.mirror];
+    const int D = pr.D;
+    for (int64_t i = 0; i < pr.n; ++i)
+      for (int d = 0; d < D; ++d) pr.x[(m.id[i] - 1) * D + d] = m.x[i * D + d];
+  }
+  for (int q : ig.probes) sort_system(sc, sc->sys[q], cutoff);
+  for (int it : ig.probe_inter) {
+    int rc = create_grid(sc, sc->inter[it]);
+    if (rc) return rc;
+  }
+  for (int q : ig.probes) {  // auto_zero_probe! (src/ProbeParticles.jl:141-148)
+    Sys &pr = sc->sys[q];
+    std::fill(pr.wsum.begin(), pr.wsum.end(), 0.0);
+    zero_fields(pr, pr.zero_mask);
+  }
+  for (int it : ig.probe_inter) {
+    int rc = sweep(sc, sc->inter[it], 1);
+    if (rc) return rc;
+  }
+  for (int q : ig.probes) {
+    int rc = update_state(sc, sc->sys[q], 1, 0.0);
+    if (rc) return rc;
+  }
+  for (int q : ig.probes) {  // _sort_probe_by_id! (:150-158)
+    Sys &pr = sc->sys[q];
+    std::vector<int64_t> perm(pr.n);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(),
+                     [&](int64_t a, int64_t b) { return pr.id[a] < pr.id[b]; });
+    const int D = pr.D;
+    permute(pr.x, perm, D);
+    permute(pr.id, perm, 1);
+    permute(pr.wsum, perm, 1);
+    permute(pr.cnt, perm, 1);
+    permute(pr.v, perm, D);
+    permute(pr.rho, perm, 1);
+    if (!pr.p.empty()) permute(pr.p, perm, 1);
+    if (!pr.stress.empty()) permute(pr.stress, perm, pr.NS);
+  }
+  return GR_OK;
+}
+
 // _xsph_correction! (src/TimeIntegration.jl:423-436): v −= v_adjustment,
 // v_adjustment ← 0, adjust_v! of every interaction, v += v_adjustment.
 int xsph_correction(orc_scene *sc, Integ &ig) {
@@ -1464,6 +1556,7 @@ int step_rk4(orc_scene *sc, Integ &ig, double dt) {
     for (size_t k = 0; k < s.x.size(); ++k) s.x[k] = s.x[k] + dt * s.v[k];
   }
   advance_virtual_positions(sc, ig, dt);
+  advance_probe_positions(sc, ig, dt);
   return GR_OK;
 }
 
@@ -1551,6 +1644,7 @@ int step(orc_scene *sc, Integ &ig, double dt) {
     for (int64_t i = 0; i < s.n; ++i)
       for (int d = 0; d < D; ++d) s.x[i * D + d] = s.x[i * D + d] + s.pv[d] * dt;
   }
+  advance_probe_positions(sc, ig, dt);
   return GR_OK;
 }
 
@@ -1569,6 +1663,7 @@ int ncomp(const orc_scene *sc, const Sys &s, int field) {
     case GR_F_DRHODT:
     case GR_F_P:
     case GR_F_W_SUM:
+    case GR_F_COUNT:
       return 1;
     case GR_F_STRESS:
     case GR_F_STRAIN_RATE:
@@ -1598,6 +1693,7 @@ std::vector<double> *field_ptr(Sys &s, int field) {
     case GR_F_V_ADJUSTMENT: return s.kind == GR_SYS_GHOST ? nullptr : &s.vadj;
     case GR_F_NORMALS: return s.kind == GR_SYS_GHOST ? &s.normals : nullptr;
     case GR_F_W_SUM: return s.virt ? &s.wsum : nullptr;
+    case GR_F_COUNT: return s.probe ? &s.cnt : nullptr;
   }
   return nullptr;
 }
@@ -1690,6 +1786,52 @@ int orc_virtual_create(orc_scene *s, int sys, int nzero, const int *zero_fields,
   for (int d = 0; d < y.D; ++d) y.pv[d] = prescribed_v ? prescribed_v[d] : 0.0;
   y.upd.clear();  // the virtual system's own updaters follow
   return GR_OK;
+}
+
+int orc_probe_create(orc_scene *s, int sys, const double *prescribed_v, int mirror, int nzero,
+                     const int *zero_fields) {
+  if (sys < 0 || sys >= (int)s->sys.size()) return fail(s, GR_ERR_ARGUMENT, "bad system");
+  Sys &y = s->sys[sys];
+  if (y.kind == GR_SYS_GHOST || y.virt) return fail(s, GR_ERR_ARGUMENT, "not a plain particle system");
+  if (mirror >= 0) {
+    if (mirror >= (int)s->sys.size() || s->sys[mirror].kind == GR_SYS_GHOST || s->sys[mirror].virt)
+      return fail(s, GR_ERR_ARGUMENT, "mirror_target must be a real particle system");
+    if (s->sys[mirror].n != y.n)
+      return fail(s, GR_ERR_ARGUMENT, "a mirroring probe has one slot per target particle");
+  }
+  y.probe = true;
+  y.mirror = mirror;
+  y.cnt.assign(y.n, 0.0);
+  int rc = orc_virtual_create(s, sys, nzero, zero_fields, prescribed_v);
+  if (rc) {
+    y.probe = false;
+    return rc;
+  }
+  return GR_OK;
+}
+
+int orc_integrator_set_probes(orc_scene *s, int integ, int nprobes, const int *probes, int ninters,
+                              const int *inters) {
+  if (integ < 0 || integ >= (int)s->integ.size()) return fail(s, GR_ERR_ARGUMENT, "bad integrator");
+  Integ &ig = s->integ[integ];
+  std::vector<int> pr(probes, probes + nprobes), it(inters, inters + ninters);
+  for (int q : pr)
+    if (q < 0 || q >= (int)s->sys.size() || !s->sys[q].probe)
+      return fail(s, GR_ERR_ARGUMENT, "probes must be ProbeParticleSystems");
+  for (int t : it) {
+    if (t < 0 || t >= (int)s->inter.size()) return fail(s, GR_ERR_ARGUMENT, "bad interaction");
+    const Inter &in = s->inter[t];
+    if (in.b < 0 || std::find(pr.begin(), pr.end(), in.b) == pr.end())
+      return fail(s, GR_ERR_ARGUMENT, "a probe interaction's system_b must be one of the probes");
+  }
+  ig.probes = pr;
+  ig.probe_inter = it;
+  return GR_OK;
+}
+
+int orc_measure_probes(orc_scene *s, int integ) {
+  if (integ < 0 || integ >= (int)s->integ.size()) return fail(s, GR_ERR_ARGUMENT, "bad integrator");
+  return measure_probes(s, s->integ[integ]);
 }
 
 int orc_system_add_updater(orc_scene *s, int sys, int stage, int kind,

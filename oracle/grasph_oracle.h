@@ -69,6 +69,11 @@ int orc_sort(orc_scene *s, int sys, double cutoff);
 int orc_create_grid(orc_scene *s, int inter);
 int orc_sweep(orc_scene *s, int inter, int stage);
 int orc_adjust_v(orc_scene *s, int inter);
+int orc_probe_create(orc_scene *s, int sys, const double *prescribed_v, int mirror, int nzero,
+                     const int *zero_fields);
+int orc_integrator_set_probes(orc_scene *s, int integ, int nprobes, const int *probes, int ninters,
+                              const int *inters);
+int orc_measure_probes(orc_scene *s, int integ);
 int orc_update_state(orc_scene *s, int sys, int stage, double dt);
 int orc_generate_ghosts(orc_scene *s, int ghost);
 int orc_update_ghost_kinematics(orc_scene *s, int ghost);

@@ -401,10 +401,18 @@ STEP_CASES = [
 ]
 
 
+# f64 state tolerance per case: the XSPH scenes sum the velocity adjustment in
+# another order than the oracle's pairwise sweep, and over ten steps of the
+# violent multiphase flow the last-bit differences grow (and flip pairs at the
+# edge of the support); a single XSPH step is held to 1e-12 below
+F64_TOL = {"bubble3_2d_rk4": 1e-7, "bubble3_xy_xsph_2d_leapfrog": 1e-7}
+
+
 @pytest.mark.parametrize("name,builder,kw,D,nsteps", STEP_CASES, ids=[c[0] for c in STEP_CASES])
 def test_steps_f64(name, builder, kw, D, nsteps):
     dev = _run_steps(builder, kw, D, "f64", nsteps, "dev")
     orc = _run_steps(builder, kw, D, "f64", nsteps, "orc")
+    base = F64_TOL.get(name, 1e-10)
     for key, ref in orc.items():
         if key.endswith(".id"):
             continue
@@ -412,8 +420,21 @@ def test_steps_f64(name, builder, kw, D, nsteps):
             assert not np.any(dev[key]), key
             continue
         e = rel(dev[key], ref)
-        tol = 1e-8 if key.endswith(("dvdt", "drhodt", "strain_rate", "vorticity")) else 1e-10
+        tol = 100 * base if key.endswith(("dvdt", "drhodt", "strain_rate", "vorticity")) else base
         assert e < tol, f"{key}: {e:.3e}"
+
+
+@pytest.mark.parametrize("rk4", [False, True])
+def test_xsph_one_step_f64(rk4):
+    """One step of bubble3 (XSPH on X, Y, X–ghost and Y–X): v_adjustment,
+    v and x equal the oracle's to the last bits."""
+    kw = dict(dx=0.2, variant="bubble3", rk4=rk4)
+    dev = _run_steps(_bubble3_xy, kw, 2, "f64", 1, "dev")
+    orc = _run_steps(_bubble3_xy, kw, 2, "f64", 1, "orc")
+    for key in (k for k in orc if k.endswith((".x", ".v", ".v_adjustment", ".rho"))):
+        assert np.any(orc[key]), key
+        e = rel(dev[key], orc[key])
+        assert e < 1e-12, f"{key}: {e:.3e}"
 
 
 @pytest.mark.parametrize("name,builder,kw,D,nsteps", STEP_CASES, ids=[c[0] for c in STEP_CASES])
