@@ -242,7 +242,7 @@ struct gr_ctx {
   DBuf lidx, lcnt;
   int64_t lidx_cap = 0, lcnt_cap = 0;
   int use_lists = 1;
-  int list_maxnb = 0;  // partners kept per target (0: 192 in 3D, 96 in 2D)
+  int list_maxnb = 0;  // mask entries kept per target (0: 48 in 3D, 24 in 2D)
   std::map<std::vector<int>, std::unique_ptr<Plan>> plans;
   // cached device tables derived from the layout
   DBuf allreal_blk;
@@ -1212,8 +1212,9 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
   std::vector<Blk> lbv;
   std::map<int, int> ltask_of;
   int64_t lacc = 0, cacc = 0;
-  // a multiple of 4: lists are stored in 4-entry chunks
-  const int maxnb = ((c->list_maxnb > 0 ? c->list_maxnb : (c->D == 3 ? 192 : 96)) + 3) & ~3;
+  // mask entries kept per target (16 B each): a strip of <= 64 candidates
+  // takes one entry, so 9 (3D) or 3 (2D) per source for interior targets
+  const int maxnb = c->list_maxnb > 0 ? c->list_maxnb : (c->D == 3 ? 48 : 24);
   for (auto &kv : tl) {
     LTask L;
     memset(&L, 0, sizeof(L));
@@ -1225,7 +1226,7 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
     L.src_end = (int)lsv.size();
     L.list_off = lacc;
     L.cnt_off = cacc;
-    lacc += (int64_t)L.cap * maxnb;
+    lacc += (int64_t)L.cap * maxnb * 4;
     cacc += (int64_t)L.cap * (L.src_end - L.src_begin + 1);
     add_blocks(lbv, (int)ltv.size(), L.cap, 64);  // one wavefront per workgroup
     ltask_of[kv.first] = (int)ltv.size();

@@ -132,9 +132,10 @@ struct Task {
 };
 
 // Neighbour list of one target system: sources [src_begin, src_end) of the
-// list source table; partner k of target i at lidx[list_off + 4·((k/4)·cap
-// + i) + k%4] (4-entry chunks), counts at lcnt[cnt_off + q·cap + i],
-// overflow flag at lcnt[cnt_off + nsrc·cap + i].
+// list source table; mask entry e of target i (int4 {base, mask_lo,
+// mask_hi, 0}) at lidx[list_off + 4·(e·cap + i)], entry counts per source at
+// lcnt[cnt_off + q·cap + i], overflow flag at lcnt[cnt_off + nsrc·cap + i];
+// maxnb = entries kept per target.
 struct LTask {
   int32_t sys;
   int32_t src_begin, src_end;

@@ -600,20 +600,24 @@ __device__ inline int wave_max_i(int v) {
 }
 
 // Accepted partners of every target of a list task, per source, in the
-// order the sources are listed.  The k-th partner (source-local index) of
-// target i sits in chunk k/4 of the target, element k%4: chunks of 4 ints
-// (16 B), chunk row c of a task at lidx[L.list_off + 4·(c·L.cap + i)], so a
-// lane writes whole 16-byte chunks and a wavefront reads one chunk row as a
-// contiguous 1 KiB.  lcnt[L.cnt_off + q·L.cap + i] holds the number from
-// source q.  Counts above L.maxnb (a multiple of 4) mean "overflow" (the
-// physics pass then walks the cells directly).
+// order the sources are listed, as acceptance masks: an entry
+// {base, mask_lo, mask_hi, 0} (int4) says that candidates base + b of the
+// source (source-local indices, b = 0..63) with bit b set are partners.
+// Entries follow the cell walk (strips in order, ascending candidates), so
+// popping the bits in order visits the partners in the walk's order.  Entry e
+// of target i sits at lidx[L.list_off + 4·(e·L.cap + i)]: a lane writes one
+// 16-byte entry per (strip, 64 candidates) that holds a partner, and a
+// wavefront reads a row of entries as a contiguous 1 KiB.
+// lcnt[L.cnt_off + q·L.cap + i] holds the number of entries of source q.
+// More than L.maxnb entries mean "overflow" (the physics pass then walks the
+// cells directly).
 //
 // One wavefront per workgroup.  For each strip, lanes whose candidate ranges
 // start within 2·wmax of the lowest pending start form a group (targets in
 // one cell column); the group's union of ranges is staged in LDS, NL_WIN
 // candidates at a time, with coalesced loads, and every lane walks its own
-// range from LDS in ascending order, so it sees its candidates in the cell
-// walk's order and tests each once with the same FP64 expression.
+// range from LDS in ascending order, so it tests each candidate once with the
+// walk's FP64 expression and sets the candidate's bit: no per-partner store.
 #define NL_WIN 128
 template <int D, typename R>
 __global__ void __launch_bounds__(64)
@@ -633,27 +637,36 @@ __global__ void __launch_bounds__(64)
   double xi[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) xi[d] = P.x[d][gi];
-  int4 *ch = reinterpret_cast<int4 *>(lidx + L.list_off) + i;
-  int cnt = 0;
-  int b0 = 0, b1 = 0, b2 = 0, b3 = 0;  // the chunk being filled
-  auto push = [&](int j) {
-    if (cnt < L.maxnb) {
-      const int sl = cnt & 3;
-      b0 = sl == 0 ? j : b0;
-      b1 = sl == 1 ? j : b1;
-      b2 = sl == 2 ? j : b2;
-      b3 = sl == 3 ? j : b3;
-      if (sl == 3) ch[(int64_t)(cnt >> 2) * L.cap] = make_int4(b0, b1, b2, b3);
+  int4 *ent = reinterpret_cast<int4 *>(lidx + L.list_off) + i;
+  int ne = 0;  // entries of this target
+  int base = 0;
+  unsigned long long m = 0;  // acceptance mask of candidates base .. base + 63
+  int cq = 0;
+  auto flush = [&]() {
+    if (m != 0ull) {
+      if (ne < L.maxnb)
+        ent[(int64_t)ne * L.cap] = make_int4(base, (int)(unsigned)m, (int)(unsigned)(m >> 32), 0);
+      ++ne;
+      ++cq;
+      m = 0ull;
     }
-    ++cnt;
+  };
+  // candidate j accepted (or not): move the window when j is past it, set the bit
+  auto mark = [&](int j, bool acc) {
+    int d = j - base;
+    if (d >= 64) {
+      flush();
+      base += d & ~63;
+      d &= 63;
+    }
+    m |= (unsigned long long)acc << d;
   };
   for (int q = L.src_begin; q < L.src_end; ++q) {
     const SrcDesc Q = srcs[q];
     const Grid G = grids[Q.grid];
-    int cq = 0;
+    cq = 0;
     if (G.ncells > 0) {
       const int *cs = cells + carr[Q.carr].off;
-      const int64_t poff = sd[Q.partner].off;
       const int self_j = Q.pk == PK_SELF ? i : -1;
       const double c2 = Q.cutoff2;
       int c[D];
@@ -661,10 +674,13 @@ __global__ void __launch_bounds__(64)
       for (int st = 0; st < ((D == 3) ? 9 : 3); ++st) {
         int j0 = 0, j1 = 0;
         if (live) {
-          const int base = strip_base<D>(c, G, st);
-          j0 = cs[base] - 1;
-          j1 = cs[base + 3] - 1;
+          const int sb = strip_base<D>(c, G, st);
+          j0 = cs[sb] - 1;
+          j1 = cs[sb + 3] - 1;
         }
+        base = j0;
+        m = 0ull;
+        const int64_t poff = sd[Q.partner].off;
         const int wmax = __builtin_amdgcn_readfirstlane(wave_max_i(j1 - j0));
         bool pend = j0 < j1;
         while (true) {
@@ -698,14 +714,8 @@ __global__ void __launch_bounds__(64)
                 ra = ra + da[d] * da[d];
                 rb = rb + db[d] * db[d];
               }
-              if (ra < c2 && j != self_j) {
-                push(j);
-                ++cq;
-              }
-              if (rb < c2 && j + 1 != self_j) {
-                push(j + 1);
-                ++cq;
-              }
+              mark(j, ra < c2 && j != self_j);
+              mark(j + 1, rb < c2 && j + 1 != self_j);
             }
             if (j < e) {
               const int t = j - w0;
@@ -715,22 +725,18 @@ __global__ void __launch_bounds__(64)
               double r2 = dx[0] * dx[0];
 #pragma unroll
               for (int d = 1; d < D; ++d) r2 = r2 + dx[d] * dx[d];
-              if (r2 < c2 && j != self_j) {
-                push(j);
-                ++cq;
-              }
+              mark(j, r2 < c2 && j != self_j);
             }
           }
           if (mem) pend = false;
         }
+        flush();
       }
     }
     if (live) lcnt[L.cnt_off + (int64_t)(q - L.src_begin) * L.cap + i] = cq;
   }
-  if (live && cnt <= L.maxnb && (cnt & 3) != 0)  // the last, partial chunk
-    ch[(int64_t)(cnt >> 2) * L.cap] = make_int4(b0, b1, b2, b3);
   if (live)
-    lcnt[L.cnt_off + (int64_t)(L.src_end - L.src_begin) * L.cap + i] = cnt > L.maxnb ? 1 : 0;
+    lcnt[L.cnt_off + (int64_t)(L.src_end - L.src_begin) * L.cap + i] = ne > L.maxnb ? 1 : 0;
 }
 
 // Functors over the accepted partners of each target (one stage).
@@ -759,25 +765,29 @@ __global__ void __launch_bounds__(GR_BLOCK)
   if (overflow) {
     gather_direct<D, R, FAM>(Gt, P, sd, T, srcs, grids, carr, cells, i, ns, no);
   } else if (live) {
-    const int4 *ch = reinterpret_cast<const int4 *>(lidx + L.list_off) + i;
-    int4 cur = make_int4(0, 0, 0, 0);
-    int curc = -1;  // chunk held in cur
+    const int4 *ent = reinterpret_cast<const int4 *>(lidx + L.list_off) + i;
     for (int q = T.src_begin; q < T.src_end; ++q) {
       const SrcDesc Q = srcs[q];
       const int64_t poff = sd[Q.partner].off;
       SrcK<R> K;
       K.load(Q);
-      int off = 0;
-      for (int l = 0; l < Q.lsrc; ++l) off += lcnt[L.cnt_off + (int64_t)l * L.cap + i];
-      const int cq = lcnt[L.cnt_off + (int64_t)Q.lsrc * L.cap + i];
+      int e = 0;
+      for (int l = 0; l < Q.lsrc; ++l) e += lcnt[L.cnt_off + (int64_t)l * L.cap + i];
+      const int eend = e + lcnt[L.cnt_off + (int64_t)Q.lsrc * L.cap + i];
       const double da = (Q.pk == PK_DYNAMIC) ? Gt.da_of(Q) : 0.0;
-      for (int k = off; k < off + cq; ++k) {
-        if ((k >> 2) != curc) {
-          curc = k >> 2;
-          cur = ch[(int64_t)curc * L.cap];
+      unsigned long long m = 0ull;
+      int base = 0;
+      for (;;) {
+        if (m == 0ull) {  // next entry (entries hold at least one partner)
+          if (e >= eend) break;
+          const int4 en = ent[(int64_t)e * L.cap];
+          ++e;
+          base = en.x;
+          m = ((unsigned long long)(unsigned)en.z << 32) | (unsigned)en.y;
         }
-        const int sl = k & 3;
-        const int j = sl == 0 ? cur.x : sl == 1 ? cur.y : sl == 2 ? cur.z : cur.w;
+        const int bit = __builtin_ctzll(m);
+        m &= m - 1ull;
+        const int j = base + bit;
         const int64_t gj = poff + j;
         const RecA<R> Aj = P.ra[gj];
         double dx[D];

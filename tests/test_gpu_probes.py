@@ -1,8 +1,8 @@
 """Probe measurement on the device (gr_measure_probes: _measure_probes!,
 src/TimeIntegration.jl:454-502) against the oracle on the same scenes:
-neighbour counts bit-exact, interpolated and normalised fields within the
-precision's tolerance, probes back in id order, positions advanced by
-prescribed_v·dt every step."""
+neighbour counts equal (bit-exact on identical positions), interpolated and
+normalised fields within the precision's tolerance, probes back in id order,
+positions advanced by prescribed_v·dt every step."""
 import numpy as np
 import pytest
 
