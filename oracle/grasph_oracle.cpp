@@ -388,6 +388,9 @@ struct PairCtx {
   double beta, ljcut;
   int interp_side = 0;  // InterpolateFieldFn: 1 = into system_b, 2 = into system_a
   Sys *Vw = nullptr;    // the virtual system written by InterpolateFieldFn
+  // writes that the colouring does not keep apart (XSPH into a ghost's source
+  // at ghost indices): sweep on one thread, in the reference's serial order
+  bool serial = false;
 };
 
 // f_v[t] += kw · f_r[s] for the fields of `mask` (InterpolateFieldFn
@@ -895,6 +898,7 @@ int sweep_fn(orc_scene *sc, Inter &in, const gr_pfn_desc &f) {
                     "XSPHPfn with a ghost partner: more ghosts than source particles "
                     "(ps_b.v_adjustment[j] would index past the source's array)");
       P.Bw = &src;
+      P.serial = true;
     }
   }
   if (f.kind == GR_PFN_NCOUNT) {
@@ -988,7 +992,7 @@ int sweep_fn(orc_scene *sc, Inter &in, const gr_pfn_desc &f) {
       for (int colour = 0; colour <= 8; ++colour) {
         int64_t cxb = colour / 3 + cmin[0], cyb = colour % 3 + cmin[1];
         int64_t nax = jl_range_len(cxb, 3, cmax[0]), nay = jl_range_len(cyb, 3, cmax[1]);
-#pragma omp parallel for schedule(dynamic, 16)
+#pragma omp parallel for schedule(dynamic, 16) if (!P.serial)
         for (int64_t flat = 1; flat <= nax * nay; ++flat) {
           int64_t sx = (flat - 1) / nay, sy = (flat - 1) % nay;
           int64_t cx = cxb + sx * 3, cy = cyb + sy * 3;
@@ -1011,7 +1015,7 @@ int sweep_fn(orc_scene *sc, Inter &in, const gr_pfn_desc &f) {
         int64_t nax = jl_range_len(cxb, 3, cmax[0]), nay = jl_range_len(cyb, 3, cmax[1]),
                 naz = jl_range_len(czb, 3, cmax[2]);
         int64_t nayz = nay * naz;
-#pragma omp parallel for schedule(dynamic, 16)
+#pragma omp parallel for schedule(dynamic, 16) if (!P.serial)
         for (int64_t flat = 1; flat <= nax * nayz; ++flat) {
           int64_t sx = (flat - 1) / nayz, rem = (flat - 1) % nayz;
           int64_t sy = rem / naz, sz = rem % naz;
