@@ -11,7 +11,8 @@ from .model import (  # noqa: F401
     ElastoPlasticParticleSystem, ElastoPlasticStressUpdater, FluidParticleSystem, FluidPfn,
     GhostCopier, GhostEntry, GhostParticleSystem, HouseholderReflect, InterpolateFieldFn,
     LeapFrogTimeIntegrator, LinearEOSUpdater, RK4TimeIntegrator, NeighborCounterPfn,
-    NeighborCountFn, PrescribedVelocityUpdater, ProbeParticleSystem,
+    NeighborCountFn, PrescribedVelocityUpdater, ProbeParticleSystem, FluidSolidPfn,
+    HookeLawStressUpdater,
     StaticBoundarySystem, StrainRatePfn, StrainRateVorticityPfn, StressParticleSystem,
     SystemInteraction, TaitEOSUpdater, ViscoPlasticMCStressUpdater, VirtualNormUpdater,
     VirtualParticleSystem, WenlandC2Kernel, XSPHPfn, ZeroFieldUpdater)

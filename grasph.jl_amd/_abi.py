@@ -38,10 +38,10 @@ INDEX_IDS = {"id": GR_I_ID, "idx_original": GR_I_IDX_ORIGINAL,
 # ---- kernels / functors / wrappers / updaters / copier modes ------------------
 GR_KERNEL_CUBIC, GR_KERNEL_WENDLAND_C2 = 0, 1
 (GR_PFN_NONE, GR_PFN_FLUID, GR_PFN_CAUCHY, GR_PFN_SR, GR_PFN_SRV, GR_PFN_COUNT,
- GR_PFN_INTERP, GR_PFN_XSPH, GR_PFN_NCOUNT) = range(9)
+ GR_PFN_INTERP, GR_PFN_XSPH, GR_PFN_NCOUNT, GR_PFN_FLUIDSOLID) = range(10)
 GR_WRAP_NONE, GR_WRAP_STATIC, GR_WRAP_DYNAMIC = range(3)
 (GR_UPD_NOTHING, GR_UPD_ZERO, GR_UPD_TAIT, GR_UPD_LINEAR, GR_UPD_EP, GR_UPD_VPMC, GR_UPD_VNORM,
- GR_UPD_PRESCRIBED_V) = range(8)
+ GR_UPD_PRESCRIBED_V, GR_UPD_HOOKE) = range(9)
 GR_COPY_PLAIN, GR_COPY_HOUSEHOLDER = 0, 1
 GR_SCHEME_LEAPFROG, GR_SCHEME_RK4 = 0, 1
 

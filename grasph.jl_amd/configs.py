@@ -342,3 +342,66 @@ def bubble_2d(dx=0.05, R=1.0, variant="bubble", cfl=1.5, rk4=True, xsph=None):
     lf = cls([fx, fy], ints, ghosts=(entry,))
     return {"integrator": lf, "fluid": fx, "light": fy, "cfl": cfl, "name": f"{variant}_2d",
             "n_particles": fx.n + fy.n, "fluid_n": fx.n}
+
+
+def dambreak_wall_2d(dx=0.5, alpha=0.02, cfl=0.05):
+    """Dam break against an elastic concrete wall (DambreakWall.jl): a
+    25 m × 25 m water column (Tait at stage 2, c = 10·sqrt(2·g·25)) in a
+    75 m × 40 m box of 3-layer dynamic boundaries (floor, left, right, top),
+    a 1.5 m × 10 m wall at x = 60 m (ElastoPlastic NS = 4, ρ = 2400,
+    E = 5e8, ν = 0.2; stages Zero(strain_rate, vorticity) then
+    HookeLawStressUpdater), cubic spline h = 1.2dx; interactions (nothing,
+    FluidPfn) for the fluid, (StrainRateVorticityPfn, CauchyFluidPfn) for the
+    wall and wall–floor, (nothing, FluidSolidPfn) fluid–wall.  The reference
+    geometry is tied to dx = 0.5 (particle counts scale with 0.5/dx)."""
+    s = 0.5 / dx
+    h = 1.2 * dx
+    rho_w, g = 1000.0, 9.81
+    c_s = 10.0 * math.sqrt(2.0 * g * 25.0)
+    rho_wall, E, nu = 2400.0, 5.0e8, 0.2
+    c_wall = math.sqrt(E * (1 - nu) / (rho_wall * (1 + nu) * (1 - 2 * nu)))
+    nf = _round(50 * s)
+    fmass = rho_w * dx * dx
+    fluid = M.FluidParticleSystem("fluid", nf * nf, 2, fmass, c_s, source_v=[0.0, -g],
+                                  state_updater=(None, M.TaitEOSUpdater(rho_w)))
+    fluid.x[:] = _lattice(nf, nf, dx)
+    fluid.rho[:] = rho_w
+    nwx, nwy = _round(3 * s), _round(20 * s)
+    wall = M.ElastoPlasticParticleSystem(
+        "wall", nwx * nwy, 2, 4, rho_wall * dx * dx, c_wall, source_v=[0.0, -g],
+        state_updater=(M.ZeroFieldUpdater("strain_rate", "vorticity"),
+                       M.HookeLawStressUpdater(E, nu)))
+    wall.x[:] = _lattice(nwx, nwy, dx) + np.array([60.0, 0.0])
+    wall.rho[:] = rho_wall
+    L = 3
+    nfx = _round(156 * s)
+    fl = np.array([(-1.25 * s * dx / 0.5 + ix * dx, -(l + 0.5) * dx)
+                   for l in range(L) for ix in range(nfx)])
+    nly = _round(80 * s)
+    le = np.array([(-(l + 0.5) * dx, (iy + 0.5) * dx) for l in range(L) for iy in range(nly)])
+    ri = np.array([(75.0 + (l + 0.5) * dx, (iy + 0.5) * dx) for l in range(L) for iy in range(nly)])
+    tp = np.array([(-1.25 * s * dx / 0.5 + ix * dx, 40.0 + (l + 0.5) * dx)
+                   for l in range(L) for ix in range(nfx)])
+    floor, floor_w = _boundary("floor", fl, fmass, c_s, (0.0, 1.0), (0.0, 0.0))
+    left, left_w = _boundary("left", le, fmass, c_s, (1.0, 0.0), (0.0, 0.0))
+    right, right_w = _boundary("right", ri, fmass, c_s, (-1.0, 0.0), (75.0, 0.0))
+    top, top_w = _boundary("top", tp, fmass, c_s, (0.0, -1.0), (0.0, 40.0))
+    kern = M.CubicSplineKernel(h, ndims=2)
+    fp = M.FluidPfn(alpha, 0.0, h)
+    cp = M.CauchyFluidPfn(alpha, 0.0, h)
+    fs = M.FluidSolidPfn(alpha, 0.0, h)
+    sr = M.StrainRateVorticityPfn()
+    ints = [M.SystemInteraction(kern, (None, fp), fluid),
+            M.SystemInteraction(kern, (sr, cp), wall),
+            M.SystemInteraction(kern, (None, fs), fluid, wall),
+            M.SystemInteraction(kern, (None, fp), fluid, floor_w),
+            M.SystemInteraction(kern, (None, fp), fluid, left_w),
+            M.SystemInteraction(kern, (None, fp), fluid, right_w),
+            M.SystemInteraction(kern, (None, fp), fluid, top_w),
+            M.SystemInteraction(kern, (sr, cp), wall, floor_w)]
+    lf = M.LeapFrogTimeIntegrator([fluid, wall, floor, left, right, top], ints)
+    # update_state!(fluid, 2): the initial pressure
+    fluid.p[:] = (rho_w * c_s * c_s / 7.0) * ((fluid.rho / rho_w) ** 7 - 1.0)
+    return {"integrator": lf, "fluid": fluid, "wall": wall, "cfl": cfl,
+            "name": "dambreak_wall_2d", "n_particles": sum(ps.n for ps in lf.systems),
+            "fluid_n": fluid.n}

@@ -705,6 +705,7 @@ int classify(gr_ctx *c, const InterH &in, int pfn) {
       return (a.kind == GR_SYS_FLUID && b.kind == GR_SYS_FLUID) ? P_TWO : P_ONE;
     case GR_PFN_CAUCHY:
     case GR_PFN_COUNT:
+    case GR_PFN_FLUIDSOLID:
       return P_TWO;
     default:
       return P_ONE;
@@ -712,7 +713,7 @@ int classify(gr_ctx *c, const InterH &in, int pfn) {
 }
 
 int family_of(int pfn) {
-  if (pfn == GR_PFN_FLUID || pfn == GR_PFN_CAUCHY) return FAM_FORCE;
+  if (pfn == GR_PFN_FLUID || pfn == GR_PFN_CAUCHY || pfn == GR_PFN_FLUIDSOLID) return FAM_FORCE;
   if (pfn == GR_PFN_SR || pfn == GR_PFN_SRV) return FAM_STRAIN;
   if (pfn == GR_PFN_COUNT) return FAM_COUNT;
   if (pfn == GR_PFN_INTERP) return FAM_INTERP;
@@ -783,6 +784,7 @@ SrcDesc make_src(gr_ctx *c, int it, int target, bool b_side, int pk) {
 
 void set_pfn(SrcDesc &q, const gr_pfn_desc &f, int pk, gr_ctx *c, const InterH &in) {
   q.pfn = f.kind;
+  q.psel = f.kind == GR_PFN_FLUIDSOLID ? (q.side == 0 ? 1 : 2) : 0;
   q.sigma = f.sigma;
   q.alpha = f.alpha;
   q.beta = f.beta;
@@ -1943,11 +1945,16 @@ int gr_system_add_updater(gr_ctx *c, int sys, int stage, int kind, const double 
       need(s.ns == 4, GR_ERR_ASSERTION,
            "ElastoPlasticStressUpdater requires NS=4 (plane-strain Voigt). Got NS=" +
                std::to_string(s.ns) + ".");
-    need(!((kind == GR_UPD_EP && s.kind != GR_SYS_EP) || (kind == GR_UPD_VPMC && s.kind < GR_SYS_STRESS) ||
+    if (kind == GR_UPD_HOOKE)
+      need(s.ns == 4, GR_ERR_ASSERTION,
+           "HookeLawStressUpdater requires NS=4 (plane-strain Voigt). Got NS=" +
+               std::to_string(s.ns) + ".");
+    need(!(((kind == GR_UPD_EP || kind == GR_UPD_HOOKE) && s.kind != GR_SYS_EP) ||
+           (kind == GR_UPD_VPMC && s.kind < GR_SYS_STRESS) ||
            ((kind == GR_UPD_TAIT || kind == GR_UPD_LINEAR) && s.kind < GR_SYS_FLUID) ||
            ((kind == GR_UPD_VNORM || kind == GR_UPD_PRESCRIBED_V) && !s.virt)),
          GR_ERR_ARGUMENT, "updater needs fields the system does not have");
-    need(kind >= GR_UPD_NOTHING && kind <= GR_UPD_PRESCRIBED_V, GR_ERR_ARGUMENT,
+    need(kind >= GR_UPD_NOTHING && kind <= GR_UPD_HOOKE, GR_ERR_ARGUMENT,
          "unknown updater kind");
     if ((int)s.upd.size() < stage) s.upd.resize(stage);
     UpdDesc u;
@@ -1984,6 +1991,10 @@ int gr_system_add_updater(gr_ctx *c, int sys, int stage, int kind, const double 
         u.prm[4] = prm[4];
         need((int)prm[0] == GR_UPD_TAIT || (int)prm[0] == GR_UPD_LINEAR, GR_ERR_ARGUMENT,
              "ViscoPlasticMCStressUpdater needs a Tait or Linear EOS");
+        break;
+      case GR_UPD_HOOKE:
+        u.prm[0] = prm[0];
+        u.prm[1] = prm[1];
         break;
       case GR_UPD_VNORM:  // v_mult, field mask
         u.prm[0] = prm[0];
@@ -2280,7 +2291,12 @@ int gr_interaction_create(gr_ctx *c, int kernel_kind, double h, int sys_a, int p
     const SysH &A = c->sys[sys_a];
     for (const gr_pfn_desc &f : in.pfns) {
       if (f.kind == GR_PFN_NONE) continue;
-      need(f.kind >= GR_PFN_FLUID && f.kind <= GR_PFN_NCOUNT, GR_ERR_ARGUMENT, "unknown functor kind");
+      need(f.kind >= GR_PFN_FLUID && f.kind <= GR_PFN_FLUIDSOLID, GR_ERR_ARGUMENT,
+           "unknown functor kind");
+      need(f.kind != GR_PFN_FLUIDSOLID ||
+               (partner >= 0 && wrap == GR_WRAP_NONE && c->sys[partner].kind != GR_SYS_GHOST &&
+                !c->sys[partner].virt && A.kind >= GR_SYS_FLUID),
+           GR_ERR_ARGUMENT, "FluidSolidPfn needs a fluid system_a and a plain solid system_b");
       need(f.kind != GR_PFN_XSPH, GR_ERR_ARGUMENT,
            "XSPHPfn runs as the interaction's velocity_adjust_pairwise_fn");
       if (f.kind == GR_PFN_FLUID)

@@ -105,7 +105,7 @@ struct SrcDesc {
   int32_t has_eps;
   int32_t side;      // 0: gathers for system_a, 1: for system_b (two-sided)
   int32_t lsrc;      // index of this source in the target's neighbour list
-  int32_t pad;
+  int32_t psel;      // FluidSolidPfn: 1 the target's pressure on both sides, 2 the partner's
   double h, norm, cutoff2, cell;  // kernel
   double alpha, beta, hf, delta, eps;
   double mass_q;                  // partner mass used by the functor

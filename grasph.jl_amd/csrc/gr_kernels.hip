@@ -191,6 +191,8 @@ __global__ void k_key_base(const int *list, int nlist, int D, int pad, const int
     base += range + (pad ? 1 : 0);
     if (base >= 0xFFFFFFFFull) flags[1] |= 1;
   }
+  // every key is below this bound: radix passes above its top bit are identities
+  flags[2] = (int)(unsigned)(base < 0xFFFFFFFFull ? base : 0xFFFFFFFFull);
 }
 
 template <int D>
@@ -248,9 +250,16 @@ __global__ void k_key_write(XP xs, const SysDesc *sd, const int *n_dev,
 #define RADIX_ITEMS 8
 #define RADIX_TILE (GR_BLOCK * RADIX_ITEMS)
 
+// A pass whose digit is zero for every key (all keys < 2^shift) keeps the
+// order: its histogram and scan are skipped and its scatter is a plain copy.
+__device__ inline bool radix_pass_idle(const int *flags, int shift) {
+  const unsigned bound = (unsigned)flags[2];
+  return bound != 0u && ((bound - 1u) >> shift) == 0u;
+}
+
 __global__ void k_radix_hist(const unsigned *keys, int N, int shift, unsigned *hist, int nblk,
                              const int *flags) {
-  if (flags[0] == 0) return;
+  if (flags[0] == 0 || radix_pass_idle(flags, shift)) return;
   __shared__ unsigned h[256];
   h[threadIdx.x] = 0;
   __syncthreads();
@@ -263,28 +272,42 @@ __global__ void k_radix_hist(const unsigned *keys, int N, int shift, unsigned *h
   hist[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of 256·nblk counters in digit-major order (one block).
-__global__ void k_radix_scan(unsigned *hist, int total, const int *flags) {
-  if (flags[0] == 0) return;
-  __shared__ unsigned part[1024];
-  const int t = threadIdx.x;
-  const int per = (total + 1023) / 1024;
-  const int b0 = t * per, b1 = min(total, b0 + per);
-  unsigned s = 0;
-  for (int k = b0; k < b1; ++k) s += hist[k];
-  part[t] = s;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {
-    unsigned v = t >= o ? part[t - o] : 0u;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
+// Exclusive scan of 256·nblk counters in digit-major order (one block of
+// 1024 threads).  Each wavefront owns a contiguous segment and walks it in
+// rows of 64 consecutive counters (coalesced loads), first summing it, then,
+// after a scan of the 16 segment sums, writing each row's exclusive prefix.
+__device__ inline unsigned wave_incl_scan(unsigned v, int lane) {
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned u = __shfl_up(v, o, 64);
+    if (lane >= o) v += u;
   }
-  unsigned run = t > 0 ? part[t - 1] : 0u;
-  for (int k = b0; k < b1; ++k) {
-    unsigned c = hist[k];
-    hist[k] = run;
-    run += c;
+  return v;
+}
+
+__global__ void __launch_bounds__(1024) k_radix_scan(unsigned *hist, int total, int shift,
+                                                     const int *flags) {
+  if (flags[0] == 0 || radix_pass_idle(flags, shift)) return;
+  __shared__ unsigned wsum[16];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int seg = (total + 15) / 16;
+  const int b0 = w * seg, b1 = min(total, b0 + seg);
+  unsigned s = 0;
+  for (int r = b0; r < b1; r += 64) {
+    const int k = r + lane;
+    s += k < b1 ? hist[k] : 0u;
+  }
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0) wsum[w] = s;
+  __syncthreads();
+  unsigned carry = 0;
+  for (int q = 0; q < w; ++q) carry += wsum[q];
+  for (int r = b0; r < b1; r += 64) {
+    const int k = r + lane;
+    const unsigned c = k < b1 ? hist[k] : 0u;
+    const unsigned inc = wave_incl_scan(c, lane);
+    if (k < b1) hist[k] = carry + inc - c;
+    carry += __shfl(inc, 63, 64);
   }
 }
 
@@ -295,6 +318,17 @@ __global__ void k_radix_scatter(const unsigned *kin, const int *vin, unsigned *k
                                 int N, int shift, const unsigned *hist, int nblk,
                                 const int *flags) {
   if (flags[0] == 0) return;
+  if (radix_pass_idle(flags, shift)) {  // identity pass: copy the tile
+    const int base = blockIdx.x * RADIX_TILE;
+    for (int r = 0; r < RADIX_ITEMS; ++r) {
+      const int e = base + r * GR_BLOCK + threadIdx.x;
+      if (e < N) {
+        kout[e] = kin[e];
+        vout[e] = vin[e];
+      }
+    }
+    return;
+  }
   __shared__ unsigned run[256];
   __shared__ unsigned wcnt[GR_BLOCK / 64][256];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
@@ -1008,6 +1042,32 @@ __global__ void k_xsph(Pool<R> P, const SysDesc *sd, const StepSys *st, const in
 // State updaters of one stage (src/StateUpdaters.jl)
 // ---------------------------------------------------------------------------
 
+// HookeLawStressUpdater (src/StateUpdaters.jl:366-405): elastic increment
+// with the Jaumann correction; p = −(σxx + σyy + σzz)/3.
+template <typename R>
+__device__ __forceinline__ void hooke_update(Pool<R> P, int64_t g, double E, double nu, double dt) {
+  const double sxx = P.stress[0][g], syy = P.stress[1][g], szz = P.stress[2][g],
+               sxy = P.stress[3][g];
+  const double dexx = (double)P.sr[0][g] * dt, deyy = (double)P.sr[1][g] * dt,
+               dezz = (double)P.sr[2][g] * dt, dexy = (double)P.sr[3][g] * dt;
+  const double D0 = E / ((1.0 + nu) * (1.0 - 2 * nu));
+  double dsxx = D0 * ((1.0 - nu) * dexx + nu * deyy + nu * dezz);
+  double dsyy = D0 * (nu * dexx + (1.0 - nu) * deyy + nu * dezz);
+  const double dszz = D0 * (nu * dexx + nu * deyy + (1.0 - nu) * dezz);
+  double dsxy = D0 * (1.0 - 2 * nu) * dexy;
+  const double wdt = (double)P.vor[0][g] * dt;
+  dsxx -= 2 * sxy * wdt;
+  dsyy += 2 * sxy * wdt;
+  dsxy += (sxx - syy) * wdt;
+  const double n[4] = {sxx + dsxx, syy + dsyy, szz + dszz, sxy + dsxy};
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    P.stress[k][g] = n[k];
+    P.stress_s[k][g] = (R)n[k];
+  }
+  P.p[g] = (R)(-(n[0] + n[1] + n[2]) / 3.0);
+}
+
 // ElastoPlasticStressUpdater (:216-335), FP64.  prm = {E, nu, a_phi, k_c,
 // a_psi} with the trigonometric constants evaluated on the host.
 template <typename R>
@@ -1173,6 +1233,9 @@ __global__ void k_updaters(Pool<R> P, const SysDesc *sd, const StepSys *st, cons
         break;
       case GR_UPD_VPMC:
         vpmc_update(P, g, S.ns, u.prm[0], u.prm[1], u.prm[2], u.prm[3], u.prm[4], S.c);
+        break;
+      case GR_UPD_HOOKE:
+        hooke_update(P, g, u.prm[0], u.prm[1], dt);
         break;
       case GR_UPD_VNORM: {  // VirtualNormUpdater (src/StateUpdaters.jl:74-84)
         const long long m = u.mask;
@@ -1367,7 +1430,7 @@ void launch_radix_sort(hipStream_t s, unsigned *keys, int *vals, unsigned *keys2
   for (int pass = 0; pass < 4; ++pass) {
     const int shift = pass * 8;
     k_radix_hist<<<nblk, GR_BLOCK, 0, s>>>(ki, N, shift, hist, nblk, flags);
-    k_radix_scan<<<1, 1024, 0, s>>>(hist, 256 * nblk, flags);
+    k_radix_scan<<<1, 1024, 0, s>>>(hist, 256 * nblk, shift, flags);
     k_radix_scatter<<<nblk, GR_BLOCK, 0, s>>>(ki, vi, ko, vo, N, shift, hist, nblk, flags);
     unsigned *tk = ki;
     ki = ko;

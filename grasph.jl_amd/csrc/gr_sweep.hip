@@ -347,18 +347,21 @@ struct Gather {
       for (int d = 1; d < D; ++d) dxg = dxg + dxr[d] * gx[d];
       psi = FA ? K.psi_coef * (ri - rj) * dxg * (ir * ir) : K.psi_coef * (ri - rj) * dxg / r2r;
     }
-    if (Q.pfn == GR_PFN_FLUID) {
+    if (Q.pfn == GR_PFN_FLUID || Q.pfn == GR_PFN_FLUIDSOLID) {
+      // FluidSolidPfn: the fluid's pressure on both sides (the target's on
+      // the fluid side, the partner's on the solid side)
+      const R pa = Q.psel == 2 ? pj : pi, pb = Q.psel == 1 ? pi : pj;
       R dh;
       if (FA) {
         if (K.sigma == 1)
-          dh = -(pi + pj) * (iri * irj);
+          dh = -(pa + pb) * (iri * irj);
         else
-          dh = -(pi * (iri * iri) + pj * (irj * irj));
+          dh = -(pa * (iri * iri) + pb * (irj * irj));
       } else {
         if (K.sigma == 1)
-          dh = -(pi + pj) / (ri * rj);
+          dh = -(pa + pb) / (ri * rj);
         else
-          dh = -(pi / (ri * ri) + pj / (rj * rj));
+          dh = -(pa / (ri * ri) + pb / (rj * rj));
       }
       if (Q.pk == PK_TWO_SIDED) {
         R ast = (R)0;

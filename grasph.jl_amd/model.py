@@ -114,6 +114,17 @@ class ElastoPlasticStressUpdater:
         return A.GR_UPD_EP, [self.E, self.nu, self.phi, self.psi, self.cohesion]
 
 
+class HookeLawStressUpdater:
+    """HookeLawStressUpdater(E, nu) (src/StateUpdaters.jl:338-405): linear
+    elastic plane-strain stress update (NS = 4) with the Jaumann rate."""
+
+    def __init__(self, E, nu):
+        self.E, self.nu = float(E), float(nu)
+
+    def encode(self):
+        return A.GR_UPD_HOOKE, [self.E, self.nu]
+
+
 class ViscoPlasticMCStressUpdater:
     """ViscoPlasticMCStressUpdater(eos, friction_angle, cohesion)
     (src/StateUpdaters.jl:144-189)."""
@@ -161,7 +172,7 @@ class PrescribedVelocityUpdater:
 
 _UPDATER_TYPES = (ZeroFieldUpdater, TaitEOSUpdater, LinearEOSUpdater,
                   ElastoPlasticStressUpdater, ViscoPlasticMCStressUpdater, VirtualNormUpdater,
-                  PrescribedVelocityUpdater)
+                  PrescribedVelocityUpdater, HookeLawStressUpdater)
 
 
 def _normalise_updaters(state_updater):
@@ -487,6 +498,22 @@ class CauchyFluidPfn:
                 _nan_or(self.delta), math.nan)
 
 
+class FluidSolidPfn:
+    """FluidSolidPfn(alpha, beta, h; sigma=2, delta=nothing)
+    (src/PairwiseFunctors.jl:463-516): FluidPfn between a fluid system_a and
+    a solid system_b with the fluid's pressure on both sides."""
+
+    def __init__(self, alpha, beta, h, sigma=2, delta=None):
+        if sigma not in (1, 2):
+            raise ValueError("FluidSolidPfn sigma must be 1 or 2 on the device path")
+        self.art_visc_alpha, self.art_visc_beta, self.h = float(alpha), float(beta), float(h)
+        self.sigma, self.delta = int(sigma), delta
+
+    def encode(self):
+        return (A.GR_PFN_FLUIDSOLID, self.sigma, self.art_visc_alpha, self.art_visc_beta, self.h,
+                _nan_or(self.delta), math.nan)
+
+
 class StrainRatePfn:
     """StrainRatePfn() (src/PairwiseFunctors.jl:18)."""
 
@@ -544,7 +571,7 @@ class NeighborCountFn:
 
 
 _PFN_TYPES = (FluidPfn, CauchyFluidPfn, StrainRatePfn, StrainRateVorticityPfn, NeighborCounterPfn,
-              InterpolateFieldFn, NeighborCountFn)
+              InterpolateFieldFn, NeighborCountFn, FluidSolidPfn)
 
 
 def encode_pfn(p):

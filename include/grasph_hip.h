@@ -88,8 +88,12 @@ enum {
                          1<<GR_F_STRESS), `alpha` != 0 accumulates w_sum  */
   GR_PFN_XSPH = 7,    /* XSPHPfn :360-390, only as an interaction's
                          velocity-adjust functor (gr_interaction_set_vadjust) */
-  GR_PFN_NCOUNT = 8   /* NeighborCountFn :443-460: system_b (a probe) counts
+  GR_PFN_NCOUNT = 8,  /* NeighborCountFn :443-460: system_b (a probe) counts
                          its system_a partners into GR_F_COUNT              */
+  GR_PFN_FLUIDSOLID = 9 /* FluidSolidPfn :463-516: FluidPfn between a fluid
+                         system_a and a plain solid system_b with the fluid's
+                         pressure on both sides (sigma, alpha, beta, h,
+                         delta as FluidPfn; no epsilon)                    */
 };
 
 /* ---- partner wrappers (src/BoundaryParticles.jl:38,68) ---------------- */
@@ -110,7 +114,10 @@ enum {
   GR_UPD_VNORM = 6,    /* VirtualNormUpdater :58-77 (virtual systems);
                           params = {v_mult[0], v_mult[1], v_mult[2],
                                     field bit mask}                        */
-  GR_UPD_PRESCRIBED_V = 7  /* PrescribedVelocityUpdater :88-92; no params */
+  GR_UPD_PRESCRIBED_V = 7, /* PrescribedVelocityUpdater :88-92; no params */
+  GR_UPD_HOOKE = 8     /* HookeLawStressUpdater :338-405 (ElastoPlastic,
+                          NS = 4); params = {E, nu}: elastic increment with
+                          the Jaumann rate, p = −tr(σ)/3                    */
 };
 
 /* ---- ghost copier modes (src/GhostParticles.jl:26,53) ------------------ */

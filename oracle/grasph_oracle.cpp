@@ -532,6 +532,24 @@ void apply_pair(const PairCtx &P, int64_t i, int64_t j, const double *dx,
       }
       return;
     }
+    case GR_PFN_FLUIDSOLID: {  // FluidSolidPfn (src/PairwiseFunctors.jl:493-516)
+      const double ri = A.rho[i], pi = A.p[i];  // the fluid's pressure, both sides
+      const double rj = B.rho[j];
+      const double mi = P.mass_a, mj = P.B.mass;
+      for (int d = 0; d < D; ++d) dv[d] = vi[d] - vj[d];
+      const double piv = art_visc(dx, dv, D, f.h, ri, rj, f.alpha, f.beta, P.c_a, P.B.c);
+      const double dh = pfc(pi, pi, ri, rj, f.sigma);
+      for (int d = 0; d < D; ++d) {
+        const double dvt = (dh - piv) * gx[d];
+        P.Aw->dvdt[i * D + d] += mj * dvt;
+        P.Bw->dvdt[j * D + d] -= mi * dvt;
+      }
+      const double dr = dotv(dv, gx, D);
+      const double psi = diffusion(dx, ri, rj, P.c_a, P.B.c, f.h, f.h, gx, D, f.delta);
+      P.Aw->drhodt[i] += mj * (dr * cdc(ri, rj, f.sigma) + psi / rj);
+      P.Bw->drhodt[j] += mi * (dr * cdc(rj, ri, f.sigma) - psi / ri);
+      return;
+    }
     case GR_PFN_CAUCHY: {
       const int NS = P.NS;
       double ri = A.rho[i];
@@ -901,6 +919,15 @@ int sweep_fn(orc_scene *sc, Inter &in, const gr_pfn_desc &f) {
       P.serial = true;
     }
   }
+  if (f.kind == GR_PFN_FLUIDSOLID) {  // only the two-system method exists
+    if (in.b < 0 || in.wrap != GR_WRAP_NONE || sc->sys[in.b].kind == GR_SYS_GHOST ||
+        sc->sys[in.b].virt || A.kind < GR_SYS_FLUID)
+      return fail(sc, GR_ERR_ARGUMENT,
+                  "FluidSolidPfn needs a fluid system_a and a plain solid system_b");
+    P.pk = P_TWO_SIDED;
+    P.B = make_view(sc, in.b);
+    P.Bw = &sc->sys[in.b];
+  }
   if (f.kind == GR_PFN_NCOUNT) {
     if (!(in.b >= 0 && sc->sys[in.b].probe && !sc->sys[in.a].virt && in.wrap == GR_WRAP_NONE))
       return fail(sc, GR_ERR_ARGUMENT,
@@ -1054,6 +1081,33 @@ void eos(Sys &s, int kind, double rho0, double gamma, int64_t i) {
 }
 
 // ElastoPlasticStressUpdater :216-335 (NS = 4, plane strain).
+// HookeLawStressUpdater (src/StateUpdaters.jl:366-405): the elastic
+// increment D:ε̇dt, the Jaumann correction with the vorticity, σ += Δσ and
+// p = −(σxx + σyy + σzz)/3.
+void hooke_update(Sys &s, const Upd &u, double dt, int64_t i) {
+  const double E = u.prm[0], nu = u.prm[1];
+  double *sg = &s.stress[i * 4];
+  const double *er = &s.sr[i * 4];
+  const double om = s.vor[i];
+  const double sxx = sg[0], syy = sg[1], szz = sg[2], sxy = sg[3];
+  const double dexx = er[0] * dt, deyy = er[1] * dt, dezz = er[2] * dt, dexy = er[3] * dt;
+  const double D0 = E / ((1.0 + nu) * (1.0 - 2 * nu));
+  double dsxx = D0 * ((1.0 - nu) * dexx + nu * deyy + nu * dezz);
+  double dsyy = D0 * (nu * dexx + (1.0 - nu) * deyy + nu * dezz);
+  const double dszz = D0 * (nu * dexx + nu * deyy + (1.0 - nu) * dezz);
+  double dsxy = D0 * (1.0 - 2 * nu) * dexy;
+  const double wdt = om * dt;
+  dsxx -= 2 * sxy * wdt;
+  dsyy += 2 * sxy * wdt;
+  dsxy += (sxx - syy) * wdt;
+  const double nxx = sxx + dsxx, nyy = syy + dsyy, nzz = szz + dszz, nxy = sxy + dsxy;
+  sg[0] = nxx;
+  sg[1] = nyy;
+  sg[2] = nzz;
+  sg[3] = nxy;
+  s.p[i] = -(nxx + nyy + nzz) / 3.0;
+}
+
 void ep_update(Sys &s, const Upd &u, double dt, int64_t i) {
   const double E = u.prm[0], nu = u.prm[1], phi = u.prm[2], psi = u.prm[3],
                coh = u.prm[4];
@@ -1196,6 +1250,10 @@ int update_state(orc_scene *sc, Sys &s, int stage, double dt) {
       case GR_UPD_VPMC:
 #pragma omp parallel for
         for (int64_t i = 0; i < s.n; ++i) vpmc_update(s, u, i);
+        break;
+      case GR_UPD_HOOKE:
+#pragma omp parallel for
+        for (int64_t i = 0; i < s.n; ++i) hooke_update(s, u, dt, i);
         break;
       case GR_UPD_VNORM: {  // VirtualNormUpdater :74-84
         if (!s.virt) return fail(sc, GR_ERR_ARGUMENT, "VirtualNormUpdater needs a virtual system");
@@ -1846,7 +1904,11 @@ int orc_system_add_updater(orc_scene *s, int sys, int stage, int kind,
   if (kind == GR_UPD_EP && y.NS != 4)
     return fail(s, GR_ERR_ASSERTION,
                 "ElastoPlasticStressUpdater requires NS=4 (plane-strain Voigt). Got NS=" + std::to_string(y.NS) + ".");
-  if ((kind == GR_UPD_EP && y.kind != GR_SYS_EP) || (kind == GR_UPD_VPMC && y.kind < GR_SYS_STRESS) ||
+  if (kind == GR_UPD_HOOKE && y.NS != 4)
+    return fail(s, GR_ERR_ASSERTION,
+                "HookeLawStressUpdater requires NS=4 (plane-strain Voigt). Got NS=" + std::to_string(y.NS) + ".");
+  if (((kind == GR_UPD_EP || kind == GR_UPD_HOOKE) && y.kind != GR_SYS_EP) ||
+      (kind == GR_UPD_VPMC && y.kind < GR_SYS_STRESS) ||
       ((kind == GR_UPD_TAIT || kind == GR_UPD_LINEAR) && y.kind < GR_SYS_FLUID) ||
       ((kind == GR_UPD_VNORM || kind == GR_UPD_PRESCRIBED_V) && !y.virt))
     return fail(s, GR_ERR_ARGUMENT, "updater needs fields the system does not have");

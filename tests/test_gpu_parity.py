@@ -398,6 +398,7 @@ STEP_CASES = [
      10),
     ("ep_column_2d_rk4", lambda **kw: _as_rk4(M.configs.ep_column_2d(**kw)),
      dict(dx=0.01, width=0.2, height=0.1), 2, 10),
+    ("dambreak_wall_2d", M.configs.dambreak_wall_2d, dict(), 2, 20),
 ]
 
 
@@ -505,6 +506,7 @@ LIST_CASES = [
     ("granular_column_3d", M.configs.granular_column_3d, dict(dx=0.01), 3, "f32"),
     ("ep_column_virtual_2d", M.configs.ep_column_virtual_2d,
      dict(dx=0.01, width=0.1, height=0.06), 2, "f32"),
+    ("dambreak_wall_2d", M.configs.dambreak_wall_2d, dict(), 2, "f32"),
 ]
 
 
