@@ -587,21 +587,6 @@ __global__ void __launch_bounds__(GR_BLOCK)
 // Neighbour lists
 // ---------------------------------------------------------------------------
 
-__device__ inline int wave_min_i(int v) {
-  for (int o = 32; o > 0; o >>= 1) {
-    int u = __shfl_xor(v, o, 64);
-    v = u < v ? u : v;
-  }
-  return v;
-}
-__device__ inline int wave_max_i(int v) {
-  for (int o = 32; o > 0; o >>= 1) {
-    int u = __shfl_xor(v, o, 64);
-    v = u > v ? u : v;
-  }
-  return v;
-}
-
 // Accepted partners of every target of a list task, per source, in the
 // order the sources are listed, as acceptance masks: an entry
 // {base, mask_lo, mask_hi, 0} (int4) says that candidates base + b of the
@@ -622,6 +607,19 @@ __device__ inline int wave_max_i(int v) {
 // range from LDS in ascending order, so it tests each candidate once with the
 // walk's FP64 expression and sets the candidate's bit: no per-partner store.
 #define NL_WIN 128
+// Wave-wide minimum / maximum (all 64 lanes active): DPP within each row of
+// 16 lanes, then the four row results through scalar registers.
+template <bool MAX>
+__device__ inline int wave_red_dpp(int v) {
+  auto op = [](int x, int y) { return MAX ? (x > y ? x : y) : (x < y ? x : y); };
+  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0xB1, 0xF, 0xF, false));   // quad_perm 1,0,3,2
+  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x4E, 0xF, 0xF, false));   // quad_perm 2,3,0,1
+  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x141, 0xF, 0xF, false));  // row_half_mirror
+  v = op(v, __builtin_amdgcn_update_dpp(v, v, 0x140, 0xF, 0xF, false));  // row_mirror
+  return op(op(__builtin_amdgcn_readlane(v, 0), __builtin_amdgcn_readlane(v, 16)),
+            op(__builtin_amdgcn_readlane(v, 32), __builtin_amdgcn_readlane(v, 48)));
+}
+
 template <int D, typename R>
 __global__ void __launch_bounds__(64)
     k_nlist(Pool<R> P, const SysDesc *__restrict__ sd, const int *__restrict__ n_dev,
@@ -642,36 +640,18 @@ __global__ void __launch_bounds__(64)
   for (int d = 0; d < D; ++d) xi[d] = P.x[d][gi];
   int4 *ent = reinterpret_cast<int4 *>(lidx + L.list_off) + i;
   int ne = 0;  // entries of this target
-  int base = 0;
-  unsigned long long m = 0;  // acceptance mask of candidates base .. base + 63
-  int cq = 0;
-  auto flush = [&]() {
-    if (m != 0ull) {
-      if (ne < L.maxnb)
-        ent[(int64_t)ne * L.cap] = make_int4(base, (int)(unsigned)m, (int)(unsigned)(m >> 32), 0);
-      ++ne;
-      ++cq;
-      m = 0ull;
-    }
-  };
-  // candidate j accepted (or not): move the window when j is past it, set the bit
-  auto mark = [&](int j, bool acc) {
-    int d = j - base;
-    if (d >= 64) {
-      flush();
-      base += d & ~63;
-      d &= 63;
-    }
-    m |= (unsigned long long)acc << d;
-  };
+  int cq = 0;  // entries of the current source
   for (int q = L.src_begin; q < L.src_end; ++q) {
     const SrcDesc Q = srcs[q];
     const Grid G = grids[Q.grid];
     cq = 0;
     if (G.ncells > 0) {
       const int *cs = cells + carr[Q.carr].off;
+      // the target itself (a self source) passes the distance test with
+      // r² = 0; its bit is cleared when a mask is stored
       const int self_j = Q.pk == PK_SELF ? i : -1;
       const double c2 = Q.cutoff2;
+      const int64_t poff = sd[Q.partner].off;
       int c[D];
       target_cell<D>(xi, G, Q.cell, c);
       for (int st = 0; st < ((D == 3) ? 9 : 3); ++st) {
@@ -681,54 +661,66 @@ __global__ void __launch_bounds__(64)
           j0 = cs[sb] - 1;
           j1 = cs[sb + 3] - 1;
         }
-        base = j0;
-        m = 0ull;
-        const int64_t poff = sd[Q.partner].off;
-        const int wmax = __builtin_amdgcn_readfirstlane(wave_max_i(j1 - j0));
         bool pend = j0 < j1;
+        if (__builtin_amdgcn_ballot_w64(pend) == 0ull) continue;
+        // mask of candidates base .. base + 63 in two words; d = j − base
+        int base = j0, d = 0;
+        unsigned lo = 0u, hi = 0u;
+        auto flush = [&]() {
+          const unsigned sdl = (unsigned)(self_j - base);
+          if (sdl < 32u) lo &= ~(1u << sdl);
+          else if (sdl < 64u) hi &= ~(1u << (sdl - 32u));
+          if ((lo | hi) != 0u) {
+            if (ne < L.maxnb) ent[(int64_t)ne * L.cap] = make_int4(base, (int)lo, (int)hi, 0);
+            ++ne;
+            ++cq;
+          }
+          lo = hi = 0u;
+        };
+        const int wmax = wave_red_dpp<true>(j1 - j0);
         while (true) {
-          const int lo = __builtin_amdgcn_readfirstlane(wave_min_i(pend ? j0 : INT_MAX));
-          if (lo == INT_MAX) break;
-          const bool mem = pend && j0 - lo <= 2 * wmax;
-          const int hi = __builtin_amdgcn_readfirstlane(wave_max_i(mem ? j1 : INT_MIN));
-          for (int w0 = lo; w0 < hi; w0 += NL_WIN) {
-            const int w1 = hi - w0 < NL_WIN ? hi : w0 + NL_WIN;
+          const int glo = wave_red_dpp<false>(pend ? j0 : INT_MAX);
+          if (glo == INT_MAX) break;
+          const bool mem = pend && j0 - glo <= 2 * wmax;
+          const int ghi = wave_red_dpp<true>(mem ? j1 : INT_MIN);
+          for (int w0 = glo; w0 < ghi; w0 += NL_WIN) {
+            const int w1 = ghi - w0 < NL_WIN ? ghi : w0 + NL_WIN;
             __syncthreads();  // the previous window has been read
             for (int t = lane; t < w1 - w0; t += 64) {
 #pragma unroll
-              for (int d = 0; d < D; ++d) sx[d][t] = P.x[d][poff + w0 + t];
+              for (int k = 0; k < D; ++k) sx[k][t] = P.x[k][poff + w0 + t];
             }
             __syncthreads();
-            const int a = mem ? (j0 > w0 ? j0 : w0) : 0;
-            const int e = mem ? (j1 < w1 ? j1 : w1) : 0;
-            // two candidates per iteration (independent chains), kept in order
-            int j = a;
-            for (; j + 1 < e; j += 2) {
-              const int t = j - w0;
-              double da[D], db[D];
-#pragma unroll
-              for (int d = 0; d < D; ++d) {
-                da[d] = xi[d] - sx[d][t];
-                db[d] = xi[d] - sx[d][t + 1];
+            // this lane's candidates in the window end before je; base + d is
+            // the next one (where the previous window stopped)
+            const int je = mem ? (j1 < w1 ? j1 : w1) : 0;
+            while (base + d < je) {
+              if (d == 64) {  // the mask is full: store it, move it on
+                flush();
+                base += 64;
+                d = 0;
               }
-              double ra = da[0] * da[0], rb = db[0] * db[0];
+              // a run of candidates within one 32-bit word of the mask
+              const int wsel = d >> 5;
+              const int lim = je - base, wend = (d & ~31) + 32;
+              const int dend = lim < wend ? lim : wend;
+              const int toff = base - w0;
+              unsigned cur = 0u;
+              for (; d < dend; ++d) {
+                const int t = d + toff;
+                const double a0 = xi[0] - sx[0][t];
+                double r2 = a0 * a0;
 #pragma unroll
-              for (int d = 1; d < D; ++d) {
-                ra = ra + da[d] * da[d];
-                rb = rb + db[d] * db[d];
+                for (int k = 1; k < D; ++k) {
+                  const double ak = xi[k] - sx[k][t];
+                  r2 = r2 + ak * ak;
+                }
+                cur |= (unsigned)(r2 < c2) << (d & 31);
               }
-              mark(j, ra < c2 && j != self_j);
-              mark(j + 1, rb < c2 && j + 1 != self_j);
-            }
-            if (j < e) {
-              const int t = j - w0;
-              double dx[D];
-#pragma unroll
-              for (int d = 0; d < D; ++d) dx[d] = xi[d] - sx[d][t];
-              double r2 = dx[0] * dx[0];
-#pragma unroll
-              for (int d = 1; d < D; ++d) r2 = r2 + dx[d] * dx[d];
-              mark(j, r2 < c2 && j != self_j);
+              if (wsel == 0)
+                lo |= cur;
+              else
+                hi |= cur;
             }
           }
           if (mem) pend = false;
