@@ -196,8 +196,8 @@ struct Gather {
   // One accepted pair (target i, partner j at pool index gj), dx = x_i − x_j
   // and r² as evaluated by _pair_self!/_pair_coupled! (src/Interaction.jl:546-553).
   __device__ __forceinline__ void pair(const Pool<R> P, const SrcDesc &Q, const SrcK<R> &K,
-                                       int64_t gj, const RecA<R> &Aj, const double *dx, double r2,
-                                       double da) {
+                                       int64_t gj, const RecA<R> &Aj, const RecB<R> &Bj,
+                                       const double *dx, double r2, double da) {
     constexpr bool FA = Fast<R>::on;
     R dxr[D], gx[D];
     const R r2r = (R)r2;
@@ -231,7 +231,6 @@ struct Gather {
       // w_sum = 0, zeroed fields and a zero density while still having
       // partners in the force sweeps
       const R w = (R)wq<double>(Q.kk, Q.norm, sqrt(r2) / Q.h);
-      const RecB<R> Bj = P.rb[gj];
       const R rj = Aj.rho;
       const R kw = FA ? w * (K.m_q * Fast<R>::rcp(rj)) : w * (K.m_q / rj);
       const int m = Q.sigma;
@@ -254,7 +253,6 @@ struct Gather {
       // the target gains du·m_partner, which is the reference's update of
       // either particle of the pair (du flips sign exactly under i <-> j)
       const R w = wq<R>(Q.kk, K.norm, FA ? r * K.ih : r / K.h);
-      const RecB<R> Bj = P.rb[gj];
       const R rs = ri + Aj.rho;
       const R ne = -K.alpha;
 #pragma unroll
@@ -279,7 +277,6 @@ struct Gather {
       rj = ri;
       pj = pi;
     } else {
-      const RecB<R> Bj = P.rb[gj];
 #pragma unroll
       for (int d = 0; d < D; ++d) vj[d] = Bj.v[d];
       rj = Aj.rho;
@@ -505,6 +502,16 @@ __device__ inline int strip_base(const int *c, const Grid &G, int st) {
   return (c[0] + st - 1) * G.ng[1] + (c[D - 1] - 1);
 }
 
+// Whether a pair of this family reads the partner's record B (velocity, and
+// pressure in FP64): a dynamic boundary partner mirrors the target instead.
+template <int FAM>
+__device__ inline bool needs_b(const SrcDesc &Q) {
+  if (FAM == FAM_COUNT) return false;
+  if (FAM == FAM_INTERP) return Q.pfn != GR_PFN_NCOUNT;
+  if (FAM == FAM_XSPH) return true;
+  return Q.pk != PK_DYNAMIC;
+}
+
 // Direct cell walk of every source of the task (no list).
 template <int D, typename R, int FAM>
 __device__ void gather_direct(Gather<D, R, FAM> &Gt, const Pool<R> P, const SysDesc *sd,
@@ -540,7 +547,7 @@ __device__ void gather_direct(Gather<D, R, FAM> &Gt, const Pool<R> P, const SysD
           ++ns;
         else
           ++no;
-        Gt.pair(P, Q, K, gj, Aj, dx, r2, da);
+        Gt.pair(P, Q, K, gj, Aj, needs_b<FAM>(Q) ? P.rb[gj] : RecB<R>{}, dx, r2, da);
       }
     }
   }
@@ -770,21 +777,42 @@ __global__ void __launch_bounds__(GR_BLOCK)
       for (int l = 0; l < Q.lsrc; ++l) e += lcnt[L.cnt_off + (int64_t)l * L.cap + i];
       const int eend = e + lcnt[L.cnt_off + (int64_t)Q.lsrc * L.cap + i];
       const double da = (Q.pk == PK_DYNAMIC) ? Gt.da_of(Q) : 0.0;
+      // the next partner's records are loaded while the current pair is
+      // evaluated, and the next entry while its predecessor is popped
+      const bool nb = needs_b<FAM>(Q);
       unsigned long long m = 0ull;
       int base = 0;
-      for (;;) {
+      int4 epre = make_int4(0, 0, 0, 0);
+      if (e < eend) epre = ent[(int64_t)e * L.cap];
+      auto next = [&](int &j) -> bool {
         if (m == 0ull) {  // next entry (entries hold at least one partner)
-          if (e >= eend) break;
-          const int4 en = ent[(int64_t)e * L.cap];
+          if (e >= eend) return false;
+          base = epre.x;
+          m = ((unsigned long long)(unsigned)epre.z << 32) | (unsigned)epre.y;
           ++e;
-          base = en.x;
-          m = ((unsigned long long)(unsigned)en.z << 32) | (unsigned)en.y;
+          if (e < eend) epre = ent[(int64_t)e * L.cap];
         }
-        const int bit = __builtin_ctzll(m);
+        j = base + __builtin_ctzll(m);
         m &= m - 1ull;
-        const int j = base + bit;
-        const int64_t gj = poff + j;
-        const RecA<R> Aj = P.ra[gj];
+        return true;
+      };
+      int jn = 0;
+      bool have = next(jn);
+      RecA<R> An{};
+      RecB<R> Bn{};
+      if (have) {
+        An = P.ra[poff + jn];
+        if (nb) Bn = P.rb[poff + jn];
+      }
+      while (have) {
+        const int64_t gj = poff + jn;
+        const RecA<R> Aj = An;
+        const RecB<R> Bj = Bn;
+        have = next(jn);
+        if (have) {
+          An = P.ra[poff + jn];
+          if (nb) Bn = P.rb[poff + jn];
+        }
         double dx[D];
 #pragma unroll
         for (int d = 0; d < D; ++d) dx[d] = Gt.xi[d] - Aj.x[d];
@@ -797,7 +825,7 @@ __global__ void __launch_bounds__(GR_BLOCK)
           else
             ++no;
         }
-        Gt.pair(P, Q, K, gj, Aj, dx, r2, da);
+        Gt.pair(P, Q, K, gj, Aj, Bj, dx, r2, da);
       }
     }
   }
