@@ -150,6 +150,7 @@ enum PK { P_SELF = PK_SELF, P_TWO = PK_TWO_SIDED, P_ONE = PK_ONE_SIDED, P_STATIC
 struct FamPlan {
   DBuf tasks, srcs, blk;
   int nblk = 0;
+  bool fluid_only = false;  // every source's functor is FluidPfn / FluidSolidPfn
 };
 struct StagePlan {
   DBuf blk_upd, urange, upd;
@@ -1293,6 +1294,9 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
       sp->fam[fam].srcs.upload(sv, c->stream);
       sp->fam[fam].blk.upload(bv, c->stream);
       sp->fam[fam].nblk = (int)bv.size();
+      bool fo = fam == FAM_FORCE;
+      for (const SrcDesc &q : sv) fo = fo && (q.pfn == GR_PFN_FLUID || q.pfn == GR_PFN_FLUIDSOLID);
+      sp->fam[fam].fluid_only = fo;
     }
     P.stages.push_back(std::move(sp));
   }
@@ -1542,6 +1546,7 @@ void phase_stage(gr_ctx *c, Plan &P, int k, bool cnt, hipEvent_t *mid) {
       a.lcnt = c->lcnt.as<int>();
       a.lidx = c->lidx.as<int>();
     }
+    a.fluid_only = F.fluid_only;
     launch_sweep<R>(s, A, a, c->D, fam, cnt);
   }
 }

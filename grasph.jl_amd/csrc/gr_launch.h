@@ -20,6 +20,7 @@ struct SweepArgs {
   const LTask *lt = nullptr;  // non-null: use the neighbour lists
   const int *lcnt = nullptr;
   const int *lidx = nullptr;
+  bool fluid_only = false;  // FORCE: every source's functor is FluidPfn / FluidSolidPfn
 };
 
 struct NlistArgs {

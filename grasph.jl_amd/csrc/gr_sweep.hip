@@ -7,7 +7,7 @@
 // neighbour cells of each interaction it takes part in, evaluates the
 // functor from its own side (which equals the reference's update of that
 // particle: the pair formulas are antisymmetric/symmetric under i <-> j),
-// accumulates in registers (Kahan-compensated in FP32) and writes once.  No
+// accumulates in registers (FP64 sums of the pair terms) and writes once.  No
 // atomics; the result is independent of scheduling.
 //
 // Two passes per step:
@@ -27,16 +27,14 @@
 
 namespace gr {
 
+// Per-target sums of the pair terms, in FP64 whatever the arithmetic type of
+// the terms: one conversion and one add per term (FP64 adds issue at the FP32
+// rate), rounded to R once when stored.
 template <typename R>
-struct Acc {  // Kahan-compensated sum
-  R s = 0, c = 0;
-  __device__ inline void add(R v) {
-    const R y = v - c;
-    const R t = s + y;
-    c = (t - s) - y;
-    s = t;
-  }
-  __device__ inline R get() const { return s - c; }
+struct Acc {
+  double s = 0;
+  __device__ inline void add(R v) { s += (double)v; }
+  __device__ inline R get() const { return (R)s; }
 };
 template <>
 struct Acc<double> {
@@ -157,9 +155,17 @@ struct SrcK {
   }
 };
 
+// FORCE family specialised for stages whose force functors are all FluidPfn
+// or FluidSolidPfn (no stress-based functor): the Cauchy branch and the
+// target's stress are compiled out.  A template argument only, not a task
+// family.
+constexpr int FAM_FORCE_FLUID = NFAM + 1;
+
 // Target particle: its state and its sums.
 template <int D, typename R, int FAM>
 struct Gather {
+  static constexpr bool FORCE = FAM == FAM_FORCE || FAM == FAM_FORCE_FLUID;
+  static constexpr bool FLUID_ONLY = FAM == FAM_FORCE_FLUID;
   double xi[D];
   R vi[D], ri, pi, iri;
   R si[GR_MAXNS];
@@ -177,9 +183,9 @@ struct Gather {
     iri = Fast<R>::rcp(ri);
     pi = 0;
     NS = T.ns;
-    if (FAM == FAM_FORCE) {
+    if (FORCE) {
       if (S.kind >= GR_SYS_FLUID) pi = P.p[gi];
-      if (T.has_stress) {
+      if (!FLUID_ONLY && T.has_stress) {
 #pragma unroll
         for (int k = 0; k < GR_MAXNS; ++k) si[k] = k < NS ? P.stress_s[k][gi] : (R)0;
       }
@@ -200,7 +206,18 @@ struct Gather {
                                        const double *dx, double r2, double da) {
     constexpr bool FA = Fast<R>::on;
     R dxr[D], gx[D];
-    const R r2r = (R)r2;
+#pragma unroll
+    for (int d = 0; d < D; ++d) dxr[d] = (R)dx[d];
+    // in FP32 the squared distance is summed from the rounded components
+    // (the FP64 r² has decided the acceptance already)
+    R r2r;
+    if constexpr (FA) {
+      r2r = dxr[0] * dxr[0];
+#pragma unroll
+      for (int d = 1; d < D; ++d) r2r = fmaf(dxr[d], dxr[d], r2r);
+    } else {
+      r2r = (R)r2;
+    }
     R r, g, ir = 0;
     if (FA) {
       ir = Fast<R>::rsq(r2r);
@@ -211,10 +228,7 @@ struct Gather {
       g = dwdq<R>(Q.kk, K.norm, r / K.h) / (r * K.h);
     }
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-      dxr[d] = (R)dx[d];
-      gx[d] = g * dxr[d];
-    }
+    for (int d = 0; d < D; ++d) gx[d] = g * dxr[d];
     if (FAM == FAM_COUNT) {
       adv[0].add((R)1);
       adr.add((R)1);
@@ -344,7 +358,7 @@ struct Gather {
       for (int d = 1; d < D; ++d) dxg = dxg + dxr[d] * gx[d];
       psi = FA ? K.psi_coef * (ri - rj) * dxg * (ir * ir) : K.psi_coef * (ri - rj) * dxg / r2r;
     }
-    if (Q.pfn == GR_PFN_FLUID || Q.pfn == GR_PFN_FLUIDSOLID) {
+    if (FLUID_ONLY || Q.pfn == GR_PFN_FLUID || Q.pfn == GR_PFN_FLUIDSOLID) {
       // FluidSolidPfn: the fluid's pressure on both sides (the target's on
       // the fluid side, the partner's on the solid side)
       const R pa = Q.psel == 2 ? pj : pi, pb = Q.psel == 1 ? pi : pj;
@@ -864,13 +878,15 @@ template <typename R>
 void launch_sweep(hipStream_t s, const Pool<R> &P, const SweepArgs &a, int D, int fam, bool cnt) {
   if (a.nblk <= 0) return;
   if (D == 2) {
-    if (fam == FAM_FORCE) launch_one<2, R, FAM_FORCE>(s, P, a, cnt);
+    if (fam == FAM_FORCE && a.fluid_only) launch_one<2, R, FAM_FORCE_FLUID>(s, P, a, cnt);
+    else if (fam == FAM_FORCE) launch_one<2, R, FAM_FORCE>(s, P, a, cnt);
     else if (fam == FAM_STRAIN) launch_one<2, R, FAM_STRAIN>(s, P, a, cnt);
     else if (fam == FAM_INTERP) launch_one<2, R, FAM_INTERP>(s, P, a, cnt);
     else if (fam == FAM_XSPH) launch_one<2, R, FAM_XSPH>(s, P, a, cnt);
     else launch_one<2, R, FAM_COUNT>(s, P, a, cnt);
   } else {
-    if (fam == FAM_FORCE) launch_one<3, R, FAM_FORCE>(s, P, a, cnt);
+    if (fam == FAM_FORCE && a.fluid_only) launch_one<3, R, FAM_FORCE_FLUID>(s, P, a, cnt);
+    else if (fam == FAM_FORCE) launch_one<3, R, FAM_FORCE>(s, P, a, cnt);
     else if (fam == FAM_STRAIN) launch_one<3, R, FAM_STRAIN>(s, P, a, cnt);
     else if (fam == FAM_INTERP) launch_one<3, R, FAM_INTERP>(s, P, a, cnt);
     else if (fam == FAM_XSPH) launch_one<3, R, FAM_XSPH>(s, P, a, cnt);
