@@ -241,8 +241,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic,
-                     "kernel": "pair sweep = k_nlist<3,float> + k_sweep_list<3,float,FORCE> "
-                               "(fused gather: self + 5 dynamic walls), one launch each per step",
+                     "kernel": "pair sweep = k_nlist<3,float> + k_sweep_list<3,float,FORCE "
+                               "fluid-only> (fused gather: self + 5 dynamic walls), one launch "
+                               "each per step",
                      "algorithmic_bytes_per_step": bytes_sweep,
                      "launch_ms": sweep_ms,
                      "k_nlist_ms": phases[8], "k_sweep_list_ms": phases[5],
