@@ -727,7 +727,20 @@ __global__ void __launch_bounds__(64)
               const int dend = lim < wend ? lim : wend;
               const int toff = base - w0;
               unsigned cur = 0u;
-              for (; d < dend; ++d) {
+              // two candidates per iteration (independent chains)
+              for (; d + 1 < dend; d += 2) {
+                const int t = d + toff;
+                const double a0 = xi[0] - sx[0][t], b0 = xi[0] - sx[0][t + 1];
+                double ra = a0 * a0, rb = b0 * b0;
+#pragma unroll
+                for (int k = 1; k < D; ++k) {
+                  const double ak = xi[k] - sx[k][t], bk = xi[k] - sx[k][t + 1];
+                  ra = ra + ak * ak;
+                  rb = rb + bk * bk;
+                }
+                cur |= ((unsigned)(ra < c2) | ((unsigned)(rb < c2) << 1)) << (d & 31);
+              }
+              if (d < dend) {
                 const int t = d + toff;
                 const double a0 = xi[0] - sx[0][t];
                 double r2 = a0 * a0;
@@ -737,6 +750,7 @@ __global__ void __launch_bounds__(64)
                   r2 = r2 + ak * ak;
                 }
                 cur |= (unsigned)(r2 < c2) << (d & 31);
+                ++d;
               }
               if (wsel == 0)
                 lo |= cur;
