@@ -727,7 +727,26 @@ __global__ void __launch_bounds__(64)
               const int dend = lim < wend ? lim : wend;
               const int toff = base - w0;
               unsigned cur = 0u;
-              // two candidates per iteration (independent chains)
+              // four, then two candidates per iteration (independent chains)
+              for (; d + 3 < dend; d += 4) {
+                const int t = d + toff;
+                double r[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                  const double a0 = xi[0] - sx[0][t + u];
+                  r[u] = a0 * a0;
+                }
+#pragma unroll
+                for (int k = 1; k < D; ++k)
+#pragma unroll
+                  for (int u = 0; u < 4; ++u) {
+                    const double ak = xi[k] - sx[k][t + u];
+                    r[u] = r[u] + ak * ak;
+                  }
+                cur |= ((unsigned)(r[0] < c2) | ((unsigned)(r[1] < c2) << 1) |
+                        ((unsigned)(r[2] < c2) << 2) | ((unsigned)(r[3] < c2) << 3))
+                       << (d & 31);
+              }
               for (; d + 1 < dend; d += 2) {
                 const int t = d + toff;
                 const double a0 = xi[0] - sx[0][t], b0 = xi[0] - sx[0][t + 1];
