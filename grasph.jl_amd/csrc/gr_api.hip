@@ -224,6 +224,8 @@ struct gr_ctx {
   int cur = 0;
   DBuf vs, rhos, rho_h, gidx, gbnd, gnrm, gxs, gidx_s, gbnd_s;
   DBuf ra, rb;  // partner records (RecA / RecB), rebuilt before every stage's sweeps
+  DBuf rc;      // compact records (RecC) of FP32 pools
+  double qscale = 1.0;  // fixed-point quantum of RecC positions
   DBuf wsum;    // w_sum of virtual systems
   DBuf pcnt;    // neighbour counts of probe systems
   DBuf probe_scr;  // scratch of the probes' sort by id
@@ -320,6 +322,9 @@ Pool<R> pool_view(gr_ctx *c, int w) {
   const int64_t cap = c->pool_cap;
   P.ra = c->ra.as<RecA<R>>();
   P.rb = c->rb.as<RecB<R>>();
+  P.rc = c->rc.as<RecC>();
+  P.qs = c->qscale;
+  P.iqs = 1.0 / c->qscale;
   P.wsum = c->wsum.as<double>();
   P.pcnt = c->pcnt.as<double>();
   for (int k = 0; k < 4; ++k) P.acc[k] = c->acc.as<double>() + k * cap;
@@ -415,6 +420,7 @@ void commit(gr_ctx *c) {
     grow(c, c->pcnt, 1, 8, old, nc);
     grow(c, c->acc, 4, 8, old, nc);
     grow(c, c->rb, 1, R == 4 ? 16 : 32, old, nc);
+    if (R == 4) grow(c, c->rc, 1, 32, old, nc);
     c->pool_cap = nc;
   }
   // v_adjustment starts at zero (src/Particles.jl:82) in both families
@@ -920,10 +926,21 @@ void ensure_cells(gr_ctx *c, const std::vector<int> &inters, const std::vector<i
   const int ni = (int)c->inter.size();
   std::vector<int2> isys(ni), icarr(ni);
   std::vector<double> cellsz(ni);
+  double cmax = 0.0;
   for (int i = 0; i < ni; ++i) {
     isys[i] = make_int2(c->inter[i].a, c->inter[i].b);
     icarr[i] = make_int2(c->inter[i].carr_b, c->inter[i].carr_a);
     cellsz[i] = c->inter[i].cell;
+    cmax = std::max(cmax, c->inter[i].cell);
+  }
+  // RecC quantum: 2^-30 of the power of two at or above the largest cutoff,
+  // so a partner's offset fits 31 bits
+  {
+    int e = 0;
+    if (cmax > 0.0 && std::isfinite(cmax)) frexp(cmax, &e);
+    const double q = cmax > 0.0 && std::isfinite(cmax) ? ldexp(1.0, e - 30) : 1.0;
+    if (q != c->qscale) drop_graphs(c);
+    c->qscale = q;
   }
   c->d_inter_sys.upload(isys, c->stream);
   c->d_inter_carr.upload(icarr, c->stream);

@@ -191,6 +191,23 @@ struct __align__(16) RecB<double> {
   double v[3];
   double p;
 };
+// Compact partner record of the FP32 fluid-only force sweep: one pair costs
+// two 16-byte loads.  Positions are fixed-point multiples of the pool's
+// quantum qs (a power of two, 2^-30 of the smallest power of two at or above
+// every interaction's cutoff), kept modulo 2^32: the difference of two
+// particles within a cutoff is exact in 32-bit arithmetic whatever their
+// absolute position.
+struct __align__(16) RecC {
+  int32_t xq[3];
+  float v[3];
+  float rho, p;
+};
+// x -> its fixed-point coordinate modulo 2^32 (as k_pack and the target side
+// both compute it)
+__device__ inline uint32_t quant_of(double x, double iqs) {
+  return (uint32_t)(unsigned long long)(long long)rint(x * iqs);
+}
+
 __device__ inline float rec_pres(const RecA<float> &a, const RecB<float> &) { return a.p; }
 __device__ inline double rec_pres(const RecA<double> &, const RecB<double> &b) { return b.p; }
 
@@ -198,6 +215,8 @@ template <typename R>
 struct Pool {
   RecA<R> *ra;
   RecB<R> *rb;
+  RecC *rc;        // FP32 pools: compact records (fluid-only force sweeps)
+  double qs, iqs;  // fixed-point quantum of RecC positions and its inverse
   double *wsum;    // virtual systems: interpolation weight sums
   double *acc[4];  // RK4 accumulators of dvdt (D components) and drhodt
   double *vadj[3];  // v_adjustment (XSPH), swapped with the sort

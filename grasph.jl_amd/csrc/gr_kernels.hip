@@ -1377,6 +1377,17 @@ __global__ void k_pack(Pool<R> P, const SysDesc *sd, const int *n_dev, const Blk
   }
   P.ra[g] = a;
   P.rb[g] = r;
+  if constexpr (sizeof(R) == 4) {
+    RecC cc;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      cc.xq[d] = d < D ? (int32_t)quant_of(a.x[d], P.iqs) : 0;
+      cc.v[d] = r.v[d];
+    }
+    cc.rho = a.rho;
+    cc.p = a.p;
+    P.rc[g] = cc;
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -166,6 +166,9 @@ template <int D, typename R, int FAM>
 struct Gather {
   static constexpr bool FORCE = FAM == FAM_FORCE || FAM == FAM_FORCE_FLUID;
   static constexpr bool FLUID_ONLY = FAM == FAM_FORCE_FLUID;
+  // FP32 fluid-only force sweeps read the compact records (RecC)
+  static constexpr bool COMPACT = FLUID_ONLY && sizeof(R) == 4;
+  uint32_t xq[D];  // COMPACT: the target's fixed-point position
   double xi[D];
   R vi[D], ri, pi, iri;
   R si[GR_MAXNS];
@@ -178,6 +181,10 @@ struct Gather {
     for (int d = 0; d < D; ++d) {
       xi[d] = P.x[d][gi];
       vi[d] = P.vs[d][gi];
+    }
+    if constexpr (COMPACT) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) xq[d] = quant_of(xi[d], P.iqs);
     }
     ri = P.rhos[gi];
     iri = Fast<R>::rcp(ri);
@@ -203,11 +210,20 @@ struct Gather {
   // and r² as evaluated by _pair_self!/_pair_coupled! (src/Interaction.jl:546-553).
   __device__ __forceinline__ void pair(const Pool<R> P, const SrcDesc &Q, const SrcK<R> &K,
                                        int64_t gj, const RecA<R> &Aj, const RecB<R> &Bj,
-                                       const double *dx, double r2, double da) {
+                                       const RecC &Cj, const double *dx, double r2, double da) {
     constexpr bool FA = Fast<R>::on;
     R dxr[D], gx[D];
+    int32_t dq[D];  // COMPACT: x_i − x_j in quanta (exact)
+    if constexpr (COMPACT) {
 #pragma unroll
-    for (int d = 0; d < D; ++d) dxr[d] = (R)dx[d];
+      for (int d = 0; d < D; ++d) {
+        dq[d] = (int32_t)(xq[d] - (uint32_t)Cj.xq[d]);
+        dxr[d] = (R)dq[d] * (R)P.qs;
+      }
+    } else {
+#pragma unroll
+      for (int d = 0; d < D; ++d) dxr[d] = (R)dx[d];
+    }
     // in FP32 the squared distance is summed from the rounded components
     // (the FP64 r² has decided the acceptance already)
     R r2r;
@@ -280,9 +296,16 @@ struct Gather {
     R vj[D], rj, pj = (R)0;
     const R m = K.m_q;
     if (Q.pk == PK_DYNAMIC) {  // DynamicBoundarySystem: mirrored velocity
-      double db = (Aj.x[0] - Q.wp[0]) * Q.wn[0];
+      double db;
+      if constexpr (COMPACT) {  // (x_j − p)·n = (x_i − p)·n − (x_i − x_j)·n
+        db = da;
 #pragma unroll
-      for (int d = 1; d < D; ++d) db = db + (Aj.x[d] - Q.wp[d]) * Q.wn[d];
+        for (int d = 0; d < D; ++d) db = db - ((double)dq[d] * P.qs) * Q.wn[d];
+      } else {
+        db = (Aj.x[0] - Q.wp[0]) * Q.wn[0];
+#pragma unroll
+        for (int d = 1; d < D; ++d) db = db + (Aj.x[d] - Q.wp[d]) * Q.wn[d];
+      }
       const double rat = fabs(db / da);
       const double mn = isnan(rat) ? rat : (rat < Q.wbeta ? rat : Q.wbeta);
       const R fac = (R)(-mn);
@@ -290,6 +313,11 @@ struct Gather {
       for (int d = 0; d < D; ++d) vj[d] = fac * vi[d];
       rj = ri;
       pj = pi;
+    } else if constexpr (COMPACT) {
+#pragma unroll
+      for (int d = 0; d < D; ++d) vj[d] = Cj.v[d];
+      rj = Cj.rho;
+      pj = Cj.p;
     } else {
 #pragma unroll
       for (int d = 0; d < D; ++d) vj[d] = Bj.v[d];
@@ -561,7 +589,9 @@ __device__ void gather_direct(Gather<D, R, FAM> &Gt, const Pool<R> P, const SysD
           ++ns;
         else
           ++no;
-        Gt.pair(P, Q, K, gj, Aj, needs_b<FAM>(Q) ? P.rb[gj] : RecB<R>{}, dx, r2, da);
+        constexpr bool CMP = Gather<D, R, FAM>::COMPACT;
+        Gt.pair(P, Q, K, gj, Aj, (!CMP && needs_b<FAM>(Q)) ? P.rb[gj] : RecB<R>{},
+                CMP ? P.rc[gj] : RecC{}, dx, r2, da);
       }
     }
   }
@@ -843,36 +873,44 @@ __global__ void __launch_bounds__(GR_BLOCK)
         m &= m - 1ull;
         return true;
       };
+      constexpr bool CMP = Gather<D, R, FAM>::COMPACT;
       int jn = 0;
       bool have = next(jn);
       RecA<R> An{};
       RecB<R> Bn{};
-      if (have) {
-        An = P.ra[poff + jn];
-        if (nb) Bn = P.rb[poff + jn];
-      }
+      RecC Cn{};
+      auto fetch = [&]() {
+        if constexpr (CMP) {
+          Cn = P.rc[poff + jn];
+        } else {
+          An = P.ra[poff + jn];
+          if (nb) Bn = P.rb[poff + jn];
+        }
+      };
+      if (have) fetch();
       while (have) {
         const int64_t gj = poff + jn;
         const RecA<R> Aj = An;
         const RecB<R> Bj = Bn;
+        const RecC Cj = Cn;
         have = next(jn);
-        if (have) {
-          An = P.ra[poff + jn];
-          if (nb) Bn = P.rb[poff + jn];
+        if (have) fetch();
+        double dx[D] = {};
+        double r2 = 0.0;
+        if constexpr (!CMP) {
+#pragma unroll
+          for (int d = 0; d < D; ++d) dx[d] = Gt.xi[d] - Aj.x[d];
+          r2 = dx[0] * dx[0];
+#pragma unroll
+          for (int d = 1; d < D; ++d) r2 = r2 + dx[d] * dx[d];
         }
-        double dx[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) dx[d] = Gt.xi[d] - Aj.x[d];
-        double r2 = dx[0] * dx[0];
-#pragma unroll
-        for (int d = 1; d < D; ++d) r2 = r2 + dx[d] * dx[d];
         if (CNT) {
           if (Q.pk == PK_SELF)
             ++ns;
           else
             ++no;
         }
-        Gt.pair(P, Q, K, gj, Aj, Bj, dx, r2, da);
+        Gt.pair(P, Q, K, gj, Aj, Bj, Cj, dx, r2, da);
       }
     }
   }
