@@ -172,6 +172,7 @@ struct Plan {
   int nblk_ghost = 0, nblk_cand = 0, n_ghost = 0, n_cand = 0;
   // grids
   std::vector<int> inters;
+  std::vector<std::pair<int, bool>> carr_state;  // (cell array, built by this plan)
   DBuf d_inter_list, blk_cells, blk_mark, carr_blocks;
   int nblk_cells = 0, nblk_mark = 0, ncarr = 0;
   // stages
@@ -223,6 +224,8 @@ struct gr_ctx {
       vor[2], strain[2], strainp[2];
   int cur = 0;
   DBuf vs, rhos, rho_h, gidx, gbnd, gnrm, gxs, gidx_s, gbnd_s;
+  std::vector<char> *carr_used = nullptr;  // set while a step plan collects its sources
+  std::vector<char> carr_valid;  // cell array holds the last grid build's starts
   DBuf ra, rb;  // partner records (RecA / RecB), rebuilt before every stage's sweeps
   DBuf rc;      // compact records (RecC) of FP32 pools
   double qscale = 1.0;  // fixed-point quantum of RecC positions
@@ -749,6 +752,10 @@ SrcDesc make_src(gr_ctx *c, int it, int target, bool b_side, int pk) {
   const int partner = b_side ? a : (in.b < 0 ? a : in.b);
   q.partner = partner;
   q.carr = b_side ? in.carr_a : in.carr_b;
+  if (c->carr_used && q.carr >= 0) {
+    if ((int)c->carr_used->size() <= q.carr) c->carr_used->resize(q.carr + 1, 0);
+    (*c->carr_used)[q.carr] = 1;
+  }
   q.side = b_side ? 1 : 0;
   q.lsrc = 0;
   q.kk = in.kk;
@@ -956,8 +963,12 @@ void ensure_cells(gr_ctx *c, const std::vector<int> &inters, const std::vector<i
       }
 }
 
-// Grid plan (block tables over cell arrays and marked particles).
-void plan_grids(gr_ctx *c, Plan &P, const std::vector<int> &inters) {
+// Grid plan (block tables over cell arrays and marked particles).  With
+// `used`, only the cell arrays some source of the plan reads are built (a
+// coupled interaction whose functor writes only system_a never walks
+// system_a's cells).
+void plan_grids(gr_ctx *c, Plan &P, const std::vector<int> &inters,
+                const std::vector<char> *used = nullptr) {
   P.inters = inters;
   P.d_inter_list.upload(inters, c->stream);
   std::vector<Blk> cb, mb;
@@ -966,6 +977,9 @@ void plan_grids(gr_ctx *c, Plan &P, const std::vector<int> &inters) {
     const InterH &in = c->inter[it];
     for (int ca : {in.carr_b, in.carr_a}) {
       if (ca < 0) continue;
+      const bool skip = used && (ca >= (int)used->size() || !(*used)[ca]);
+      P.carr_state.emplace_back(ca, !skip);
+      if (skip) continue;
       const int first = (int)cb.size();
       add_blocks(cb, ca, c->carr[ca].cap, GR_BLOCK * 4);
       cblocks.push_back(make_int2(first, (int)cb.size()));
@@ -1404,8 +1418,16 @@ Plan &get_plan(gr_ctx *c, const std::vector<int> &integs) {
     P->nblk_probe = (int)bp.size();
   }
   plan_ghosts(c, *P, ghosts);
-  plan_grids(c, *P, inters);
-  plan_stages(c, *P, false);
+  std::vector<char> used;
+  c->carr_used = &used;
+  try {
+    plan_stages(c, *P, false);
+  } catch (...) {
+    c->carr_used = nullptr;
+    throw;
+  }
+  c->carr_used = nullptr;
+  plan_grids(c, *P, inters, &used);
   HIPCHK(hipStreamSynchronize(c->stream));
   Plan &ref = *P;
   c->plans[integs] = std::move(P);
@@ -1503,6 +1525,14 @@ void phase_ghosts(gr_ctx *c, Plan &P, bool generate, bool sort) {
                        xpw_pool(c, c->cur), A.gidx, A.gbnd, c->d_sd.as<SysDesc>(),
                        c->d_n.as<int>(), P.blk_ghost.as<Blk>(), P.nblk_ghost,
                        P.d_bbase_ghost.as<int>(), c->vals.as<int>(), c->d_flags.as<int>(), c->D);
+}
+
+// Host record of which cell arrays the plan's grid build fills.
+void mark_grids(gr_ctx *c, const Plan &P) {
+  for (const auto &e : P.carr_state) {
+    if ((int)c->carr_valid.size() <= e.first) c->carr_valid.resize(e.first + 1, 0);
+    c->carr_valid[e.first] = e.second ? 1 : 0;
+  }
 }
 
 void phase_grids(gr_ctx *c, Plan &P) {
@@ -1749,6 +1779,7 @@ void do_steps(gr_ctx *c, const std::vector<int> &integs, int nsteps) {
     all_real(c, b, nb);  // build cached tables before any capture
   }
   if (c->count_pairs) launch_fill_i32(c->stream, c->d_pairs.as<int>(), 0, 4);
+  if (nsteps > 0) mark_grids(c, P);
   const int nst = (int)P.stages.size();
   for (int k = 0; k < nsteps; ++k) {
     if (c->timing)
@@ -2615,6 +2646,7 @@ int gr_create_grid(gr_ctx *c, int inter) {
     Plan P;
     plan_grids(c, P, {inter});
     phase_grids(c, P);
+    mark_grids(c, P);
     int flags[4];
     HIPCHK(hipMemcpyAsync(flags, c->d_flags.p, sizeof(flags), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -2937,6 +2969,9 @@ int gr_get_cell_start(gr_ctx *c, int inter, int which, int64_t *out, int64_t cou
     const InterH &in = c->inter[inter];
     const int ca = which ? in.carr_a : in.carr_b;
     need(ca >= 0, GR_ERR_ARGUMENT, "no such cell array");
+    need(ca < (int)c->carr_valid.size() && c->carr_valid[ca], GR_ERR_STATE,
+         "cell array not built by the last grid build (a step builds only the cells its "
+         "functors walk): call gr_create_grid first");
     Grid G;
     HIPCHK(hipMemcpyAsync(&G, c->d_grids.as<Grid>() + inter, sizeof(G), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
