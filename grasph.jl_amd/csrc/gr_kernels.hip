@@ -216,30 +216,31 @@ __global__ void k_key_write(XP xs, const SysDesc *sd, const int *n_dev,
                             unsigned *keys, int *vals, int *flags) {
   const Blk b = blk[blockIdx.x];
   const int s = b.item;
+  const int lane = threadIdx.x & 63;
   const int i = b.first + threadIdx.x;
   const int cap = sd[s].cap;
-  if (i >= cap) return;
   const int n = n_dev[s];
   const int64_t gi = sd[s].off + i;
-  const int p = bbase[s] + i;
   const double cutoff = st[s].sort_cutoff;
   unsigned key;
   if (i < n) {
-    unsigned rng = 1;
-    for (int d = 0; d < D; ++d) rng *= krange[s * 3 + d];
     key = kbase[s] + lin_key<D>(xs, gi, cutoff, kmin, krange, s);
-    if (i + 1 < n) {
-      unsigned k2 = kbase[s] + lin_key<D>(xs, gi + 1, cutoff, kmin, krange, s);
-      if (k2 < key) atomicOr(&flags[0], 1);
-    }
-    (void)rng;
   } else {
     unsigned rng = 1;
     for (int d = 0; d < D; ++d) rng *= krange[s * 3 + d];
     key = kbase[s] + (n > 0 ? rng : 0u);
   }
-  keys[p] = key;
-  vals[p] = (int)gi;
+  // the next particle's key: from the next lane, computed only by the last
+  // lane of each wavefront (every lane is active here)
+  unsigned k2 = __shfl_down(key, 1, 64);
+  if (lane == 63 && i + 1 < n) k2 = kbase[s] + lin_key<D>(xs, gi + 1, cutoff, kmin, krange, s);
+  const bool inv = i + 1 < n && k2 < key;
+  if (__ballot(inv) != 0ull && lane == 0) atomicOr(&flags[0], 1);
+  if (i < cap) {
+    const int p = bbase[s] + i;
+    keys[p] = key;
+    vals[p] = (int)gi;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1400,7 +1401,9 @@ void launch_key_minmax(hipStream_t s, XP xs, const SysDesc *sd, const int *n, co
                        const Blk *blk, int nblk, int D, int *kmin, int *kmax, long long *bbmin,
                        long long *bbmax, int with_keys) {
   if (nblk <= 0) return;
-  const int per = 16;
+  // 4 table entries (1024 particles) per workgroup: enough workgroups to fill
+  // the chip, few enough that the per-system atomics stay cheap
+  const int per = 4;
   const unsigned g = nb_of(nblk, per);
   if (D == 2)
     k_key_minmax<2><<<g, GR_BLOCK, 0, s>>>(xs, sd, n, st, blk, nblk, per, kmin, kmax, bbmin,
