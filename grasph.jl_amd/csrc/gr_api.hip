@@ -1559,11 +1559,16 @@ void phase_stage(gr_ctx *c, Plan &P, int k, bool cnt, hipEvent_t *mid) {
   launch_ghost_copy<R>(s, A, c->d_sd.as<SysDesc>(), c->d_bnd.as<Bnd>(), c->d_n.as<int>(),
                        S.blk_gcopy.as<Blk>(), S.nblk_gcopy, S.gcopy.as<int2>(), c->D);
   if (mid) HIPCHK(hipEventRecord(*mid, s));
+  // a stage whose only sweeps are FP32 fluid-only force sweeps reads the
+  // compact records alone
+  const bool c_only = sizeof(R) == 4 && S.fam[FAM_FORCE].fluid_only &&
+                      S.fam[FAM_INTERP].nblk == 0 && S.fam[FAM_COUNT].nblk == 0 &&
+                      S.fam[FAM_STRAIN].nblk == 0;
   auto pack = [&]() {
     launch_pack<R>(s, A, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), P.blk_real.as<Blk>(),
-                   P.nblk_real, c->D);
+                   P.nblk_real, c->D, c_only);
     launch_pack<R>(s, A, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), P.blk_ghost.as<Blk>(),
-                   P.nblk_ghost, c->D);
+                   P.nblk_ghost, c->D, c_only);
   };
   const int nother = S.fam[FAM_FORCE].nblk + S.fam[FAM_COUNT].nblk + S.fam[FAM_STRAIN].nblk;
   if (nother + S.fam[FAM_INTERP].nblk > 0) pack();

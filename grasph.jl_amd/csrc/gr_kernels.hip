@@ -1356,7 +1356,7 @@ __global__ void k_fill_i32(int *p, int v, long long n) {
 
 // Partner records of the live particles of the block's system (positions,
 // shadows of density and velocity, pressure), read by the pair sweeps.
-template <typename R, int D>
+template <typename R, int D, bool C_ONLY>
 __global__ void k_pack(Pool<R> P, const SysDesc *sd, const int *n_dev, const Blk *blk) {
   const Blk b = blk[blockIdx.x];
   const int i = b.first + threadIdx.x;
@@ -1376,8 +1376,10 @@ __global__ void k_pack(Pool<R> P, const SysDesc *sd, const int *n_dev, const Blk
   } else {
     r.p = P.p[g];
   }
-  P.ra[g] = a;
-  P.rb[g] = r;
+  if (!C_ONLY) {
+    P.ra[g] = a;
+    P.rb[g] = r;
+  }
   if constexpr (sizeof(R) == 4) {
     RecC cc;
 #pragma unroll
@@ -1647,17 +1649,24 @@ template void launch_virtual<double>(hipStream_t, const Pool<double> &, const Sy
 
 template <typename R>
 void launch_pack(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const int *n, const Blk *blk,
-                 int nblk, int D) {
+                 int nblk, int D, bool c_only) {
   if (nblk <= 0) return;
+  if (c_only && sizeof(R) == 4) {
+    if (D == 2)
+      k_pack<R, 2, true><<<nblk, GR_BLOCK, 0, s>>>(P, sd, n, blk);
+    else
+      k_pack<R, 3, true><<<nblk, GR_BLOCK, 0, s>>>(P, sd, n, blk);
+    return;
+  }
   if (D == 2)
-    k_pack<R, 2><<<nblk, GR_BLOCK, 0, s>>>(P, sd, n, blk);
+    k_pack<R, 2, false><<<nblk, GR_BLOCK, 0, s>>>(P, sd, n, blk);
   else
-    k_pack<R, 3><<<nblk, GR_BLOCK, 0, s>>>(P, sd, n, blk);
+    k_pack<R, 3, false><<<nblk, GR_BLOCK, 0, s>>>(P, sd, n, blk);
 }
 template void launch_pack<float>(hipStream_t, const Pool<float> &, const SysDesc *, const int *,
-                                 const Blk *, int, int);
+                                 const Blk *, int, int, bool);
 template void launch_pack<double>(hipStream_t, const Pool<double> &, const SysDesc *, const int *,
-                                  const Blk *, int, int);
+                                  const Blk *, int, int, bool);
 
 // Probe slot id[i] − 1 takes the position of mirror particle i
 // (_measure_probes!, src/TimeIntegration.jl:469-479).

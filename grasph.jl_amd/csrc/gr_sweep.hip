@@ -577,10 +577,18 @@ __device__ void gather_direct(Gather<D, R, FAM> &Gt, const Pool<R> P, const SysD
       for (int j = j0; j < j1; ++j) {
         if (Q.pk == PK_SELF && j == i) continue;
         const int64_t gj = poff + j;
-        const RecA<R> Aj = P.ra[gj];
+        constexpr bool CMP = Gather<D, R, FAM>::COMPACT;
+        // compact sweeps read no RecA: the walk's FP64 test uses the positions
+        RecA<R> Aj{};
         double dx[D];
+        if constexpr (CMP) {
 #pragma unroll
-        for (int d = 0; d < D; ++d) dx[d] = Gt.xi[d] - Aj.x[d];
+          for (int d = 0; d < D; ++d) dx[d] = Gt.xi[d] - P.x[d][gj];
+        } else {
+          Aj = P.ra[gj];
+#pragma unroll
+          for (int d = 0; d < D; ++d) dx[d] = Gt.xi[d] - Aj.x[d];
+        }
         double r2 = dx[0] * dx[0];
 #pragma unroll
         for (int d = 1; d < D; ++d) r2 = r2 + dx[d] * dx[d];
@@ -589,7 +597,6 @@ __device__ void gather_direct(Gather<D, R, FAM> &Gt, const Pool<R> P, const SysD
           ++ns;
         else
           ++no;
-        constexpr bool CMP = Gather<D, R, FAM>::COMPACT;
         Gt.pair(P, Q, K, gj, Aj, (!CMP && needs_b<FAM>(Q)) ? P.rb[gj] : RecB<R>{},
                 CMP ? P.rc[gj] : RecC{}, dx, r2, da);
       }
