@@ -273,10 +273,10 @@ __global__ void k_radix_hist(const unsigned *keys, int N, int shift, unsigned *h
   hist[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of 256·nblk counters in digit-major order (one block of
-// 1024 threads).  Each wavefront owns a contiguous segment and walks it in
-// rows of 64 consecutive counters (coalesced loads), first summing it, then,
-// after a scan of the 16 segment sums, writing each row's exclusive prefix.
+// Exclusive scan of each digit's column of per-block counts (one wavefront
+// per digit, all 256 in parallel): hist[d·nblk + b] becomes the count of
+// digit d in blocks before b, and hist[256·nblk + d] the digit's total.  The
+// scatter adds the exclusive prefix of the totals.
 __device__ inline unsigned wave_incl_scan(unsigned v, int lane) {
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
@@ -286,30 +286,20 @@ __device__ inline unsigned wave_incl_scan(unsigned v, int lane) {
   return v;
 }
 
-__global__ void __launch_bounds__(1024) k_radix_scan(unsigned *hist, int total, int shift,
-                                                     const int *flags) {
+__global__ void __launch_bounds__(64) k_radix_scan(unsigned *hist, int nblk, int shift,
+                                                   const int *flags) {
   if (flags[0] == 0 || radix_pass_idle(flags, shift)) return;
-  __shared__ unsigned wsum[16];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int seg = (total + 15) / 16;
-  const int b0 = w * seg, b1 = min(total, b0 + seg);
-  unsigned s = 0;
-  for (int r = b0; r < b1; r += 64) {
-    const int k = r + lane;
-    s += k < b1 ? hist[k] : 0u;
-  }
-  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-  if (lane == 0) wsum[w] = s;
-  __syncthreads();
+  const int lane = threadIdx.x;
+  unsigned *col = hist + (size_t)blockIdx.x * nblk;
   unsigned carry = 0;
-  for (int q = 0; q < w; ++q) carry += wsum[q];
-  for (int r = b0; r < b1; r += 64) {
+  for (int r = 0; r < nblk; r += 64) {
     const int k = r + lane;
-    const unsigned c = k < b1 ? hist[k] : 0u;
+    const unsigned c = k < nblk ? col[k] : 0u;
     const unsigned inc = wave_incl_scan(c, lane);
-    if (k < b1) hist[k] = carry + inc - c;
+    if (k < nblk) col[k] = carry + inc - c;
     carry += __shfl(inc, 63, 64);
   }
+  if (lane == 0) hist[(size_t)256 * nblk + blockIdx.x] = carry;
 }
 
 // Stable scatter: element e of the tile is ranked after every earlier
@@ -332,8 +322,17 @@ __global__ void k_radix_scatter(const unsigned *kin, const int *vin, unsigned *k
   }
   __shared__ unsigned run[256];
   __shared__ unsigned wcnt[GR_BLOCK / 64][256];
+  __shared__ unsigned wtot[GR_BLOCK / 64];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  run[t] = hist[t * nblk + blockIdx.x];
+  {  // digit t starts after the totals of the digits below it
+    const unsigned tot = hist[(size_t)256 * nblk + t];
+    const unsigned inc = wave_incl_scan(tot, lane);
+    if (lane == 63) wtot[w] = inc;
+    __syncthreads();
+    unsigned off = 0;
+    for (int q = 0; q < w; ++q) off += wtot[q];
+    run[t] = hist[t * nblk + blockIdx.x] + off + inc - tot;
+  }
   for (int q = 0; q < GR_BLOCK / 64; ++q) wcnt[q][t] = 0;
   __syncthreads();
   const int base = blockIdx.x * RADIX_TILE;
@@ -1446,7 +1445,7 @@ void launch_radix_sort(hipStream_t s, unsigned *keys, int *vals, unsigned *keys2
   for (int pass = 0; pass < 4; ++pass) {
     const int shift = pass * 8;
     k_radix_hist<<<nblk, GR_BLOCK, 0, s>>>(ki, N, shift, hist, nblk, flags);
-    k_radix_scan<<<1, 1024, 0, s>>>(hist, 256 * nblk, shift, flags);
+    k_radix_scan<<<256, 64, 0, s>>>(hist, nblk, shift, flags);
     k_radix_scatter<<<nblk, GR_BLOCK, 0, s>>>(ki, vi, ko, vo, N, shift, hist, nblk, flags);
     unsigned *tk = ki;
     ki = ko;
@@ -1456,7 +1455,7 @@ void launch_radix_sort(hipStream_t s, unsigned *keys, int *vals, unsigned *keys2
     vo = tv;
   }
 }
-int radix_hist_entries(int N) { return 256 * (int)nb_of(N > 0 ? N : 1, RADIX_TILE); }
+int radix_hist_entries(int N) { return 256 * (int)nb_of(N > 0 ? N : 1, RADIX_TILE) + 256; }
 
 template <typename R>
 void launch_permute_real(hipStream_t s, const Pool<R> &A, const Pool<R> &B, const SysDesc *sd,
