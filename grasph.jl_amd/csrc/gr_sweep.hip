@@ -712,12 +712,22 @@ __global__ void __launch_bounds__(64)
       const int64_t poff = sd[Q.partner].off;
       int c[D];
       target_cell<D>(xi, G, Q.cell, c);
-      for (int st = 0; st < ((D == 3) ? 9 : 3); ++st) {
-        int j0 = 0, j1 = 0;
-        if (live) {
-          const int sb = strip_base<D>(c, G, st);
-          j0 = cs[sb] - 1;
-          j1 = cs[sb + 3] - 1;
+      constexpr int NST = (D == 3) ? 9 : 3;
+      // the next strip's range is loaded while the current strip is tested;
+      // a lane past the system's end keeps the empty range [0, 0) (a
+      // negative base would run its candidate loop)
+      int n0 = 1, n1 = 1;
+      if (live) {
+        const int sb = strip_base<D>(c, G, 0);
+        n0 = cs[sb];
+        n1 = cs[sb + 3];
+      }
+      for (int st = 0; st < NST; ++st) {
+        const int j0 = n0 - 1, j1 = n1 - 1;
+        if (live && st + 1 < NST) {
+          const int sb = strip_base<D>(c, G, st + 1);
+          n0 = cs[sb];
+          n1 = cs[sb + 3];
         }
         bool pend = j0 < j1;
         if (__builtin_amdgcn_ballot_w64(pend) == 0ull) continue;
@@ -728,7 +738,7 @@ __global__ void __launch_bounds__(64)
           const unsigned sdl = (unsigned)(self_j - base);
           if (sdl < 32u) lo &= ~(1u << sdl);
           else if (sdl < 64u) hi &= ~(1u << (sdl - 32u));
-          if ((lo | hi) != 0u) {
+          if (live && (lo | hi) != 0u) {
             if (ne < L.maxnb) ent[(int64_t)ne * L.cap] = make_int4(base, (int)lo, (int)hi, 0);
             ++ne;
             ++cq;
