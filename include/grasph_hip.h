@@ -297,7 +297,10 @@ int gr_update_ghost_kinematics(gr_ctx *ctx, int ghost);
 int gr_update_ghost(gr_ctx *ctx, int ghost, int stage);
 /* Grid introspection (src/Interaction.jl:29-34): mingridx[ndims],
  * ngridx[ndims], cell_start (which = 0: _cell_start, 1: _cell_start_a),
- * length ncells+1, 1-based values. */
+ * length ncells+1, 1-based values.  A step builds only the cell arrays its
+ * functors walk; asking for one the last grid build skipped (system_a's of
+ * a coupled interaction whose functor writes only system_a) returns
+ * GR_ERR_STATE, and gr_create_grid builds both. */
 int gr_get_grid(gr_ctx *ctx, int inter, double *mingridx, int64_t *ngridx,
                 int64_t *ncells);
 int gr_get_cell_start(gr_ctx *ctx, int inter, int which, int64_t *out,
