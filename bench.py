@@ -113,6 +113,21 @@ def sweep_bytes(cfg, grids):
     return nf * (8 * D + 4 * D + 4 + 4) + nf * (4 * D + 4) + nb * 8 * D + 4 * cells
 
 
+def sort_bytes(cfg, key_cells, precision):
+    """Algorithmic bytes of the sort of one step: the key pass reads x (8D)
+    and writes a key (4), the ranking reads and writes a pool index (8), the
+    histogram and its scan touch 8 B per key cell, and the permutation reads
+    and writes every per-particle array once (2·S_state: x 8D, v 8D, ρ 8, id
+    4, dvdt and drhodt in the state precision, p for fluids)."""
+    D = 3
+    r = 4 if precision == "f32" else 8
+    nf, nb = cfg["fluid_n"], cfg["boundary_n"]
+    n = nf + nb
+    s_fluid = 8 * D + 8 * D + 8 + 4 + r * D + r + r
+    s_bnd = 8 * D + 8 * D + 8 + 4 + r * D + r
+    return n * (8 * D + 4) + 8 * n + 8 * key_cells + 2 * (nf * s_fluid + nb * s_bnd)
+
+
 def load_traffic():
     """HBM bytes of the sweep per step (the k_nlist and k_sweep_list
     dispatches of one step) from the committed rocprofv3 --pmc summary
@@ -213,6 +228,9 @@ def main():
     bytes_sweep = sweep_bytes(cfg, grids)
     sweep_ms = phases[5] + phases[8]
     achieved = bytes_sweep / (sweep_ms * 1e-3) / 1e9
+    # the sort's keys span the fluid's cells of side 2h (its self grid)
+    bytes_sort = sort_bytes(cfg, grids[0] if grids else 0, args.precision)
+    sort_gbs = bytes_sort / (phases[0] * 1e-3) / 1e9 if phases[0] > 0 else None
     traffic = load_traffic()
     cpu = cpu_baseline(g.configs.dambreak_3d, kw, args.cpu_steps, pairs)
     ens = ensemble_leg(args.ensemble, args.ensemble_steps, args.precision) if args.ensemble else None
@@ -248,6 +266,12 @@ def main():
                      "launch_ms": sweep_ms,
                      "k_nlist_ms": phases[8], "k_sweep_list_ms": phases[5],
                      "ordered_pairs": self_ordered + coupled},
+        "sort_roofline": {"bound": "hbm", "achieved": sort_gbs, "peak": HBM_PEAK_GBS,
+                          "unit": "GB/s",
+                          "frac": sort_gbs / HBM_PEAK_GBS if sort_gbs else None,
+                          "algorithmic_bytes_per_step": bytes_sort, "phase_ms": phases[0],
+                          "kernels": "key min/max, key write, LSD radix (4 x 8 bits), "
+                                     "permute of every per-particle array"},
         "cpu_baseline": cpu,
         "steps_per_s": 1e3 / ms_step,
         "phase_ms": {k: round(v, 4) for k, v in zip(
