@@ -498,6 +498,64 @@ def test_graph_mode_equals_eager():
         assert np.array_equal(res[0][f], res[1][f]), f
 
 
+def _translated_dambreak(offset, **kw):
+    """The 2D dam break moved by `offset` (particles and wall points)."""
+    cfg = M.configs.dambreak_2d(**kw)
+    lf = cfg["integrator"]
+    off = np.asarray(offset, float)
+    for ps in lf.systems:
+        ps.x[:] = ps.x + off
+    for si in lf.interactions:
+        b = si.system_b
+        if isinstance(b, M.DynamicBoundarySystem):
+            b.boundary_point = b.boundary_point + off
+    return cfg
+
+
+@pytest.mark.parametrize("offset", [(0.0, 0.0), (1.0e5, -3.0e4)])
+def test_fp32_steps_translation_invariant(offset):
+    """FP32 steps far from the origin: the fluid-only force sweep takes pair
+    offsets from 32-bit fixed-point positions kept modulo 2^32 (here wrapping
+    about a million times), and must match the oracle as closely as at the
+    origin."""
+    kw = dict(dx=0.04)
+    dev = _run_steps(lambda **k: _translated_dambreak(offset, **k), kw, 2, "f32", 20, "dev")
+    orc = _run_steps(lambda **k: _translated_dambreak(offset, **k), kw, 2, "f64", 20, "orc")
+    for key in (k for k in orc if k.endswith((".rho", ".v", ".p"))):
+        ref = orc[key]
+        if not np.any(ref):
+            continue
+        e = rel(dev[key], ref)
+        tol = 2e-4 if key.endswith((".v", ".p")) else 1e-5
+        assert e < tol, f"{key}: {e:.3e}"
+    f = "fluid.x"
+    assert np.max(np.abs(dev[f] - orc[f])) < 1e-9 * (1.0 + np.max(np.abs(orc[f])))
+
+
+def test_step_skips_unwalked_cells_and_create_grid_builds_them():
+    """A step builds no fluid cells for the fluid–wall grids (the dynamic
+    wall functor writes only the fluid); reading them then fails with a
+    state error instead of returning stale starts, and create_grid builds
+    them."""
+    cfg = M.configs.dambreak_2d(dx=0.04)
+    lf = cfg["integrator"]
+    sc = M.Session(2, "f32").scene
+    h = sc.integrator(lf)
+    sc.upload_all(lf)
+    sc.be.step(h, 2, sc.be.integrator_dt(h, cfg["cfl"]))
+    si = lf.interactions[1]
+    assert si.system_b is not None
+    hi = sc.interaction(si)
+    _, _, ncells = sc.be.get_grid(hi, 2)
+    assert len(sc.be.get_cell_start(hi, 0, ncells + 1)) == ncells + 1
+    with pytest.raises(Exception, match="not built"):
+        sc.be.get_cell_start(hi, 1, ncells + 1)
+    sc.be.create_grid(hi)
+    _, _, ncells = sc.be.get_grid(hi, 2)
+    cs = sc.be.get_cell_start(hi, 1, ncells + 1)
+    assert cs[-1] == cfg["fluid"].n + 1
+
+
 LIST_CASES = [
     ("ep_column_2d", M.configs.ep_column_2d, dict(dx=0.01, width=0.2, height=0.1), 2, "f32"),
     ("dambreak_3d", M.configs.dambreak_3d, dict(dx=0.04, water=(0.24, 0.2, 0.24),
