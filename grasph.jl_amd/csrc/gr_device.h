@@ -165,8 +165,8 @@ struct Pool;
 // Partner records: what a pair functor reads of a partner particle, packed
 // per particle so that one pair costs three 16-byte loads (RecA: position and
 // the scalar fields, RecB: half-step velocity) instead of eight scattered
-// 4/8-byte gathers from separate arrays.  Written by k_pack before the
-// sweeps of each stage, from the arrays below.
+// 4/8-byte gathers from separate arrays (two loads with RecC below).  Written
+// by k_pack before the sweeps of each stage, from the arrays below.
 template <typename R>
 struct RecA;
 template <>

@@ -16,7 +16,9 @@
 //    in its own CSR ranges with r² < (2h)², evaluated in FP64 in the
 //    reference's order, so the accepted pair set equals the reference's.
 //  * k_sweep_list: the functors over each lane's accepted partners, once per
-//    stage, so every lane does useful arithmetic on every iteration.
+//    stage, so every lane does useful arithmetic on every iteration.  The
+//    FP32 fluid-only force sweep reads 32-byte compact partner records
+//    (fixed-point positions, see RecC) with two loads per pair.
 // k_sweep walks the cells directly (fine-grained sweep! calls, and targets
 // whose list overflowed).
 #include <hip/hip_runtime.h>
