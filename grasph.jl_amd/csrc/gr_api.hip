@@ -228,6 +228,7 @@ struct gr_ctx {
   std::vector<char> carr_valid;  // cell array holds the last grid build's starts
   DBuf ra, rb;  // partner records (RecA / RecB), rebuilt before every stage's sweeps
   DBuf rc;      // compact records (RecC) of FP32 pools
+  DBuf rs;      // partner stress records (RecS) of FP32 pools
   double qscale = 1.0;  // fixed-point quantum of RecC positions
   DBuf wsum;    // w_sum of virtual systems
   DBuf pcnt;    // neighbour counts of probe systems
@@ -326,6 +327,7 @@ Pool<R> pool_view(gr_ctx *c, int w) {
   P.ra = c->ra.as<RecA<R>>();
   P.rb = c->rb.as<RecB<R>>();
   P.rc = c->rc.as<RecC>();
+  P.rs = c->rs.as<RecS>();
   P.qs = c->qscale;
   P.iqs = 1.0 / c->qscale;
   P.wsum = c->wsum.as<double>();
@@ -424,6 +426,7 @@ void commit(gr_ctx *c) {
     grow(c, c->acc, 4, 8, old, nc);
     grow(c, c->rb, 1, R == 4 ? 16 : 32, old, nc);
     if (R == 4) grow(c, c->rc, 1, 32, old, nc);
+    if (R == 4) grow(c, c->rs, 1, 32, old, nc);
     c->pool_cap = nc;
   }
   // v_adjustment starts at zero (src/Particles.jl:82) in both families
@@ -1559,11 +1562,10 @@ void phase_stage(gr_ctx *c, Plan &P, int k, bool cnt, hipEvent_t *mid) {
   launch_ghost_copy<R>(s, A, c->d_sd.as<SysDesc>(), c->d_bnd.as<Bnd>(), c->d_n.as<int>(),
                        S.blk_gcopy.as<Blk>(), S.nblk_gcopy, S.gcopy.as<int2>(), c->D);
   if (mid) HIPCHK(hipEventRecord(*mid, s));
-  // a stage whose only sweeps are FP32 fluid-only force sweeps reads the
+  // a stage whose only sweeps are FP32 force and strain-rate sweeps reads the
   // compact records alone
-  const bool c_only = sizeof(R) == 4 && S.fam[FAM_FORCE].fluid_only &&
-                      S.fam[FAM_INTERP].nblk == 0 && S.fam[FAM_COUNT].nblk == 0 &&
-                      S.fam[FAM_STRAIN].nblk == 0;
+  const bool c_only = sizeof(R) == 4 && S.fam[FAM_INTERP].nblk == 0 &&
+                      S.fam[FAM_COUNT].nblk == 0;
   auto pack = [&]() {
     launch_pack<R>(s, A, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), P.blk_real.as<Blk>(),
                    P.nblk_real, c->D, c_only);

@@ -202,6 +202,11 @@ struct __align__(16) RecC {
   float v[3];
   float rho, p;
 };
+// Partner stress of compact Cauchy sweeps (NS <= 4 in the first half).
+struct __align__(16) RecS {
+  float4 a, b;
+};
+
 // x -> its fixed-point coordinate modulo 2^32 (as k_pack and the target side
 // both compute it)
 __device__ inline uint32_t quant_of(double x, double iqs) {
@@ -215,7 +220,8 @@ template <typename R>
 struct Pool {
   RecA<R> *ra;
   RecB<R> *rb;
-  RecC *rc;        // FP32 pools: compact records (fluid-only force sweeps)
+  RecC *rc;        // FP32 pools: compact records (force and strain-rate sweeps)
+  RecS *rs;        // FP32 pools: partner stress of compact Cauchy sweeps
   double qs, iqs;  // fixed-point quantum of RecC positions and its inverse
   double *wsum;    // virtual systems: interpolation weight sums
   double *acc[4];  // RK4 accumulators of dvdt (D components) and drhodt

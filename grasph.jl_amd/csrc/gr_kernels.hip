@@ -1389,6 +1389,17 @@ __global__ void k_pack(Pool<R> P, const SysDesc *sd, const int *n_dev, const Blk
     cc.rho = a.rho;
     cc.p = a.p;
     P.rc[g] = cc;
+    const SysDesc S = sd[b.item];
+    float sv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (S.kind >= GR_SYS_STRESS) {
+#pragma unroll
+      for (int k = 0; k < GR_MAXNS; ++k)
+        if (k < S.ns) sv[k] = (float)P.stress_s[k][g];
+    }
+    RecS rs;
+    rs.a = make_float4(sv[0], sv[1], sv[2], sv[3]);
+    rs.b = make_float4(sv[4], sv[5], sv[6], sv[7]);
+    P.rs[g] = rs;
   }
 }
 

@@ -169,7 +169,7 @@ struct Gather {
   static constexpr bool FORCE = FAM == FAM_FORCE || FAM == FAM_FORCE_FLUID;
   static constexpr bool FLUID_ONLY = FAM == FAM_FORCE_FLUID;
   // FP32 fluid-only force sweeps read the compact records (RecC)
-  static constexpr bool COMPACT = FLUID_ONLY && sizeof(R) == 4;
+  static constexpr bool COMPACT = (FORCE || FAM == FAM_STRAIN) && sizeof(R) == 4;
   uint32_t xq[D];  // COMPACT: the target's fixed-point position
   double xi[D];
   R vi[D], ri, pi, iri;
@@ -212,7 +212,8 @@ struct Gather {
   // and r² as evaluated by _pair_self!/_pair_coupled! (src/Interaction.jl:546-553).
   __device__ __forceinline__ void pair(const Pool<R> P, const SrcDesc &Q, const SrcK<R> &K,
                                        int64_t gj, const RecA<R> &Aj, const RecB<R> &Bj,
-                                       const RecC &Cj, const double *dx, double r2, double da) {
+                                       const RecC &Cj, const RecS &Sj, const double *dx,
+                                       double r2, double da) {
     constexpr bool FA = Fast<R>::on;
     R dxr[D], gx[D];
     int32_t dq[D];  // COMPACT: x_i − x_j in quanta (exact)
@@ -426,7 +427,14 @@ struct Gather {
         for (int k = 0; k < GR_MAXNS; ++k) sj[k] = si[k];
       } else {
 #pragma unroll
-        for (int k = 0; k < GR_MAXNS; ++k) sj[k] = k < NS ? P.stress_s[k][gj] : (R)0;
+        for (int k = 0; k < GR_MAXNS; ++k) {
+          if constexpr (COMPACT) {
+            const float sk = k < 4 ? (&Sj.a.x)[k] : (&Sj.b.x)[k - 4];
+            sj[k] = k < NS ? (R)sk : (R)0;
+          } else {
+            sj[k] = k < NS ? P.stress_s[k][gj] : (R)0;
+          }
+        }
       }
       const R ci = FA ? iri * iri : (R)1 / (ri * ri), cj = FA ? irj * irj : (R)1 / (rj * rj);
       R hv[D];
@@ -556,6 +564,14 @@ __device__ inline bool needs_b(const SrcDesc &Q) {
   return Q.pk != PK_DYNAMIC;
 }
 
+// Whether a compact force pair reads the partner's stress record: Cauchy
+// functors against a partner with its own stress (a dynamic boundary mirrors
+// the target's).
+template <int FAM>
+__device__ inline bool needs_s(const SrcDesc &Q) {
+  return FAM == FAM_FORCE && Q.pfn == GR_PFN_CAUCHY && Q.pk != PK_DYNAMIC;
+}
+
 // Direct cell walk of every source of the task (no list).
 template <int D, typename R, int FAM>
 __device__ void gather_direct(Gather<D, R, FAM> &Gt, const Pool<R> P, const SysDesc *sd,
@@ -600,7 +616,8 @@ __device__ void gather_direct(Gather<D, R, FAM> &Gt, const Pool<R> P, const SysD
         else
           ++no;
         Gt.pair(P, Q, K, gj, Aj, (!CMP && needs_b<FAM>(Q)) ? P.rb[gj] : RecB<R>{},
-                CMP ? P.rc[gj] : RecC{}, dx, r2, da);
+                CMP ? P.rc[gj] : RecC{}, (CMP && needs_s<FAM>(Q)) ? P.rs[gj] : RecS{}, dx,
+                r2, da);
       }
     }
   }
@@ -893,14 +910,20 @@ __global__ void __launch_bounds__(GR_BLOCK)
         return true;
       };
       constexpr bool CMP = Gather<D, R, FAM>::COMPACT;
+      // compact Cauchy sources also read the partner's stress (the second
+      // half only for NS > 4)
+      const bool ns_a = CMP && needs_s<FAM>(Q), ns_b = ns_a && Gt.NS > 4;
       int jn = 0;
       bool have = next(jn);
       RecA<R> An{};
       RecB<R> Bn{};
       RecC Cn{};
+      RecS Sn{};
       auto fetch = [&]() {
         if constexpr (CMP) {
           Cn = P.rc[poff + jn];
+          if (ns_a) Sn.a = P.rs[poff + jn].a;
+          if (ns_b) Sn.b = P.rs[poff + jn].b;
         } else {
           An = P.ra[poff + jn];
           if (nb) Bn = P.rb[poff + jn];
@@ -912,6 +935,7 @@ __global__ void __launch_bounds__(GR_BLOCK)
         const RecA<R> Aj = An;
         const RecB<R> Bj = Bn;
         const RecC Cj = Cn;
+        const RecS Sj = Sn;
         have = next(jn);
         if (have) fetch();
         double dx[D] = {};
@@ -929,7 +953,7 @@ __global__ void __launch_bounds__(GR_BLOCK)
           else
             ++no;
         }
-        Gt.pair(P, Q, K, gj, Aj, Bj, Cj, dx, r2, da);
+        Gt.pair(P, Q, K, gj, Aj, Bj, Cj, Sj, dx, r2, da);
       }
     }
   }
