@@ -171,28 +171,45 @@ __global__ void k_minmax_reset(const int *list, int nlist, int *kmin, int *kmax,
 __global__ void k_key_base(const int *list, int nlist, int D, int pad, const int *n_dev,
                            const int *kmin, const int *kmax, unsigned *kbase, unsigned *krange,
                            int *flags) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  unsigned long long base = 0;
-  flags[0] = 0;
-  for (int q = 0; q < nlist; ++q) {
-    int s = list[q];
-    unsigned long long range = 1;
-    if (n_dev[s] > 0) {
-      for (int d = 0; d < D; ++d) {
-        long long r = (long long)kmax[s * 3 + d] - (long long)kmin[s * 3 + d] + 1;
-        krange[s * 3 + d] = (unsigned)r;
-        range *= (unsigned long long)r;
+  // one wavefront: the systems' key ranges in parallel, 64 at a time, and
+  // their exclusive prefix sum (each system's keys start after the ranges of
+  // the systems listed before it, plus one padding key each when `pad`)
+  const int lane = threadIdx.x;
+  unsigned long long carry = 0;
+  for (int q0 = 0; q0 < nlist; q0 += 64) {
+    const int q = q0 + lane;
+    unsigned long long range = 0;
+    int s = -1;
+    if (q < nlist) {
+      s = list[q];
+      if (n_dev[s] > 0) {
+        range = 1;
+        for (int d = 0; d < D; ++d) {
+          const long long r = (long long)kmax[s * 3 + d] - (long long)kmin[s * 3 + d] + 1;
+          krange[s * 3 + d] = (unsigned)r;
+          range *= (unsigned long long)r;
+        }
+      } else {
+        for (int d = 0; d < D; ++d) krange[s * 3 + d] = 1;
       }
-    } else {
-      for (int d = 0; d < D; ++d) krange[s * 3 + d] = 1;
-      range = 0;
+      range += pad ? 1 : 0;
     }
-    kbase[s] = (unsigned)base;
-    base += range + (pad ? 1 : 0);
-    if (base >= 0xFFFFFFFFull) flags[1] |= 1;
+    unsigned long long inc = range;
+    for (int o = 1; o < 64; o <<= 1) {
+      const unsigned long long u = __shfl_up(inc, o, 64);
+      if (lane >= o) inc += u;
+    }
+    if (q < nlist) kbase[s] = (unsigned)(carry + inc - range);
+    carry += __shfl(inc, 63, 64);
   }
-  // every key is below this bound: radix passes above its top bit are identities
-  flags[2] = (int)(unsigned)(base < 0xFFFFFFFFull ? base : 0xFFFFFFFFull);
+  if (lane == 0) {
+    flags[0] = 0;
+    // the running base only grows, so overflowing at some system means the
+    // total overflows
+    if (carry >= 0xFFFFFFFFull) flags[1] |= 1;
+    // every key is below this bound: radix passes above its top bit are identities
+    flags[2] = (int)(unsigned)(carry < 0xFFFFFFFFull ? carry : 0xFFFFFFFFull);
+  }
 }
 
 template <int D>
