@@ -1691,6 +1691,8 @@ void run_step(gr_ctx *c, Plan &P, bool cnt, std::vector<hipEvent_t> *ev) {
     a.nblk = P.nblk_list;
     a.lcnt = c->lcnt.as<int>();
     a.lidx = c->lidx.as<int>();
+    a.bbmin = c->d_bbmin.as<long long>();
+    a.bbmax = c->d_bbmax.as<long long>();
     launch_nlist<R>(s, A, a, c->D);
   }
   rec(4);

@@ -35,6 +35,8 @@ struct NlistArgs {
   int nblk;
   int *lcnt;
   int *lidx;
+  const long long *bbmin = nullptr;  // per-system position bounds of this step
+  const long long *bbmax = nullptr;  // (ordered int64, see ord_of)
 };
 
 void launch_key_minmax(hipStream_t s, XP xs, const SysDesc *sd, const int *n,

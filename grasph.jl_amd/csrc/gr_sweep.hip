@@ -703,7 +703,8 @@ __global__ void __launch_bounds__(64)
             const LTask *__restrict__ lt, const SrcDesc *__restrict__ srcs,
             const Grid *__restrict__ grids, const CellArr *__restrict__ carr,
             const int *__restrict__ cells, const Blk *__restrict__ blk, int nblk, int *lcnt,
-            int *lidx) {
+            int *lidx, const long long *__restrict__ bbmin,
+            const long long *__restrict__ bbmax) {
   __shared__ double sx[D][NL_WIN];
   const Blk b = blk[xcd_block(nblk)];
   const LTask L = lt[b.item];
@@ -729,6 +730,19 @@ __global__ void __launch_bounds__(64)
       const int self_j = Q.pk == PK_SELF ? i : -1;
       const double c2 = Q.cutoff2;
       const int64_t poff = sd[Q.partner].off;
+      // a dynamic boundary (a sorted real system, its bounds taken this
+      // step) more than a cutoff from the target along some axis holds no
+      // partner: the margin keeps every FP64 r² of such a pair above c2
+      bool act = live;
+      if (Q.pk == PK_DYNAMIC && bbmin) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+          const double mn = double_of(bbmin[Q.partner * 3 + d]);
+          const double mx = double_of(bbmax[Q.partner * 3 + d]);
+          const double gap = mn - xi[d] > xi[d] - mx ? mn - xi[d] : xi[d] - mx;
+          act = act && !(gap > Q.cell * (1.0 + 1e-9));
+        }
+      }
       int c[D];
       target_cell<D>(xi, G, Q.cell, c);
       constexpr int NST = (D == 3) ? 9 : 3;
@@ -736,14 +750,14 @@ __global__ void __launch_bounds__(64)
       // a lane past the system's end keeps the empty range [0, 0) (a
       // negative base would run its candidate loop)
       int n0 = 1, n1 = 1;
-      if (live) {
+      if (act) {
         const int sb = strip_base<D>(c, G, 0);
         n0 = cs[sb];
         n1 = cs[sb + 3];
       }
-      for (int st = 0; st < NST; ++st) {
+      for (int st = 0; st < (__builtin_amdgcn_ballot_w64(act) != 0ull ? NST : 0); ++st) {
         const int j0 = n0 - 1, j1 = n1 - 1;
-        if (live && st + 1 < NST) {
+        if (act && st + 1 < NST) {
           const int sb = strip_base<D>(c, G, st + 1);
           n0 = cs[sb];
           n1 = cs[sb + 3];
@@ -1017,10 +1031,10 @@ void launch_nlist(hipStream_t s, const Pool<R> &P, const NlistArgs &a, int D) {
   if (a.nblk <= 0) return;
   if (D == 2)
     k_nlist<2, R><<<a.nblk, 64, 0, s>>>(P, a.sd, a.n, a.lt, a.srcs, a.grids, a.carr, a.cells,
-                                        a.blk, a.nblk, a.lcnt, a.lidx);
+                                        a.blk, a.nblk, a.lcnt, a.lidx, a.bbmin, a.bbmax);
   else
     k_nlist<3, R><<<a.nblk, 64, 0, s>>>(P, a.sd, a.n, a.lt, a.srcs, a.grids, a.carr, a.cells,
-                                        a.blk, a.nblk, a.lcnt, a.lidx);
+                                        a.blk, a.nblk, a.lcnt, a.lidx, a.bbmin, a.bbmax);
 }
 template void launch_nlist<float>(hipStream_t, const Pool<float> &, const NlistArgs &, int);
 template void launch_nlist<double>(hipStream_t, const Pool<double> &, const NlistArgs &, int);
