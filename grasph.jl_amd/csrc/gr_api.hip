@@ -151,6 +151,7 @@ struct FamPlan {
   DBuf tasks, srcs, blk;
   int nblk = 0;
   bool fluid_only = false;  // every source's functor is FluidPfn / FluidSolidPfn
+  bool cauchy = false;      // some source's functor is CauchyFluidPfn
 };
 struct StagePlan {
   DBuf blk_upd, urange, upd;
@@ -1331,6 +1332,9 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
       bool fo = fam == FAM_FORCE;
       for (const SrcDesc &q : sv) fo = fo && (q.pfn == GR_PFN_FLUID || q.pfn == GR_PFN_FLUIDSOLID);
       sp->fam[fam].fluid_only = fo;
+      bool cy = false;
+      for (const SrcDesc &q : sv) cy = cy || q.pfn == GR_PFN_CAUCHY;
+      sp->fam[fam].cauchy = cy;
     }
     P.stages.push_back(std::move(sp));
   }
@@ -1566,11 +1570,13 @@ void phase_stage(gr_ctx *c, Plan &P, int k, bool cnt, hipEvent_t *mid) {
   // compact records alone
   const bool c_only = sizeof(R) == 4 && S.fam[FAM_INTERP].nblk == 0 &&
                       S.fam[FAM_COUNT].nblk == 0;
+  // the partner stress records only for stages with Cauchy force sweeps
+  const bool with_s = S.fam[FAM_FORCE].cauchy;
   auto pack = [&]() {
     launch_pack<R>(s, A, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), P.blk_real.as<Blk>(),
-                   P.nblk_real, c->D, c_only);
+                   P.nblk_real, c->D, c_only, with_s);
     launch_pack<R>(s, A, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), P.blk_ghost.as<Blk>(),
-                   P.nblk_ghost, c->D, c_only);
+                   P.nblk_ghost, c->D, c_only, with_s);
   };
   const int nother = S.fam[FAM_FORCE].nblk + S.fam[FAM_COUNT].nblk + S.fam[FAM_STRAIN].nblk;
   if (nother + S.fam[FAM_INTERP].nblk > 0) pack();

@@ -1372,7 +1372,7 @@ __global__ void k_fill_i32(int *p, int v, long long n) {
 
 // Partner records of the live particles of the block's system (positions,
 // shadows of density and velocity, pressure), read by the pair sweeps.
-template <typename R, int D, bool C_ONLY>
+template <typename R, int D, bool C_ONLY, bool WITH_S>
 __global__ void k_pack(Pool<R> P, const SysDesc *sd, const int *n_dev, const Blk *blk) {
   const Blk b = blk[blockIdx.x];
   const int i = b.first + threadIdx.x;
@@ -1406,6 +1406,8 @@ __global__ void k_pack(Pool<R> P, const SysDesc *sd, const int *n_dev, const Blk
     cc.rho = a.rho;
     cc.p = a.p;
     P.rc[g] = cc;
+  }
+  if constexpr (sizeof(R) == 4 && WITH_S) {  // stages with Cauchy sweeps
     const SysDesc S = sd[b.item];
     float sv[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (S.kind >= GR_SYS_STRESS) {
@@ -1674,26 +1676,33 @@ template void launch_virtual<float>(hipStream_t, const Pool<float> &, const SysD
 template void launch_virtual<double>(hipStream_t, const Pool<double> &, const SysDesc *,
                                      const StepSys *, const int *, const Blk *, int, int, int);
 
+template <typename R, bool C_ONLY, bool WITH_S>
+static void launch_pack_d(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const int *n,
+                          const Blk *blk, int nblk, int D) {
+  if (D == 2)
+    k_pack<R, 2, C_ONLY, WITH_S><<<nblk, GR_BLOCK, 0, s>>>(P, sd, n, blk);
+  else
+    k_pack<R, 3, C_ONLY, WITH_S><<<nblk, GR_BLOCK, 0, s>>>(P, sd, n, blk);
+}
+
 template <typename R>
 void launch_pack(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const int *n, const Blk *blk,
-                 int nblk, int D, bool c_only) {
+                 int nblk, int D, bool c_only, bool with_s) {
   if (nblk <= 0) return;
-  if (c_only && sizeof(R) == 4) {
-    if (D == 2)
-      k_pack<R, 2, true><<<nblk, GR_BLOCK, 0, s>>>(P, sd, n, blk);
-    else
-      k_pack<R, 3, true><<<nblk, GR_BLOCK, 0, s>>>(P, sd, n, blk);
-    return;
-  }
-  if (D == 2)
-    k_pack<R, 2, false><<<nblk, GR_BLOCK, 0, s>>>(P, sd, n, blk);
+  c_only = c_only && sizeof(R) == 4;
+  if (c_only && with_s)
+    launch_pack_d<R, true, true>(s, P, sd, n, blk, nblk, D);
+  else if (c_only)
+    launch_pack_d<R, true, false>(s, P, sd, n, blk, nblk, D);
+  else if (with_s)
+    launch_pack_d<R, false, true>(s, P, sd, n, blk, nblk, D);
   else
-    k_pack<R, 3, false><<<nblk, GR_BLOCK, 0, s>>>(P, sd, n, blk);
+    launch_pack_d<R, false, false>(s, P, sd, n, blk, nblk, D);
 }
 template void launch_pack<float>(hipStream_t, const Pool<float> &, const SysDesc *, const int *,
-                                 const Blk *, int, int, bool);
+                                 const Blk *, int, int, bool, bool);
 template void launch_pack<double>(hipStream_t, const Pool<double> &, const SysDesc *, const int *,
-                                  const Blk *, int, int, bool);
+                                  const Blk *, int, int, bool, bool);
 
 // Probe slot id[i] − 1 takes the position of mirror particle i
 // (_measure_probes!, src/TimeIntegration.jl:469-479).

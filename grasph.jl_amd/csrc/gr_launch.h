@@ -109,7 +109,7 @@ void launch_virtual(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const St
                     const int *n, const Blk *blk, int nblk, int mode, int D);
 template <typename R>
 void launch_pack(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const int *n, const Blk *blk,
-                 int nblk, int D, bool c_only = false);
+                 int nblk, int D, bool c_only = false, bool with_s = true);
 template <typename R>
 void launch_nlist(hipStream_t s, const Pool<R> &P, const NlistArgs &a, int D);
 void launch_fill_i32(hipStream_t s, int *p, int v, long long n);
