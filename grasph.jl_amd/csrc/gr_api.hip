@@ -848,12 +848,11 @@ void host_bbox(gr_ctx *c, const std::vector<int> &systems, std::vector<double> &
   DBuf dl, db;
   dl.upload(list, c->stream);
   db.upload(blk, c->stream);
-  launch_minmax_reset(c->stream, dl.as<int>(), (int)list.size(), c->d_kmin.as<int>(),
-                      c->d_kmax.as<int>(), c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>());
+  launch_minmax_reset(c->stream, dl.as<int>(), (int)list.size(), c->d_bbmin.as<long long>(),
+                      c->d_bbmax.as<long long>());
   launch_key_minmax(c->stream, xp_pool(c, c->cur), c->d_sd.as<SysDesc>(), c->d_n.as<int>(),
-                    c->d_st.as<StepSys>(), db.as<Blk>(), (int)blk.size(), c->D,
-                    c->d_kmin.as<int>(), c->d_kmax.as<int>(), c->d_bbmin.as<long long>(),
-                    c->d_bbmax.as<long long>(), 0);
+                    db.as<Blk>(), (int)blk.size(), c->D, c->d_bbmin.as<long long>(),
+                    c->d_bbmax.as<long long>());
   const int ns = (int)c->sys.size();
   std::vector<long long> bmn(ns * 3), bmx(ns * 3);
   HIPCHK(hipMemcpyAsync(bmn.data(), c->d_bbmin.p, ns * 3 * 8, hipMemcpyDeviceToHost, c->stream));
@@ -1468,12 +1467,12 @@ void phase_sort_real(gr_ctx *c, Plan &P) {
   if (P.real_list.empty()) return;
   hipStream_t s = c->stream;
   const XP xs = xp_pool(c, c->cur);
-  launch_minmax_reset(s, P.d_real_list.as<int>(), (int)P.real_list.size(), c->d_kmin.as<int>(),
-                      c->d_kmax.as<int>(), c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>());
-  launch_key_minmax(s, xs, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), c->d_st.as<StepSys>(),
-                    P.blk_real.as<Blk>(), P.nblk_real, c->D, c->d_kmin.as<int>(),
-                    c->d_kmax.as<int>(), c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>(), 1);
+  launch_minmax_reset(s, P.d_real_list.as<int>(), (int)P.real_list.size(),
+                      c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>());
+  launch_key_minmax(s, xs, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), P.blk_real.as<Blk>(),
+                    P.nblk_real, c->D, c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>());
   launch_key_base(s, P.d_real_list.as<int>(), (int)P.real_list.size(), c->D, 0, c->d_n.as<int>(),
+                  c->d_st.as<StepSys>(), c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>(),
                   c->d_kmin.as<int>(), c->d_kmax.as<int>(), c->d_kbase.as<unsigned>(),
                   c->d_krange.as<unsigned>(), c->d_flags.as<int>());
   launch_key_write(s, xs, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), c->d_st.as<StepSys>(),
@@ -1509,15 +1508,14 @@ void phase_ghosts(gr_ctx *c, Plan &P, bool generate, bool sort) {
                          c->cand_scan.as<int>(), c->D, c->d_n.as<int>());
   }
   const XP gx = xp_gxs(c);
-  launch_minmax_reset(s, P.d_ghost_list.as<int>(), (int)P.ghost_list.size(), c->d_kmin.as<int>(),
-                      c->d_kmax.as<int>(), c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>());
-  launch_key_minmax(s, gx, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), c->d_st.as<StepSys>(),
-                    P.blk_ghost.as<Blk>(), P.nblk_ghost, c->D, c->d_kmin.as<int>(),
-                    c->d_kmax.as<int>(), c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>(),
-                    sort ? 1 : 0);
+  launch_minmax_reset(s, P.d_ghost_list.as<int>(), (int)P.ghost_list.size(),
+                      c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>());
+  launch_key_minmax(s, gx, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), P.blk_ghost.as<Blk>(),
+                    P.nblk_ghost, c->D, c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>());
   if (sort) {
     launch_key_base(s, P.d_ghost_list.as<int>(), (int)P.ghost_list.size(), c->D, 1,
-                    c->d_n.as<int>(), c->d_kmin.as<int>(), c->d_kmax.as<int>(),
+                    c->d_n.as<int>(), c->d_st.as<StepSys>(), c->d_bbmin.as<long long>(),
+                    c->d_bbmax.as<long long>(), c->d_kmin.as<int>(), c->d_kmax.as<int>(),
                     c->d_kbase.as<unsigned>(), c->d_krange.as<unsigned>(), c->d_flags.as<int>());
     launch_key_write(s, gx, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), c->d_st.as<StepSys>(),
                      P.blk_ghost.as<Blk>(), P.nblk_ghost, P.d_bbase_ghost.as<int>(), c->D,
@@ -2652,12 +2650,11 @@ int gr_create_grid(gr_ctx *c, int inter) {
     DBuf db, dl;
     db.upload(blk, c->stream);
     dl.upload(syss, c->stream);
-    launch_minmax_reset(c->stream, dl.as<int>(), (int)syss.size(), c->d_kmin.as<int>(), c->d_kmax.as<int>(),
-                        c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>());
-    launch_key_minmax(c->stream, xp_pool(c, c->cur), c->d_sd.as<SysDesc>(),
-                      c->d_n.as<int>(), c->d_st.as<StepSys>(), db.as<Blk>(), (int)blk.size(), c->D,
-                      c->d_kmin.as<int>(), c->d_kmax.as<int>(), c->d_bbmin.as<long long>(),
-                      c->d_bbmax.as<long long>(), 0);
+    launch_minmax_reset(c->stream, dl.as<int>(), (int)syss.size(), c->d_bbmin.as<long long>(),
+                        c->d_bbmax.as<long long>());
+    launch_key_minmax(c->stream, xp_pool(c, c->cur), c->d_sd.as<SysDesc>(), c->d_n.as<int>(),
+                      db.as<Blk>(), (int)blk.size(), c->D, c->d_bbmin.as<long long>(),
+                      c->d_bbmax.as<long long>());
     Plan P;
     plan_grids(c, P, {inter});
     phase_grids(c, P);

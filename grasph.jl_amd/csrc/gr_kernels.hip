@@ -16,19 +16,27 @@ namespace gr {
 // Wave / block reductions (64-lane wavefronts)
 // ---------------------------------------------------------------------------
 
-__device__ inline long long wave_min_ll(long long v) {
-  for (int o = 32; o > 0; o >>= 1) {
-    long long u = __shfl_xor(v, o, 64);
-    v = u < v ? u : v;
-  }
-  return v;
+// 64-bit wave-wide minimum / maximum (all 64 lanes active): DPP exchanges
+// within each row of 16 lanes, then the four row results through readlane.
+template <int CTRL>
+__device__ __forceinline__ long long dpp_ll(long long v) {
+  const int lo = __builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(v >> 32), (int)(v >> 32), CTRL, 0xF, 0xF, false);
+  return (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
 }
-__device__ inline long long wave_max_ll(long long v) {
-  for (int o = 32; o > 0; o >>= 1) {
-    long long u = __shfl_xor(v, o, 64);
-    v = u > v ? u : v;
-  }
-  return v;
+__device__ __forceinline__ long long readlane_ll(long long v, int l) {
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)v, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(v >> 32), l);
+  return (long long)(((unsigned long long)hi << 32) | lo);
+}
+template <bool MAX>
+__device__ inline long long wave_red_ll(long long v) {
+  auto op = [](long long x, long long y) { return MAX ? (x > y ? x : y) : (x < y ? x : y); };
+  v = op(v, dpp_ll<0xB1>(v));   // quad_perm 1,0,3,2
+  v = op(v, dpp_ll<0x4E>(v));   // quad_perm 2,3,0,1
+  v = op(v, dpp_ll<0x141>(v));  // row_half_mirror
+  v = op(v, dpp_ll<0x140>(v));  // row_mirror
+  return op(op(readlane_ll(v, 0), readlane_ll(v, 16)), op(readlane_ll(v, 32), readlane_ll(v, 48)));
 }
 __device__ inline int wave_min_i(int v) {
   for (int o = 32; o > 0; o >>= 1) {
@@ -50,6 +58,14 @@ __device__ inline long long ord_d(double d) {
   return i < 0 ? (i ^ 0x7FFFFFFFFFFFFFFFLL) : i;
 }
 
+// Coherent (device-scope, relaxed) read of a value other workgroups update
+// with atomics.  Used to skip atomics that cannot change the value: a stale
+// read only costs an atomic that turns out to be redundant.
+template <typename T>
+__device__ __forceinline__ T ld_dev(const T *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // floor(Int, x / cutoff) (src/Sorting.jl:122-124), clamped to int32.
 __device__ inline int key_comp(double x, double cutoff) {
   double f = floor(x / cutoff);
@@ -62,90 +78,83 @@ __device__ inline int key_comp(double x, double cutoff) {
 // Keys: per-system key range and position bounding box
 // ---------------------------------------------------------------------------
 
-// Min/max of the integer cell key and of the position per dimension, per
-// system.  Each workgroup reduces `per` consecutive entries of the block
-// table in registers and flushes (workgroup reduction + one atomic per
-// quantity) whenever the system changes and at the end, so the number of
-// global atomics is ~12 per `per` table entries.
-template <int D>
-__global__ void k_key_minmax(XP xs, const SysDesc *sd, const int *n_dev, const StepSys *st,
-                             const Blk *blk, int nblk, int per, int *kmin, int *kmax,
-                             long long *bbmin, long long *bbmax, int with_keys) {
-  __shared__ long long red[4][GR_BLOCK / 64][3];
+// Bounding box of the positions per dimension and system, as order-preserving
+// integers (ord_d).  The cell keys' ranges follow from it (the key component
+// is a monotone function of the position) and are derived in k_key_base.
+// Each workgroup reduces PER consecutive entries of the block table in
+// registers and flushes (workgroup reduction, then an atomic for each bound
+// that improves on the current value) whenever the system changes and at the
+// end.  The loads of all PER entries (table entry -> system descriptor ->
+// positions) are issued before any is reduced, so their latencies overlap.
+template <int D, int PER>
+__global__ void k_key_minmax(XP xs, const SysDesc *sd, const int *n_dev, const Blk *blk, int nblk,
+                             long long *bbmin, long long *bbmax) {
+  __shared__ long long red[2][GR_BLOCK / 64][3];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int b0 = blockIdx.x * per;
-  const int b1 = b0 + per < nblk ? b0 + per : nblk;
+  const int b0 = blockIdx.x * PER;
+  int item[PER];
+  bool live[PER];
+  double xe[PER][D];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    const Blk b = blk[b0 + k < nblk ? b0 + k : nblk - 1];
+    item[k] = b.item;
+    const int i = b.first + threadIdx.x;
+    live[k] = b0 + k < nblk && i < n_dev[b.item];
+    const int64_t gi = sd[b.item].off + i;
+#pragma unroll
+    for (int d = 0; d < D; ++d) xe[k][d] = live[k] ? xs[d][gi] : 0.0;
+  }
   long long mn[D], mx[D];
-  int kmn[D], kmx[D];
   auto reset = [&]() {
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       mn[d] = LLONG_MAX;
       mx[d] = LLONG_MIN;
-      kmn[d] = INT_MAX;
-      kmx[d] = INT_MIN;
     }
   };
   auto flush = [&](int s) {
 #pragma unroll
     for (int d = 0; d < D; ++d) {
-      const long long a = wave_min_ll(mn[d]), bq = wave_max_ll(mx[d]);
-      const int c = wave_min_i(kmn[d]), e = wave_max_i(kmx[d]);
+      const long long a = wave_red_ll<false>(mn[d]), bq = wave_red_ll<true>(mx[d]);
       if (lane == 0) {
         red[0][w][d] = a;
         red[1][w][d] = bq;
-        red[2][w][d] = c;
-        red[3][w][d] = e;
       }
     }
     __syncthreads();
     if (threadIdx.x < D) {
       const int d = threadIdx.x;
       long long a = LLONG_MAX, bq = LLONG_MIN;
-      int c = INT_MAX, e = INT_MIN;
       for (int q = 0; q < GR_BLOCK / 64; ++q) {
         a = red[0][q][d] < a ? red[0][q][d] : a;
         bq = red[1][q][d] > bq ? red[1][q][d] : bq;
-        c = (int)red[2][q][d] < c ? (int)red[2][q][d] : c;
-        e = (int)red[3][q][d] > e ? (int)red[3][q][d] : e;
       }
+      // every workgroup of a system reduces into the same few words; only
+      // a partial result that improves on the current value takes the atomic
       if (a != LLONG_MAX) {
-        atomicMin(&bbmin[s * 3 + d], a);
-        atomicMax(&bbmax[s * 3 + d], bq);
-        if (with_keys) {
-          atomicMin(&kmin[s * 3 + d], c);
-          atomicMax(&kmax[s * 3 + d], e);
-        }
+        if (a < ld_dev(&bbmin[s * 3 + d])) atomicMin(&bbmin[s * 3 + d], a);
+        if (bq > ld_dev(&bbmax[s * 3 + d])) atomicMax(&bbmax[s * 3 + d], bq);
       }
     }
     __syncthreads();
   };
   reset();
-  int cur = blk[b0].item;
-  for (int bb = b0; bb < b1; ++bb) {
-    const Blk b = blk[bb];
-    if (b.item != cur) {  // uniform over the workgroup
+  int cur = item[0];
+#pragma unroll
+  for (int k = 0; k < PER; ++k) {
+    if (b0 + k >= nblk) break;  // uniform over the workgroup
+    if (item[k] != cur) {       // uniform over the workgroup
       flush(cur);
       reset();
-      cur = b.item;
+      cur = item[k];
     }
-    const int i = b.first + threadIdx.x;
-    if (i < n_dev[cur]) {
-      const int64_t gi = sd[cur].off + i;
-      const double cutoff = st[cur].sort_cutoff;
-      double xl[D];
-#pragma unroll
-      for (int d = 0; d < D; ++d) xl[d] = xs[d][gi];
+    if (live[k]) {
 #pragma unroll
       for (int d = 0; d < D; ++d) {
-        const long long o = ord_d(xl[d]);
+        const long long o = ord_d(xe[k][d]);
         mn[d] = o < mn[d] ? o : mn[d];
         mx[d] = o > mx[d] ? o : mx[d];
-        if (with_keys) {
-          const int k = key_comp(xl[d], cutoff);
-          kmn[d] = k < kmn[d] ? k : kmn[d];
-          kmx[d] = k > kmx[d] ? k : kmx[d];
-        }
       }
     }
   }
@@ -153,13 +162,10 @@ __global__ void k_key_minmax(XP xs, const SysDesc *sd, const int *n_dev, const S
 }
 
 // Reset the reduction slots of the listed systems.
-__global__ void k_minmax_reset(const int *list, int nlist, int *kmin, int *kmax, long long *bbmin,
-                               long long *bbmax) {
+__global__ void k_minmax_reset(const int *list, int nlist, long long *bbmin, long long *bbmax) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nlist * 3) return;
   int s = list[t / 3], d = t % 3;
-  kmin[s * 3 + d] = INT_MAX;
-  kmax[s * 3 + d] = INT_MIN;
   bbmin[s * 3 + d] = LLONG_MAX;
   bbmax[s * 3 + d] = LLONG_MIN;
 }
@@ -167,10 +173,16 @@ __global__ void k_minmax_reset(const int *list, int nlist, int *kmin, int *kmax,
 // Key space of each listed system: lexicographic linearisation of
 // (k1 - kmin1, ..., kD - kminD), bases prefix-summed in list order.  Ghost
 // systems reserve one extra key for padding slots.  flags[0] (the inversion
-// flag) is cleared here; flags[1] records overflow.
+// flag) is cleared here; flags[1] records overflow.  kmin/kmax are the key
+// components of the bounding box's corners: key_comp is monotone in the
+// position, so they are the extremes of the particles' key components.  A
+// NaN position leaves kmax below kmin or a range beyond 32 bits: overflow.
+__device__ inline double unord_d(long long o) {
+  return __longlong_as_double(o < 0 ? (o ^ 0x7FFFFFFFFFFFFFFFLL) : o);
+}
 __global__ void k_key_base(const int *list, int nlist, int D, int pad, const int *n_dev,
-                           const int *kmin, const int *kmax, unsigned *kbase, unsigned *krange,
-                           int *flags) {
+                           const StepSys *st, const long long *bbmin, const long long *bbmax,
+                           int *kmin, int *kmax, unsigned *kbase, unsigned *krange, int *flags) {
   // one wavefront: the systems' key ranges in parallel, 64 at a time, and
   // their exclusive prefix sum (each system's keys start after the ranges of
   // the systems listed before it, plus one padding key each when `pad`)
@@ -184,11 +196,22 @@ __global__ void k_key_base(const int *list, int nlist, int D, int pad, const int
       s = list[q];
       if (n_dev[s] > 0) {
         range = 1;
+        const double cutoff = st[s].sort_cutoff;
         for (int d = 0; d < D; ++d) {
-          const long long r = (long long)kmax[s * 3 + d] - (long long)kmin[s * 3 + d] + 1;
+          const int ka = key_comp(unord_d(bbmin[s * 3 + d]), cutoff);
+          const int kb = key_comp(unord_d(bbmax[s * 3 + d]), cutoff);
+          const int lo = ka < kb ? ka : kb, hi = ka < kb ? kb : ka;
+          kmin[s * 3 + d] = lo;
+          kmax[s * 3 + d] = hi;
+          long long r = (long long)hi - (long long)lo + 1;
+          if (!(unord_d(bbmin[s * 3 + d]) <= unord_d(bbmax[s * 3 + d]))) r = 1LL << 32;  // NaN
           krange[s * 3 + d] = (unsigned)r;
-          range *= (unsigned long long)r;
+          // any product of 2^32 or more overflows the 32-bit keys: saturate
+          // there so that neither this product nor the prefix sum wraps
+          range = (range > 0xFFFFFFFFull || r > 0xFFFFFFFFLL) ? (1ull << 32)
+                                                               : range * (unsigned long long)r;
         }
+        if (range > (1ull << 32)) range = 1ull << 32;
       } else {
         for (int d = 0; d < D; ++d) krange[s * 3 + d] = 1;
       }
@@ -239,20 +262,31 @@ __global__ void k_key_write(XP xs, const SysDesc *sd, const int *n_dev,
   const int n = n_dev[s];
   const int64_t gi = sd[s].off + i;
   const double cutoff = st[s].sort_cutoff;
-  unsigned key;
+  __shared__ unsigned kw[GR_BLOCK / 64];
+  const int w = threadIdx.x >> 6;
+  const bool last = threadIdx.x == GR_BLOCK - 1 && i + 1 < n;
+  unsigned key, kn = 0u;
   if (i < n) {
     key = kbase[s] + lin_key<D>(xs, gi, cutoff, kmin, krange, s);
+    // the key after the workgroup's last particle, the only neighbour key
+    // that no thread of this workgroup computes for itself
+    if (last) kn = kbase[s] + lin_key<D>(xs, gi + 1, cutoff, kmin, krange, s);
   } else {
     unsigned rng = 1;
     for (int d = 0; d < D; ++d) rng *= krange[s * 3 + d];
     key = kbase[s] + (n > 0 ? rng : 0u);
   }
-  // the next particle's key: from the next lane, computed only by the last
-  // lane of each wavefront (every lane is active here)
+  // the next particle's key: from the next lane, from the next wavefront's
+  // first lane (through LDS) for lane 63, and kn for the workgroup's last
+  // thread (every thread of the workgroup is active here)
+  if (lane == 0) kw[w] = key;
+  __syncthreads();
   unsigned k2 = __shfl_down(key, 1, 64);
-  if (lane == 63 && i + 1 < n) k2 = kbase[s] + lin_key<D>(xs, gi + 1, cutoff, kmin, krange, s);
+  if (lane == 63) k2 = w + 1 < GR_BLOCK / 64 ? kw[w + 1] : kn;
   const bool inv = i + 1 < n && k2 < key;
-  if (__ballot(inv) != 0ull && lane == 0) atomicOr(&flags[0], 1);
+  // most workgroups of a moving scene hold an inversion: one atomic per
+  // workgroup at most, none once the flag reads as raised
+  if (__syncthreads_or(inv) && threadIdx.x == 0 && ld_dev(&flags[0]) == 0) atomicOr(&flags[0], 1);
   if (i < cap) {
     const int p = bbase[s] + i;
     keys[p] = key;
@@ -1428,31 +1462,28 @@ __global__ void k_pack(Pool<R> P, const SysDesc *sd, const int *n_dev, const Blk
 
 static inline unsigned nb_of(long long n, int per) { return (unsigned)((n + per - 1) / per); }
 
-void launch_key_minmax(hipStream_t s, XP xs, const SysDesc *sd, const int *n, const StepSys *st,
-                       const Blk *blk, int nblk, int D, int *kmin, int *kmax, long long *bbmin,
-                       long long *bbmax, int with_keys) {
+void launch_key_minmax(hipStream_t s, XP xs, const SysDesc *sd, const int *n, const Blk *blk,
+                       int nblk, int D, long long *bbmin, long long *bbmax) {
   if (nblk <= 0) return;
   // 4 table entries (1024 particles) per workgroup: enough workgroups to fill
   // the chip, few enough that the per-system atomics stay cheap
-  const int per = 4;
+  constexpr int per = 4;
   const unsigned g = nb_of(nblk, per);
   if (D == 2)
-    k_key_minmax<2><<<g, GR_BLOCK, 0, s>>>(xs, sd, n, st, blk, nblk, per, kmin, kmax, bbmin,
-                                           bbmax, with_keys);
+    k_key_minmax<2, per><<<g, GR_BLOCK, 0, s>>>(xs, sd, n, blk, nblk, bbmin, bbmax);
   else
-    k_key_minmax<3><<<g, GR_BLOCK, 0, s>>>(xs, sd, n, st, blk, nblk, per, kmin, kmax, bbmin,
-                                           bbmax, with_keys);
+    k_key_minmax<3, per><<<g, GR_BLOCK, 0, s>>>(xs, sd, n, blk, nblk, bbmin, bbmax);
 }
-void launch_minmax_reset(hipStream_t s, const int *list, int nlist, int *kmin, int *kmax,
-                         long long *bbmin, long long *bbmax) {
+void launch_minmax_reset(hipStream_t s, const int *list, int nlist, long long *bbmin,
+                         long long *bbmax) {
   if (nlist > 0)
-    k_minmax_reset<<<nb_of(nlist * 3, GR_BLOCK), GR_BLOCK, 0, s>>>(list, nlist, kmin, kmax, bbmin,
-                                                                   bbmax);
+    k_minmax_reset<<<nb_of(nlist * 3, GR_BLOCK), GR_BLOCK, 0, s>>>(list, nlist, bbmin, bbmax);
 }
 void launch_key_base(hipStream_t s, const int *list, int nlist, int D, int pad, const int *n,
-                     const int *kmin, const int *kmax, unsigned *kbase, unsigned *krange,
-                     int *flags) {
-  k_key_base<<<1, 64, 0, s>>>(list, nlist, D, pad, n, kmin, kmax, kbase, krange, flags);
+                     const StepSys *st, const long long *bbmin, const long long *bbmax, int *kmin,
+                     int *kmax, unsigned *kbase, unsigned *krange, int *flags) {
+  k_key_base<<<1, 64, 0, s>>>(list, nlist, D, pad, n, st, bbmin, bbmax, kmin, kmax, kbase, krange,
+                              flags);
 }
 void launch_key_write(hipStream_t s, XP xs, const SysDesc *sd, const int *n, const StepSys *st,
                       const Blk *blk, int nblk, const int *bbase, int D, const int *kmin,

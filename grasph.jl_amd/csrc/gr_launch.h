@@ -39,14 +39,13 @@ struct NlistArgs {
   const long long *bbmax = nullptr;  // (ordered int64, see ord_of)
 };
 
-void launch_key_minmax(hipStream_t s, XP xs, const SysDesc *sd, const int *n,
-                       const StepSys *st, const Blk *blk, int nblk, int D, int *kmin, int *kmax,
-                       long long *bbmin, long long *bbmax, int with_keys);
-void launch_minmax_reset(hipStream_t s, const int *list, int nlist, int *kmin, int *kmax,
-                         long long *bbmin, long long *bbmax);
+void launch_key_minmax(hipStream_t s, XP xs, const SysDesc *sd, const int *n, const Blk *blk,
+                       int nblk, int D, long long *bbmin, long long *bbmax);
+void launch_minmax_reset(hipStream_t s, const int *list, int nlist, long long *bbmin,
+                         long long *bbmax);
 void launch_key_base(hipStream_t s, const int *list, int nlist, int D, int pad, const int *n,
-                     const int *kmin, const int *kmax, unsigned *kbase, unsigned *krange,
-                     int *flags);
+                     const StepSys *st, const long long *bbmin, const long long *bbmax, int *kmin,
+                     int *kmax, unsigned *kbase, unsigned *krange, int *flags);
 void launch_key_write(hipStream_t s, XP xs, const SysDesc *sd, const int *n,
                       const StepSys *st, const Blk *blk, int nblk, const int *bbase, int D,
                       const int *kmin, const unsigned *krange, const unsigned *kbase,
