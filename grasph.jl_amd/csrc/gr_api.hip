@@ -239,6 +239,7 @@ struct gr_ctx {
   bool vadj_any = false;  // some v_adjustment may be nonzero: permute and correct it
   // tables
   DBuf d_sd, d_n, d_st, d_bnd, d_kmin, d_kmax, d_bbmin, d_bbmax, d_kbase, d_krange, d_flags;
+  int bb_rs = 0;  // entries per replica of d_bbmin / d_bbmax (GR_BB_REP replicas)
   DBuf d_grids, d_carr, d_cellsz, d_inter_sys, d_inter_carr, cells;
   std::vector<CellArr> carr;
   int64_t cells_cap = 0;
@@ -493,11 +494,12 @@ void commit(gr_ctx *c) {
   if (bnds.empty()) bnds.resize(1);
   c->d_bnd.upload(bnds, c->stream);
   std::vector<int> z(ns * 3, 0);
-  std::vector<long long> zl(ns * 3, 0);
+  std::vector<long long> zl((size_t)ns * 3 * GR_BB_REP, 0);
   c->d_kmin.upload(z, c->stream);
   c->d_kmax.upload(z, c->stream);
   c->d_bbmin.upload(zl, c->stream);
   c->d_bbmax.upload(zl, c->stream);
+  c->bb_rs = ns * 3;
   std::vector<unsigned> zu(ns * 3, 0);
   c->d_kbase.upload(zu, c->stream);
   c->d_krange.upload(zu, c->stream);
@@ -849,10 +851,10 @@ void host_bbox(gr_ctx *c, const std::vector<int> &systems, std::vector<double> &
   dl.upload(list, c->stream);
   db.upload(blk, c->stream);
   launch_minmax_reset(c->stream, dl.as<int>(), (int)list.size(), c->d_bbmin.as<long long>(),
-                      c->d_bbmax.as<long long>());
+                      c->d_bbmax.as<long long>(), c->bb_rs);
   launch_key_minmax(c->stream, xp_pool(c, c->cur), c->d_sd.as<SysDesc>(), c->d_n.as<int>(),
                     db.as<Blk>(), (int)blk.size(), c->D, c->d_bbmin.as<long long>(),
-                    c->d_bbmax.as<long long>());
+                    c->d_bbmax.as<long long>(), c->bb_rs, dl.as<int>(), (int)list.size(), true);
   const int ns = (int)c->sys.size();
   std::vector<long long> bmn(ns * 3), bmx(ns * 3);
   HIPCHK(hipMemcpyAsync(bmn.data(), c->d_bbmin.p, ns * 3 * 8, hipMemcpyDeviceToHost, c->stream));
@@ -1468,12 +1470,13 @@ void phase_sort_real(gr_ctx *c, Plan &P) {
   hipStream_t s = c->stream;
   const XP xs = xp_pool(c, c->cur);
   launch_minmax_reset(s, P.d_real_list.as<int>(), (int)P.real_list.size(),
-                      c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>());
+                      c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>(), c->bb_rs);
   launch_key_minmax(s, xs, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), P.blk_real.as<Blk>(),
-                    P.nblk_real, c->D, c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>());
+                    P.nblk_real, c->D, c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>(),
+                    c->bb_rs, P.d_real_list.as<int>(), (int)P.real_list.size(), false);
   launch_key_base(s, P.d_real_list.as<int>(), (int)P.real_list.size(), c->D, 0, c->d_n.as<int>(),
                   c->d_st.as<StepSys>(), c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>(),
-                  c->d_kmin.as<int>(), c->d_kmax.as<int>(), c->d_kbase.as<unsigned>(),
+                  c->bb_rs, c->d_kmin.as<int>(), c->d_kmax.as<int>(), c->d_kbase.as<unsigned>(),
                   c->d_krange.as<unsigned>(), c->d_flags.as<int>());
   launch_key_write(s, xs, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), c->d_st.as<StepSys>(),
                    P.blk_real.as<Blk>(), P.nblk_real, P.d_bbase_real.as<int>(), c->D,
@@ -1509,13 +1512,14 @@ void phase_ghosts(gr_ctx *c, Plan &P, bool generate, bool sort) {
   }
   const XP gx = xp_gxs(c);
   launch_minmax_reset(s, P.d_ghost_list.as<int>(), (int)P.ghost_list.size(),
-                      c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>());
+                      c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>(), c->bb_rs);
   launch_key_minmax(s, gx, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), P.blk_ghost.as<Blk>(),
-                    P.nblk_ghost, c->D, c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>());
+                    P.nblk_ghost, c->D, c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>(),
+                    c->bb_rs, P.d_ghost_list.as<int>(), (int)P.ghost_list.size(), !sort);
   if (sort) {
     launch_key_base(s, P.d_ghost_list.as<int>(), (int)P.ghost_list.size(), c->D, 1,
                     c->d_n.as<int>(), c->d_st.as<StepSys>(), c->d_bbmin.as<long long>(),
-                    c->d_bbmax.as<long long>(), c->d_kmin.as<int>(), c->d_kmax.as<int>(),
+                    c->d_bbmax.as<long long>(), c->bb_rs, c->d_kmin.as<int>(), c->d_kmax.as<int>(),
                     c->d_kbase.as<unsigned>(), c->d_krange.as<unsigned>(), c->d_flags.as<int>());
     launch_key_write(s, gx, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), c->d_st.as<StepSys>(),
                      P.blk_ghost.as<Blk>(), P.nblk_ghost, P.d_bbase_ghost.as<int>(), c->D,
@@ -2651,10 +2655,10 @@ int gr_create_grid(gr_ctx *c, int inter) {
     db.upload(blk, c->stream);
     dl.upload(syss, c->stream);
     launch_minmax_reset(c->stream, dl.as<int>(), (int)syss.size(), c->d_bbmin.as<long long>(),
-                        c->d_bbmax.as<long long>());
+                        c->d_bbmax.as<long long>(), c->bb_rs);
     launch_key_minmax(c->stream, xp_pool(c, c->cur), c->d_sd.as<SysDesc>(), c->d_n.as<int>(),
                       db.as<Blk>(), (int)blk.size(), c->D, c->d_bbmin.as<long long>(),
-                      c->d_bbmax.as<long long>());
+                      c->d_bbmax.as<long long>(), c->bb_rs, dl.as<int>(), (int)syss.size(), true);
     Plan P;
     plan_grids(c, P, {inter});
     phase_grids(c, P);

@@ -86,9 +86,16 @@ __device__ inline int key_comp(double x, double cutoff) {
 // that improves on the current value) whenever the system changes and at the
 // end.  The loads of all PER entries (table entry -> system descriptor ->
 // positions) are issued before any is reduced, so their latencies overlap.
+//
+// The workgroups of a launch run concurrently, so they all see the initial
+// bounds and most of them take their atomics; those atomics are spread over
+// GR_BB_REP replicas of the bounds (stride `rs`), one for the workgroups of
+// each XCD, and folded into replica 0 afterwards (k_key_base, k_bb_fold).
 template <int D, int PER>
 __global__ void k_key_minmax(XP xs, const SysDesc *sd, const int *n_dev, const Blk *blk, int nblk,
-                             long long *bbmin, long long *bbmax) {
+                             long long *bbmin, long long *bbmax, int rs) {
+  bbmin += (blockIdx.x % GR_BB_REP) * rs;
+  bbmax += (blockIdx.x % GR_BB_REP) * rs;
   __shared__ long long red[2][GR_BLOCK / 64][3];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const int b0 = blockIdx.x * PER;
@@ -162,12 +169,32 @@ __global__ void k_key_minmax(XP xs, const SysDesc *sd, const int *n_dev, const B
 }
 
 // Reset the reduction slots of the listed systems.
-__global__ void k_minmax_reset(const int *list, int nlist, long long *bbmin, long long *bbmax) {
+__global__ void k_minmax_reset(const int *list, int nlist, long long *bbmin, long long *bbmax,
+                               int rs) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nlist * 3) return;
+  if (t >= nlist * 3 * GR_BB_REP) return;
+  const int r = t / (nlist * 3);
+  t -= r * nlist * 3;
   int s = list[t / 3], d = t % 3;
-  bbmin[s * 3 + d] = LLONG_MAX;
-  bbmax[s * 3 + d] = LLONG_MIN;
+  bbmin[r * rs + s * 3 + d] = LLONG_MAX;
+  bbmax[r * rs + s * 3 + d] = LLONG_MIN;
+}
+
+// Fold the replicas of the listed systems' bounds into replica 0.
+__device__ __forceinline__ void bb_fold(long long *bbmin, long long *bbmax, int rs, int e) {
+  long long a = bbmin[e], b = bbmax[e];
+#pragma unroll
+  for (int r = 1; r < GR_BB_REP; ++r) {
+    const long long u = bbmin[r * rs + e], v = bbmax[r * rs + e];
+    a = u < a ? u : a;
+    b = v > b ? v : b;
+  }
+  bbmin[e] = a;
+  bbmax[e] = b;
+}
+__global__ void k_bb_fold(const int *list, int nlist, long long *bbmin, long long *bbmax, int rs) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < nlist * 3) bb_fold(bbmin, bbmax, rs, list[t / 3] * 3 + t % 3);
 }
 
 // Key space of each listed system: lexicographic linearisation of
@@ -181,7 +208,7 @@ __device__ inline double unord_d(long long o) {
   return __longlong_as_double(o < 0 ? (o ^ 0x7FFFFFFFFFFFFFFFLL) : o);
 }
 __global__ void k_key_base(const int *list, int nlist, int D, int pad, const int *n_dev,
-                           const StepSys *st, const long long *bbmin, const long long *bbmax,
+                           const StepSys *st, long long *bbmin, long long *bbmax, int rs,
                            int *kmin, int *kmax, unsigned *kbase, unsigned *krange, int *flags) {
   // one wavefront: the systems' key ranges in parallel, 64 at a time, and
   // their exclusive prefix sum (each system's keys start after the ranges of
@@ -194,6 +221,7 @@ __global__ void k_key_base(const int *list, int nlist, int D, int pad, const int
     int s = -1;
     if (q < nlist) {
       s = list[q];
+      for (int d = 0; d < 3; ++d) bb_fold(bbmin, bbmax, rs, s * 3 + d);
       if (n_dev[s] > 0) {
         range = 1;
         const double cutoff = st[s].sort_cutoff;
@@ -1463,27 +1491,32 @@ __global__ void k_pack(Pool<R> P, const SysDesc *sd, const int *n_dev, const Blk
 static inline unsigned nb_of(long long n, int per) { return (unsigned)((n + per - 1) / per); }
 
 void launch_key_minmax(hipStream_t s, XP xs, const SysDesc *sd, const int *n, const Blk *blk,
-                       int nblk, int D, long long *bbmin, long long *bbmax) {
-  if (nblk <= 0) return;
-  // 4 table entries (1024 particles) per workgroup: enough workgroups to fill
-  // the chip, few enough that the per-system atomics stay cheap
-  constexpr int per = 4;
-  const unsigned g = nb_of(nblk, per);
-  if (D == 2)
-    k_key_minmax<2, per><<<g, GR_BLOCK, 0, s>>>(xs, sd, n, blk, nblk, bbmin, bbmax);
-  else
-    k_key_minmax<3, per><<<g, GR_BLOCK, 0, s>>>(xs, sd, n, blk, nblk, bbmin, bbmax);
+                       int nblk, int D, long long *bbmin, long long *bbmax, int rs,
+                       const int *list, int nlist, bool fold) {
+  if (nblk > 0) {
+    // 4 table entries (1024 particles) per workgroup: enough workgroups to
+    // fill the chip, few enough that the per-system atomics stay cheap
+    constexpr int per = 4;
+    const unsigned g = nb_of(nblk, per);
+    if (D == 2)
+      k_key_minmax<2, per><<<g, GR_BLOCK, 0, s>>>(xs, sd, n, blk, nblk, bbmin, bbmax, rs);
+    else
+      k_key_minmax<3, per><<<g, GR_BLOCK, 0, s>>>(xs, sd, n, blk, nblk, bbmin, bbmax, rs);
+  }
+  if (fold && nlist > 0)
+    k_bb_fold<<<nb_of(nlist * 3, GR_BLOCK), GR_BLOCK, 0, s>>>(list, nlist, bbmin, bbmax, rs);
 }
 void launch_minmax_reset(hipStream_t s, const int *list, int nlist, long long *bbmin,
-                         long long *bbmax) {
+                         long long *bbmax, int rs) {
   if (nlist > 0)
-    k_minmax_reset<<<nb_of(nlist * 3, GR_BLOCK), GR_BLOCK, 0, s>>>(list, nlist, bbmin, bbmax);
+    k_minmax_reset<<<nb_of(nlist * 3 * GR_BB_REP, GR_BLOCK), GR_BLOCK, 0, s>>>(list, nlist, bbmin,
+                                                                               bbmax, rs);
 }
 void launch_key_base(hipStream_t s, const int *list, int nlist, int D, int pad, const int *n,
-                     const StepSys *st, const long long *bbmin, const long long *bbmax, int *kmin,
+                     const StepSys *st, long long *bbmin, long long *bbmax, int rs, int *kmin,
                      int *kmax, unsigned *kbase, unsigned *krange, int *flags) {
-  k_key_base<<<1, 64, 0, s>>>(list, nlist, D, pad, n, st, bbmin, bbmax, kmin, kmax, kbase, krange,
-                              flags);
+  k_key_base<<<1, 64, 0, s>>>(list, nlist, D, pad, n, st, bbmin, bbmax, rs, kmin, kmax, kbase,
+                              krange, flags);
 }
 void launch_key_write(hipStream_t s, XP xs, const SysDesc *sd, const int *n, const StepSys *st,
                       const Blk *blk, int nblk, const int *bbase, int D, const int *kmin,

@@ -39,12 +39,18 @@ struct NlistArgs {
   const long long *bbmax = nullptr;  // (ordered int64, see ord_of)
 };
 
+// Position bounds per system and dimension are kept in GR_BB_REP replicas
+// of `rs` entries each (the key pass's atomics are spread over them); replica
+// 0 holds the folded bounds that every reader uses.  launch_key_minmax folds
+// them itself when `fold`, launch_key_base always does.
+#define GR_BB_REP 8
 void launch_key_minmax(hipStream_t s, XP xs, const SysDesc *sd, const int *n, const Blk *blk,
-                       int nblk, int D, long long *bbmin, long long *bbmax);
+                       int nblk, int D, long long *bbmin, long long *bbmax, int rs,
+                       const int *list, int nlist, bool fold);
 void launch_minmax_reset(hipStream_t s, const int *list, int nlist, long long *bbmin,
-                         long long *bbmax);
+                         long long *bbmax, int rs);
 void launch_key_base(hipStream_t s, const int *list, int nlist, int D, int pad, const int *n,
-                     const StepSys *st, const long long *bbmin, const long long *bbmax, int *kmin,
+                     const StepSys *st, long long *bbmin, long long *bbmax, int rs, int *kmin,
                      int *kmax, unsigned *kbase, unsigned *krange, int *flags);
 void launch_key_write(hipStream_t s, XP xs, const SysDesc *sd, const int *n,
                       const StepSys *st, const Blk *blk, int nblk, const int *bbase, int D,
