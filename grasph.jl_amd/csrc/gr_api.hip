@@ -174,8 +174,8 @@ struct Plan {
   // grids
   std::vector<int> inters;
   std::vector<std::pair<int, bool>> carr_state;  // (cell array, built by this plan)
-  DBuf d_inter_list, blk_cells, blk_mark, carr_blocks;
-  int nblk_cells = 0, nblk_mark = 0, ncarr = 0;
+  DBuf d_inter_list, blk_cells, blk_mark;
+  int nblk_cells = 0, nblk_mark = 0;
   // stages
   std::vector<std::unique_ptr<StagePlan>> stages;
   std::vector<std::pair<int, StepSys>> stepsys;  // per-system step parameters
@@ -245,7 +245,7 @@ struct gr_ctx {
   int64_t cells_cap = 0;
   std::vector<StepSys> stepsys;
   // sort / scan scratch
-  DBuf keys, vals, keys2, vals2, hist, cand, cand_scan, scan_bsum, bagg, carry;
+  DBuf keys, vals, keys2, vals2, hist, cand, cand_scan, scan_bsum, bagg;
   int64_t sort_cap = 0, cand_cap = 0, cellblk_cap = 0;
   DBuf d_pairs;
   DBuf lidx, lcnt;
@@ -977,7 +977,6 @@ void plan_grids(gr_ctx *c, Plan &P, const std::vector<int> &inters,
   P.inters = inters;
   P.d_inter_list.upload(inters, c->stream);
   std::vector<Blk> cb, mb;
-  std::vector<int2> cblocks;
   for (int it : inters) {
     const InterH &in = c->inter[it];
     for (int ca : {in.carr_b, in.carr_a}) {
@@ -985,21 +984,16 @@ void plan_grids(gr_ctx *c, Plan &P, const std::vector<int> &inters,
       const bool skip = used && (ca >= (int)used->size() || !(*used)[ca]);
       P.carr_state.emplace_back(ca, !skip);
       if (skip) continue;
-      const int first = (int)cb.size();
       add_blocks(cb, ca, c->carr[ca].cap, GR_BLOCK * 4);
-      cblocks.push_back(make_int2(first, (int)cb.size()));
       add_blocks(mb, ca, c->sys[c->carr[ca].sys].cap, GR_BLOCK);
     }
   }
   P.blk_cells.upload(cb, c->stream);
   P.blk_mark.upload(mb, c->stream);
-  P.carr_blocks.upload(cblocks, c->stream);
   P.nblk_cells = (int)cb.size();
   P.nblk_mark = (int)mb.size();
-  P.ncarr = (int)cblocks.size();
   if ((int64_t)cb.size() + 1 > c->cellblk_cap) {
     c->bagg.alloc((cb.size() + 1) * 4);
-    c->carry.alloc((cb.size() + 1) * 4);
     c->cellblk_cap = cb.size() + 1;
   }
 }
@@ -1554,8 +1548,7 @@ void phase_grids(gr_ctx *c, Plan &P) {
   launch_cells_build(s, xp_pool(c, c->cur), c->cells.as<int>(),
                      c->d_carr.as<CellArr>(), c->d_grids.as<Grid>(), c->d_cellsz.as<double>(),
                      c->d_sd.as<SysDesc>(), c->d_n.as<int>(), P.blk_cells.as<Blk>(), P.nblk_cells,
-                     P.blk_mark.as<Blk>(), P.nblk_mark, P.carr_blocks.as<int2>(), P.ncarr,
-                     c->bagg.as<int>(), c->carry.as<int>(), c->D);
+                     P.blk_mark.as<Blk>(), P.nblk_mark, c->bagg.as<int>(), c->D);
 }
 
 template <typename R>

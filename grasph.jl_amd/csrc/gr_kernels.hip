@@ -845,76 +845,85 @@ __global__ void k_cells_mark(XP x, int *cells, const CellArr *carr, const Grid *
 }
 
 // Backward pass (:231-235): every empty cell takes the first non-empty start
-// to its right.  Three kernels: block-local suffix, per-array carry, apply.
+// to its right.  Two kernels: block-local suffix (with the block's first
+// non-empty start as its aggregate), then each block finds the first
+// non-empty aggregate among the blocks after it and applies it.
 #define FILL_TILE (GR_BLOCK * 4)
 
 __device__ inline int first_nz(int a, int b) { return a != 0 ? a : b; }
 
 __global__ void k_fill_local(int *cells, const CellArr *carr, const Grid *grids, const Blk *blk,
                              int *bagg) {
-  __shared__ int s[GR_BLOCK];
+  __shared__ int s[GR_BLOCK / 64];
   const Blk b = blk[blockIdx.x];
   const CellArr C = carr[b.item];
   const int ncells = grids[C.grid].ncells;
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int c0 = b.first + t * 4;
   int v[4];
   for (int q = 0; q < 4; ++q) {
     int c = c0 + q;
     v[q] = (c < C.cap && c <= ncells) ? cells[C.off + c] : 0;
   }
-  // thread-local suffix
+  // thread-local suffix, then the wavefront's (lanes lane..63) by shuffles
   for (int q = 2; q >= 0; --q) v[q] = first_nz(v[q], v[q + 1]);
-  s[t] = v[0];
-  __syncthreads();
-  for (int o = 1; o < GR_BLOCK; o <<= 1) {
-    int r = (t + o < GR_BLOCK) ? s[t + o] : 0;
-    __syncthreads();
-    s[t] = first_nz(s[t], r);
-    __syncthreads();
+  int a = v[0];
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_down(a, o, 64);
+    if (lane + o < 64) a = first_nz(a, u);
   }
-  const int right = (t + 1 < GR_BLOCK) ? s[t + 1] : 0;
+  if (lane == 0) s[w] = a;
+  __syncthreads();
+  int rw = 0;  // first non-zero in the wavefronts after this one
+  for (int q = GR_BLOCK / 64 - 1; q > w; --q) rw = first_nz(s[q], rw);
+  const int nx = __shfl_down(a, 1, 64);
+  const int right = first_nz(lane < 63 ? nx : 0, rw);
   for (int q = 0; q < 4; ++q) {
     int c = c0 + q;
     if (c < C.cap && c <= ncells) cells[C.off + c] = first_nz(v[q], right);
   }
-  if (t == 0) bagg[blockIdx.x] = s[0];
+  if (t == 0) bagg[blockIdx.x] = first_nz(a, rw);
 }
 
-// carry[blk] = first non-zero start in the blocks to the right of blk
-// (blocks of one cell array are consecutive in the table).  One workgroup
-// per cell array: chunk aggregates, a suffix scan over the chunks, then each
-// chunk's carries right to left.
-__global__ void k_fill_carry(const int2 *carr_blocks, int ncarr, const int *bagg, int *carry) {
-  __shared__ int s[GR_BLOCK];
-  const int2 r = carr_blocks[blockIdx.x];  // [first block, end block)
-  const int t = threadIdx.x;
-  const int per = (r.y - r.x + GR_BLOCK - 1) / GR_BLOCK;
-  const int k0 = r.x + t * per;
-  const int k1 = k0 + per < r.y ? k0 + per : r.y;
-  int agg = 0;
-  for (int k = k1 - 1; k >= k0; --k) agg = first_nz(bagg[k], agg);
-  s[t] = agg;
-  __syncthreads();
-  for (int o = 1; o < GR_BLOCK; o <<= 1) {
-    const int v = (t + o < GR_BLOCK) ? s[t + o] : 0;
-    __syncthreads();
-    s[t] = first_nz(s[t], v);
-    __syncthreads();
-  }
-  int c = (t + 1 < GR_BLOCK) ? s[t + 1] : 0;
-  for (int k = k1 - 1; k >= k0; --k) {
-    carry[k] = c;
-    c = first_nz(bagg[k], c);
-  }
-}
-
+// The carry of a block is the first non-zero aggregate among the following
+// blocks of its cell array (consecutive in the table, `first` restarting at 0
+// for the next array), found 256 blocks at a time by each wavefront (4 per
+// lane, loads independent of each other).  The array's sentinel cell (n + 1)
+// is non-zero, so the search ends at the latest in the sentinel's block.
 __global__ void k_fill_apply(int *cells, const CellArr *carr, const Grid *grids, const Blk *blk,
-                             const int *carry) {
+                             int nblk, const int *bagg) {
   const Blk b = blk[blockIdx.x];
   const CellArr C = carr[b.item];
   const int ncells = grids[C.grid].ncells;
-  const int cr = carry[blockIdx.x];
+  if (b.first > ncells) return;  // uniform: no cell of this block is in the grid
+  const int lane = threadIdx.x & 63;
+  int cr = 0;
+  for (int k0 = blockIdx.x + 1; k0 < nblk; k0 += 256) {
+    Blk bk[4];
+    int g[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int k = k0 + lane * 4 + q;
+      bk[q] = k < nblk ? blk[k] : Blk{-1, 0};
+      g[q] = k < nblk ? bagg[k] : 0;
+    }
+    // this lane's first stop (end of the array or a non-zero aggregate)
+    bool stop = false;
+    int gs = 0;
+#pragma unroll
+    for (int q = 3; q >= 0; --q) {
+      const bool same = bk[q].item == b.item && bk[q].first != 0;
+      if (!same || g[q] != 0) {
+        stop = true;
+        gs = same ? g[q] : 0;
+      }
+    }
+    const unsigned long long m = __ballot(stop);
+    if (m != 0ull) {
+      cr = __shfl(gs, __ffsll((long long)m) - 1, 64);  // 0 when the array ended first
+      break;
+    }
+  }
   if (cr == 0) return;
   for (int q = 0; q < 4; ++q) {
     int c = b.first + threadIdx.x * 4 + q;
@@ -1642,8 +1651,7 @@ void launch_grid_params(hipStream_t s, const int *inter_list, int n, const int2 
 void launch_cells_build(hipStream_t s, XP x, int *cells, const CellArr *carr,
                         const Grid *grids, const double *cellsz, const SysDesc *sd,
                         const int *n_dev, const Blk *cblk, int ncblk, const Blk *mblk,
-                        int nmblk, const int2 *carr_blocks, int ncarr, int *bagg, int *carry,
-                        int D) {
+                        int nmblk, int *bagg, int D) {
   if (ncblk > 0) k_cells_clear<<<ncblk, GR_BLOCK, 0, s>>>(cells, carr, grids, n_dev, cblk);
   if (nmblk > 0) {
     if (D == 2)
@@ -1653,8 +1661,7 @@ void launch_cells_build(hipStream_t s, XP x, int *cells, const CellArr *carr,
   }
   if (ncblk > 0) {
     k_fill_local<<<ncblk, GR_BLOCK, 0, s>>>(cells, carr, grids, cblk, bagg);
-    if (ncarr > 0) k_fill_carry<<<ncarr, GR_BLOCK, 0, s>>>(carr_blocks, ncarr, bagg, carry);
-    k_fill_apply<<<ncblk, GR_BLOCK, 0, s>>>(cells, carr, grids, cblk, carry);
+    k_fill_apply<<<ncblk, GR_BLOCK, 0, s>>>(cells, carr, grids, cblk, ncblk, bagg);
   }
 }
 

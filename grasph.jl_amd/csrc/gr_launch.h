@@ -87,8 +87,7 @@ void launch_grid_params(hipStream_t s, const int *inter_list, int n, const int2 
 void launch_cells_build(hipStream_t s, XP x, int *cells, const CellArr *carr,
                         const Grid *grids, const double *cellsz, const SysDesc *sd,
                         const int *n_dev, const Blk *cblk, int ncblk, const Blk *mblk,
-                        int nmblk, const int2 *carr_blocks, int ncarr, int *bagg, int *carry,
-                        int D);
+                        int nmblk, int *bagg, int D);
 template <typename R>
 void launch_halfstep(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
                      const Blk *blk, int nblk, int D, double mult, int keep);

@@ -120,6 +120,30 @@ def test_grid_bit_exact(D, sorted_first):
             assert np.array_equal(gdev[3], gorc[3])  # cell_start_a
 
 
+def test_grid_bit_exact_long_empty_run():
+    """Two clusters 2000 apart along the slowest grid axis: ~400k empty cells
+    (hundreds of fill blocks) between them, so an empty cell's start comes
+    from a block far to its right."""
+    rng = np.random.default_rng(11)
+    h = 0.02
+    xa = rng.uniform(0.0, 0.3, (6000, 2))
+    xa[3000:, 0] += 2000.0
+    res = []
+    for make in ("dev", "orc"):
+        a = M.FluidParticleSystem("a", len(xa), 2, 1.0, 1.0)
+        a.x[:] = xa
+        si = M.SystemInteraction(M.CubicSplineKernel(h, ndims=2), M.NeighborCounterPfn(), a)
+        sc = M.Session(2, "f64").scene if make == "dev" else oracle_scene(2)
+        sc.upload(a)
+        sc.be.sort(sc.system(a), 2 * h)
+        sc.be.create_grid(sc.interaction(si))
+        res.append(sc.grid(si))
+    gdev, gorc = res
+    assert len(gorc[2]) > 300000
+    assert np.array_equal(gdev[1], gorc[1])
+    assert np.array_equal(gdev[2], gorc[2])
+
+
 # ---------------------------------------------------------------------------
 # Neighbour counts (test/test_interaction_sweep.jl:85-164, 215-305 analogues
 # at scale): device == oracle == brute force
