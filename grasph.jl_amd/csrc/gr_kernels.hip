@@ -834,13 +834,20 @@ __global__ void k_cells_mark(XP x, int *cells, const CellArr *carr, const Grid *
   const int n = n_dev[C.sys];
   const Grid G = grids[C.grid];
   if (G.ncells <= 0) return;  // uniform over the block
+  __shared__ int cw[GR_BLOCK / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   const bool live = i < n;
   const double cell = cellsz[C.grid];
   const int64_t gi = sd[C.sys].off + i;
   const int c = live ? cell_flat0<D>(x, gi, G, cell) : -1;
-  // the cell of particle i − 1 is the previous lane's (lane 0 computes it)
+  // the cell of particle i − 1: the previous lane's, the previous wavefront's
+  // last lane's (through LDS), and for the workgroup's first thread its own
+  // computation (the only one no other thread makes)
+  const int c0 = (threadIdx.x == 0 && live && i > 0) ? cell_flat0<D>(x, gi - 1, G, cell) : -1;
+  if (lane == 63) cw[w] = c;
+  __syncthreads();
   int prev = __shfl_up(c, 1, 64);
-  if ((threadIdx.x & 63) == 0) prev = (live && i > 0) ? cell_flat0<D>(x, gi - 1, G, cell) : -1;
+  if (lane == 0) prev = w > 0 ? cw[w - 1] : c0;
   if (live && c != prev && c >= 0 && c < G.ncells) atomicMax(&cells[C.off + c], i + 1);
 }
 
