@@ -1510,9 +1510,9 @@ void launch_key_minmax(hipStream_t s, XP xs, const SysDesc *sd, const int *n, co
                        int nblk, int D, long long *bbmin, long long *bbmax, int rs,
                        const int *list, int nlist, bool fold) {
   if (nblk > 0) {
-    // 4 table entries (1024 particles) per workgroup: enough workgroups to
+    // 8 table entries (2048 particles) per workgroup: enough workgroups to
     // fill the chip, few enough that the per-system atomics stay cheap
-    constexpr int per = 4;
+    constexpr int per = 8;
     const unsigned g = nb_of(nblk, per);
     if (D == 2)
       k_key_minmax<2, per><<<g, GR_BLOCK, 0, s>>>(xs, sd, n, blk, nblk, bbmin, bbmax, rs);
