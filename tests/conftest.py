@@ -17,6 +17,10 @@ grasph = __graft_entry__.package()
 A = grasph._abi
 
 ORACLE_LIB = ROOT / "oracle" / "libgrasph_oracle.so"
+# tests/test_oracle_mutants.py points this at oracles built with a planted
+# mistake, to show that the pins fail on them
+if os.environ.get("GRASPH_ORACLE_LIB"):
+    ORACLE_LIB = Path(os.environ["GRASPH_ORACLE_LIB"])
 
 
 def pytest_configure(config):
