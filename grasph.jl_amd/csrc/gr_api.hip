@@ -311,6 +311,9 @@ int guard(gr_ctx *c, F &&f) {
   } catch (const std::exception &e) {
     c->err = e.what();
     return GR_ERR_STATE;
+  } catch (...) {  // nothing may unwind across the C ABI
+    c->err = "unknown exception";
+    return GR_ERR_STATE;
   }
 }
 

@@ -42,7 +42,8 @@ enum {
 
 /* ---- arithmetic precision of the pair sums (gr_ctx_create) ------------ */
 enum {
-  GR_PREC_F32 = 0,  /* FP32 pair math + Kahan sums; FP64 x, v, rho, stress */
+  GR_PREC_F32 = 0,  /* FP32 pair math, FP64 sums of the FP32 pair terms;  */
+                    /* FP64 x, v, rho, stress                             */
   GR_PREC_F64 = 1   /* FP64 everywhere                                    */
 };
 

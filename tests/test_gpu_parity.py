@@ -6,7 +6,8 @@ indices, mirrored positions) must be bit-exact.  Floating-point results are
 compared with max|Δ| / max|ref| per field:
   * GR_PREC_F64 ("f64"): 1e-11 for one call, 1e-9 after short trajectories
     (only the summation order differs from the oracle);
-  * GR_PREC_F32 ("f32"): 2e-5 for one call (FP32 pair math, Kahan sums).
+  * GR_PREC_F32 ("f32"): 2e-5 for one call (FP32 pair math, FP64 sums of
+    the FP32 pair terms).
 """
 import math
 
