@@ -44,24 +44,33 @@ def parse():
     ap.add_argument("--graph", type=int, default=1, help="replay the step as a hipGraph")
     ap.add_argument("--calibrate-copy", type=int, default=0,
                     help="also run the plain copy kernel on this many MiB (counter calibration)")
-    ap.add_argument("--ensemble", type=int, default=0,
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="pick the launch shapes with the autotuner before timing (0: defaults)")
+    ap.add_argument("--ensemble", type=int, default=256,
                     help="also step an ensemble of this many 2D columns in shared launches "
-                         "and the same columns one by one")
-    ap.add_argument("--ensemble-steps", type=int, default=50)
-    ap.add_argument("--serve", type=int, default=0,
+                         "and the same columns one by one (0 disables)")
+    ap.add_argument("--ensemble-steps", type=int, default=200,
+                    help="timed steps of the batched ensemble")
+    ap.add_argument("--ensemble-serial-steps", type=int, default=20,
+                    help="timed steps of each column run alone")
+    ap.add_argument("--serve", type=int, default=100,
                     help="also serve this many column-collapse jobs (continuous batching) "
-                         "and the same jobs serially")
-    ap.add_argument("--serve-step-scale", type=float, default=0.2,
+                         "and the same jobs serially (0 disables)")
+    ap.add_argument("--serve-step-scale", type=float, default=0.25,
                     help="step counts of the served sample are scaled by this factor")
-    ap.add_argument("--serve-size-scale", type=float, default=0.25,
+    ap.add_argument("--serve-size-scale", type=float, default=1.0,
                     help="particle counts of the served sample are scaled by this factor")
     return ap.parse_args()
 
 
-def ensemble_leg(n_sims, nsteps, precision):
-    """`n_sims` 2D granular columns (ensemble parameters) advanced together
-    (gr_step_many, one launch per phase) vs one at a time."""
+def ensemble_leg(n_sims, nsteps, serial_steps, precision, tune):
+    """`n_sims` 2D granular columns (the ensemble workload: aspect, width, φ
+    and ρ drawn per simulation, dx = 2.5 mm) advanced together (gr_step_many:
+    one graph replay per step, one launch per phase over the packed pool)
+    vs each column alone, one after another (graph replay too).  The
+    batched run times `nsteps` steps, each column alone `serial_steps`."""
     g = __graft_entry__.package()
+    t0 = time.perf_counter()
     sims = g.configs.ensemble_columns(n_sims=n_sims)
     npart = sum(c["n_particles"] for c in sims)
     s = g.Session(2, precision)
@@ -72,32 +81,59 @@ def ensemble_leg(n_sims, nsteps, precision):
         s.scene.upload_all(cfg["integrator"])
         s.backend.integrator_set_dt(h, s.backend.integrator_dt(h, cfg["cfl"]))
         hs.append(h)
+    build_s = time.perf_counter() - t0
+    rec = (g.autotune.autotune(s, [c["integrator"] for c in sims], steps=2, reps=1)
+           if tune else None)
     s.backend.step_many(hs, 2)  # plan, capture
     t0 = time.perf_counter()
     s.backend.step_many(hs, nsteps)
     batched = time.perf_counter() - t0
+    for h in hs:  # plan and capture of each column alone, untimed
+        s.backend.step_many([h], 1)
     t0 = time.perf_counter()
     for h in hs:
-        s.backend.step_many([h], nsteps)
+        s.backend.step_many([h], serial_steps)
     serial = time.perf_counter() - t0
-    return {"sims": n_sims, "particles": npart, "steps": nsteps,
-            "batched_s": batched, "serial_s": serial,
-            "sim_steps_per_s_batched": n_sims * nsteps / batched,
-            "sim_steps_per_s_serial": n_sims * nsteps / serial,
-            "particle_steps_per_s_batched": npart * nsteps / batched,
-            "speedup": serial / batched}
+    s.close()
+    b_rate = n_sims * nsteps / batched
+    s_rate = n_sims * serial_steps / serial
+    out = {"workload": f"{n_sims} 2D granular columns (EP, Drucker-Prager, ghost walls), "
+                       f"aspect U[0.5,4], width U[0.1,0.4] m, phi U[20,40] deg, "
+                       f"rho U[1500,2000], dx 2.5 mm",
+           "sims": n_sims, "particles": npart, "steps_batched": nsteps,
+           "steps_serial_each": serial_steps, "batched_s": batched, "serial_s": serial,
+           "sim_steps_per_s_batched": b_rate, "sim_steps_per_s_serial": s_rate,
+           "particle_steps_per_s_batched": npart * nsteps / batched,
+           "ms_per_batched_step": batched / nsteps * 1e3,
+           "speedup": b_rate / s_rate, "build_s": build_s,
+           "projected_5000_steps_s": 5000 * batched / nsteps}
+    if rec:
+        out["autotune"] = rec
+    return out
 
 
 def serve_leg(n_jobs, step_scale, size_scale, precision):
+    """The served stream: Poisson arrivals at 20 jobs/s, 80 % 2D elasto-plastic
+    columns (the ensemble's parameter ranges) and 20 % 3D visco-plastic
+    columns (50k-500k particles, dx 5 mm, phi U[25, 40] deg), step counts
+    log-uniform in [500, 20000] times `step_scale`, particle counts times
+    `size_scale`, an HBM budget of 200 GB.  Continuous batching (one context
+    per dimension, shared launches, admission by footprint, retirement)
+    against the same jobs run one at a time, each alone on the GPU, in
+    arrival order (graph replay and scene building overlapped in both)."""
     g = __graft_entry__.package()
     S = g.serving
     mk = lambda: S.column_jobs(n_jobs, step_scale=step_scale, size_scale=size_scale)  # noqa: E731
-    served = S.Server(precision=precision, quantum=50, graph=True).run(mk())
+    served = S.Server(precision=precision, quantum=50, graph=True, hbm_budget=200e9).run(mk())
     plain = S.run_plain(mk(), precision=precision, graph=True)
-    return {"sample": f"{n_jobs} jobs, Poisson 20/s, 80% 2D / 20% 3D columns, steps x{step_scale}, "
-                      f"particles x{size_scale}",
+    return {"workload": "served stream of 2D/3D column-collapse jobs (Poisson 20/s, 80% 2D / "
+                        "20% 3D, steps log-U[500, 20000], HBM budget 200 GB)",
+            "jobs": n_jobs, "step_scale": step_scale, "size_scale": size_scale,
             "served": served, "plain": plain,
-            "jobs_per_h_ratio": served["jobs_per_h"] / plain["jobs_per_h"]}
+            "jobs_per_h_served": served["jobs_per_h"], "jobs_per_h_plain": plain["jobs_per_h"],
+            "jobs_per_h_ratio": served["jobs_per_h"] / plain["jobs_per_h"],
+            "latency_p50_s": {"served": served["latency_p50_s"], "plain": plain["latency_p50_s"]},
+            "latency_p99_s": {"served": served["latency_p99_s"], "plain": plain["latency_p99_s"]}}
 
 
 def sweep_bytes(cfg, grids):
@@ -197,7 +233,11 @@ def main():
     be.integrator_set_dt(h, dt)
     be.set_graph_mode(bool(args.graph))
     info = be.ctx_info()
-    # warm-up (includes the first sort of the shuffled order and graph capture)
+    # first sort of the shuffled order, then the launch shapes (the tuner
+    # restores the state it found)
+    be.step_many([h], 1)
+    tune = g.autotune.autotune(s, [lf], steps=3, reps=2) if args.autotune else None
+    # warm-up (graph capture)
     if args.warmup:
         be.step_many([h], args.warmup)
     # pair count of the current configuration (diagnostic counting kernels)
@@ -233,7 +273,8 @@ def main():
     sort_gbs = bytes_sort / (phases[0] * 1e-3) / 1e9 if phases[0] > 0 else None
     traffic = load_traffic()
     cpu = cpu_baseline(g.configs.dambreak_3d, kw, args.cpu_steps, pairs)
-    ens = ensemble_leg(args.ensemble, args.ensemble_steps, args.precision) if args.ensemble else None
+    ens = (ensemble_leg(args.ensemble, args.ensemble_steps, args.ensemble_serial_steps,
+                        args.precision, bool(args.autotune)) if args.ensemble else None)
     srv = (serve_leg(args.serve, args.serve_step_scale, args.serve_size_scale, args.precision)
            if args.serve else None)
     total_pairs = pairs * args.steps
@@ -282,6 +323,8 @@ def main():
         "ms_per_step_graph": el_graph / args.steps * 1e3,
         "device": info,
     }
+    if tune:
+        res["autotune"] = tune
     if ens:
         res["ensemble"] = ens
     if srv:

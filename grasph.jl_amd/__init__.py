@@ -22,3 +22,4 @@ from .integration import (  # noqa: F401
 from . import configs  # noqa: F401
 from . import serving  # noqa: F401
 from . import snapshot  # noqa: F401
+from . import autotune  # noqa: F401

@@ -16,7 +16,10 @@ from pathlib import Path
 import numpy as np
 
 # ---- status codes -------------------------------------------------------
-GR_OK, GR_ERR_ARGUMENT, GR_ERR_ASSERTION, GR_ERR_HIP, GR_ERR_CAPACITY, GR_ERR_STATE = range(6)
+(GR_OK, GR_ERR_ARGUMENT, GR_ERR_ASSERTION, GR_ERR_HIP, GR_ERR_CAPACITY, GR_ERR_STATE,
+ GR_ERR_FAULT) = range(7)
+# ---- per-integrator fault bits ---------------------------------------------
+GR_FAULT_NONFINITE, GR_FAULT_DOMAIN = 1, 2
 # ---- precision ------------------------------------------------------------
 GR_PREC_F32, GR_PREC_F64 = 0, 1
 # ---- system kinds -----------------------------------------------------------
@@ -44,6 +47,7 @@ GR_WRAP_NONE, GR_WRAP_STATIC, GR_WRAP_DYNAMIC = range(3)
  GR_UPD_PRESCRIBED_V, GR_UPD_HOOKE) = range(9)
 GR_COPY_PLAIN, GR_COPY_HOUSEHOLDER = 0, 1
 GR_SCHEME_LEAPFROG, GR_SCHEME_RK4 = 0, 1
+GR_TUNE_NL_WIN, GR_TUNE_NL_SPAN, GR_TUNE_SWEEP_BLOCK, GR_TUNE_MAXNB, GR_TUNE_NL_KERNEL = range(5)
 
 
 class gr_pfn_desc(C.Structure):
@@ -105,6 +109,9 @@ _GPU_SIGS = {
     "get_pair_stats": [_i64p, _i64p],
     "ctx_info": [C.c_char_p, _i64],
     "bench_copy": [_i64, _i],
+    "integrator_status": [_i, _ip],
+    "tune_set": [_i, _i],
+    "tune_get": [_i, _ip],
 }
 
 PKG_DIR = Path(__file__).resolve().parent
@@ -121,6 +128,11 @@ def _i64ptr(a):
 
 def _iptr(a):
     return a.ctypes.data_as(_ip)
+
+
+class FaultError(RuntimeError):
+    """gr_step / gr_step_many completed, but some of the integrators stepped
+    faulted (GR_ERR_FAULT); `integrator_status` tells which."""
 
 
 class CBackend:
@@ -171,6 +183,8 @@ class CBackend:
                 raise ValueError(msg)
             if rc == GR_ERR_ASSERTION:
                 raise AssertionError(msg)
+            if rc == GR_ERR_FAULT:
+                raise FaultError(msg)
             raise RuntimeError(f"{self.prefix}{name} failed ({rc}): {msg}")
         return rc
 
@@ -316,6 +330,19 @@ class CBackend:
 
     def integrator_set_dt(self, integ, dt):
         self.call("integrator_set_dt", integ, dt)
+
+    def tune_set(self, param, value):
+        self.call("tune_set", param, int(value))
+
+    def tune_get(self, param):
+        out = C.c_int()
+        self.call("tune_get", param, C.byref(out))
+        return out.value
+
+    def integrator_status(self, integ):
+        out = C.c_int()
+        self.call("integrator_status", integ, C.byref(out))
+        return out.value
 
     def step_many(self, integs, nsteps):
         a = np.ascontiguousarray(integs, dtype=np.int32)

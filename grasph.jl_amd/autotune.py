@@ -1,0 +1,94 @@
+"""Autotuner of the launch shapes of the neighbour search and functor gather.
+
+The reference has no counterpart: its hot loop (src/Interaction.jl:477-541)
+runs on Polyester threads with no launch shape.  On the device the knobs
+that matter per scene are (include/grasph_hip.h, GR_TUNE_*):
+
+  * GR_TUNE_SWEEP_BLOCK — target particles per workgroup of the functor
+    gather (64, 128, 256);
+  * GR_TUNE_NL_WIN — candidates staged in LDS per window of the neighbour
+    search (the staging tile: 64, 128, 256);
+  * GR_TUNE_NL_SPAN — how far apart the lanes' candidate ranges may start
+    and still share one staging pass (1, 2, 4: a cell-column grouping);
+  * GR_TUNE_MAXNB — list entries kept per target before it walks its cells
+    directly (memory against overflow walks);
+  * GR_TUNE_NL_KERNEL — the neighbour search with one target per lane, or
+    with the candidates in the lanes tested against each target of a cell.
+
+None of them changes a result (bit-identical lists, sums and states), so
+the tuner only times.  It runs a coordinate descent: for each knob in turn,
+every candidate value is timed over a few steps of the caller's own
+integrators (HIP events around each step, the whole step's mean), the best
+value is kept, and the next knob is tried with it.  The simulations' state
+is saved to the host before the trials and restored afterwards, so tuning
+does not advance them.
+"""
+from __future__ import annotations
+
+import time
+
+from . import _abi as A
+
+KNOB_NAMES = {A.GR_TUNE_NL_KERNEL: "nl_kernel", A.GR_TUNE_SWEEP_BLOCK: "sweep_block",
+              A.GR_TUNE_NL_WIN: "nl_win", A.GR_TUNE_NL_SPAN: "nl_span",
+              A.GR_TUNE_MAXNB: "maxnb"}
+
+
+def default_space(ndims):
+    """Candidate values per knob, in the order the descent visits them."""
+    mx = (32, 48, 64) if ndims == 3 else (16, 24, 32)
+    return [(A.GR_TUNE_NL_KERNEL, (0, 1)),
+            (A.GR_TUNE_SWEEP_BLOCK, (256, 128, 64)),
+            (A.GR_TUNE_NL_WIN, (128, 64, 256)),
+            (A.GR_TUNE_NL_SPAN, (2, 1, 4)),
+            (A.GR_TUNE_MAXNB, mx)]
+
+
+def _time_steps(be, integs, steps, reps):
+    """Mean ms per step (HIP events around each step), best of `reps`."""
+    best = float("inf")
+    be.step_many(integs, 1)  # plan and lists for the current shape
+    for _ in range(reps):
+        be.set_timing(1)
+        be.step_many(integs, steps)
+        ms = be.get_timing()[7]
+        be.set_timing(0)
+        best = min(best, ms)
+    return best
+
+
+def current(be):
+    return {KNOB_NAMES[k]: be.tune_get(k) for k in KNOB_NAMES}
+
+
+def autotune(session, integrators, steps=3, reps=2, space=None, restore=True):
+    """Pick the launch shapes for `integrators` (host LeapFrogTimeIntegrator
+    objects registered on `session`'s scene) and leave them set on the
+    context.  Returns a record: the chosen values, the default's and the
+    tuned time per step, and every trial."""
+    be, sc = session.backend, session.scene
+    hs = [sc.integrator(lf) for lf in integrators]
+    t0 = time.perf_counter()
+    if restore:
+        for lf in integrators:
+            sc.download_all(lf)
+    space = space or default_space(sc.ndims)
+    start = current(be)
+    ms_start = _time_steps(be, hs, steps, reps)
+    trials = []
+    for knob, values in space:
+        best_v, best_ms = be.tune_get(knob), None
+        for v in values:
+            be.tune_set(knob, v)
+            ms = _time_steps(be, hs, steps, reps)
+            trials.append([KNOB_NAMES[knob], int(v), ms])
+            if best_ms is None or ms < best_ms:
+                best_v, best_ms = v, ms
+        be.tune_set(knob, best_v)
+    ms_tuned = _time_steps(be, hs, steps, reps)
+    if restore:
+        for lf in integrators:
+            sc.upload_all(lf)
+    return {"chosen": current(be), "start": start, "ms_per_step_start": ms_start,
+            "ms_per_step_tuned": ms_tuned, "trials": trials, "steps_per_trial": steps,
+            "reps": reps, "tuning_s": time.perf_counter() - t0}

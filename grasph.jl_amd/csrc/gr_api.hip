@@ -37,24 +37,90 @@ struct GrError : std::runtime_error {
       throw GrError(GR_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));     \
   } while (0)
 
+// Device blocks of one context kept for reuse instead of hipFree/hipMalloc
+// (each of which synchronises the device): plans are rebuilt whenever a
+// simulation is admitted or released, and their tables come and go.  Blocks
+// up to kCacheBlock bytes are kept in power-of-two size classes, at most
+// kCacheTotal bytes in all; a larger block is freed.  All work of a context
+// runs on its one stream, so a block returned while that stream may still
+// read it is only handed out again to work queued behind it.
+struct DevCache {
+  static constexpr size_t kCacheBlock = 32u << 20;
+  static constexpr size_t kCacheTotal = 1u << 30;
+  std::map<size_t, std::vector<void *>> bins;
+  size_t cached = 0;
+  DevCache() = default;
+  DevCache(const DevCache &) = delete;
+  DevCache &operator=(const DevCache &) = delete;
+  ~DevCache() {
+    for (auto &kv : bins)
+      for (void *q : kv.second) (void)hipFree(q);
+  }
+  static size_t size_class(size_t b) {
+    size_t c = 256;
+    while (c < b) c <<= 1;
+    return c;
+  }
+  void *take(size_t b, size_t &got) {
+    if (b <= kCacheBlock) {
+      got = size_class(b);
+      auto it = bins.find(got);
+      if (it != bins.end() && !it->second.empty()) {
+        void *q = it->second.back();
+        it->second.pop_back();
+        cached -= got;
+        return q;
+      }
+    } else {
+      got = b;
+    }
+    void *q = nullptr;
+    HIPCHK(hipMalloc(&q, got));
+    return q;
+  }
+  void give(void *q, size_t b) {
+    if (b <= kCacheBlock && cached + b <= kCacheTotal) {
+      bins[b].push_back(q);
+      cached += b;
+    } else {
+      (void)hipFree(q);
+    }
+  }
+};
+// The cache of the context whose C-ABI call is running on this thread
+// (set by guard()); null outside a call.
+thread_local DevCache *tl_cache = nullptr;
+
 // Owned device buffer.
 struct DBuf {
   void *p = nullptr;
   size_t bytes = 0;
+  DevCache *cache = nullptr;  // where the block goes back (null: hipFree)
   DBuf() = default;
   DBuf(const DBuf &) = delete;
   DBuf &operator=(const DBuf &) = delete;
   ~DBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      if (cache)
+        cache->give(p, bytes);
+      else
+        (void)hipFree(p);
+    }
     p = nullptr;
     bytes = 0;
+    cache = nullptr;
   }
   void alloc(size_t b) {
     release();
     if (b == 0) b = 16;
-    HIPCHK(hipMalloc(&p, b));
-    bytes = b;
+    if (tl_cache) {
+      p = tl_cache->take(b, bytes);
+      cache = tl_cache;
+    } else {
+      HIPCHK(hipMalloc(&p, b));
+      bytes = b;
+    }
   }
   template <typename T>
   T *as() const {
@@ -113,6 +179,7 @@ struct IntegH {
   int nst = 0;
   bool dead = false;
   int scheme = GR_SCHEME_LEAPFROG;
+  int fault = 0;  // GR_FAULT_* bits collected by its steps
 };
 
 // Free ranges [off, off + len) of a pool, kept sorted and coalesced.
@@ -191,6 +258,8 @@ struct Plan {
   std::vector<std::unique_ptr<FamPlan>> xsph_al;
   std::vector<int2> xsph_alias;  // (ghost, source n) of those tasks
   int vadj_live = 0;
+  // targets per workgroup of the plan's sweep block tables
+  int sweep_block = GR_BLOCK;
   // integrated (non-virtual) real systems; virtual systems
   DBuf blk_int, blk_virt, blk_probe;
   int nblk_int = 0, nblk_virt = 0, nblk_probe = 0;
@@ -206,6 +275,8 @@ struct Plan {
 }  // namespace
 
 struct gr_ctx {
+  // first member: destroyed last, after every buffer has returned its block
+  DevCache cache;
   int device = 0, D = 2, prec = GR_PREC_F32;
   hipStream_t stream = nullptr;
   std::string err;
@@ -239,6 +310,7 @@ struct gr_ctx {
   bool vadj_any = false;  // some v_adjustment may be nonzero: permute and correct it
   // tables
   DBuf d_sd, d_n, d_st, d_bnd, d_kmin, d_kmax, d_bbmin, d_bbmax, d_kbase, d_krange, d_flags;
+  DBuf d_sysfault;  // per system: GR_FAULT_* bits raised by the step's kernels
   int bb_rs = 0;  // entries per replica of d_bbmin / d_bbmax (GR_BB_REP replicas)
   DBuf d_grids, d_carr, d_cellsz, d_inter_sys, d_inter_carr, cells;
   std::vector<CellArr> carr;
@@ -252,6 +324,8 @@ struct gr_ctx {
   int64_t lidx_cap = 0, lcnt_cap = 0;
   int use_lists = 1;
   int list_maxnb = 0;  // mask entries kept per target (0: 48 in 3D, 24 in 2D)
+  // launch shapes chosen by gr_tune_set (results do not depend on them)
+  int tune_win = 128, tune_span = 2, tune_block = GR_BLOCK, tune_nlk = 0;
   std::map<std::vector<int>, std::unique_ptr<Plan>> plans;
   // cached device tables derived from the layout
   DBuf allreal_blk;
@@ -302,6 +376,11 @@ int fail(gr_ctx *c, int code, const std::string &m) {
 template <typename F>
 int guard(gr_ctx *c, F &&f) {
   if (!c) return GR_ERR_ARGUMENT;
+  struct CacheScope {  // device blocks of this call come from c's cache
+    DevCache *prev;
+    explicit CacheScope(DevCache *cc) : prev(tl_cache) { tl_cache = cc; }
+    ~CacheScope() { tl_cache = prev; }
+  } scope(&c->cache);
   try {
     f();
     return GR_OK;
@@ -379,6 +458,7 @@ void grow(gr_ctx *c, DBuf &b, int comps, size_t esz, int64_t old_cap, int64_t ne
                             hipMemcpyDeviceToDevice, c->stream));
   std::swap(b.p, nb.p);
   std::swap(b.bytes, nb.bytes);
+  std::swap(b.cache, nb.cache);
 }
 
 void commit(gr_ctx *c) {
@@ -506,6 +586,7 @@ void commit(gr_ctx *c) {
   std::vector<unsigned> zu(ns * 3, 0);
   c->d_kbase.upload(zu, c->stream);
   c->d_krange.upload(zu, c->stream);
+  c->d_sysfault.upload(std::vector<int>(std::max(ns, 1), 0), c->stream);
   if (!c->d_flags.p) {
     std::vector<int> f(4, 0);
     c->d_flags.upload(f, c->stream);
@@ -1320,7 +1401,7 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
         t.src_begin = (int)sv.size();
         for (auto &q : kv.second.src) sv.push_back(q);
         t.src_end = (int)sv.size();
-        add_blocks(bv, (int)tv.size(), c->sys[t.sys].cap, GR_BLOCK);
+        add_blocks(bv, (int)tv.size(), c->sys[t.sys].cap, P.sweep_block);
         tv.push_back(t);
       }
       sp->fam[fam].tasks.upload(tv, c->stream);
@@ -1348,7 +1429,7 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
         t.src_begin = (int)sv.size();
         for (auto &q : kv.second.src) sv.push_back(q);
         t.src_end = (int)sv.size();
-        add_blocks(bv, (int)tv.size(), c->sys[t.sys].cap, GR_BLOCK);
+        add_blocks(bv, (int)tv.size(), c->sys[t.sys].cap, P.sweep_block);
         tv.push_back(t);
       }
       F.tasks.upload(tv, c->stream);
@@ -1381,6 +1462,7 @@ Plan &get_plan(gr_ctx *c, const std::vector<int> &integs) {
   if (it != c->plans.end()) return *it->second;
   auto P = std::make_unique<Plan>();
   P->integs = integs;
+  P->sweep_block = c->tune_block;
   std::vector<int> reals, ghosts, inters, all;
   for (int ig : integs) {
     const IntegH &I = c->integ[ig];
@@ -1474,7 +1556,7 @@ void phase_sort_real(gr_ctx *c, Plan &P) {
   launch_key_base(s, P.d_real_list.as<int>(), (int)P.real_list.size(), c->D, 0, c->d_n.as<int>(),
                   c->d_st.as<StepSys>(), c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>(),
                   c->bb_rs, c->d_kmin.as<int>(), c->d_kmax.as<int>(), c->d_kbase.as<unsigned>(),
-                  c->d_krange.as<unsigned>(), c->d_flags.as<int>());
+                  c->d_krange.as<unsigned>(), c->d_flags.as<int>(), c->d_sysfault.as<int>());
   launch_key_write(s, xs, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), c->d_st.as<StepSys>(),
                    P.blk_real.as<Blk>(), P.nblk_real, P.d_bbase_real.as<int>(), c->D,
                    c->d_kmin.as<int>(), c->d_krange.as<unsigned>(), c->d_kbase.as<unsigned>(),
@@ -1517,7 +1599,8 @@ void phase_ghosts(gr_ctx *c, Plan &P, bool generate, bool sort) {
     launch_key_base(s, P.d_ghost_list.as<int>(), (int)P.ghost_list.size(), c->D, 1,
                     c->d_n.as<int>(), c->d_st.as<StepSys>(), c->d_bbmin.as<long long>(),
                     c->d_bbmax.as<long long>(), c->bb_rs, c->d_kmin.as<int>(), c->d_kmax.as<int>(),
-                    c->d_kbase.as<unsigned>(), c->d_krange.as<unsigned>(), c->d_flags.as<int>());
+                    c->d_kbase.as<unsigned>(), c->d_krange.as<unsigned>(), c->d_flags.as<int>(),
+                    c->d_sysfault.as<int>());
     launch_key_write(s, gx, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), c->d_st.as<StepSys>(),
                      P.blk_ghost.as<Blk>(), P.nblk_ghost, P.d_bbase_ghost.as<int>(), c->D,
                      c->d_kmin.as<int>(), c->d_krange.as<unsigned>(), c->d_kbase.as<unsigned>(),
@@ -1547,7 +1630,8 @@ void phase_grids(gr_ctx *c, Plan &P) {
   launch_grid_params(s, P.d_inter_list.as<int>(), (int)P.inters.size(), c->d_inter_sys.as<int2>(),
                      c->d_cellsz.as<double>(), c->d_bbmin.as<long long>(),
                      c->d_bbmax.as<long long>(), c->d_n.as<int>(), c->d_grids.as<Grid>(),
-                     c->d_inter_carr.as<int2>(), c->d_carr.as<CellArr>(), c->D, c->d_flags.as<int>());
+                     c->d_inter_carr.as<int2>(), c->d_carr.as<CellArr>(), c->D, c->d_flags.as<int>(),
+                     c->d_sysfault.as<int>());
   launch_cells_build(s, xp_pool(c, c->cur), c->cells.as<int>(),
                      c->d_carr.as<CellArr>(), c->d_grids.as<Grid>(), c->d_cellsz.as<double>(),
                      c->d_sd.as<SysDesc>(), c->d_n.as<int>(), P.blk_cells.as<Blk>(), P.nblk_cells,
@@ -1605,6 +1689,7 @@ void phase_stage(gr_ctx *c, Plan &P, int k, bool cnt, hipEvent_t *mid) {
       a.lidx = c->lidx.as<int>();
     }
     a.fluid_only = F.fluid_only;
+    a.block = P.sweep_block;
     launch_sweep<R>(s, A, a, c->D, fam, cnt);
   }
 }
@@ -1633,6 +1718,7 @@ void phase_xsph(gr_ctx *c, Plan &P) {
     a.cells = c->cells.as<int>();
     a.blk = P.xsph.blk.as<Blk>();
     a.nblk = P.xsph.nblk;
+    a.block = P.sweep_block;
     a.pair_count = c->d_pairs.as<unsigned long long>();
     if (P.nblk_list > 0) {
       a.lt = P.ltasks.as<LTask>();
@@ -1697,6 +1783,9 @@ void run_step(gr_ctx *c, Plan &P, bool cnt, std::vector<hipEvent_t> *ev) {
     a.lidx = c->lidx.as<int>();
     a.bbmin = c->d_bbmin.as<long long>();
     a.bbmax = c->d_bbmax.as<long long>();
+    a.win = c->tune_win;
+    a.span = c->tune_span;
+    a.cand = c->tune_nlk;
     launch_nlist<R>(s, A, a, c->D);
   }
   rec(4);
@@ -1813,10 +1902,35 @@ void do_steps(gr_ctx *c, const std::vector<int> &integs, int nsteps) {
     c->last_pairs[0] = (long long)(pc[0] / std::max(nsteps, 1));
     c->last_pairs[1] = (long long)(pc[1] / std::max(nsteps, 1));
   }
-  if (flags[1]) {
+  std::string faulted;
+  if (flags[1] & (2 | 8)) {
+    // per-system faults → the integrators they belong to (the kernels have
+    // isolated those systems; every other system stepped normally)
+    const size_t ns = c->sys.size();
+    std::vector<int> sf(ns, 0);
+    HIPCHK(hipMemcpyAsync(sf.data(), c->d_sysfault.p, ns * sizeof(int), hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipMemsetAsync(c->d_sysfault.p, 0, ns * sizeof(int), c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (int ig : integs) {
+      IntegH &I = c->integ[ig];
+      int f = 0;
+      for (const auto *lst : {&I.sys, &I.ghost, &I.virt, &I.probes})
+        for (int q : *lst)
+          if (q >= 0 && q < (int)ns) f |= sf[q];
+      if (f) {
+        I.fault |= f;
+        faulted += (faulted.empty() ? "" : ", ") + std::to_string(ig);
+      }
+    }
+    if (!faulted.empty()) flags[1] &= ~(2 | 8);
+  }
+  if (flags[1] || !faulted.empty()) {
     int z[4] = {0, 0, 0, 0};
     HIPCHK(hipMemcpyAsync(c->d_flags.p, z, sizeof(z), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  if (flags[1]) {
     if (flags[1] & 4)
       throw GrError(GR_ERR_ARGUMENT,
                     "XSPHPfn with a ghost partner: more ghosts than source particles "
@@ -1825,6 +1939,11 @@ void do_steps(gr_ctx *c, const std::vector<int> &integs, int nsteps) {
                   (flags[1] & 2) ? "cell grid outgrew its capacity (particles left the planned domain)"
                                  : "cell-key range exceeds 32 bits");
   }
+  if (!faulted.empty())
+    throw GrError(GR_ERR_FAULT,
+                  "integrator(s) " + faulted +
+                      " faulted (non-finite positions or particles beyond the cell range); "
+                      "the others advanced normally");
 }
 
 void validate_sys(gr_ctx *c, int s) {
@@ -2573,6 +2692,61 @@ int gr_step_many(gr_ctx *c, int ninteg, const int *integs, int nsteps) {
   });
 }
 
+int gr_integrator_status(gr_ctx *c, int integ, int *fault_out) {
+  return guard(c, [&] {
+    need(integ >= 0 && integ < (int)c->integ.size(), GR_ERR_ARGUMENT, "bad integrator");
+    need(fault_out != nullptr, GR_ERR_ARGUMENT, "fault_out is NULL");
+    *fault_out = c->integ[integ].fault;
+  });
+}
+
+int gr_tune_set(gr_ctx *c, int param, int value) {
+  return guard(c, [&] {
+    switch (param) {
+      case GR_TUNE_NL_WIN:
+        need(value == 64 || value == 128 || value == 256, GR_ERR_ARGUMENT,
+             "GR_TUNE_NL_WIN must be 64, 128 or 256");
+        c->tune_win = value;
+        break;
+      case GR_TUNE_NL_SPAN:
+        need(value >= 1 && value <= 16, GR_ERR_ARGUMENT, "GR_TUNE_NL_SPAN must be in 1..16");
+        c->tune_span = value;
+        break;
+      case GR_TUNE_SWEEP_BLOCK:
+        need(value == 64 || value == 128 || value == 256, GR_ERR_ARGUMENT,
+             "GR_TUNE_SWEEP_BLOCK must be 64, 128 or 256");
+        c->tune_block = value;
+        break;
+      case GR_TUNE_MAXNB:
+        need(value == 0 || (value >= 4 && value <= 1024), GR_ERR_ARGUMENT,
+             "GR_TUNE_MAXNB must be 0 (default) or in 4..1024");
+        c->list_maxnb = value;
+        break;
+      case GR_TUNE_NL_KERNEL:
+        need(value == 0 || value == 1, GR_ERR_ARGUMENT, "GR_TUNE_NL_KERNEL must be 0 or 1");
+        c->tune_nlk = value;
+        break;
+      default:
+        throw GrError(GR_ERR_ARGUMENT, "unknown tuning parameter");
+    }
+    c->plans.clear();  // block tables, list layout and captured graphs change
+  });
+}
+
+int gr_tune_get(gr_ctx *c, int param, int *value) {
+  return guard(c, [&] {
+    need(value != nullptr, GR_ERR_ARGUMENT, "value is NULL");
+    switch (param) {
+      case GR_TUNE_NL_WIN: *value = c->tune_win; break;
+      case GR_TUNE_NL_SPAN: *value = c->tune_span; break;
+      case GR_TUNE_SWEEP_BLOCK: *value = c->tune_block; break;
+      case GR_TUNE_MAXNB: *value = c->list_maxnb > 0 ? c->list_maxnb : (c->D == 3 ? 48 : 24); break;
+      case GR_TUNE_NL_KERNEL: *value = c->tune_nlk; break;
+      default: throw GrError(GR_ERR_ARGUMENT, "unknown tuning parameter");
+    }
+  });
+}
+
 int gr_set_graph_mode(gr_ctx *c, int on) {
   return guard(c, [&] { c->graph_mode = on != 0; });
 }
@@ -2629,7 +2803,24 @@ int gr_sort(gr_ctx *c, int sys, double cutoff) {
         phase_sort_real<R>(c, P);
       });
     }
+    // floor(Int, x / cutoff) of a non-finite position raises in the
+    // reference (src/Sorting.jl:122-124); here the system keeps its order
+    // and the call reports it
+    int flags[4];
+    HIPCHK(hipMemcpyAsync(flags, c->d_flags.p, sizeof(flags), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    if (flags[1] & (1 | 8)) {
+      int sf = 0;
+      HIPCHK(hipMemcpyAsync(&sf, c->d_sysfault.as<int>() + sys, sizeof(int), hipMemcpyDeviceToHost,
+                            c->stream));
+      int z[4] = {0, 0, 0, 0};
+      HIPCHK(hipMemcpyAsync(c->d_flags.p, z, sizeof(z), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipMemsetAsync(c->d_sysfault.p, 0, c->sys.size() * sizeof(int), c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      if (sf & GR_FAULT_NONFINITE)
+        throw GrError(GR_ERR_ARGUMENT, "non-finite position: its cell key floor(x / cutoff) is undefined");
+      throw GrError(GR_ERR_CAPACITY, "cell-key range exceeds 32 bits");
+    }
   });
 }
 

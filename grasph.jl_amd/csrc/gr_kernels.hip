@@ -209,7 +209,8 @@ __device__ inline double unord_d(long long o) {
 }
 __global__ void k_key_base(const int *list, int nlist, int D, int pad, const int *n_dev,
                            const StepSys *st, long long *bbmin, long long *bbmax, int rs,
-                           int *kmin, int *kmax, unsigned *kbase, unsigned *krange, int *flags) {
+                           int *kmin, int *kmax, unsigned *kbase, unsigned *krange, int *flags,
+                           int *sysfault) {
   // one wavefront: the systems' key ranges in parallel, 64 at a time, and
   // their exclusive prefix sum (each system's keys start after the ranges of
   // the systems listed before it, plus one padding key each when `pad`)
@@ -240,6 +241,24 @@ __global__ void k_key_base(const int *list, int nlist, int D, int pad, const int
                                                                : range * (unsigned long long)r;
         }
         if (range > (1ull << 32)) range = 1ull << 32;
+        // a system whose bounds are not finite, or whose own keys need 32
+        // bits or more, is faulted: all its keys become its base (the
+        // stable sort keeps its order), so the systems after it keep
+        // their keys and the step goes on for them
+        bool nonfinite = false;
+        for (int d = 0; d < D; ++d)
+          nonfinite = nonfinite || !(unord_d(bbmin[s * 3 + d]) <= unord_d(bbmax[s * 3 + d])) ||
+                      !isfinite(unord_d(bbmin[s * 3 + d])) || !isfinite(unord_d(bbmax[s * 3 + d]));
+        if (nonfinite || range >= 0xFFFFFFFFull) {
+          atomicOr(&sysfault[s], nonfinite ? GR_FAULT_NONFINITE : GR_FAULT_DOMAIN);
+          atomicOr(&flags[1], 8);
+          for (int d = 0; d < D; ++d) {
+            kmin[s * 3 + d] = 0;
+            kmax[s * 3 + d] = 0;
+            krange[s * 3 + d] = d == 0 ? 0u : 1u;  // krange[0] == 0 marks it
+          }
+          range = 1;
+        }
       } else {
         for (int d = 0; d < D; ++d) krange[s * 3 + d] = 1;
       }
@@ -267,6 +286,7 @@ template <int D>
 __device__ inline unsigned lin_key(XP xs, int64_t gi, double cutoff, const int *kmin,
                                    const unsigned *krange, int s) {
   unsigned k = 0;
+  if (krange[s * 3] == 0u) return 0u;  // faulted system (k_key_base)
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     int c = key_comp(xs[d][gi], cutoff) - kmin[s * 3 + d];
@@ -757,7 +777,8 @@ __global__ void k_scan_add(int *out, const int *bsum, int N) {
 __global__ void k_grid_params(const int *inter_list, int n, const int2 *inter_sys,
                               const double *cellsz, const long long *bbmin,
                               const long long *bbmax, const int *n_dev, Grid *grids,
-                              const int2 *inter_carr, const CellArr *carr, int D, int *flags) {
+                              const int2 *inter_carr, const CellArr *carr, int D, int *flags,
+                              int *sysfault) {
   int q = blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= n) return;
   const int it = inter_list[q];
@@ -777,7 +798,9 @@ __global__ void k_grid_params(const int *inter_list, int n, const int2 *inter_sy
     double ngd = floor((maxg - G.ming[d]) / cutoff) + 1.0;  // :179
     if (!(ngd >= 1.0) || ngd > 1.0e9) ngd = 1.0e9;
     G.ng[d] = (int)ngd;
-    ncells *= (long long)G.ng[d];
+    // saturate at 2^40 cells (far beyond any capacity): three factors of up
+    // to 1e9 would overflow a 64-bit product
+    ncells = ncells > (1LL << 40) / G.ng[d] ? (1LL << 40) : ncells * (long long)G.ng[d];
   }
   for (int d = D; d < 3; ++d) {
     G.ming[d] = 0.0;
@@ -788,8 +811,12 @@ __global__ void k_grid_params(const int *inter_list, int n, const int2 *inter_sy
   long long cap = carr[ca.x].cap;
   if (ca.y >= 0 && carr[ca.y].cap < cap) cap = carr[ca.y].cap;
   if (ncells + 1 > cap || ncells < 1) {
+    // an empty grid: the sweeps find no partner for these systems; the
+    // systems are faulted, the other systems' grids are unaffected
     G.ok = 0;
     atomicOr(&flags[1], 2);
+    atomicOr(&sysfault[ab.x], GR_FAULT_DOMAIN);
+    if (ab.y >= 0) atomicOr(&sysfault[ab.y], GR_FAULT_DOMAIN);
     ncells = 0;
   }
   G.ncells = (int)ncells;
@@ -1530,9 +1557,9 @@ void launch_minmax_reset(hipStream_t s, const int *list, int nlist, long long *b
 }
 void launch_key_base(hipStream_t s, const int *list, int nlist, int D, int pad, const int *n,
                      const StepSys *st, long long *bbmin, long long *bbmax, int rs, int *kmin,
-                     int *kmax, unsigned *kbase, unsigned *krange, int *flags) {
+                     int *kmax, unsigned *kbase, unsigned *krange, int *flags, int *sysfault) {
   k_key_base<<<1, 64, 0, s>>>(list, nlist, D, pad, n, st, bbmin, bbmax, rs, kmin, kmax, kbase,
-                              krange, flags);
+                              krange, flags, sysfault);
 }
 void launch_key_write(hipStream_t s, XP xs, const SysDesc *sd, const int *n, const StepSys *st,
                       const Blk *blk, int nblk, const int *bbase, int D, const int *kmin,
@@ -1650,10 +1677,11 @@ template void launch_ghost_copy<double>(hipStream_t, const Pool<double> &, const
 void launch_grid_params(hipStream_t s, const int *inter_list, int n, const int2 *inter_sys,
                         const double *cellsz, const long long *bbmin, const long long *bbmax,
                         const int *n_dev, Grid *grids, const int2 *inter_carr,
-                        const CellArr *carr, int D, int *flags) {
+                        const CellArr *carr, int D, int *flags, int *sysfault) {
   if (n > 0)
     k_grid_params<<<nb_of(n, 64), 64, 0, s>>>(inter_list, n, inter_sys, cellsz, bbmin, bbmax,
-                                              n_dev, grids, inter_carr, carr, D, flags);
+                                              n_dev, grids, inter_carr, carr, D, flags,
+                                              sysfault);
 }
 void launch_cells_build(hipStream_t s, XP x, int *cells, const CellArr *carr,
                         const Grid *grids, const double *cellsz, const SysDesc *sd,

@@ -21,6 +21,7 @@ struct SweepArgs {
   const int *lcnt = nullptr;
   const int *lidx = nullptr;
   bool fluid_only = false;  // FORCE: every source's functor is FluidPfn / FluidSolidPfn
+  int block = GR_BLOCK;     // targets per workgroup (the block table's step)
 };
 
 struct NlistArgs {
@@ -37,6 +38,9 @@ struct NlistArgs {
   int *lidx;
   const long long *bbmin = nullptr;  // per-system position bounds of this step
   const long long *bbmax = nullptr;  // (ordered int64, see ord_of)
+  int win = 128;  // candidates staged in LDS per window (64, 128 or 256)
+  int span = 2;   // lanes whose ranges start within span·(widest range) group
+  int cand = 0;   // 1: candidates in the lanes (k_nlist_cand), 0: one target per lane
 };
 
 // Position bounds per system and dimension are kept in GR_BB_REP replicas
@@ -51,7 +55,7 @@ void launch_minmax_reset(hipStream_t s, const int *list, int nlist, long long *b
                          long long *bbmax, int rs);
 void launch_key_base(hipStream_t s, const int *list, int nlist, int D, int pad, const int *n,
                      const StepSys *st, long long *bbmin, long long *bbmax, int rs, int *kmin,
-                     int *kmax, unsigned *kbase, unsigned *krange, int *flags);
+                     int *kmax, unsigned *kbase, unsigned *krange, int *flags, int *sysfault);
 void launch_key_write(hipStream_t s, XP xs, const SysDesc *sd, const int *n,
                       const StepSys *st, const Blk *blk, int nblk, const int *bbase, int D,
                       const int *kmin, const unsigned *krange, const unsigned *kbase,
@@ -83,7 +87,7 @@ void launch_ghost_copy(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const
 void launch_grid_params(hipStream_t s, const int *inter_list, int n, const int2 *inter_sys,
                         const double *cellsz, const long long *bbmin, const long long *bbmax,
                         const int *n_dev, Grid *grids, const int2 *inter_carr,
-                        const CellArr *carr, int D, int *flags);
+                        const CellArr *carr, int D, int *flags, int *sysfault);
 void launch_cells_build(hipStream_t s, XP x, int *cells, const CellArr *carr,
                         const Grid *grids, const double *cellsz, const SysDesc *sd,
                         const int *n_dev, const Blk *cblk, int ncblk, const Blk *mblk,

@@ -15,6 +15,19 @@ library calls).  A context loop:
      (gr_integrator_release, so later jobs reuse their pool segments) and
      returns their footprint to the budget.
 
+A failed job does not disturb the others:
+  * a job whose footprint can never fit the budget is rejected at arrival;
+  * a job whose scene cannot be built, registered or uploaded fails alone;
+  * a job whose simulation faults on the device (positions turning NaN or
+    infinite, or particles leaving the cell range) is isolated by the
+    library inside the shared launches (GR_ERR_FAULT, gr_integrator_status):
+    the other jobs of the batch complete the quantum exactly as they would
+    alone, and the faulted job is retired as failed;
+  * only an error of the context itself (a HIP failure) ends that context's
+    loop, failing the jobs it holds; the other context goes on.
+The reference's only recovery is a restart from a snapshot
+(src/Driver.jl:79-96).
+
 `run_plain` serves the same jobs one at a time, each alone on the GPU, in
 arrival order: the "plain single-sim kernel run serially" baseline.
 
@@ -27,11 +40,13 @@ from __future__ import annotations
 import math
 import threading
 import time
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 from typing import Callable, Optional
 
 import numpy as np
 
+from . import _abi as A
 from .integration import Session
 
 
@@ -100,7 +115,7 @@ class Server:
     """Continuous batching over one context per spatial dimension."""
 
     def __init__(self, precision="f32", hbm_budget=200e9, quantum=50, graph=False,
-                 keep_results=False, session_factory=None):
+                 keep_results=False, session_factory=None, builders=2):
         self.precision = precision
         self._make_session = session_factory or (lambda D: Session(D, precision))
         self.budget = _Budget(hbm_budget)
@@ -109,6 +124,12 @@ class Server:
         self.keep_results = keep_results
         self.sessions = {}
         self.errors = []
+        # per context: seconds in admission (register, upload; the scene is
+        # built on a builder thread as soon as the job arrives), in the shared
+        # steps, in retirement, and the number of shared calls
+        self.stats = {}
+        self.builders = max(1, int(builders))
+        self._pool = None
 
     def _session(self, D):
         if D not in self.sessions:
@@ -117,19 +138,76 @@ class Server:
             self.sessions[D] = s
         return self.sessions[D]
 
+    def _fail(self, job, t0, why, taken=True):
+        job.meta["failed"] = why
+        job.meta.pop("cfg", None)
+        job.finished = time.perf_counter() - t0
+        if taken:
+            self.budget.give(job.bytes)
+
+    def _retire(self, sc, job, cfg, t0):
+        lf = cfg["integrator"]
+        if self.keep_results:
+            sc.download_all(lf)
+            job.result = {f: getattr(cfg["fluid"], f).copy() for f in cfg["fluid"].field_names()}
+        sc.release(lf)
+        job.finished = time.perf_counter() - t0
+        self.budget.give(job.bytes)
+
+    def _admit(self, sc, be, job, t0):
+        """Build (if not yet), register and upload one job; None on failure
+        (the job is marked failed and its budget returned)."""
+        cfg = job.meta.pop("cfg", None)
+        lf = None
+        try:
+            if cfg is None:
+                cfg = job.build()
+            lf = cfg["integrator"]
+            h = sc.integrator(lf)
+            sc.upload_all(lf)
+            be.integrator_set_dt(h, be.integrator_dt(h, cfg["cfl"]))
+        except (ValueError, AssertionError, KeyError, TypeError) as e:
+            if lf is not None:
+                try:
+                    sc.release(lf)
+                except Exception:
+                    pass
+            self._fail(job, t0, f"admission: {e}")
+            return None
+        job.admitted = time.perf_counter() - t0
+        return cfg, h
+
     def _loop(self, D, jobs, t0):
+        queue = sorted(jobs, key=lambda j: (j.arrival, j.jid))
+        running = []  # (job, cfg, handle)
+        st = self.stats.setdefault(D, {"admit_s": 0.0, "step_s": 0.0, "retire_s": 0.0,
+                                       "calls": 0, "max_running": 0})
         try:
             s = self._session(D)
             be, sc = s.backend, s.scene
-            queue = sorted(jobs, key=lambda j: (j.arrival, j.jid))
-            running = []  # (job, cfg, handle)
             while queue or running:
                 now = time.perf_counter() - t0
+                # the scene of every arrived job is built on a builder thread
+                for job in queue:
+                    if job.arrival > now:
+                        break
+                    if not job.bytes and "fut" not in job.meta:
+                        job.meta["fut"] = self._pool.submit(job.build)
                 # admit arrived jobs in order while they fit
                 while queue and queue[0].arrival <= now:
                     job = queue[0]
                     if not job.bytes:
-                        cfg = job.build()
+                        fut = job.meta["fut"]
+                        if not fut.done() and running:
+                            break  # step the running jobs meanwhile
+                        try:
+                            cfg = fut.result()
+                        except (ValueError, AssertionError, KeyError, TypeError) as e:
+                            queue.pop(0)
+                            job.meta.pop("fut", None)
+                            self._fail(job, t0, f"build: {e}", taken=False)
+                            continue
+                        job.meta.pop("fut", None)
                         job.meta["cfg"] = cfg
                         job.n_particles = cfg["n_particles"]
                         job.bytes = footprint_bytes(cfg, self.precision)
@@ -143,13 +221,11 @@ class Server:
                             time.sleep(0.001)  # the other context will free memory
                         break
                     queue.pop(0)
-                    cfg = job.meta.pop("cfg", None) or job.build()
-                    lf = cfg["integrator"]
-                    h = sc.integrator(lf)
-                    sc.upload_all(lf)
-                    be.integrator_set_dt(h, be.integrator_dt(h, cfg["cfl"]))
-                    job.admitted = time.perf_counter() - t0
-                    running.append((job, cfg, h))
+                    ta = time.perf_counter()
+                    got = self._admit(sc, be, job, t0)
+                    st["admit_s"] += time.perf_counter() - ta
+                    if got is not None:
+                        running.append((job,) + got)
                 if not running:
                     if queue:
                         wait = queue[0].arrival - (time.perf_counter() - t0)
@@ -157,24 +233,41 @@ class Server:
                             time.sleep(min(wait, 0.05))
                     continue
                 q = min(self.quantum, min(j.steps - j.done_steps for j, _, _ in running))
-                be.step_many([h for _, _, h in running], q)
+                faulted = set()
+                ts = time.perf_counter()
+                st["max_running"] = max(st["max_running"], len(running))
+                try:
+                    be.step_many([h for _, _, h in running], q)
+                except A.FaultError:
+                    # the library isolated the faulted simulations: the
+                    # others advanced all q steps
+                    faulted = {h for _, _, h in running if be.integrator_status(h)}
+                st["step_s"] += time.perf_counter() - ts
+                st["calls"] += 1
+                tr = time.perf_counter()
                 keep = []
                 for job, cfg, h in running:
+                    if h in faulted:
+                        bits = be.integrator_status(h)
+                        why = ("non-finite positions" if bits & A.GR_FAULT_NONFINITE
+                               else "particles beyond the cell range")
+                        sc.release(cfg["integrator"])
+                        self._fail(job, t0, f"device fault after step <= {job.done_steps + q}: "
+                                            f"{why}")
+                        continue
                     job.done_steps += q
                     if job.done_steps < job.steps:
                         keep.append((job, cfg, h))
                         continue
-                    lf = cfg["integrator"]
-                    if self.keep_results:
-                        sc.download_all(lf)
-                        job.result = {f: getattr(cfg["fluid"], f).copy()
-                                      for f in cfg["fluid"].field_names()}
-                    sc.release(lf)
-                    job.finished = time.perf_counter() - t0
-                    self.budget.give(job.bytes)
+                    self._retire(sc, job, cfg, t0)
                 running = keep
-        except Exception as e:  # surfaced by run()
+                st["retire_s"] += time.perf_counter() - tr
+        except Exception as e:  # an error of the context itself
             self.errors.append(e)
+            for job, _, _ in running:
+                self._fail(job, t0, f"context error: {e}")
+            for job in queue:
+                self._fail(job, t0, f"context error: {e}", taken=False)
 
     def run(self, jobs):
         """Serve `jobs` (arrival times in seconds from now); returns a report."""
@@ -184,34 +277,48 @@ class Server:
             by_dim.setdefault(j.ndims, []).append(j)
         for D in by_dim:  # create contexts before the clock matters
             self._session(D)
+        self._pool = ThreadPoolExecutor(max_workers=self.builders)
         t0 = time.perf_counter()
         threads = [threading.Thread(target=self._loop, args=(D, js, t0)) for D, js in by_dim.items()]
         for t in threads:
             t.start()
         for t in threads:
             t.join()
-        if self.errors:
-            raise self.errors[0]
-        return report(jobs, time.perf_counter() - t0)
+        self._pool.shutdown(wait=True)
+        rep = report(jobs, time.perf_counter() - t0)
+        rep["errors"] = [str(e) for e in self.errors]
+        rep["contexts"] = {f"{D}d": v for D, v in self.stats.items()}
+        return rep
 
 
 def run_plain(jobs, precision="f32", keep_results=False, session_factory=None, graph=False):
     """Each job alone on the GPU, one after another in arrival order (a job
     starts when it has arrived and the previous one has finished); `graph`
-    replays each job's step as a captured graph, as the server can."""
+    replays each job's step as a captured graph, as the server can.  The
+    next job's scene is built on a helper thread while the current job runs
+    (as soon as it has arrived), so the baseline loses no GPU time to scene
+    building that the server would hide."""
     make = session_factory or (lambda D: Session(D, precision))
     sessions = {}
     t0 = time.perf_counter()
     for D in sorted({j.ndims for j in jobs}):
         sessions[D] = make(D)
         sessions[D].backend.set_graph_mode(bool(graph))
-    t0 = time.perf_counter()
-    for job in sorted(jobs, key=lambda j: (j.arrival, j.jid)):
-        wait = job.arrival - (time.perf_counter() - t0)
+    order = sorted(jobs, key=lambda j: (j.arrival, j.jid))
+    pool = ThreadPoolExecutor(max_workers=1)
+
+    def build_after_arrival(job, t_start):
+        wait = job.arrival - (time.perf_counter() - t_start)
         if wait > 0:
             time.sleep(wait)
+        return job.build()
+
+    t0 = time.perf_counter()
+    fut = pool.submit(build_after_arrival, order[0], t0) if order else None
+    for k, job in enumerate(order):
+        cfg = fut.result()
+        fut = pool.submit(build_after_arrival, order[k + 1], t0) if k + 1 < len(order) else None
         s = sessions[job.ndims]
-        cfg = job.build()
         job.n_particles = cfg["n_particles"]
         lf = cfg["integrator"]
         h = s.scene.integrator(lf)
@@ -225,14 +332,17 @@ def run_plain(jobs, precision="f32", keep_results=False, session_factory=None, g
             job.result = {f: getattr(cfg["fluid"], f).copy() for f in cfg["fluid"].field_names()}
         s.scene.release(lf)
         job.finished = time.perf_counter() - t0
+    pool.shutdown(wait=True)
     return report(jobs, time.perf_counter() - t0)
 
 
 def report(jobs, makespan):
-    done = [j for j in jobs if not j.meta.get("rejected")]
+    done = [j for j in jobs if not j.meta.get("rejected") and not j.meta.get("failed")]
+    failed = [j for j in jobs if j.meta.get("failed")]
     lat = np.array([j.latency for j in done]) if done else np.array([math.nan])
     psteps = sum(j.n_particles * j.steps for j in done)
-    return {"jobs": len(done), "rejected": len(jobs) - len(done), "makespan_s": makespan,
+    return {"jobs": len(done), "rejected": sum(1 for j in jobs if j.meta.get("rejected")),
+            "failed": len(failed), "makespan_s": makespan,
             "jobs_per_h": len(done) / makespan * 3600.0,
             "latency_p50_s": float(np.percentile(lat, 50)),
             "latency_p99_s": float(np.percentile(lat, 99)),

@@ -37,7 +37,17 @@ enum {
   GR_ERR_ASSERTION = 2,  /* reference: AssertionError (@assert)          */
   GR_ERR_HIP = 3,        /* HIP runtime failure                           */
   GR_ERR_CAPACITY = 4,   /* a grid/key range outgrew its device capacity  */
-  GR_ERR_STATE = 5       /* call made in the wrong state                  */
+  GR_ERR_STATE = 5,      /* call made in the wrong state                  */
+  GR_ERR_FAULT = 6       /* gr_step / gr_step_many: some integrators
+                            faulted (gr_integrator_status); the others
+                            advanced all their steps normally             */
+};
+
+/* ---- per-integrator fault bits (gr_integrator_status) ------------------ */
+enum {
+  GR_FAULT_NONFINITE = 1,  /* a position became NaN or infinite            */
+  GR_FAULT_DOMAIN = 2      /* particles spread beyond the cell-key range or
+                              the planned cell-grid capacity               */
 };
 
 /* ---- arithmetic precision of the pair sums (gr_ctx_create) ------------ */
@@ -271,11 +281,47 @@ int gr_integrator_set_dt(gr_ctx *ctx, int integ, double dt);
  * print/save cadence stays with the caller). */
 int gr_step(gr_ctx *ctx, int integ, int nsteps, double dt);
 /* Advance every listed integrator `nsteps` steps with its own dt, all of
- * them in the same kernel launches (packed pool). */
+ * them in the same kernel launches (packed pool).
+ * Fault isolation (no counterpart in the reference, whose only recovery is a
+ * restart from a snapshot, src/Driver.jl:79-96): a simulation whose
+ * positions become non-finite, or spread beyond its cell-key range or its
+ * planned grid capacity, is faulted: from that step on its particles keep
+ * their order and find no partners, and every other integrator of the call
+ * advances exactly as it would alone.  The call then completes all nsteps
+ * and returns GR_ERR_FAULT naming the faulted integrators; their fault bits
+ * stay readable through gr_integrator_status until they are released.  The
+ * same holds for gr_step. */
 int gr_step_many(gr_ctx *ctx, int ninteg, const int *integs, int nsteps);
+/* Fault bits (GR_FAULT_*) an integrator has collected since it was created;
+ * 0 for a healthy one. */
+int gr_integrator_status(gr_ctx *ctx, int integ, int *fault_out);
 /* Capture one step of the listed integrators in a hipGraph and replay it
  * for subsequent gr_step_many calls (0 = off). */
 int gr_set_graph_mode(gr_ctx *ctx, int on);
+
+/* ---- launch-shape tuning (no counterpart in the reference: the knobs of
+ * the neighbour search and functor gather that replace the hot loop of
+ * src/Interaction.jl:477-541; grasph.jl_amd/autotune.py picks them from
+ * timed trials).  No value changes any result: the pair lists, summation
+ * order and every output are bit-identical for all of them. */
+enum {
+  GR_TUNE_NL_WIN = 0,       /* candidates staged in LDS per window of the
+                               neighbour search: 64, 128 (default), 256     */
+  GR_TUNE_NL_SPAN = 1,      /* lanes whose candidate ranges start within
+                               span × the widest range share one staging
+                               pass: 1..16 (default 2)                      */
+  GR_TUNE_SWEEP_BLOCK = 2,  /* target particles per workgroup of the
+                               functor gather: 64, 128, 256 (default)       */
+  GR_TUNE_MAXNB = 3,        /* list entries kept per target before it walks
+                               its cells directly: 0 = default (48 in 3D,
+                               24 in 2D) or 4..1024                         */
+  GR_TUNE_NL_KERNEL = 4     /* neighbour-search kernel: 0 one target per
+                               lane (default), 1 the candidates in the
+                               lanes, tested against each target of a cell
+                               at once                                      */
+};
+int gr_tune_set(gr_ctx *ctx, int param, int value);
+int gr_tune_get(gr_ctx *ctx, int param, int *value);
 
 /* ---- fine-grained drop-ins (one reference call each) --------------------- */
 /* sort_particles!(ps, cutoff, ...) (src/Sorting.jl:157). */

@@ -705,6 +705,14 @@ inline int64_t jl_range_len(int64_t a, int64_t s, int64_t b) {
 
 inline int64_t jl_floor_int(double v) { return (int64_t)std::floor(v); }
 
+// floor(Int, x / cutoff) raises InexactError for a non-finite position
+// (src/Sorting.jl:122-124); the sort is refused before any key is formed.
+bool nonfinite_x(const std::vector<double> &x, int64_t n, int D) {
+  for (int64_t i = 0; i < n * D; ++i)
+    if (!std::isfinite(x[i])) return true;
+  return false;
+}
+
 // ---------------------------------------------------------------------------
 // Sorting (src/Sorting.jl)
 // ---------------------------------------------------------------------------
@@ -1560,6 +1568,9 @@ int xsph_correction(orc_scene *sc, Integ &ig) {
 // Scalars use the fused axpy (@fastmath, src/Utils.jl:19-29), vectors do not.
 int step_rk4(orc_scene *sc, Integ &ig, double dt) {
   const double cutoff = 2.0 * ig.h;
+  for (int id : ig.sys)
+    if (nonfinite_x(sc->sys[id].x, sc->sys[id].n, sc->sys[id].D))
+      return fail(sc, GR_ERR_ARGUMENT, "sort_particles!: non-finite position (InexactError)");
   for (int id : ig.sys) sort_system(sc, sc->sys[id], cutoff);
   const size_t ns = ig.sys.size();
   std::vector<std::vector<double>> v0(ns), r0(ns), av(ns), ar(ns);
@@ -1625,6 +1636,9 @@ int step_rk4(orc_scene *sc, Integ &ig, double dt) {
 int step(orc_scene *sc, Integ &ig, double dt) {
   if (ig.scheme == GR_SCHEME_RK4) return step_rk4(sc, ig, dt);
   const double cutoff = 2.0 * ig.h;  // sort_cutoff :612
+  for (int id : ig.sys)
+    if (nonfinite_x(sc->sys[id].x, sc->sys[id].n, sc->sys[id].D))
+      return fail(sc, GR_ERR_ARGUMENT, "sort_particles!: non-finite position (InexactError)");
   for (int id : ig.sys) sort_system(sc, sc->sys[id], cutoff);
   std::vector<std::vector<double>> v0(ig.sys.size()), r0(ig.sys.size());
   for (size_t k = 0; k < ig.sys.size(); ++k) {
@@ -2133,6 +2147,8 @@ int orc_step(orc_scene *s, int integ, int nsteps, double dt) {
 }
 
 int orc_sort(orc_scene *s, int sys, double cutoff) {
+  if (nonfinite_x(s->sys[sys].x, s->sys[sys].n, s->sys[sys].D))
+    return fail(s, GR_ERR_ARGUMENT, "sort_particles!: non-finite position (InexactError)");
   sort_system(s, s->sys[sys], cutoff);
   return GR_OK;
 }

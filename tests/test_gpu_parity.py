@@ -505,6 +505,47 @@ def test_step_many_equals_alone():
             assert np.array_equal(getattr(cfg["fluid"], f), getattr(ref, f)), f
 
 
+@pytest.mark.parametrize("prec,tol", [("f64", 1e-10), ("f32", 2e-4)])
+def test_ensemble_members_match_the_oracle_alone(prec, tol):
+    """Eight granular columns of the ensemble workload (aspect, width, φ and ρ
+    drawn per simulation, dx = 5 mm) advanced together in shared launches
+    (gr_step_many, graph replay); four members with different φ and ρ each
+    equal the oracle running that column alone (src/TimeIntegration.jl:
+    645-693): particle order bit for bit, state to the step tolerance."""
+    nsteps = 25
+    sims = M.configs.ensemble_columns(n_sims=8, dx=0.005, seed=3)
+    s = M.Session(2, prec)
+    s.backend.set_graph_mode(True)
+    hs = []
+    for cfg in sims:
+        h = s.scene.integrator(cfg["integrator"])
+        s.scene.upload_all(cfg["integrator"])
+        s.backend.integrator_set_dt(h, s.backend.integrator_dt(h, cfg["cfl"]))
+        hs.append(h)
+    s.backend.step_many(hs, 10)
+    s.backend.step_many(hs, nsteps - 10)
+    phis, rhos = [], []
+    for k in (0, 2, 5, 7):
+        cfg = sims[k]
+        s.scene.download_all(cfg["integrator"])
+        dev = cfg["fluid"]
+        ref = M.configs.ensemble_columns(n_sims=8, dx=0.005, seed=3)[k]
+        sc = oracle_scene(2)
+        h = sc.integrator(ref["integrator"])
+        sc.upload_all(ref["integrator"])
+        sc.be.step(h, nsteps, sc.be.integrator_dt(h, ref["cfl"]))
+        sc.download_all(ref["integrator"])
+        o = ref["fluid"]
+        upd = o.state_updater[1]
+        phis.append(upd.phi)
+        rhos.append(float(o.rho.max()))
+        assert np.array_equal(dev.id, o.id), k
+        assert np.array_equal(dev.x, o.x) or rel(dev.x, o.x) <= tol, k
+        for f in ("x", "v", "rho", "stress", "strain_p"):
+            assert rel(getattr(dev, f), getattr(o, f)) <= tol, (k, f, rel(getattr(dev, f), getattr(o, f)))
+    assert len(set(np.round(phis, 6))) == 4 and len(set(np.round(rhos, 3))) == 4
+
+
 def test_graph_mode_equals_eager():
     res = []
     for graph in (False, True):
@@ -710,6 +751,70 @@ def test_device_snapshot_restart_continues_bit_identically(tmp_path, prec):
     run(b, 5)
     for f in ["x"] + SN.sim_field_names(ref["fluid"]):
         assert np.array_equal(getattr(b["fluid"], f), getattr(ref["fluid"], f)), f
+
+
+@pytest.mark.parametrize("writer", ["orc", "dev"])
+def test_snapshot_cross_restart_device_oracle(tmp_path, writer):
+    """One backend runs 7 steps and writes an HDF5 snapshot in the reference
+    layout (src/Particles.jl:711-735, src/TimeIntegration.jl:520-549); the
+    other restarts from that file (src/Particles.jl:747-787,
+    src/Driver.jl:79-96) and runs 5 more steps, while the writer continues
+    its own run.  The two continuations agree to the f64 step tolerance, the
+    particle order bit for bit; the file's state equals what the reader's own
+    run reaches at step 7 to the same tolerance."""
+    SN = M.snapshot
+    reader = "dev" if writer == "orc" else "orc"
+
+    def cfg():
+        return M.configs.ep_column_2d(dx=0.01, width=0.1, height=0.06)
+
+    def session(make):
+        return M.Session(2, "f64").scene if make == "dev" else oracle_scene(2)
+
+    def start(make, c):
+        lf = c["integrator"]
+        sc = session(make)
+        h = sc.integrator(lf)
+        sc.upload_all(lf)
+        return sc, h, sc.be.integrator_dt(h, c["cfl"])
+
+    def state(c):
+        ps = c["fluid"]
+        return {f: getattr(ps, f).copy() for f in ["x", "id"] + SN.sim_field_names(ps)}
+
+    # writer: 7 steps, snapshot, 5 more steps
+    w = cfg()
+    sc, h, dt = start(writer, w)
+    sc.be.step(h, 7, dt)
+    sc.download_all(w["integrator"])
+    path = str(tmp_path / f"{writer}_07.h5")
+    SN.save_snapshot(path, w["integrator"].systems, w["integrator"].ghosts, 7, dt)
+    at7_writer = state(w)
+    sc.be.step(h, 5, dt)
+    sc.download_all(w["integrator"])
+    cont_writer = state(w)
+    # the reader's own run to step 7 agrees with the file
+    r7 = cfg()
+    sc7, h7, dt7 = start(reader, r7)
+    assert dt7 == dt
+    sc7.be.step(h7, 7, dt7)
+    sc7.download_all(r7["integrator"])
+    own7 = state(r7)
+    assert np.array_equal(own7["id"], at7_writer["id"])
+    for f, ref in at7_writer.items():
+        if f != "id":
+            assert rel(own7[f], ref) <= 1e-10, (f, rel(own7[f], ref))
+    # reader: restart from the writer's file, 5 steps
+    r = cfg()
+    assert SN.load_restart(path, r["integrator"].systems) == 7
+    scr, hr, dtr = start(reader, r)
+    scr.be.step(hr, 5, dtr)
+    scr.download_all(r["integrator"])
+    cont_reader = state(r)
+    assert np.array_equal(cont_reader["id"], cont_writer["id"])
+    for f, ref in cont_writer.items():
+        if f != "id":
+            assert rel(cont_reader[f], ref) <= 1e-10, (f, rel(cont_reader[f], ref))
 
 
 @pytest.mark.parametrize("prec,tol", [("f64", 1e-12), ("f32", 2e-6)])

@@ -1,7 +1,9 @@
 """Host logic of the continuous-batching server (grasph.jl_amd/serving.py)
 on the CPU: the oracle stands in for the device through a thin adapter that
 adds the batching calls the oracle lacks (each integrator stepped with its
-own dt, release as a no-op)."""
+own dt, release as a no-op, and the device's fault contract: a simulation
+the oracle refuses to step is reported through GR_ERR_FAULT and
+integrator_status while the others complete the call)."""
 import types
 
 import numpy as np
@@ -16,6 +18,7 @@ class OracleDevice:
     def __init__(self, D):
         self.be = oracle_backend(D)
         self.dt = {}
+        self.status = {}
         self.calls = []
 
     def __getattr__(self, name):
@@ -29,11 +32,25 @@ class OracleDevice:
 
     def step_many(self, hs, n):
         self.calls.append((tuple(hs), n))
+        bad = []
         for h in hs:
-            self.be.step(h, n, self.dt[h])
+            if self.status.get(h):
+                bad.append(h)
+                continue
+            try:
+                self.be.step(h, n, self.dt[h])
+            except ValueError:  # the oracle refuses a non-finite position
+                self.status[h] = M._abi.GR_FAULT_NONFINITE
+                bad.append(h)
+        if bad:
+            raise M._abi.FaultError(f"integrator(s) {bad} faulted")
+
+    def integrator_status(self, h):
+        return self.status.get(h, 0)
 
     def integrator_release(self, h):
         self.dt.pop(h)
+        self.status.pop(h, None)
 
 
 def session(D):
@@ -100,3 +117,50 @@ def test_column_jobs_stream_statistics():
     assert 0.13 < frac3 < 0.27
     st = np.array([j.steps for j in jobs])
     assert st.min() >= 500 and st.max() <= 20000
+
+
+def _nan_rho_column(k):
+    cfg = M.configs.ep_column_2d(dx=0.01, width=0.07, height=0.05, name=f"nan{k}")
+    cfg["fluid"].rho[7] = float("nan")  # NaN density: positions turn NaN a step later
+    return cfg
+
+
+def _bad_column(k):
+    raise ValueError(f"job {k}: width must be positive")
+
+
+def test_failed_jobs_do_not_disturb_the_others():
+    """A job whose simulation turns NaN mid-run, a job whose scene cannot be
+    built and a job that can never fit the budget fail or are rejected
+    alone; every other job finishes with the state it reaches alone."""
+    S = M.serving
+    served = small_jobs()
+    healthy = {j.jid for j in served}
+    one = max(S.footprint_bytes(j.build()) for j in served)
+    served.append(S.Job(10, 2, lambda: _nan_rho_column(10), 9, arrival=0.0))
+    served.append(S.Job(11, 2, lambda: _bad_column(11), 9, arrival=0.0))
+    big = S.Job(12, 2, lambda: M.configs.ep_column_2d(dx=0.01, width=0.06, height=0.05), 5,
+                arrival=0.0)
+    big.bytes = int(one * 100)  # declared footprint beyond the budget
+    served.append(big)
+    srv = S.Server(quantum=4, keep_results=True, hbm_budget=one * 20, session_factory=session)
+    rep = srv.run(served)
+    assert rep["errors"] == []
+    assert rep["jobs"] == len(healthy) and rep["failed"] == 2 and rep["rejected"] == 1
+    by = {j.jid: j for j in served}
+    assert "non-finite" in by[10].meta["failed"]
+    assert by[10].done_steps < by[10].steps
+    assert "build" in by[11].meta["failed"]
+    assert by[12].meta.get("rejected")
+    # the NaN job ran batched with healthy jobs before it failed
+    calls = srv.sessions[2].backend.calls
+    assert any(len(hs) > 1 for hs, _ in calls)
+    alone = small_jobs()
+    S.run_plain(alone, keep_results=True, session_factory=session)
+    for b in alone:
+        a = by[b.jid]
+        assert a.done_steps == a.steps
+        for f, v in b.result.items():
+            assert np.array_equal(a.result[f], v), (a.jid, f)
+    # the budget is whole again
+    assert abs(srv.budget.used) < 1e-6
