@@ -111,6 +111,7 @@ _GPU_SIGS = {
     "bench_copy": [_i64, _i],
     "integrator_status": [_i, _ip],
     "tune_set": [_i, _i],
+    "sort_static": [_i, _d],
     "tune_get": [_i, _ip],
 }
 
@@ -330,6 +331,9 @@ class CBackend:
 
     def integrator_set_dt(self, integ, dt):
         self.call("integrator_set_dt", integ, dt)
+
+    def sort_static(self, sys, cutoff):
+        self.call("sort_static", sys, cutoff)
 
     def tune_set(self, param, value):
         self.call("tune_set", param, int(value))

@@ -13,6 +13,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -22,6 +23,61 @@
 #include "gr_launch.h"
 
 using namespace gr;
+
+// ---------------------------------------------------------------------------
+// Scratch of gr_sort_static: a file-scope device array, one copy per GPU,
+// held until the process ends.  Host code never passes the array's name to a
+// kernel or a copy (on the host it only stands for the symbol): it uses the
+// device address hipGetSymbolAddress returns, taken once per device in
+// gr_ctx_create with that device current.
+//   keys, vals, keys2, vals2  4 × GR_STATIC_SORT_MAXN × 4 B
+//   radix histogram           radix_hist_entries(GR_STATIC_SORT_MAXN) × 4 B
+//   per-system key bases      GR_STATIC_MAXSYS × 4 B
+//   the sorted system's list and block table, every real system's blocks
+// Every call on a GPU shares that GPU's array: calls that overlap on one
+// GPU (two streams, two threads) race.
+// ---------------------------------------------------------------------------
+#define GR_STATIC_SORT_MAXN (1 << 21)  // particles of the sorted system
+#define GR_STATIC_MAXSYS 4096          // systems of the context
+#define GR_STATIC_MAXBLK (1 << 16)     // real-system blocks of the pool (16.7M particles)
+namespace {
+constexpr size_t kScrKeys = 0;
+constexpr size_t kScrVals = kScrKeys + (size_t)GR_STATIC_SORT_MAXN * 4;
+constexpr size_t kScrKeys2 = kScrVals + (size_t)GR_STATIC_SORT_MAXN * 4;
+constexpr size_t kScrVals2 = kScrKeys2 + (size_t)GR_STATIC_SORT_MAXN * 4;
+constexpr size_t kScrHist = kScrVals2 + (size_t)GR_STATIC_SORT_MAXN * 4;
+// 256 entries per tile of 2048 keys plus 256 (radix_hist_entries)
+constexpr size_t kScrHistEntries = 256 * ((size_t)GR_STATIC_SORT_MAXN / 2048 + 1) + 256;
+constexpr size_t kScrBase = kScrHist + kScrHistEntries * 4;
+constexpr size_t kScrList = kScrBase + (size_t)GR_STATIC_MAXSYS * 4;
+constexpr size_t kScrBlkSys = kScrList + 256;
+constexpr size_t kScrBlkAll = kScrBlkSys + ((size_t)GR_STATIC_SORT_MAXN / GR_BLOCK + 1) * 8;
+constexpr size_t kScrBytes = kScrBlkAll + (size_t)GR_STATIC_MAXBLK * 8;
+}  // namespace
+static __device__ __align__(256) unsigned char scratch[kScrBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch` per GPU
+
+// hipGetSymbolAddress once per device, with that device current.
+hipError_t scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return hipSuccess;
+  void *ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress((void **)&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch_ptr[device] = ptr;
+  return hipSuccess;
+}
+void *scratch_of(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+}  // namespace
 
 namespace {
 
@@ -2044,6 +2100,10 @@ int gr_ctx_create(gr_ctx **out, int device, int ndims, int precision) {
     g_create_err = "hipSetDevice failed";
     return GR_ERR_HIP;
   }
+  if (hipError_t e = scratch_setup(device); e != hipSuccess) {
+    g_create_err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+    return GR_ERR_HIP;
+  }
   gr_ctx *c = new gr_ctx();
   c->device = device;
   c->D = ndims;
@@ -2817,6 +2877,93 @@ int gr_sort(gr_ctx *c, int sys, double cutoff) {
       HIPCHK(hipMemcpyAsync(c->d_flags.p, z, sizeof(z), hipMemcpyHostToDevice, c->stream));
       HIPCHK(hipMemsetAsync(c->d_sysfault.p, 0, c->sys.size() * sizeof(int), c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
+      if (sf & GR_FAULT_NONFINITE)
+        throw GrError(GR_ERR_ARGUMENT, "non-finite position: its cell key floor(x / cutoff) is undefined");
+      throw GrError(GR_ERR_CAPACITY, "cell-key range exceeds 32 bits");
+    }
+  });
+}
+
+int gr_sort_static(gr_ctx *c, int sys, double cutoff) {
+  return guard(c, [&] {
+    validate_sys(c, sys);
+    need(c->sys[sys].kind != GR_SYS_GHOST, GR_ERR_ARGUMENT,
+         "gr_sort_static sorts real systems (ghosts: gr_sort)");
+    need(!c->dirty_layout, GR_ERR_STATE,
+         "systems created since the last upload are not laid out yet: upload them first");
+    unsigned char *scr = (unsigned char *)scratch_of(c->device);
+    need(scr != nullptr, GR_ERR_STATE, "static scratch not set up on this device");
+    const int ns = (int)c->sys.size();
+    const int64_t cap = c->sys[sys].cap;
+    need(cap <= GR_STATIC_SORT_MAXN, GR_ERR_CAPACITY,
+         "gr_sort_static: system of " + std::to_string(cap) + " particles exceeds the static "
+         "scratch limit of " + std::to_string(GR_STATIC_SORT_MAXN) + " (use gr_sort)");
+    need(ns <= GR_STATIC_MAXSYS, GR_ERR_CAPACITY,
+         "gr_sort_static: more than " + std::to_string(GR_STATIC_MAXSYS) + " systems (use gr_sort)");
+    need((size_t)radix_hist_entries((int)cap) <= kScrHistEntries, GR_ERR_CAPACITY,
+         "gr_sort_static: histogram exceeds the static scratch");
+    // tables (host vectors; device memory is only the static scratch)
+    std::vector<Blk> bsys, ball;
+    add_blocks(bsys, sys, cap, GR_BLOCK);
+    for (int q = 0; q < ns; ++q)
+      if (c->sys[q].kind != GR_SYS_GHOST && !c->sys[q].dead) add_blocks(ball, q, c->sys[q].cap, GR_BLOCK);
+    need(ball.size() <= (size_t)GR_STATIC_MAXBLK, GR_ERR_CAPACITY,
+         "gr_sort_static: the pool's real systems exceed the static block table (use gr_sort)");
+    need(c->d_st.p && c->d_st.bytes >= (size_t)ns * sizeof(StepSys) && c->d_flags.p &&
+             c->d_sysfault.p && c->d_sysfault.bytes >= (size_t)ns * sizeof(int),
+         GR_ERR_STATE, "context tables not laid out: upload the system first");
+    std::vector<int> bbase(ns, -1);
+    bbase[sys] = 0;
+    hipStream_t s = c->stream;
+    c->stepsys[sys].sort_cutoff = cutoff;
+    HIPCHK(hipMemcpyAsync(c->d_st.as<StepSys>() + sys, &c->stepsys[sys], sizeof(StepSys),
+                          hipMemcpyHostToDevice, s));
+    int *d_list = (int *)(scr + kScrList);
+    int *d_bbase = (int *)(scr + kScrBase);
+    Blk *d_bsys = (Blk *)(scr + kScrBlkSys);
+    Blk *d_ball = (Blk *)(scr + kScrBlkAll);
+    unsigned *keys = (unsigned *)(scr + kScrKeys), *keys2 = (unsigned *)(scr + kScrKeys2);
+    int *vals = (int *)(scr + kScrVals), *vals2 = (int *)(scr + kScrVals2);
+    unsigned *hist = (unsigned *)(scr + kScrHist);
+    HIPCHK(hipMemcpyAsync(d_list, &sys, sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(d_bbase, bbase.data(), ns * sizeof(int), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(d_bsys, bsys.data(), bsys.size() * sizeof(Blk), hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(d_ball, ball.data(), ball.size() * sizeof(Blk), hipMemcpyHostToDevice, s));
+    // the kernel sequence of phase_sort_real (sort_particles!, src/Sorting.jl:157-195)
+    const XP xs = xp_pool(c, c->cur);
+    launch_minmax_reset(s, d_list, 1, c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>(),
+                        c->bb_rs);
+    launch_key_minmax(s, xs, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), d_bsys, (int)bsys.size(),
+                      c->D, c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>(), c->bb_rs,
+                      d_list, 1, false);
+    launch_key_base(s, d_list, 1, c->D, 0, c->d_n.as<int>(), c->d_st.as<StepSys>(),
+                    c->d_bbmin.as<long long>(), c->d_bbmax.as<long long>(), c->bb_rs,
+                    c->d_kmin.as<int>(), c->d_kmax.as<int>(), c->d_kbase.as<unsigned>(),
+                    c->d_krange.as<unsigned>(), c->d_flags.as<int>(), c->d_sysfault.as<int>());
+    launch_key_write(s, xs, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), c->d_st.as<StepSys>(), d_bsys,
+                     (int)bsys.size(), d_bbase, c->D, c->d_kmin.as<int>(),
+                     c->d_krange.as<unsigned>(), c->d_kbase.as<unsigned>(), keys, vals,
+                     c->d_flags.as<int>());
+    launch_radix_sort(s, keys, vals, keys2, vals2, hist, (int)cap, c->d_flags.as<int>());
+    by_prec(c, [&](auto *tag) {
+      using R = std::remove_pointer_t<decltype(tag)>;
+      Pool<R> A = pool_view<R>(c, c->cur), B = pool_view<R>(c, 1 - c->cur);
+      launch_permute_real<R>(s, A, B, c->d_sd.as<SysDesc>(), d_ball, (int)ball.size(), d_bbase,
+                             vals, c->d_flags.as<int>(), c->vadj_any ? 1 : 0, c->D);
+    });
+    c->cur = 1 - c->cur;
+    // as gr_sort: a non-finite position or a key range of 32 bits or more is
+    // reported (the system keeps its order)
+    int flags[4];
+    HIPCHK(hipMemcpyAsync(flags, c->d_flags.p, sizeof(flags), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (flags[1] & (1 | 8)) {
+      int sf = 0;
+      HIPCHK(hipMemcpyAsync(&sf, c->d_sysfault.as<int>() + sys, sizeof(int), hipMemcpyDeviceToHost, s));
+      int z[4] = {0, 0, 0, 0};
+      HIPCHK(hipMemcpyAsync(c->d_flags.p, z, sizeof(z), hipMemcpyHostToDevice, s));
+      HIPCHK(hipMemsetAsync(c->d_sysfault.p, 0, (size_t)ns * sizeof(int), s));
+      HIPCHK(hipStreamSynchronize(s));
       if (sf & GR_FAULT_NONFINITE)
         throw GrError(GR_ERR_ARGUMENT, "non-finite position: its cell key floor(x / cutoff) is undefined");
       throw GrError(GR_ERR_CAPACITY, "cell-key range exceeds 32 bits");

@@ -326,6 +326,18 @@ int gr_tune_get(gr_ctx *ctx, int param, int *value);
 /* ---- fine-grained drop-ins (one reference call each) --------------------- */
 /* sort_particles!(ps, cutoff, ...) (src/Sorting.jl:157). */
 int gr_sort(gr_ctx *ctx, int sys, double cutoff);
+/* gr_sort for a real system without allocating: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under
+ * stream capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same result as gr_sort (bit for bit).  Its scratch (keys, ranks, radix
+ * histogram, block tables) is a static device array of the library, about
+ * 34 MB per GPU, held until the process ends; limits: the system holds at
+ * most 2,097,152 particles, the context at most 4096 systems and its real
+ * systems at most 65536 blocks of 256 particles (beyond: GR_ERR_CAPACITY).
+ * The system must be laid out (uploaded) already (else GR_ERR_STATE).
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (other threads, other streams) race. */
+int gr_sort_static(gr_ctx *ctx, int sys, double cutoff);
 /* create_grid!(si) (src/Interaction.jl:123). */
 int gr_create_grid(gr_ctx *ctx, int inter);
 /* sweep!(si, stage) (src/Interaction.jl:292): accumulates into the target

@@ -62,7 +62,7 @@ def test_oracle_mirrors_the_device_api():
                    "integrator_release",
                    "step_many", "set_graph_mode", "set_timing", "get_timing",
                    "get_pair_count", "get_pair_stats", "bench_copy", "integrator_status",
-                   "tune_set", "tune_get"}
+                   "tune_set", "tune_get", "sort_static"}
     assert sorted(dev - orc - device_only) == []
 
 
