@@ -32,6 +32,11 @@ HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 
 
+def progress(msg):
+    """One progress line on stderr (the JSON result stays alone on stdout)."""
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,12 +87,15 @@ def ensemble_leg(n_sims, nsteps, serial_steps, precision, tune):
         s.backend.integrator_set_dt(h, s.backend.integrator_dt(h, cfg["cfl"]))
         hs.append(h)
     build_s = time.perf_counter() - t0
+    progress(f"ensemble: {n_sims} columns, {npart} particles built in {build_s:.1f} s")
     rec = (g.autotune.autotune(s, [c["integrator"] for c in sims], steps=2, reps=1)
            if tune else None)
+    progress("ensemble: tuned; batched run")
     s.backend.step_many(hs, 2)  # plan, capture
     t0 = time.perf_counter()
     s.backend.step_many(hs, nsteps)
     batched = time.perf_counter() - t0
+    progress(f"ensemble: batched {nsteps} steps in {batched:.2f} s; columns alone")
     for h in hs:  # plan and capture of each column alone, untimed
         s.backend.step_many([h], 1)
     t0 = time.perf_counter()
@@ -124,8 +132,12 @@ def serve_leg(n_jobs, step_scale, size_scale, precision):
     g = __graft_entry__.package()
     S = g.serving
     mk = lambda: S.column_jobs(n_jobs, step_scale=step_scale, size_scale=size_scale)  # noqa: E731
+    progress(f"serving: {n_jobs} jobs, continuous batching")
     served = S.Server(precision=precision, quantum=50, graph=True, hbm_budget=200e9).run(mk())
-    plain = S.run_plain(mk(), precision=precision, graph=True)
+    progress(f"serving: served in {served['makespan_s']:.1f} s; the same jobs one at a time")
+    plain = S.run_plain(mk(), precision=precision, graph=True,
+                        progress=lambda m: progress("serving (plain): " + m))
+    progress(f"serving: plain in {plain['makespan_s']:.1f} s")
     return {"workload": "served stream of 2D/3D column-collapse jobs (Poisson 20/s, 80% 2D / "
                         "20% 3D, steps log-U[500, 20000], HBM budget 200 GB)",
             "jobs": n_jobs, "step_scale": step_scale, "size_scale": size_scale,
@@ -223,6 +235,7 @@ def main():
         sys.exit(1)
     g = __graft_entry__.package()
     kw = dict(dx=args.dx)
+    progress("3D dam break: building the scene")
     cfg = g.configs.dambreak_3d(**kw)
     lf = cfg["integrator"]
     s = g.Session(3, args.precision)
@@ -237,6 +250,7 @@ def main():
     # restores the state it found)
     be.step_many([h], 1)
     tune = g.autotune.autotune(s, [lf], steps=3, reps=2) if args.autotune else None
+    progress("3D dam break: tuned; timed steps")
     # warm-up (graph capture)
     if args.warmup:
         be.step_many([h], args.warmup)
@@ -272,6 +286,7 @@ def main():
     bytes_sort = sort_bytes(cfg, grids[0] if grids else 0, args.precision)
     sort_gbs = bytes_sort / (phases[0] * 1e-3) / 1e9 if phases[0] > 0 else None
     traffic = load_traffic()
+    progress(f"3D dam break: {el / args.steps * 1e3:.3f} ms/step; CPU baseline")
     cpu = cpu_baseline(g.configs.dambreak_3d, kw, args.cpu_steps, pairs)
     ens = (ensemble_leg(args.ensemble, args.ensemble_steps, args.ensemble_serial_steps,
                         args.precision, bool(args.autotune)) if args.ensemble else None)

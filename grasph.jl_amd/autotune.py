@@ -11,9 +11,7 @@ that matter per scene are (include/grasph_hip.h, GR_TUNE_*):
   * GR_TUNE_NL_SPAN — how far apart the lanes' candidate ranges may start
     and still share one staging pass (1, 2, 4: a cell-column grouping);
   * GR_TUNE_MAXNB — list entries kept per target before it walks its cells
-    directly (memory against overflow walks);
-  * GR_TUNE_NL_KERNEL — the neighbour search with one target per lane, or
-    with the candidates in the lanes tested against each target of a cell.
+    directly (memory against overflow walks).
 
 None of them changes a result (bit-identical lists, sums and states), so
 the tuner only times.  It runs a coordinate descent: for each knob in turn,
@@ -29,16 +27,14 @@ import time
 
 from . import _abi as A
 
-KNOB_NAMES = {A.GR_TUNE_NL_KERNEL: "nl_kernel", A.GR_TUNE_SWEEP_BLOCK: "sweep_block",
-              A.GR_TUNE_NL_WIN: "nl_win", A.GR_TUNE_NL_SPAN: "nl_span",
-              A.GR_TUNE_MAXNB: "maxnb"}
+KNOB_NAMES = {A.GR_TUNE_SWEEP_BLOCK: "sweep_block", A.GR_TUNE_NL_WIN: "nl_win",
+              A.GR_TUNE_NL_SPAN: "nl_span", A.GR_TUNE_MAXNB: "maxnb"}
 
 
 def default_space(ndims):
     """Candidate values per knob, in the order the descent visits them."""
     mx = (32, 48, 64) if ndims == 3 else (16, 24, 32)
-    return [(A.GR_TUNE_NL_KERNEL, (0, 1)),
-            (A.GR_TUNE_SWEEP_BLOCK, (256, 128, 64)),
+    return [(A.GR_TUNE_SWEEP_BLOCK, (256, 128, 64)),
             (A.GR_TUNE_NL_WIN, (128, 64, 256)),
             (A.GR_TUNE_NL_SPAN, (2, 1, 4)),
             (A.GR_TUNE_MAXNB, mx)]

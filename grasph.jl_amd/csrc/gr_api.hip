@@ -368,6 +368,7 @@ struct gr_ctx {
   DBuf d_sd, d_n, d_st, d_bnd, d_kmin, d_kmax, d_bbmin, d_bbmax, d_kbase, d_krange, d_flags;
   DBuf d_sysfault;  // per system: GR_FAULT_* bits raised by the step's kernels
   int bb_rs = 0;  // entries per replica of d_bbmin / d_bbmax (GR_BB_REP replicas)
+  bool bb_valid = false;  // d_bbmin / d_bbmax hold the bounds of a step's key pass
   DBuf d_grids, d_carr, d_cellsz, d_inter_sys, d_inter_carr, cells;
   std::vector<CellArr> carr;
   int64_t cells_cap = 0;
@@ -381,7 +382,7 @@ struct gr_ctx {
   int use_lists = 1;
   int list_maxnb = 0;  // mask entries kept per target (0: 48 in 3D, 24 in 2D)
   // launch shapes chosen by gr_tune_set (results do not depend on them)
-  int tune_win = 128, tune_span = 2, tune_block = GR_BLOCK, tune_nlk = 0;
+  int tune_win = 128, tune_span = 2, tune_block = GR_BLOCK;
   std::map<std::vector<int>, std::unique_ptr<Plan>> plans;
   // cached device tables derived from the layout
   DBuf allreal_blk;
@@ -638,6 +639,7 @@ void commit(gr_ctx *c) {
   c->d_kmax.upload(z, c->stream);
   c->d_bbmin.upload(zl, c->stream);
   c->d_bbmax.upload(zl, c->stream);
+  c->bb_valid = false;
   c->bb_rs = ns * 3;
   std::vector<unsigned> zu(ns * 3, 0);
   c->d_kbase.upload(zu, c->stream);
@@ -1795,6 +1797,90 @@ void phase_xsph(gr_ctx *c, Plan &P) {
   launch_xsph<R>(s, A, sd, st, c->d_n.as<int>(), P.blk_int.as<Blk>(), P.nblk_int, c->D, 2);
 }
 
+// Steps between two checks of the cell arrays' capacity, and the cells kept
+// free on every side of the systems' current extent.  With dt = CFL·h/c a
+// particle at speed v moves CFL·(v/c)·h per step: a chunk moves a subsonic
+// particle (v < c, CFL <= 1) at most 16 h = 8 cells.
+constexpr int kGrowChunk = 16;
+constexpr int kGrowMargin = 8;
+
+// The bounds of the last key pass (replica 0 of d_bbmin / d_bbmax, exact
+// doubles) against each interaction's cell arrays: an interaction whose grid,
+// kGrowMargin cells wider on every side, would not fit gets new arrays sized
+// for its current extent (cell_capacity).  Cell arrays are rebuilt every
+// step, so nothing is copied; plans and captured graphs refer to the tables
+// and are dropped.  Returns whether anything was reallocated.
+bool grow_cells(gr_ctx *c, const Plan &P) {
+  if (P.inters.empty()) return false;
+  const int ns = (int)c->sys.size();
+  std::vector<long long> bmn((size_t)ns * 3), bmx((size_t)ns * 3);
+  HIPCHK(hipMemcpyAsync(bmn.data(), c->d_bbmin.p, bmn.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(bmx.data(), c->d_bbmax.p, bmx.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  std::vector<int> nn(ns);
+  HIPCHK(hipMemcpyAsync(nn.data(), c->d_n.p, ns * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  auto dbl = [](long long i) {  // ordered int64 -> double (ord_of's inverse)
+    const long long b = i < 0 ? (i ^ 0x7FFFFFFFFFFFFFFFLL) : i;
+    double d;
+    memcpy(&d, &b, 8);
+    return d;
+  };
+  bool grew = false;
+  for (int it : P.inters) {
+    InterH &in = c->inter[it];
+    if (in.carr_b < 0) continue;
+    double mn[3] = {0, 0, 0}, mx[3] = {0, 0, 0};
+    bool any = false, finite = true;
+    for (int sid : {in.a, in.b}) {
+      if (sid < 0 || nn[sid] <= 0) continue;
+      for (int d = 0; d < c->D; ++d) {
+        const double lo = dbl(bmn[sid * 3 + d]), hi = dbl(bmx[sid * 3 + d]);
+        finite = finite && std::isfinite(lo) && std::isfinite(hi);
+        mn[d] = any ? std::min(mn[d], lo) : lo;
+        mx[d] = any ? std::max(mx[d], hi) : hi;
+      }
+      any = true;
+    }
+    if (!any || !finite) continue;  // a faulted system is reported, not grown for
+    const double cell = in.cell;
+    double need = 1.0;
+    for (int d = 0; d < c->D; ++d) {
+      const double ming = std::floor((mn[d] - 2.0 * cell) / cell) * cell;
+      need *= std::floor((mx[d] + 2.0 * cell - ming) / cell) + 1.0 + 2.0 * kGrowMargin;
+    }
+    int64_t cap = c->carr[in.carr_b].cap;
+    if (in.carr_a >= 0) cap = std::min<int64_t>(cap, c->carr[in.carr_a].cap);
+    if (need + 1.0 <= (double)cap) continue;
+    double lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+      lo[d] = d < c->D ? mn[d] - kGrowMargin * cell : 0.0;
+      hi[d] = d < c->D ? mx[d] + kGrowMargin * cell : 0.0;
+    }
+    const int64_t ncap = cell_capacity(c, in, lo, hi);
+    if (ncap <= cap) continue;
+    for (int ca : {in.carr_b, in.carr_a}) {
+      if (ca < 0) continue;
+      CellArr &a = c->carr[ca];
+      c->free_cells.put(a.off, a.cap);
+      a.off = c->free_cells.take(ncap);
+      if (a.off < 0) {
+        a.off = c->cells_cap;
+        c->cells_cap += ncap;
+      }
+      a.cap = (int32_t)ncap;
+    }
+    grew = true;
+  }
+  if (!grew) return false;
+  drop_graphs(c);
+  c->plans.clear();
+  if ((size_t)std::max<int64_t>(c->cells_cap, 1) * 4 > c->cells.bytes)
+    c->cells.alloc(std::max<int64_t>(c->cells_cap, 1) * 4);
+  c->d_carr.upload(c->carr, c->stream);
+  std::fill(c->carr_valid.begin(), c->carr_valid.end(), 0);
+  return true;
+}
+
 // Phase events of one step (timing mode): 0 start, 1 sorted, 2 ghosts,
 // 3 grids, 4 half-step, then per stage (updaters end, sweeps end), last
 // full-step end.  Recorded without synchronisation; read after the call.
@@ -1841,7 +1927,6 @@ void run_step(gr_ctx *c, Plan &P, bool cnt, std::vector<hipEvent_t> *ev) {
     a.bbmax = c->d_bbmax.as<long long>();
     a.win = c->tune_win;
     a.span = c->tune_span;
-    a.cand = c->tune_nlk;
     launch_nlist<R>(s, A, a, c->D);
   }
   rec(4);
@@ -1927,9 +2012,9 @@ void step_graph(gr_ctx *c, Plan &P) {
 
 template <typename R>
 void do_steps(gr_ctx *c, const std::vector<int> &integs, int nsteps) {
-  Plan &P = get_plan(c, integs);
+  Plan *P = &get_plan(c, integs);
   // the plan's step parameters (fine-grained calls may have changed them)
-  for (auto &e : P.stepsys) c->stepsys[e.first] = e.second;
+  for (auto &e : P->stepsys) c->stepsys[e.first] = e.second;
   upload_stepsys(c);
   {
     const Blk *b;
@@ -1937,15 +2022,26 @@ void do_steps(gr_ctx *c, const std::vector<int> &integs, int nsteps) {
     all_real(c, b, nb);  // build cached tables before any capture
   }
   if (c->count_pairs) launch_fill_i32(c->stream, c->d_pairs.as<int>(), 0, 4);
-  if (nsteps > 0) mark_grids(c, P);
-  const int nst = (int)P.stages.size();
-  for (int k = 0; k < nsteps; ++k) {
-    if (c->timing)
-      run_step<R>(c, P, c->count_pairs, &step_events(c, k, nst));
-    else if (c->graph_mode && !c->count_pairs && !P.real_list.empty())
-      step_graph<R>(c, P);
-    else
-      run_step<R>(c, P, c->count_pairs, nullptr);
+  if (nsteps > 0) mark_grids(c, *P);
+  const int nst = (int)P->stages.size();
+  // chunks of at most kGrowChunk steps; between chunks the cell arrays
+  // follow the systems' spread (grow_cells), so a flow that runs out over
+  // thousands of steps never outgrows its grids
+  for (int k0 = 0; k0 < nsteps; k0 += kGrowChunk) {
+    if ((k0 > 0 || c->bb_valid) && grow_cells(c, *P)) {
+      P = &get_plan(c, integs);
+      mark_grids(c, *P);
+    }
+    const int k1 = std::min(nsteps, k0 + kGrowChunk);
+    for (int k = k0; k < k1; ++k) {
+      if (c->timing)
+        run_step<R>(c, *P, c->count_pairs, &step_events(c, k, nst));
+      else if (c->graph_mode && !c->count_pairs && !P->real_list.empty())
+        step_graph<R>(c, *P);
+      else
+        run_step<R>(c, *P, c->count_pairs, nullptr);
+    }
+    c->bb_valid = !P->real_list.empty();
   }
   int flags[4];
   HIPCHK(hipMemcpyAsync(flags, c->d_flags.p, sizeof(flags), hipMemcpyDeviceToHost, c->stream));
@@ -2782,10 +2878,6 @@ int gr_tune_set(gr_ctx *c, int param, int value) {
              "GR_TUNE_MAXNB must be 0 (default) or in 4..1024");
         c->list_maxnb = value;
         break;
-      case GR_TUNE_NL_KERNEL:
-        need(value == 0 || value == 1, GR_ERR_ARGUMENT, "GR_TUNE_NL_KERNEL must be 0 or 1");
-        c->tune_nlk = value;
-        break;
       default:
         throw GrError(GR_ERR_ARGUMENT, "unknown tuning parameter");
     }
@@ -2801,7 +2893,6 @@ int gr_tune_get(gr_ctx *c, int param, int *value) {
       case GR_TUNE_NL_SPAN: *value = c->tune_span; break;
       case GR_TUNE_SWEEP_BLOCK: *value = c->tune_block; break;
       case GR_TUNE_MAXNB: *value = c->list_maxnb > 0 ? c->list_maxnb : (c->D == 3 ? 48 : 24); break;
-      case GR_TUNE_NL_KERNEL: *value = c->tune_nlk; break;
       default: throw GrError(GR_ERR_ARGUMENT, "unknown tuning parameter");
     }
   });

@@ -40,7 +40,6 @@ struct NlistArgs {
   const long long *bbmax = nullptr;  // (ordered int64, see ord_of)
   int win = 128;  // candidates staged in LDS per window (64, 128 or 256)
   int span = 2;   // lanes whose ranges start within span·(widest range) group
-  int cand = 0;   // 1: candidates in the lanes (k_nlist_cand), 0: one target per lane
 };
 
 // Position bounds per system and dimension are kept in GR_BB_REP replicas

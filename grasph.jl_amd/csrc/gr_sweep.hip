@@ -869,198 +869,6 @@ __global__ void __launch_bounds__(64)
     lcnt[L.cnt_off + (int64_t)(L.src_end - L.src_begin) * L.cap + i] = ne > L.maxnb ? 1 : 0;
 }
 
-// The same lists with the candidates in the lanes.  Targets of a wavefront
-// that lie in one cell of a source's grid share their strips (a "group").
-// The group's strips are cut into pieces of at most 64 candidates (a longer
-// strip in pieces of 64 from its start, as in k_nlist), and the pieces are
-// packed in walk order into chunks of 64 lanes, a piece never split over two
-// chunks.  For each chunk every lane loads one candidate's position
-// (coalesced), and every target of the group, its position broadcast from
-// LDS, tests all 64 at once: the wave's compare mask is that target's
-// acceptance mask over the chunk.  The target's lane keeps it; after the
-// group each lane cuts its masks at the piece boundaries into list entries,
-// in walk order.  r² is the per-lane kernel's expression, so the pair set,
-// the entries and their order are bit-identical to k_nlist's.  A group whose
-// pieces need more than NLC_MAXCH chunks marks its targets as overflowed
-// (they walk their cells in the functor pass).
-#define NLC_MAXCH 16
-#define NLC_MAXP 48
-template <int D, typename R>
-__global__ void __launch_bounds__(64)
-    k_nlist_cand(Pool<R> P, const SysDesc *__restrict__ sd, const int *__restrict__ n_dev,
-                 const LTask *__restrict__ lt, const SrcDesc *__restrict__ srcs,
-                 const Grid *__restrict__ grids, const CellArr *__restrict__ carr,
-                 const int *__restrict__ cells, const Blk *__restrict__ blk, int nblk, int *lcnt,
-                 int *lidx, const long long *__restrict__ bbmin,
-                 const long long *__restrict__ bbmax) {
-  constexpr int NST = (D == 3) ? 9 : 3;
-  __shared__ double sxt[D][64];            // target positions by lane
-  __shared__ int pj[NLC_MAXP];             // piece: first candidate (source-local)
-  __shared__ int plen[NLC_MAXP];           // piece: candidates (1..64)
-  __shared__ int plane[NLC_MAXP];          // piece: first lane in its chunk
-  __shared__ int cfirst[NLC_MAXCH + 1];    // chunk: first piece
-  const Blk b = blk[xcd_block(nblk)];
-  const LTask L = lt[b.item];
-  const int lane = threadIdx.x;
-  const int i = b.first + lane;
-  const SysDesc S = sd[L.sys];
-  const bool live = i < n_dev[L.sys];
-  const int64_t gi = S.off + (live ? i : 0);
-  double xi[D];
-#pragma unroll
-  for (int d = 0; d < D; ++d) {
-    xi[d] = P.x[d][gi];
-    sxt[d][lane] = xi[d];
-  }
-  __syncthreads();
-  int4 *ent = reinterpret_cast<int4 *>(lidx + L.list_off) + i;
-  int ne = 0;  // entries of this target
-  bool ovf = false;
-  for (int q = L.src_begin; q < L.src_end; ++q) {
-    const SrcDesc Q = srcs[q];
-    const Grid G = grids[Q.grid];
-    int cq = 0;  // entries of this source
-    if (G.ncells > 0) {
-      const int *cs = cells + carr[Q.carr].off;
-      const int self_j = Q.pk == PK_SELF ? i : -1;
-      const double c2 = Q.cutoff2;
-      const int64_t poff = sd[Q.partner].off;
-      bool act = live;
-      if (Q.pk == PK_DYNAMIC && bbmin) {
-#pragma unroll
-        for (int d = 0; d < D; ++d) {
-          const double mn = double_of(bbmin[Q.partner * 3 + d]);
-          const double mx = double_of(bbmax[Q.partner * 3 + d]);
-          const double gap = mn - xi[d] > xi[d] - mx ? mn - xi[d] : xi[d] - mx;
-          act = act && !(gap > Q.cell * (1.0 + 1e-9));
-        }
-      }
-      int c[D];
-      target_cell<D>(xi, G, Q.cell, c);
-      int flat = c[0];
-#pragma unroll
-      for (int d = 1; d < D; ++d) flat = flat * G.ng[d] + c[d];
-      unsigned long long pending = __builtin_amdgcn_ballot_w64(act);
-      while (pending) {
-        const int leader = __ffsll((long long)pending) - 1;
-        const int lflat = __shfl(flat, leader, 64);
-        const bool ing = act && flat == lflat;
-        const unsigned long long grp = __builtin_amdgcn_ballot_w64(ing);
-        pending &= ~grp;
-        int cl[D];
-#pragma unroll
-        for (int d = 0; d < D; ++d) cl[d] = __shfl(c[d], leader, 64);
-        // pieces and chunks (uniform; written by lane 0)
-        int npc = 0, nch = 0, used = 64;
-        bool fits = true;
-        for (int st = 0; st < NST; ++st) {
-          const int sb = strip_base<D>(cl, G, st);
-          int j0 = cs[sb] - 1;
-          const int j1 = cs[sb + 3] - 1;
-          while (j0 < j1) {
-            const int len = j1 - j0 < 64 ? j1 - j0 : 64;
-            if (used + len > 64) {  // a new chunk
-              if (nch == NLC_MAXCH) {
-                fits = false;
-                break;
-              }
-              if (lane == 0) cfirst[nch] = npc;
-              ++nch;
-              used = 0;
-            }
-            if (npc == NLC_MAXP) {
-              fits = false;
-              break;
-            }
-            if (lane == 0) {
-              pj[npc] = j0;
-              plen[npc] = len;
-              plane[npc] = used;
-            }
-            ++npc;
-            used += len;
-            j0 += len;
-          }
-          if (!fits) break;
-        }
-        if (lane == 0) cfirst[nch] = npc;
-        __syncthreads();
-        if (!fits) {  // the group's targets walk their cells
-          if (ing) ovf = true;
-          continue;
-        }
-        // this lane's masks of its own target, one per chunk
-        unsigned mlo[NLC_MAXCH], mhi[NLC_MAXCH];
-#pragma unroll
-        for (int k = 0; k < NLC_MAXCH; ++k) {
-          mlo[k] = 0u;
-          mhi[k] = 0u;
-        }
-#pragma unroll
-        for (int k = 0; k < NLC_MAXCH; ++k) {
-          if (k < nch) {
-            // this lane's candidate in chunk k
-            int sel = cfirst[k];
-            for (int p = cfirst[k] + 1; p < cfirst[k + 1]; ++p)
-              if (lane >= plane[p]) sel = p;
-            const int off = lane - plane[sel];
-            const bool valid = off < plen[sel];
-            const int j = pj[sel] + (valid ? off : 0);
-            double xj[D];
-#pragma unroll
-            for (int d = 0; d < D; ++d) xj[d] = P.x[d][poff + j];
-            const unsigned long long vmask = __builtin_amdgcn_ballot_w64(valid);
-            unsigned long long g = grp;
-            while (g) {
-              const int t = __ffsll((long long)g) - 1;
-              g &= g - 1ull;
-              const double a0 = sxt[0][t] - xj[0];
-              double r2 = a0 * a0;
-#pragma unroll
-              for (int d = 1; d < D; ++d) {
-                const double ad = sxt[d][t] - xj[d];
-                r2 = r2 + ad * ad;
-              }
-              const unsigned long long m = __builtin_amdgcn_ballot_w64(r2 < c2) & vmask;
-              const bool me = lane == t;
-              mlo[k] = me ? (unsigned)m : mlo[k];
-              mhi[k] = me ? (unsigned)(m >> 32) : mhi[k];
-            }
-          }
-        }
-        // entries of this lane's target, in walk order
-        if (ing) {
-#pragma unroll
-          for (int k = 0; k < NLC_MAXCH; ++k) {
-            if (k < nch) {
-              const unsigned long long m = ((unsigned long long)mhi[k] << 32) | mlo[k];
-              for (int p = cfirst[k]; p < cfirst[k + 1]; ++p) {
-                const int len = plen[p], base = pj[p];
-                unsigned long long sub = m >> plane[p];
-                if (len < 64) sub &= (1ull << len) - 1ull;
-                const unsigned sdl = (unsigned)(self_j - base);
-                if (sdl < 64u) sub &= ~(1ull << sdl);
-                if (sub != 0ull) {
-                  if (ne < L.maxnb)
-                    ent[(int64_t)ne * L.cap] =
-                        make_int4(base, (int)(unsigned)sub, (int)(unsigned)(sub >> 32), 0);
-                  ++ne;
-                  ++cq;
-                }
-              }
-            }
-          }
-        }
-        __syncthreads();  // the piece table is rewritten by the next group
-      }
-    }
-    if (live) lcnt[L.cnt_off + (int64_t)(q - L.src_begin) * L.cap + i] = cq;
-  }
-  if (live)
-    lcnt[L.cnt_off + (int64_t)(L.src_end - L.src_begin) * L.cap + i] =
-        (ne > L.maxnb || ovf) ? 1 : 0;
-}
-
 // Functors over the accepted partners of each target (one stage).
 template <int D, typename R, int FAM, bool CNT>
 __global__ void __launch_bounds__(GR_BLOCK)
@@ -1237,17 +1045,6 @@ static void nlist_one(hipStream_t s, const Pool<R> &P, const NlistArgs &a) {
 template <typename R>
 void launch_nlist(hipStream_t s, const Pool<R> &P, const NlistArgs &a, int D) {
   if (a.nblk <= 0) return;
-  if (a.cand) {
-    if (D == 2)
-      k_nlist_cand<2, R><<<a.nblk, 64, 0, s>>>(P, a.sd, a.n, a.lt, a.srcs, a.grids, a.carr,
-                                               a.cells, a.blk, a.nblk, a.lcnt, a.lidx, a.bbmin,
-                                               a.bbmax);
-    else
-      k_nlist_cand<3, R><<<a.nblk, 64, 0, s>>>(P, a.sd, a.n, a.lt, a.srcs, a.grids, a.carr,
-                                               a.cells, a.blk, a.nblk, a.lcnt, a.lidx, a.bbmin,
-                                               a.bbmax);
-    return;
-  }
   if (D == 2) {
     if (a.win == 64) nlist_one<2, R, 64>(s, P, a);
     else if (a.win == 256) nlist_one<2, R, 256>(s, P, a);

@@ -291,7 +291,8 @@ class Server:
         return rep
 
 
-def run_plain(jobs, precision="f32", keep_results=False, session_factory=None, graph=False):
+def run_plain(jobs, precision="f32", keep_results=False, session_factory=None, graph=False,
+              progress=None):
     """Each job alone on the GPU, one after another in arrival order (a job
     starts when it has arrived and the previous one has finished); `graph`
     replays each job's step as a captured graph, as the server can.  The
@@ -325,13 +326,24 @@ def run_plain(jobs, precision="f32", keep_results=False, session_factory=None, g
         s.scene.upload_all(lf)
         s.backend.integrator_set_dt(h, s.backend.integrator_dt(h, cfg["cfl"]))
         job.admitted = time.perf_counter() - t0
-        s.backend.step_many([h], job.steps)
+        try:
+            s.backend.step_many([h], job.steps)
+        except A.FaultError:  # the same failure as in the server: retired alone
+            bits = s.backend.integrator_status(h)
+            s.scene.release(lf)
+            job.meta["failed"] = ("device fault: " + ("non-finite positions"
+                                  if bits & A.GR_FAULT_NONFINITE
+                                  else "particles beyond the cell range"))
+            job.finished = time.perf_counter() - t0
+            continue
         job.done_steps = job.steps
         if keep_results:
             s.scene.download_all(lf)
             job.result = {f: getattr(cfg["fluid"], f).copy() for f in cfg["fluid"].field_names()}
         s.scene.release(lf)
         job.finished = time.perf_counter() - t0
+        if progress and (k + 1) % 20 == 0:
+            progress(f"{k + 1}/{len(order)} jobs done")
     pool.shutdown(wait=True)
     return report(jobs, time.perf_counter() - t0)
 

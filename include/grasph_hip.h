@@ -312,13 +312,9 @@ enum {
                                pass: 1..16 (default 2)                      */
   GR_TUNE_SWEEP_BLOCK = 2,  /* target particles per workgroup of the
                                functor gather: 64, 128, 256 (default)       */
-  GR_TUNE_MAXNB = 3,        /* list entries kept per target before it walks
+  GR_TUNE_MAXNB = 3         /* list entries kept per target before it walks
                                its cells directly: 0 = default (48 in 3D,
                                24 in 2D) or 4..1024                         */
-  GR_TUNE_NL_KERNEL = 4     /* neighbour-search kernel: 0 one target per
-                               lane (default), 1 the candidates in the
-                               lanes, tested against each target of a cell
-                               at once                                      */
 };
 int gr_tune_set(gr_ctx *ctx, int param, int value);
 int gr_tune_get(gr_ctx *ctx, int param, int *value);

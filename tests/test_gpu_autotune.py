@@ -39,8 +39,8 @@ def test_every_launch_shape_gives_the_same_state(builder, kw, D):
     shapes = [((A.GR_TUNE_SWEEP_BLOCK, 64),), ((A.GR_TUNE_SWEEP_BLOCK, 128),),
               ((A.GR_TUNE_NL_WIN, 64), (A.GR_TUNE_NL_SPAN, 1)),
               ((A.GR_TUNE_NL_WIN, 256), (A.GR_TUNE_NL_SPAN, 4)),
-              ((A.GR_TUNE_MAXNB, 4),), ((A.GR_TUNE_NL_KERNEL, 1),),
-              ((A.GR_TUNE_NL_KERNEL, 1), (A.GR_TUNE_MAXNB, 6), (A.GR_TUNE_SWEEP_BLOCK, 64))]
+              ((A.GR_TUNE_MAXNB, 4),),
+              ((A.GR_TUNE_MAXNB, 6), (A.GR_TUNE_SWEEP_BLOCK, 64), (A.GR_TUNE_NL_WIN, 64))]
     for knobs in shapes:
         got, _ = _run(builder, kw, D, 12, knobs)
         for f, v in ref.items():
@@ -52,7 +52,7 @@ def test_autotune_records_a_choice_and_leaves_the_state_alone():
     got, rec = _run(M.configs.dambreak_3d, dict(dx=0.03), 3, 10, tune=True)
     for f, v in ref.items():
         assert np.array_equal(got[f], v), f
-    assert set(rec["chosen"]) == {"nl_kernel", "sweep_block", "nl_win", "nl_span", "maxnb"}
-    assert len(rec["trials"]) == 14 and rec["ms_per_step_tuned"] > 0
+    assert set(rec["chosen"]) == {"sweep_block", "nl_win", "nl_span", "maxnb"}
+    assert len(rec["trials"]) == 12 and rec["ms_per_step_tuned"] > 0
     with pytest.raises(ValueError):
         M.Session(3, "f32").backend.tune_set(A.GR_TUNE_NL_WIN, 100)

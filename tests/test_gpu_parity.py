@@ -840,3 +840,46 @@ def test_virtual_interpolation_matches_oracle(prec, tol):
         if k == "id":
             continue
         assert rel(res["dev"][k], ref) <= tol, (k, rel(res["dev"][k], ref))
+
+
+def test_cell_arrays_follow_a_spreading_flow():
+    """Two fluid clusters fly apart at 20 m/s each (the sound speed): their
+    common grid spreads to ten times its first extent over the run, far beyond the cell arrays
+    planned at creation.  The device grows them between chunks of steps
+    and matches the oracle (which has no capacity) at the f64 tolerance,
+    with no fault."""
+    def scene():
+        dx, h = 0.01, 0.013
+        n1 = 12
+        xs = (np.stack(np.meshgrid(np.arange(n1), np.arange(n1), indexing="ij"), -1)
+              .reshape(-1, 2) + 0.5) * dx
+        x = np.concatenate([xs, xs + [0.3, 0.0]])
+        f = M.FluidParticleSystem("f", len(x), 2, 1000 * dx * dx, 20.0,
+                                  state_updater=M.TaitEOSUpdater(1000.0))
+        f.x[:] = x
+        rng = np.random.default_rng(4)
+        f.rho[:] = 1000.0 * (1 + rng.uniform(-1e-3, 1e-3, len(x)))
+        f.v[:] = rng.uniform(-0.2, 0.2, (len(x), 2))
+        f.v[: len(xs), 0] -= 20.0
+        f.v[len(xs):, 0] += 20.0
+        si = M.SystemInteraction(M.WenlandC2Kernel(h, ndims=2), M.FluidPfn(0.1, 0.0, h), f)
+        return f, M.LeapFrogTimeIntegrator([f], [si])
+
+    res = {}
+    for make in ("dev", "orc"):
+        f, lf = scene()
+        sc = M.Session(2, "f64").scene if make == "dev" else oracle_scene(2)
+        h = sc.integrator(lf)
+        sc.upload_all(lf)
+        dt = sc.be.integrator_dt(h, 0.2)
+        for _ in range(4):
+            sc.be.step(h, 200, dt)
+        sc.download_all(lf)
+        if make == "dev":
+            assert sc.be.integrator_status(h) == 0
+        res[make] = (f.id.copy(), f.x.copy(), f.v.copy(), f.rho.copy())
+    assert np.array_equal(res["dev"][0], res["orc"][0])
+    span = res["orc"][1][:, 0].max() - res["orc"][1][:, 0].min()
+    assert span > 10 * 0.42  # the flow spread far beyond its first extent
+    for a, b in zip(res["dev"][1:], res["orc"][1:]):
+        assert rel(a, b) <= 1e-10, rel(a, b)
