@@ -7,9 +7,9 @@ that matter per scene are (include/grasph_hip.h, GR_TUNE_*):
   * GR_TUNE_SWEEP_BLOCK — target particles per workgroup of the functor
     gather (64, 128, 256);
   * GR_TUNE_NL_WIN — candidates staged in LDS per window of the neighbour
-    search (the staging tile: 64, 128, 256);
+    search (the staging tile: 64, 128, 256, 512);
   * GR_TUNE_NL_SPAN — how far apart the lanes' candidate ranges may start
-    and still share one staging pass (1, 2, 4: a cell-column grouping);
+    and still share one staging pass (a cell-column grouping);
   * GR_TUNE_MAXNB — list entries kept per target before it walks its cells
     directly (memory against overflow walks).
 
@@ -35,8 +35,8 @@ def default_space(ndims):
     """Candidate values per knob, in the order the descent visits them."""
     mx = (32, 48, 64) if ndims == 3 else (16, 24, 32)
     return [(A.GR_TUNE_SWEEP_BLOCK, (256, 128, 64)),
-            (A.GR_TUNE_NL_WIN, (128, 64, 256)),
-            (A.GR_TUNE_NL_SPAN, (2, 1, 4)),
+            (A.GR_TUNE_NL_WIN, (128, 256, 512)),
+            (A.GR_TUNE_NL_SPAN, (2, 4, 8)),
             (A.GR_TUNE_MAXNB, mx)]
 
 

@@ -275,6 +275,7 @@ struct FamPlan {
   int nblk = 0;
   bool fluid_only = false;  // every source's functor is FluidPfn / FluidSolidPfn
   bool cauchy = false;      // some source's functor is CauchyFluidPfn
+  bool inv = false;         // every source: FluidPfn S = 2, no δ-SPH, no surface tension
 };
 struct StagePlan {
   DBuf blk_upd, urange, upd;
@@ -382,7 +383,7 @@ struct gr_ctx {
   int use_lists = 1;
   int list_maxnb = 0;  // mask entries kept per target (0: 48 in 3D, 24 in 2D)
   // launch shapes chosen by gr_tune_set (results do not depend on them)
-  int tune_win = 128, tune_span = 2, tune_block = GR_BLOCK;
+  int tune_win = 128, tune_span = 2, tune_block = GR_BLOCK, tune_inv = 1;
   std::map<std::vector<int>, std::unique_ptr<Plan>> plans;
   // cached device tables derived from the layout
   DBuf allreal_blk;
@@ -1472,6 +1473,11 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
       bool cy = false;
       for (const SrcDesc &q : sv) cy = cy || q.pfn == GR_PFN_CAUCHY;
       sp->fam[fam].cauchy = cy;
+      bool iv = fo && c->tune_inv && !sv.empty();
+      for (const SrcDesc &q : sv)
+        iv = iv && q.pfn == GR_PFN_FLUID && q.sigma == 2 && !q.has_delta && !q.has_eps &&
+             q.psel == 0;
+      sp->fam[fam].inv = iv;
     }
     P.stages.push_back(std::move(sp));
   }
@@ -1712,11 +1718,14 @@ void phase_stage(gr_ctx *c, Plan &P, int k, bool cnt, hipEvent_t *mid) {
                       S.fam[FAM_COUNT].nblk == 0;
   // the partner stress records only for stages with Cauchy force sweeps
   const bool with_s = S.fam[FAM_FORCE].cauchy;
+  // records of 1/ρ and p/ρ² when the stage's only sweeps are FluidPfn S = 2
+  // force sweeps without δ-SPH or surface tension
+  const bool inv = c_only && S.fam[FAM_FORCE].inv && S.fam[FAM_STRAIN].nblk == 0;
   auto pack = [&]() {
     launch_pack<R>(s, A, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), P.blk_real.as<Blk>(),
-                   P.nblk_real, c->D, c_only, with_s);
+                   P.nblk_real, c->D, c_only, with_s, inv);
     launch_pack<R>(s, A, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), P.blk_ghost.as<Blk>(),
-                   P.nblk_ghost, c->D, c_only, with_s);
+                   P.nblk_ghost, c->D, c_only, with_s, inv);
   };
   const int nother = S.fam[FAM_FORCE].nblk + S.fam[FAM_COUNT].nblk + S.fam[FAM_STRAIN].nblk;
   if (nother + S.fam[FAM_INTERP].nblk > 0) pack();
@@ -1747,6 +1756,7 @@ void phase_stage(gr_ctx *c, Plan &P, int k, bool cnt, hipEvent_t *mid) {
       a.lidx = c->lidx.as<int>();
     }
     a.fluid_only = F.fluid_only;
+    a.inv = inv && fam == FAM_FORCE;
     a.block = P.sweep_block;
     launch_sweep<R>(s, A, a, c->D, fam, cnt);
   }
@@ -2860,8 +2870,8 @@ int gr_tune_set(gr_ctx *c, int param, int value) {
   return guard(c, [&] {
     switch (param) {
       case GR_TUNE_NL_WIN:
-        need(value == 64 || value == 128 || value == 256, GR_ERR_ARGUMENT,
-             "GR_TUNE_NL_WIN must be 64, 128 or 256");
+        need(value == 64 || value == 128 || value == 256 || value == 512, GR_ERR_ARGUMENT,
+             "GR_TUNE_NL_WIN must be 64, 128, 256 or 512");
         c->tune_win = value;
         break;
       case GR_TUNE_NL_SPAN:
@@ -2878,6 +2888,10 @@ int gr_tune_set(gr_ctx *c, int param, int value) {
              "GR_TUNE_MAXNB must be 0 (default) or in 4..1024");
         c->list_maxnb = value;
         break;
+      case GR_TUNE_INV_RECORDS:
+        need(value == 0 || value == 1, GR_ERR_ARGUMENT, "GR_TUNE_INV_RECORDS must be 0 or 1");
+        c->tune_inv = value;
+        break;
       default:
         throw GrError(GR_ERR_ARGUMENT, "unknown tuning parameter");
     }
@@ -2893,6 +2907,7 @@ int gr_tune_get(gr_ctx *c, int param, int *value) {
       case GR_TUNE_NL_SPAN: *value = c->tune_span; break;
       case GR_TUNE_SWEEP_BLOCK: *value = c->tune_block; break;
       case GR_TUNE_MAXNB: *value = c->list_maxnb > 0 ? c->list_maxnb : (c->D == 3 ? 48 : 24); break;
+      case GR_TUNE_INV_RECORDS: *value = c->tune_inv; break;
       default: throw GrError(GR_ERR_ARGUMENT, "unknown tuning parameter");
     }
   });

@@ -132,8 +132,8 @@ struct Task {
 };
 
 // Neighbour list of one target system: sources [src_begin, src_end) of the
-// list source table; mask entry e of target i (int4 {base, mask_lo,
-// mask_hi, 0}) at lidx[list_off + 4·(e·cap + i)], entry counts per source at
+// list source table; mask entry e of target i (int4 {mask_lo, mask_hi,
+// base, 0}) at lidx[list_off + 4·(e·cap + i)], entry counts per source at
 // lcnt[cnt_off + q·cap + i], overflow flag at lcnt[cnt_off + nsrc·cap + i];
 // maxnb = entries kept per target.
 struct LTask {

@@ -1477,7 +1477,7 @@ __global__ void k_fill_i32(int *p, int v, long long n) {
 
 // Partner records of the live particles of the block's system (positions,
 // shadows of density and velocity, pressure), read by the pair sweeps.
-template <typename R, int D, bool C_ONLY, bool WITH_S>
+template <typename R, int D, bool C_ONLY, bool WITH_S, bool INV = false>
 __global__ void k_pack(Pool<R> P, const SysDesc *sd, const int *n_dev, const Blk *blk) {
   const Blk b = blk[blockIdx.x];
   const int i = b.first + threadIdx.x;
@@ -1508,8 +1508,14 @@ __global__ void k_pack(Pool<R> P, const SysDesc *sd, const int *n_dev, const Blk
       cc.xq[d] = d < D ? (int32_t)quant_of(a.x[d], P.iqs) : 0;
       cc.v[d] = r.v[d];
     }
-    cc.rho = a.rho;
-    cc.p = a.p;
+    if constexpr (INV) {  // 1/ρ and p/ρ² (the FluidPfn S = 2 force sweep)
+      const float ir = __builtin_amdgcn_rcpf(a.rho);
+      cc.rho = ir;
+      cc.p = a.p * (ir * ir);
+    } else {
+      cc.rho = a.rho;
+      cc.p = a.p;
+    }
     P.rc[g] = cc;
   }
   if constexpr (sizeof(R) == 4 && WITH_S) {  // stages with Cauchy sweeps
@@ -1793,9 +1799,16 @@ static void launch_pack_d(hipStream_t s, const Pool<R> &P, const SysDesc *sd, co
 
 template <typename R>
 void launch_pack(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const int *n, const Blk *blk,
-                 int nblk, int D, bool c_only, bool with_s) {
+                 int nblk, int D, bool c_only, bool with_s, bool inv) {
   if (nblk <= 0) return;
   c_only = c_only && sizeof(R) == 4;
+  if (inv && c_only && !with_s && sizeof(R) == 4) {
+    if (D == 2)
+      k_pack<R, 2, true, false, true><<<nblk, GR_BLOCK, 0, s>>>(P, sd, n, blk);
+    else
+      k_pack<R, 3, true, false, true><<<nblk, GR_BLOCK, 0, s>>>(P, sd, n, blk);
+    return;
+  }
   if (c_only && with_s)
     launch_pack_d<R, true, true>(s, P, sd, n, blk, nblk, D);
   else if (c_only)
@@ -1806,9 +1819,9 @@ void launch_pack(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const int *
     launch_pack_d<R, false, false>(s, P, sd, n, blk, nblk, D);
 }
 template void launch_pack<float>(hipStream_t, const Pool<float> &, const SysDesc *, const int *,
-                                 const Blk *, int, int, bool, bool);
+                                 const Blk *, int, int, bool, bool, bool);
 template void launch_pack<double>(hipStream_t, const Pool<double> &, const SysDesc *, const int *,
-                                  const Blk *, int, int, bool, bool);
+                                  const Blk *, int, int, bool, bool, bool);
 
 // Probe slot id[i] − 1 takes the position of mirror particle i
 // (_measure_probes!, src/TimeIntegration.jl:469-479).

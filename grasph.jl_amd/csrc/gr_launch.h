@@ -22,6 +22,7 @@ struct SweepArgs {
   const int *lidx = nullptr;
   bool fluid_only = false;  // FORCE: every source's functor is FluidPfn / FluidSolidPfn
   int block = GR_BLOCK;     // targets per workgroup (the block table's step)
+  bool inv = false;         // FP32 FluidPfn S = 2 stage whose records hold 1/ρ, p/ρ²
 };
 
 struct NlistArgs {
@@ -116,7 +117,7 @@ void launch_virtual(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const St
                     const int *n, const Blk *blk, int nblk, int mode, int D);
 template <typename R>
 void launch_pack(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const int *n, const Blk *blk,
-                 int nblk, int D, bool c_only = false, bool with_s = true);
+                 int nblk, int D, bool c_only = false, bool with_s = true, bool inv = false);
 template <typename R>
 void launch_nlist(hipStream_t s, const Pool<R> &P, const NlistArgs &a, int D);
 void launch_fill_i32(hipStream_t s, int *p, int v, long long n);

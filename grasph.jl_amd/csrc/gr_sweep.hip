@@ -162,12 +162,19 @@ struct SrcK {
 // target's stress are compiled out.  A template argument only, not a task
 // family.
 constexpr int FAM_FORCE_FLUID = NFAM + 1;
+// The same for FluidPfn with S = 2, no δ-SPH and no surface tension, whose
+// partner records hold 1/ρ and p/ρ² in place of ρ and p (k_pack INV): the
+// reciprocal and the products the pair would compute are done once per
+// particle, with the same FP32 operations, so the sums are unchanged.
+constexpr int FAM_FORCE_FLUID_INV = NFAM + 2;
 
 // Target particle: its state and its sums.
 template <int D, typename R, int FAM>
 struct Gather {
-  static constexpr bool FORCE = FAM == FAM_FORCE || FAM == FAM_FORCE_FLUID;
-  static constexpr bool FLUID_ONLY = FAM == FAM_FORCE_FLUID;
+  static constexpr bool FORCE =
+      FAM == FAM_FORCE || FAM == FAM_FORCE_FLUID || FAM == FAM_FORCE_FLUID_INV;
+  static constexpr bool FLUID_ONLY = FAM == FAM_FORCE_FLUID || FAM == FAM_FORCE_FLUID_INV;
+  static constexpr bool INV = FAM == FAM_FORCE_FLUID_INV;
   // FP32 fluid-only force sweeps read the compact records (RecC)
   static constexpr bool COMPACT = (FORCE || FAM == FAM_STRAIN) && sizeof(R) == 4;
   uint32_t xq[D];  // COMPACT: the target's fixed-point position
@@ -327,7 +334,11 @@ struct Gather {
       rj = Aj.rho;
       pj = rec_pres(Aj, Bj);
     }
-    const R irj = (FA && Q.pk == PK_DYNAMIC) ? iri : Fast<R>::rcp(rj);
+    R irj;
+    if constexpr (INV)  // the record's 1/ρ (a dynamic partner mirrors the target)
+      irj = Q.pk == PK_DYNAMIC ? iri : rj;
+    else
+      irj = (FA && Q.pk == PK_DYNAMIC) ? iri : Fast<R>::rcp(rj);
     if (FAM == FAM_STRAIN) {  // strain_rate_tensor / vorticity_tensor
       R dv[D];
 #pragma unroll
@@ -394,7 +405,9 @@ struct Gather {
       // the fluid side, the partner's on the solid side)
       const R pa = Q.psel == 2 ? pj : pi, pb = Q.psel == 1 ? pi : pj;
       R dh;
-      if (FA) {
+      if constexpr (INV) {  // S = 2; the record's p/ρ² for a real partner
+        dh = -(pa * (iri * iri) + (Q.pk == PK_DYNAMIC ? pb * (irj * irj) : pj));
+      } else if (FA) {
         if (K.sigma == 1)
           dh = -(pa + pb) * (iri * irj);
         else
@@ -666,7 +679,8 @@ __global__ void __launch_bounds__(GR_BLOCK)
 
 // Accepted partners of every target of a list task, per source, in the
 // order the sources are listed, as acceptance masks: an entry
-// {base, mask_lo, mask_hi, 0} (int4) says that candidates base + b of the
+// {mask_lo, mask_hi, base, 0} (int4; the mask first, so a load of the entry
+// puts it in an aligned register pair) says that candidates base + b of the
 // source (source-local indices, b = 0..63) with bit b set are partners.
 // Entries follow the cell walk (strips in order, ascending candidates), so
 // popping the bits in order visits the partners in the walk's order.  Entry e
@@ -773,7 +787,7 @@ __global__ void __launch_bounds__(64)
           if (sdl < 32u) lo &= ~(1u << sdl);
           else if (sdl < 64u) hi &= ~(1u << (sdl - 32u));
           if (live && (lo | hi) != 0u) {
-            if (ne < L.maxnb) ent[(int64_t)ne * L.cap] = make_int4(base, (int)lo, (int)hi, 0);
+            if (ne < L.maxnb) ent[(int64_t)ne * L.cap] = make_int4((int)lo, (int)hi, base, 0);
             ++ne;
             ++cq;
           }
@@ -915,8 +929,8 @@ __global__ void __launch_bounds__(GR_BLOCK)
       auto next = [&](int &j) -> bool {
         if (m == 0ull) {  // next entry (entries hold at least one partner)
           if (e >= eend) return false;
-          base = epre.x;
-          m = ((unsigned long long)(unsigned)epre.z << 32) | (unsigned)epre.y;
+          base = epre.z;
+          m = ((unsigned long long)(unsigned)epre.y << 32) | (unsigned)epre.x;
           ++e;
           if (e < eend) epre = ent[(int64_t)e * L.cap];
         }
@@ -928,28 +942,36 @@ __global__ void __launch_bounds__(GR_BLOCK)
       // compact Cauchy sources also read the partner's stress (the second
       // half only for NS > 4)
       const bool ns_a = CMP && needs_s<FAM>(Q), ns_b = ns_a && Gt.NS > 4;
-      struct PRec {
-        RecA<R> A;
-        RecB<R> B;
-        RecC C;
-        RecS S;
-      };
-      auto fetch = [&](int j, PRec &r) {
+      int jn = 0;
+      bool have = next(jn);
+      RecA<R> An{};
+      RecB<R> Bn{};
+      RecC Cn{};
+      RecS Sn{};
+      auto fetch = [&]() {
         if constexpr (CMP) {
-          r.C = P.rc[poff + j];
-          if (ns_a) r.S.a = P.rs[poff + j].a;
-          if (ns_b) r.S.b = P.rs[poff + j].b;
+          Cn = P.rc[poff + jn];
+          if (ns_a) Sn.a = P.rs[poff + jn].a;
+          if (ns_b) Sn.b = P.rs[poff + jn].b;
         } else {
-          r.A = P.ra[poff + j];
-          if (nb) r.B = P.rb[poff + j];
+          An = P.ra[poff + jn];
+          if (nb) Bn = P.rb[poff + jn];
         }
       };
-      auto eval = [&](int j, const PRec &r) {
+      if (have) fetch();
+      while (have) {
+        const int64_t gj = poff + jn;
+        const RecA<R> Aj = An;
+        const RecB<R> Bj = Bn;
+        const RecC Cj = Cn;
+        const RecS Sj = Sn;
+        have = next(jn);
+        if (have) fetch();
         double dx[D] = {};
         double r2 = 0.0;
         if constexpr (!CMP) {
 #pragma unroll
-          for (int d = 0; d < D; ++d) dx[d] = Gt.xi[d] - r.A.x[d];
+          for (int d = 0; d < D; ++d) dx[d] = Gt.xi[d] - Aj.x[d];
           r2 = dx[0] * dx[0];
 #pragma unroll
           for (int d = 1; d < D; ++d) r2 = r2 + dx[d] * dx[d];
@@ -960,22 +982,7 @@ __global__ void __launch_bounds__(GR_BLOCK)
           else
             ++no;
         }
-        Gt.pair(P, Q, K, poff + j, r.A, r.B, r.C, r.S, dx, r2, da);
-      };
-      // two record buffers used in turn: the next partner's record is in
-      // flight while the current pair is evaluated, with no register copies
-      PRec r0{}, r1{};
-      int j0 = 0, j1 = 0;
-      bool h0 = next(j0);
-      if (h0) fetch(j0, r0);
-      while (h0) {
-        const bool h1 = next(j1);
-        if (h1) fetch(j1, r1);
-        eval(j0, r0);
-        if (!h1) break;
-        h0 = next(j0);
-        if (h0) fetch(j0, r0);
-        eval(j1, r1);
+        Gt.pair(P, Q, K, gj, Aj, Bj, Cj, Sj, dx, r2, da);
       }
     }
   }
@@ -1015,14 +1022,16 @@ template <typename R>
 void launch_sweep(hipStream_t s, const Pool<R> &P, const SweepArgs &a, int D, int fam, bool cnt) {
   if (a.nblk <= 0) return;
   if (D == 2) {
-    if (fam == FAM_FORCE && a.fluid_only) launch_one<2, R, FAM_FORCE_FLUID>(s, P, a, cnt);
+    if (fam == FAM_FORCE && a.fluid_only && a.inv) launch_one<2, R, FAM_FORCE_FLUID_INV>(s, P, a, cnt);
+    else if (fam == FAM_FORCE && a.fluid_only) launch_one<2, R, FAM_FORCE_FLUID>(s, P, a, cnt);
     else if (fam == FAM_FORCE) launch_one<2, R, FAM_FORCE>(s, P, a, cnt);
     else if (fam == FAM_STRAIN) launch_one<2, R, FAM_STRAIN>(s, P, a, cnt);
     else if (fam == FAM_INTERP) launch_one<2, R, FAM_INTERP>(s, P, a, cnt);
     else if (fam == FAM_XSPH) launch_one<2, R, FAM_XSPH>(s, P, a, cnt);
     else launch_one<2, R, FAM_COUNT>(s, P, a, cnt);
   } else {
-    if (fam == FAM_FORCE && a.fluid_only) launch_one<3, R, FAM_FORCE_FLUID>(s, P, a, cnt);
+    if (fam == FAM_FORCE && a.fluid_only && a.inv) launch_one<3, R, FAM_FORCE_FLUID_INV>(s, P, a, cnt);
+    else if (fam == FAM_FORCE && a.fluid_only) launch_one<3, R, FAM_FORCE_FLUID>(s, P, a, cnt);
     else if (fam == FAM_FORCE) launch_one<3, R, FAM_FORCE>(s, P, a, cnt);
     else if (fam == FAM_STRAIN) launch_one<3, R, FAM_STRAIN>(s, P, a, cnt);
     else if (fam == FAM_INTERP) launch_one<3, R, FAM_INTERP>(s, P, a, cnt);
@@ -1048,10 +1057,12 @@ void launch_nlist(hipStream_t s, const Pool<R> &P, const NlistArgs &a, int D) {
   if (D == 2) {
     if (a.win == 64) nlist_one<2, R, 64>(s, P, a);
     else if (a.win == 256) nlist_one<2, R, 256>(s, P, a);
+    else if (a.win == 512) nlist_one<2, R, 512>(s, P, a);
     else nlist_one<2, R, 128>(s, P, a);
   } else {
     if (a.win == 64) nlist_one<3, R, 64>(s, P, a);
     else if (a.win == 256) nlist_one<3, R, 256>(s, P, a);
+    else if (a.win == 512) nlist_one<3, R, 512>(s, P, a);
     else nlist_one<3, R, 128>(s, P, a);
   }
 }

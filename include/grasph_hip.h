@@ -306,15 +306,21 @@ int gr_set_graph_mode(gr_ctx *ctx, int on);
  * order and every output are bit-identical for all of them. */
 enum {
   GR_TUNE_NL_WIN = 0,       /* candidates staged in LDS per window of the
-                               neighbour search: 64, 128 (default), 256     */
+                               neighbour search: 64, 128 (default), 256,
+                               512                                          */
   GR_TUNE_NL_SPAN = 1,      /* lanes whose candidate ranges start within
                                span × the widest range share one staging
                                pass: 1..16 (default 2)                      */
   GR_TUNE_SWEEP_BLOCK = 2,  /* target particles per workgroup of the
                                functor gather: 64, 128, 256 (default)       */
-  GR_TUNE_MAXNB = 3         /* list entries kept per target before it walks
+  GR_TUNE_MAXNB = 3,        /* list entries kept per target before it walks
                                its cells directly: 0 = default (48 in 3D,
                                24 in 2D) or 4..1024                         */
+  GR_TUNE_INV_RECORDS = 4   /* 1 (default): in FP32 stages whose sweeps are
+                               all FluidPfn S = 2 without δ-SPH or surface
+                               tension, the partner records hold 1/ρ and
+                               p/ρ² (the same FP32 operations, done once per
+                               particle); 0: ρ and p                        */
 };
 int gr_tune_set(gr_ctx *ctx, int param, int value);
 int gr_tune_get(gr_ctx *ctx, int param, int *value);
