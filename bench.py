@@ -43,6 +43,10 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--dx", type=float, default=0.006)
+    ap.add_argument("--start-step", type=int, default=300,
+                    help="untimed steps of the dam break before tuning, warm-up and timing: "
+                         "from rest no particle crosses a cell face and the sort skips its "
+                         "radix passes, which a flowing dam break does not")
     ap.add_argument("--precision", default="f32", choices=["f32", "f64"])
     ap.add_argument("--cpu-steps", type=int, default=60,
                     help="timed oracle steps on the host (0 disables the CPU baseline)")
@@ -51,6 +55,9 @@ def parse():
                     help="also run the plain copy kernel on this many MiB (counter calibration)")
     ap.add_argument("--autotune", type=int, default=1,
                     help="pick the launch shapes with the autotuner before timing (0: defaults)")
+    ap.add_argument("--shapes", default="",
+                    help="fixed launch shapes instead of the autotuner's, e.g. "
+                         "'nl_win=256,nl_span=4,sweep_block=128,maxnb=32'")
     ap.add_argument("--ensemble", type=int, default=256,
                     help="also step an ensemble of this many 2D columns in shared launches "
                          "and the same columns one by one (0 disables)")
@@ -246,10 +253,16 @@ def main():
     be.integrator_set_dt(h, dt)
     be.set_graph_mode(bool(args.graph))
     info = be.ctx_info()
-    # first sort of the shuffled order, then the launch shapes (the tuner
-    # restores the state it found)
-    be.step_many([h], 1)
-    tune = g.autotune.autotune(s, [lf], steps=3, reps=2) if args.autotune else None
+    # first sort of the shuffled order and the start of the flow, then the
+    # launch shapes (the tuner restores the state it found)
+    be.step_many([h], max(1, args.start_step))
+    if args.shapes:
+        knob = {v: k for k, v in g.autotune.KNOB_NAMES.items()}
+        for kv in args.shapes.split(","):
+            k, v = kv.split("=")
+            be.tune_set(knob[k.strip()], int(v))
+    tune = (g.autotune.autotune(s, [lf], steps=3, reps=2)
+            if args.autotune and not args.shapes else None)
     progress("3D dam break: tuned; timed steps")
     # warm-up (graph capture)
     if args.warmup:
@@ -308,7 +321,8 @@ def main():
         "dtype": "fp32 pair math + fp64 positions/velocity/density" if args.precision == "f32"
         else "fp64",
         "data": "synthetic (seeded lattice, shuffled order)",
-        "config": {"workload": "dambreak_3d", "n_fluid": cfg["fluid_n"],
+        "config": {"workload": "dambreak_3d", "timed_from_step": max(1, args.start_step),
+                   "n_fluid": cfg["fluid_n"],
                    "n_boundary": cfg["boundary_n"], "dx": args.dx, "kernel": "WendlandC2",
                    "h_over_dx": 1.3, "boundary": "3-layer DynamicBoundarySystem x5",
                    "pairs_per_step": pairs, "graph": bool(args.graph)},
