@@ -31,7 +31,7 @@ using namespace gr;
 // device address hipGetSymbolAddress returns, taken once per device in
 // gr_ctx_create with that device current.
 //   keys, vals, keys2, vals2  4 × GR_STATIC_SORT_MAXN × 4 B
-//   radix histogram           radix_hist_entries(GR_STATIC_SORT_MAXN) × 4 B
+//   radix scratch             radix_scratch_bytes(GR_STATIC_SORT_MAXN)
 //   per-system key bases      GR_STATIC_MAXSYS × 4 B
 //   the sorted system's list and block table, every real system's blocks
 // Every call on a GPU shares that GPU's array: calls that overlap on one
@@ -46,9 +46,10 @@ constexpr size_t kScrVals = kScrKeys + (size_t)GR_STATIC_SORT_MAXN * 4;
 constexpr size_t kScrKeys2 = kScrVals + (size_t)GR_STATIC_SORT_MAXN * 4;
 constexpr size_t kScrVals2 = kScrKeys2 + (size_t)GR_STATIC_SORT_MAXN * 4;
 constexpr size_t kScrHist = kScrVals2 + (size_t)GR_STATIC_SORT_MAXN * 4;
-// 256 entries per tile of 2048 keys plus 256 (radix_hist_entries)
-constexpr size_t kScrHistEntries = 256 * ((size_t)GR_STATIC_SORT_MAXN / 2048 + 1) + 256;
-constexpr size_t kScrBase = kScrHist + kScrHistEntries * 4;
+// radix scratch: 4 KB of digit totals, then two arrays of 256 tile counts of
+// 4 B per tile of 4096 keys (radix_scratch_bytes)
+constexpr size_t kScrHistBytes = 4096 + 2 * ((size_t)GR_STATIC_SORT_MAXN / 4096 + 1) * 256 * 4;
+constexpr size_t kScrBase = kScrHist + kScrHistBytes;
 constexpr size_t kScrList = kScrBase + (size_t)GR_STATIC_MAXSYS * 4;
 constexpr size_t kScrBlkSys = kScrList + 256;
 constexpr size_t kScrBlkAll = kScrBlkSys + ((size_t)GR_STATIC_SORT_MAXN / GR_BLOCK + 1) * 8;
@@ -670,7 +671,9 @@ void commit(gr_ctx *c) {
     c->vals.alloc(sc * 4);
     c->keys2.alloc(sc * 4);
     c->vals2.alloc(sc * 4);
-    c->hist.alloc((size_t)radix_hist_entries((int)sc) * 4);
+    // the radix sort's tile counts start at zero (each sort leaves them so)
+    c->hist.alloc(radix_scratch_bytes((int)sc));
+    HIPCHK(hipMemsetAsync(c->hist.p, 0, c->hist.bytes, c->stream));
     c->sort_cap = sc;
   }
   if (ncand + 1 > c->cand_cap) {
@@ -1624,7 +1627,8 @@ void phase_sort_real(gr_ctx *c, Plan &P) {
   launch_key_write(s, xs, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), c->d_st.as<StepSys>(),
                    P.blk_real.as<Blk>(), P.nblk_real, P.d_bbase_real.as<int>(), c->D,
                    c->d_kmin.as<int>(), c->d_krange.as<unsigned>(), c->d_kbase.as<unsigned>(),
-                   c->keys.as<unsigned>(), c->vals.as<int>(), c->d_flags.as<int>());
+                   c->keys.as<unsigned>(), c->vals.as<int>(), c->d_flags.as<int>(),
+                   c->hist.as<unsigned>());
   launch_radix_sort(s, c->keys.as<unsigned>(), c->vals.as<int>(), c->keys2.as<unsigned>(),
                     c->vals2.as<int>(), c->hist.as<unsigned>(), P.n_real, c->d_flags.as<int>());
   const Blk *arb;
@@ -1632,8 +1636,8 @@ void phase_sort_real(gr_ctx *c, Plan &P) {
   all_real(c, arb, arn);
   Pool<R> A = pool_view<R>(c, c->cur), B = pool_view<R>(c, 1 - c->cur);
   launch_permute_real<R>(s, A, B, c->d_sd.as<SysDesc>(), arb, arn,
-                         P.d_bbase_real.as<int>(), c->vals.as<int>(), c->d_flags.as<int>(),
-                         c->vadj_any ? 1 : 0, c->D);
+                         P.d_bbase_real.as<int>(), c->vals.as<int>(), c->vals2.as<int>(),
+                         c->d_flags.as<int>(), c->vadj_any ? 1 : 0, c->D);
   c->cur = 1 - c->cur;
 }
 
@@ -1668,7 +1672,8 @@ void phase_ghosts(gr_ctx *c, Plan &P, bool generate, bool sort) {
     launch_key_write(s, gx, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), c->d_st.as<StepSys>(),
                      P.blk_ghost.as<Blk>(), P.nblk_ghost, P.d_bbase_ghost.as<int>(), c->D,
                      c->d_kmin.as<int>(), c->d_krange.as<unsigned>(), c->d_kbase.as<unsigned>(),
-                     c->keys.as<unsigned>(), c->vals.as<int>(), c->d_flags.as<int>());
+                     c->keys.as<unsigned>(), c->vals.as<int>(), c->d_flags.as<int>(),
+                     c->hist.as<unsigned>());
     launch_radix_sort(s, c->keys.as<unsigned>(), c->vals.as<int>(), c->keys2.as<unsigned>(),
                       c->vals2.as<int>(), c->hist.as<unsigned>(), P.n_ghost, c->d_flags.as<int>());
   } else {
@@ -1677,7 +1682,8 @@ void phase_ghosts(gr_ctx *c, Plan &P, bool generate, bool sort) {
   launch_permute_ghost(s, gx, c->gidx_s.as<int>(), c->gbnd_s.as<int>(),
                        xpw_pool(c, c->cur), A.gidx, A.gbnd, c->d_sd.as<SysDesc>(),
                        c->d_n.as<int>(), P.blk_ghost.as<Blk>(), P.nblk_ghost,
-                       P.d_bbase_ghost.as<int>(), c->vals.as<int>(), c->d_flags.as<int>(), c->D);
+                       P.d_bbase_ghost.as<int>(), c->vals.as<int>(), c->vals2.as<int>(),
+                       c->d_flags.as<int>(), c->D);
 }
 
 // Host record of which cell arrays the plan's grid build fills.
@@ -2093,6 +2099,8 @@ void do_steps(gr_ctx *c, const std::vector<int> &integs, int nsteps) {
     HIPCHK(hipStreamSynchronize(c->stream));
   }
   if (flags[1]) {
+    if (flags[1] & 16)
+      throw GrError(GR_ERR_HIP, "device radix sort: the tile counts did not place every key");
     if (flags[1] & 4)
       throw GrError(GR_ERR_ARGUMENT,
                     "XSPHPfn with a ghost partner: more ghosts than source particles "
@@ -2987,6 +2995,12 @@ int gr_sort(gr_ctx *c, int sys, double cutoff) {
         throw GrError(GR_ERR_ARGUMENT, "non-finite position: its cell key floor(x / cutoff) is undefined");
       throw GrError(GR_ERR_CAPACITY, "cell-key range exceeds 32 bits");
     }
+    if (flags[1] & 16) {
+      int z[4] = {0, 0, 0, 0};
+      HIPCHK(hipMemcpyAsync(c->d_flags.p, z, sizeof(z), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      throw GrError(GR_ERR_HIP, "device radix sort: the tile counts did not place every key");
+    }
   });
 }
 
@@ -3006,8 +3020,8 @@ int gr_sort_static(gr_ctx *c, int sys, double cutoff) {
          "scratch limit of " + std::to_string(GR_STATIC_SORT_MAXN) + " (use gr_sort)");
     need(ns <= GR_STATIC_MAXSYS, GR_ERR_CAPACITY,
          "gr_sort_static: more than " + std::to_string(GR_STATIC_MAXSYS) + " systems (use gr_sort)");
-    need((size_t)radix_hist_entries((int)cap) <= kScrHistEntries, GR_ERR_CAPACITY,
-         "gr_sort_static: histogram exceeds the static scratch");
+    need(radix_scratch_bytes((int)cap) <= kScrHistBytes, GR_ERR_CAPACITY,
+         "gr_sort_static: radix scratch exceeds the static scratch");
     // tables (host vectors; device memory is only the static scratch)
     std::vector<Blk> bsys, ball;
     add_blocks(bsys, sys, cap, GR_BLOCK);
@@ -3049,13 +3063,13 @@ int gr_sort_static(gr_ctx *c, int sys, double cutoff) {
     launch_key_write(s, xs, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), c->d_st.as<StepSys>(), d_bsys,
                      (int)bsys.size(), d_bbase, c->D, c->d_kmin.as<int>(),
                      c->d_krange.as<unsigned>(), c->d_kbase.as<unsigned>(), keys, vals,
-                     c->d_flags.as<int>());
+                     c->d_flags.as<int>(), hist);
     launch_radix_sort(s, keys, vals, keys2, vals2, hist, (int)cap, c->d_flags.as<int>());
     by_prec(c, [&](auto *tag) {
       using R = std::remove_pointer_t<decltype(tag)>;
       Pool<R> A = pool_view<R>(c, c->cur), B = pool_view<R>(c, 1 - c->cur);
       launch_permute_real<R>(s, A, B, c->d_sd.as<SysDesc>(), d_ball, (int)ball.size(), d_bbase,
-                             vals, c->d_flags.as<int>(), c->vadj_any ? 1 : 0, c->D);
+                             vals, vals2, c->d_flags.as<int>(), c->vadj_any ? 1 : 0, c->D);
     });
     c->cur = 1 - c->cur;
     // as gr_sort: a non-finite position or a key range of 32 bits or more is
@@ -3073,6 +3087,12 @@ int gr_sort_static(gr_ctx *c, int sys, double cutoff) {
       if (sf & GR_FAULT_NONFINITE)
         throw GrError(GR_ERR_ARGUMENT, "non-finite position: its cell key floor(x / cutoff) is undefined");
       throw GrError(GR_ERR_CAPACITY, "cell-key range exceeds 32 bits");
+    }
+    if (flags[1] & 16) {
+      int z[4] = {0, 0, 0, 0};
+      HIPCHK(hipMemcpyAsync(c->d_flags.p, z, sizeof(z), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+      throw GrError(GR_ERR_HIP, "device radix sort: the tile counts did not place every key");
     }
   });
 }

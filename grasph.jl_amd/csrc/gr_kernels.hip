@@ -75,6 +75,62 @@ __device__ inline int key_comp(double x, double cutoff) {
 }
 
 // ---------------------------------------------------------------------------
+// LSD radix sort, 8-bit digits, stable: two kernels per digit pass.
+//
+// The keys are cut into tiles of OS_TILE.  Each pass needs, for every tile
+// and digit, the number of keys with that digit in the tiles before it.  The
+// counts of pass p's tiles are taken by the kernel that writes pass p's
+// input: k_key_write for pass 0, and pass p-1's scatter for pass p, which
+// knows where each key lands and so which tile of the next pass holds it.
+// Both count runs of equal (tile, digit) along a wavefront with one atomic
+// per run (the keys are nearly in order: a wavefront holds few runs).
+// k_radix_scan turns each digit's column of tile counts into exclusive
+// prefixes and the digit's total; k_radix_scatter ranks its tile's keys per
+// digit (stable), adds the prefixes and the digit starts, and scatters.
+//
+// Scratch (radix_scratch_bytes): digit totals [4][256], then two count
+// arrays [tiles][256] used by alternate passes.  Every reader of a tile's
+// row zeroes it, so the arrays are zero again after each sort (and are
+// zeroed when allocated).
+//
+// Passes above the top bit of the largest key keep the order and are
+// skipped; flags[3] is the number of passes that run, so the sorted pairs
+// end in (keys2, vals2) when it is odd (radix_sorted_vals).  The whole sort
+// is skipped when flags[0] == 0: the keys are already in order.
+// ---------------------------------------------------------------------------
+
+#ifndef OS_ITEMS
+#define OS_ITEMS 16
+#endif
+#define OS_WAVE_KEYS (64 * OS_ITEMS)
+#define OS_TILE (GR_BLOCK * OS_ITEMS)
+#define RS_TOT 0      // words: digit totals [4][256]
+#define RS_CNT 1024   // words: tile counts [2][tiles][256]
+
+__device__ inline int radix_passes(unsigned bound) {
+  // passes whose digit is not zero for some key below `bound`
+  if (bound <= 1u) return 0;
+  const int bits = 32 - __clz((int)(bound - 1u));
+  return (bits + 7) >> 3;
+}
+
+// Add the lanes' bins to `cnt`: one atomic for each run of equal bins along
+// the valid lanes, made by the run's first lane.
+__device__ inline void radix_add_runs(unsigned bin, bool valid, unsigned *cnt) {
+  const int lane = threadIdx.x & 63;
+  const unsigned prev = __shfl_up(bin, 1, 64);
+  const unsigned long long V = __ballot(valid);
+  const bool prev_valid = lane > 0 && ((V >> (lane - 1)) & 1ull);
+  const bool head = valid && (!prev_valid || prev != bin);
+  const unsigned long long H = __ballot(head);
+  if (head) {
+    const unsigned long long ends = (H | ~V) & (lane == 63 ? 0ull : (~0ull << (lane + 1)));
+    const int next = ends ? __ffsll((long long)ends) - 1 : 64;
+    atomicAdd(&cnt[bin], (unsigned)(next - lane));
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Keys: per-system key range and position bounding box
 // ---------------------------------------------------------------------------
 
@@ -279,6 +335,7 @@ __global__ void k_key_base(const int *list, int nlist, int D, int pad, const int
     if (carry >= 0xFFFFFFFFull) flags[1] |= 1;
     // every key is below this bound: radix passes above its top bit are identities
     flags[2] = (int)(unsigned)(carry < 0xFFFFFFFFull ? carry : 0xFFFFFFFFull);
+    flags[3] = radix_passes((unsigned)flags[2]);
   }
 }
 
@@ -301,7 +358,7 @@ template <int D>
 __global__ void k_key_write(XP xs, const SysDesc *sd, const int *n_dev,
                             const StepSys *st, const Blk *blk, const int *bbase,
                             const int *kmin, const unsigned *krange, const unsigned *kbase,
-                            unsigned *keys, int *vals, int *flags) {
+                            unsigned *keys, int *vals, int *flags, unsigned *rscr) {
   const Blk b = blk[blockIdx.x];
   const int s = b.item;
   const int lane = threadIdx.x & 63;
@@ -340,42 +397,29 @@ __global__ void k_key_write(XP xs, const SysDesc *sd, const int *n_dev,
     keys[p] = key;
     vals[p] = (int)gi;
   }
-}
-
-// ---------------------------------------------------------------------------
-// LSD radix sort, 8-bit digits, stable.  Skipped entirely (early return)
-// when flags[0] == 0: the keys are already in order.
-// ---------------------------------------------------------------------------
-
-#define RADIX_ITEMS 8
-#define RADIX_TILE (GR_BLOCK * RADIX_ITEMS)
-
-// A pass whose digit is zero for every key (all keys < 2^shift) keeps the
-// order: its histogram and scan are skipped and its scatter is a plain copy.
-__device__ inline bool radix_pass_idle(const int *flags, int shift) {
-  const unsigned bound = (unsigned)flags[2];
-  return bound != 0u && ((bound - 1u) >> shift) == 0u;
-}
-
-__global__ void k_radix_hist(const unsigned *keys, int N, int shift, unsigned *hist, int nblk,
-                             const int *flags) {
-  if (flags[0] == 0 || radix_pass_idle(flags, shift)) return;
-  __shared__ unsigned h[256];
-  h[threadIdx.x] = 0;
-  __syncthreads();
-  const int base = blockIdx.x * RADIX_TILE;
-  for (int r = 0; r < RADIX_ITEMS; ++r) {
-    int e = base + r * GR_BLOCK + threadIdx.x;
-    if (e < N) atomicAdd(&h[(keys[e] >> shift) & 255u], 1u);
+  // the radix sort's pass-0 tile counts (its entries are the slots below cap)
+  if (flags[3] > 0) {
+    const unsigned p = (unsigned)(bbase[s] + i);
+    radix_add_runs((p / OS_TILE) * 256u + (key & 255u), i < cap, rscr + RS_CNT);
   }
-  __syncthreads();
-  hist[threadIdx.x * nblk + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of each digit's column of per-block counts (one wavefront
-// per digit, all 256 in parallel): hist[d·nblk + b] becomes the count of
-// digit d in blocks before b, and hist[256·nblk + d] the digit's total.  The
-// scatter adds the exclusive prefix of the totals.
+// The radix sort's passes (described above the key kernels).
+__device__ inline unsigned block_excl_scan256(unsigned v, unsigned *wsum) {
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  unsigned inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const unsigned u = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += u;
+  }
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  unsigned off = 0;
+  for (int q = 0; q < w; ++q) off += wsum[q];
+  return off + inc - v;
+}
+
 __device__ inline unsigned wave_incl_scan(unsigned v, int lane) {
 #pragma unroll
   for (int o = 1; o < 64; o <<= 1) {
@@ -385,89 +429,145 @@ __device__ inline unsigned wave_incl_scan(unsigned v, int lane) {
   return v;
 }
 
-__global__ void __launch_bounds__(64) k_radix_scan(unsigned *hist, int nblk, int shift,
-                                                   const int *flags) {
-  if (flags[0] == 0 || radix_pass_idle(flags, shift)) return;
-  const int lane = threadIdx.x;
-  unsigned *col = hist + (size_t)blockIdx.x * nblk;
-  unsigned carry = 0;
-  for (int r = 0; r < nblk; r += 64) {
-    const int k = r + lane;
-    const unsigned c = k < nblk ? col[k] : 0u;
-    const unsigned inc = wave_incl_scan(c, lane);
-    if (k < nblk) col[k] = carry + inc - c;
-    carry += __shfl(inc, 63, 64);
-  }
-  if (lane == 0) hist[(size_t)256 * nblk + blockIdx.x] = carry;
-}
-
-// Stable scatter: element e of the tile is ranked after every earlier
-// element with the same digit (round-major, then lane order), using 64-lane
-// ballots to find equal digits within a wave.
-__global__ void k_radix_scatter(const unsigned *kin, const int *vin, unsigned *kout, int *vout,
-                                int N, int shift, const unsigned *hist, int nblk,
-                                const int *flags) {
-  if (flags[0] == 0) return;
-  if (radix_pass_idle(flags, shift)) {  // identity pass: copy the tile
-    const int base = blockIdx.x * RADIX_TILE;
-    for (int r = 0; r < RADIX_ITEMS; ++r) {
-      const int e = base + r * GR_BLOCK + threadIdx.x;
-      if (e < N) {
-        kout[e] = kin[e];
-        vout[e] = vin[e];
-      }
-    }
+// One wavefront per digit: the digit's tile counts become exclusive
+// prefixes in place, and its total goes to the totals of the pass.
+__global__ void __launch_bounds__(GR_BLOCK) k_radix_scan(int N, int pass, unsigned *scr,
+                                                         const int *flags) {
+  if (pass >= flags[3]) return;
+  const int ntile = (N + OS_TILE - 1) / OS_TILE;
+  const int lane = threadIdx.x & 63;
+  const int d = blockIdx.x * (GR_BLOCK / 64) + (threadIdx.x >> 6);
+  unsigned *cnt = scr + RS_CNT + (size_t)(pass & 1) * ntile * 256 + d;
+  if (flags[0] == 0) {
+    // keys already in order: nothing runs, so clear what k_key_write counted
+    if (pass == 0)
+      for (int j = lane; j < ntile; j += 64) cnt[(size_t)j * 256] = 0u;
     return;
   }
-  __shared__ unsigned run[256];
+  unsigned carry = 0;
+  for (int j0 = 0; j0 < ntile; j0 += 64 * 8) {
+    unsigned c[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int j = j0 + k * 64 + lane;
+      c[k] = j < ntile ? cnt[(size_t)j * 256] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int j = j0 + k * 64 + lane;
+      const unsigned inc = wave_incl_scan(c[k], lane);
+      if (j < ntile) cnt[(size_t)j * 256] = carry + inc - c[k];
+      carry += __shfl(inc, 63, 64);
+    }
+  }
+  if (lane == 0) scr[RS_TOT + pass * 256 + d] = carry;
+}
+
+// Stable scatter of one tile: wavefront w holds the tile's keys
+// [w·OS_WAVE_KEYS, (w+1)·OS_WAVE_KEYS) in rounds of 64, and a key's rank
+// among the wavefront's keys with its digit counts the earlier rounds, then
+// the lower lanes.  A round whose digits never decrease along the lanes holds
+// each digit as one run (the common case for keys nearly in order); other
+// rounds match digits bit by bit with ballots.
+__global__ void __launch_bounds__(GR_BLOCK)
+    k_radix_scatter(const unsigned *__restrict__ kin, const int *__restrict__ vin,
+                    unsigned *__restrict__ kout, int *__restrict__ vout, int N, int pass,
+                    unsigned *scr, int *flags) {
+  const int npass = flags[3];
+  if (flags[0] == 0 || pass >= npass) return;
+  const int shift = 8 * pass;
+  // digit bits that vary: the top pass may use fewer than 8
+  const int bits = 32 - __clz((int)((unsigned)flags[2] - 1u));
+  const int nb = bits - shift < 8 ? bits - shift : 8;
+  const int ntile = (N + OS_TILE - 1) / OS_TILE;
   __shared__ unsigned wcnt[GR_BLOCK / 64][256];
-  __shared__ unsigned wtot[GR_BLOCK / 64];
+  __shared__ unsigned wsum[GR_BLOCK / 64];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-  {  // digit t starts after the totals of the digits below it
-    const unsigned tot = hist[(size_t)256 * nblk + t];
-    const unsigned inc = wave_incl_scan(tot, lane);
-    if (lane == 63) wtot[w] = inc;
-    __syncthreads();
-    unsigned off = 0;
-    for (int q = 0; q < w; ++q) off += wtot[q];
-    run[t] = hist[t * nblk + blockIdx.x] + off + inc - tot;
+  const unsigned tile = blockIdx.x;
+  const long long e0 = (long long)tile * OS_TILE + w * OS_WAVE_KEYS + lane;
+  unsigned key[OS_ITEMS];
+  int val[OS_ITEMS];
+#pragma unroll
+  for (int r = 0; r < OS_ITEMS; ++r) {
+    const long long e = e0 + r * 64;
+    key[r] = e < N ? kin[e] : 0u;
+    val[r] = e < N ? vin[e] : 0;
   }
+  // this tile's start within digit t; the row is left zeroed
+  unsigned *row = scr + RS_CNT + (size_t)(pass & 1) * ntile * 256 + (size_t)tile * 256;
+  const unsigned ex = row[t];
+  row[t] = 0u;
+#pragma unroll
   for (int q = 0; q < GR_BLOCK / 64; ++q) wcnt[q][t] = 0;
-  __syncthreads();
-  const int base = blockIdx.x * RADIX_TILE;
-  const unsigned long long lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  for (int r = 0; r < RADIX_ITEMS; ++r) {
-    const int e = base + r * GR_BLOCK + t;
-    const bool valid = e < N;
-    unsigned key = valid ? kin[e] : 0u;
-    int val = valid ? vin[e] : 0;
-    unsigned dig = (key >> shift) & 255u;
-    unsigned long long match = __ballot(valid);
-    for (int bit = 0; bit < 8; ++bit) {
-      unsigned long long bal = __ballot((dig >> bit) & 1u);
-      match &= ((dig >> bit) & 1u) ? bal : ~bal;
-    }
-    const int rank = __popcll(match & lt_mask);
-    if (valid && rank == 0) wcnt[w][dig] = (unsigned)__popcll(match);
-    __syncthreads();
-    if (valid) {
-      unsigned pre = run[dig];
-      for (int q = 0; q < w; ++q) pre += wcnt[q][dig];
-      unsigned pos = pre + rank;
-      kout[pos] = key;
-      vout[pos] = val;
-    }
-    __syncthreads();
-    {
-      unsigned add = 0;
-      for (int q = 0; q < GR_BLOCK / 64; ++q) {
-        add += wcnt[q][t];
-        wcnt[q][t] = 0;
+  // global start of digit t: totals of the digits below it
+  const unsigned gstart = block_excl_scan256(scr[RS_TOT + pass * 256 + t], wsum);  // (synchronises)
+  const unsigned long long lt_mask = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  const unsigned long long le_mask = (2ull << lane) - 1ull;
+  unsigned rank[OS_ITEMS];
+#pragma unroll
+  for (int r = 0; r < OS_ITEMS; ++r) {
+    const bool valid = e0 + r * 64 < N;
+    const unsigned d = (key[r] >> shift) & 255u;
+    const unsigned dp = __shfl_up(d, 1, 64);
+    const unsigned long long V = __ballot(valid);
+    unsigned rk, run;
+    if (__ballot(valid && lane > 0 && d < dp) == 0ull) {
+      // equal digits are adjacent: runs start where the digit changes
+      const unsigned long long H = __ballot(valid && (lane == 0 || d != dp));
+      const int head = 63 - __clzll((long long)(H & le_mask));
+      const unsigned long long ends = (H | ~V) & ~le_mask;
+      const int next = ends ? __ffsll((long long)ends) - 1 : 64;
+      rk = (unsigned)(lane - head);
+      run = (unsigned)(next - lane);
+    } else {
+      unsigned long long match = V;
+      for (int b = 0; b < nb; ++b) {
+        const unsigned long long bal = __ballot((d >> b) & 1u);
+        match &= ((d >> b) & 1u) ? bal : ~bal;
       }
-      run[t] += add;
+      rk = (unsigned)__popcll(match & lt_mask);
+      run = (unsigned)__popcll(match);
     }
-    __syncthreads();
+    const unsigned before = wcnt[w][d];
+    rank[r] = before + rk;
+    if (valid && rk == 0) wcnt[w][d] = before + run;
   }
+  __syncthreads();
+  // output start of (wavefront q, digit t)
+  unsigned start = gstart + ex;
+#pragma unroll
+  for (int q = 0; q < GR_BLOCK / 64; ++q) {
+    const unsigned c = wcnt[q][t];
+    wcnt[q][t] = start;
+    start += c;
+  }
+  __syncthreads();
+  unsigned *cnext = scr + RS_CNT + (size_t)((pass + 1) & 1) * ntile * 256;
+  const bool count_next = pass + 1 < npass;
+  bool bad = false;
+#pragma unroll
+  for (int r = 0; r < OS_ITEMS; ++r) {
+    const bool valid = e0 + r * 64 < N;
+    const unsigned pos = wcnt[w][(key[r] >> shift) & 255u] + rank[r];
+    // the counts place every key below N; a position beyond it is
+    // reported, never written
+    const bool ok = valid && pos < (unsigned)N;
+    if (ok) {
+      kout[pos] = key[r];
+      vout[pos] = val[r];
+    }
+    bad = bad || (valid && !ok);
+    // the next pass's tile counts
+    if (count_next)
+      radix_add_runs((pos / OS_TILE) * 256u + ((key[r] >> (shift + 8)) & 255u), ok, cnext);
+  }
+  if (bad) atomicOr(&flags[1], 16);
+}
+
+// The buffer that holds the sorted values (see the sort's header).
+__device__ __forceinline__ const int *radix_sorted_vals(const int *vals, const int *vals2,
+                                                        const int *flags) {
+  return (flags[3] & 1) ? vals2 : vals;
 }
 
 // ---------------------------------------------------------------------------
@@ -477,7 +577,8 @@ __global__ void k_radix_scatter(const unsigned *kin, const int *vin, unsigned *k
 
 template <typename R, int D>
 __global__ void k_permute_real(Pool<R> A, Pool<R> B, const SysDesc *sd, const Blk *blk,
-                               const int *bbase, const int *vals, const int *flags, int vadj) {
+                               const int *bbase, const int *vals, const int *vals2,
+                               const int *flags, int vadj) {
   const Blk b = blk[blockIdx.x];
   const int s = b.item;
   const int i = b.first + threadIdx.x;
@@ -485,7 +586,7 @@ __global__ void k_permute_real(Pool<R> A, Pool<R> B, const SysDesc *sd, const Bl
   if (i >= S.cap) return;
   const int64_t dst = S.off + i;
   int64_t src = dst;
-  if (flags[0] != 0 && bbase[s] >= 0) src = vals[bbase[s] + i];
+  if (flags[0] != 0 && bbase[s] >= 0) src = radix_sorted_vals(vals, vals2, flags)[bbase[s] + i];
 #pragma unroll
   for (int d = 0; d < D; ++d) {
     B.x[d][dst] = A.x[d][src];
@@ -581,7 +682,8 @@ __global__ void k_ghost_scatter(XP x, XPW gx, int *gidx, int *gbnd, int *gnrm, c
 template <int D>
 __global__ void k_permute_ghost(XP gxs, const int *gidx_s, const int *gbnd_s, XPW x, int *gidx,
                                 int *gbnd, const SysDesc *sd, const int *n_dev, const Blk *blk,
-                                const int *bbase, const int *vals, const int *flags) {
+                                const int *bbase, const int *vals, const int *vals2,
+                                const int *flags) {
   const Blk b = blk[blockIdx.x];
   const int g = b.item;
   const int k = b.first + threadIdx.x;
@@ -589,7 +691,7 @@ __global__ void k_permute_ghost(XP gxs, const int *gidx_s, const int *gbnd_s, XP
   if (k >= n_dev[g]) return;
   const int64_t dst = G.off + k;
   int64_t src = dst;
-  if (flags[0] != 0) src = vals[bbase[g] + k];
+  if (flags[0] != 0) src = radix_sorted_vals(vals, vals2, flags)[bbase[g] + k];
 #pragma unroll
   for (int d = 0; d < D; ++d) x[d][dst] = gxs[d][src];
   gidx[dst] = gidx_s[src];
@@ -1570,52 +1672,49 @@ void launch_key_base(hipStream_t s, const int *list, int nlist, int D, int pad, 
 void launch_key_write(hipStream_t s, XP xs, const SysDesc *sd, const int *n, const StepSys *st,
                       const Blk *blk, int nblk, const int *bbase, int D, const int *kmin,
                       const unsigned *krange, const unsigned *kbase, unsigned *keys, int *vals,
-                      int *flags) {
+                      int *flags, unsigned *rscr) {
   if (nblk <= 0) return;
   if (D == 2)
     k_key_write<2><<<nblk, GR_BLOCK, 0, s>>>(xs, sd, n, st, blk, bbase, kmin, krange, kbase, keys,
-                                             vals, flags);
+                                             vals, flags, rscr);
   else
     k_key_write<3><<<nblk, GR_BLOCK, 0, s>>>(xs, sd, n, st, blk, bbase, kmin, krange, kbase, keys,
-                                             vals, flags);
+                                             vals, flags, rscr);
 }
 void launch_radix_sort(hipStream_t s, unsigned *keys, int *vals, unsigned *keys2, int *vals2,
-                       unsigned *hist, int N, const int *flags) {
+                       unsigned *rscr, int N, int *flags) {
   if (N <= 0) return;
-  const int nblk = (int)nb_of(N, RADIX_TILE);
-  unsigned *ki = keys, *ko = keys2;
-  int *vi = vals, *vo = vals2;
+  const unsigned ntile = nb_of(N, OS_TILE);
   for (int pass = 0; pass < 4; ++pass) {
-    const int shift = pass * 8;
-    k_radix_hist<<<nblk, GR_BLOCK, 0, s>>>(ki, N, shift, hist, nblk, flags);
-    k_radix_scan<<<256, 64, 0, s>>>(hist, nblk, shift, flags);
-    k_radix_scatter<<<nblk, GR_BLOCK, 0, s>>>(ki, vi, ko, vo, N, shift, hist, nblk, flags);
-    unsigned *tk = ki;
-    ki = ko;
-    ko = tk;
-    int *tv = vi;
-    vi = vo;
-    vo = tv;
+    const bool even = (pass & 1) == 0;
+    k_radix_scan<<<256 / (GR_BLOCK / 64), GR_BLOCK, 0, s>>>(N, pass, rscr, flags);
+    k_radix_scatter<<<ntile, GR_BLOCK, 0, s>>>(even ? keys : keys2, even ? vals : vals2,
+                                               even ? keys2 : keys, even ? vals2 : vals, N, pass,
+                                               rscr, flags);
   }
 }
-int radix_hist_entries(int N) { return 256 * (int)nb_of(N > 0 ? N : 1, RADIX_TILE) + 256; }
+size_t radix_scratch_bytes(int N) {
+  const size_t ntile = nb_of(N > 0 ? N : 1, OS_TILE);
+  return ((size_t)RS_CNT + 2 * ntile * 256) * 4;
+}
 
 template <typename R>
 void launch_permute_real(hipStream_t s, const Pool<R> &A, const Pool<R> &B, const SysDesc *sd,
                          const Blk *blk, int nblk, const int *bbase, const int *vals,
-                         const int *flags, int vadj, int D) {
+                         const int *vals2, const int *flags, int vadj, int D) {
   if (nblk <= 0) return;
   if (D == 2)
-    k_permute_real<R, 2><<<nblk, GR_BLOCK, 0, s>>>(A, B, sd, blk, bbase, vals, flags, vadj);
+    k_permute_real<R, 2><<<nblk, GR_BLOCK, 0, s>>>(A, B, sd, blk, bbase, vals, vals2, flags, vadj);
   else
-    k_permute_real<R, 3><<<nblk, GR_BLOCK, 0, s>>>(A, B, sd, blk, bbase, vals, flags, vadj);
+    k_permute_real<R, 3><<<nblk, GR_BLOCK, 0, s>>>(A, B, sd, blk, bbase, vals, vals2, flags, vadj);
 }
 template void launch_permute_real<float>(hipStream_t, const Pool<float> &, const Pool<float> &,
                                          const SysDesc *, const Blk *, int, const int *,
-                                         const int *, const int *, int, int);
+                                         const int *, const int *, const int *, int, int);
 template void launch_permute_real<double>(hipStream_t, const Pool<double> &,
                                           const Pool<double> &, const SysDesc *, const Blk *,
-                                          int, const int *, const int *, const int *, int, int);
+                                          int, const int *, const int *, const int *, const int *,
+                                          int, int);
 
 void launch_ghost_flags(hipStream_t s, XP x, const SysDesc *sd, const Bnd *bnd, const Blk *blk,
                         int nblk, const int *cand_off, int D, int *flags) {
@@ -1645,14 +1744,15 @@ void launch_ghost_scatter(hipStream_t s, XP x, XPW gx, int *gidx, int *gbnd, int
 }
 void launch_permute_ghost(hipStream_t s, XP gxs, const int *gidx_s, const int *gbnd_s, XPW x,
                           int *gidx, int *gbnd, const SysDesc *sd, const int *n, const Blk *blk,
-                          int nblk, const int *bbase, const int *vals, const int *flags, int D) {
+                          int nblk, const int *bbase, const int *vals, const int *vals2,
+                          const int *flags, int D) {
   if (nblk <= 0) return;
   if (D == 2)
     k_permute_ghost<2><<<nblk, GR_BLOCK, 0, s>>>(gxs, gidx_s, gbnd_s, x, gidx, gbnd, sd, n, blk,
-                                                 bbase, vals, flags);
+                                                 bbase, vals, vals2, flags);
   else
     k_permute_ghost<3><<<nblk, GR_BLOCK, 0, s>>>(gxs, gidx_s, gbnd_s, x, gidx, gbnd, sd, n, blk,
-                                                 bbase, vals, flags);
+                                                 bbase, vals, vals2, flags);
 }
 template <typename R>
 void launch_ghost_kin(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const Bnd *bnd,

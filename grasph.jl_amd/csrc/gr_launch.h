@@ -59,14 +59,20 @@ void launch_key_base(hipStream_t s, const int *list, int nlist, int D, int pad, 
 void launch_key_write(hipStream_t s, XP xs, const SysDesc *sd, const int *n,
                       const StepSys *st, const Blk *blk, int nblk, const int *bbase, int D,
                       const int *kmin, const unsigned *krange, const unsigned *kbase,
-                      unsigned *keys, int *vals, int *flags);
+                      unsigned *keys, int *vals, int *flags, unsigned *rscr);
+// Stable radix sort of (keys, vals) by key, N entries.  `rscr` is the sort's
+// scratch (radix_scratch_bytes(N), zeroed when allocated and left zeroed by
+// every sort); launch_key_write counts the first pass into it and must run
+// before it with the same scratch.  The sorted values end in vals when
+// flags[3] is even and in vals2 when it is odd; the permutations take both.
+// flags[1] bit 16: the counts did not place every key (not expected).
 void launch_radix_sort(hipStream_t s, unsigned *keys, int *vals, unsigned *keys2, int *vals2,
-                       unsigned *hist, int N, const int *flags);
-int radix_hist_entries(int N);
+                       unsigned *rscr, int N, int *flags);
+size_t radix_scratch_bytes(int N);
 template <typename R>
 void launch_permute_real(hipStream_t s, const Pool<R> &A, const Pool<R> &B, const SysDesc *sd,
                          const Blk *blk, int nblk, const int *bbase, const int *vals,
-                         const int *flags, int vadj, int D);
+                         const int *vals2, const int *flags, int vadj, int D);
 void launch_ghost_flags(hipStream_t s, XP x, const SysDesc *sd, const Bnd *bnd,
                         const Blk *blk, int nblk, const int *cand_off, int D, int *flags);
 void launch_scan(hipStream_t s, const int *in, int *out, int *bsum, int N);
@@ -77,7 +83,8 @@ void launch_ghost_scatter(hipStream_t s, XP x, XPW gx, int *gidx,
 void launch_permute_ghost(hipStream_t s, XP gxs, const int *gidx_s,
                           const int *gbnd_s, XPW x, int *gidx, int *gbnd,
                           const SysDesc *sd, const int *n, const Blk *blk, int nblk,
-                          const int *bbase, const int *vals, const int *flags, int D);
+                          const int *bbase, const int *vals, const int *vals2, const int *flags,
+                          int D);
 template <typename R>
 void launch_ghost_kin(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const Bnd *bnd,
                       const int *n, const Blk *blk, int nblk, int D);
