@@ -320,6 +320,7 @@ struct Plan {
   int sweep_block = GR_BLOCK;
   // integrated (non-virtual) real systems; virtual systems
   DBuf blk_int, blk_virt, blk_probe;
+  DBuf d_halfsys;  // per system: 1 if integrated by this plan (its half step can ride the sort)
   int nblk_int = 0, nblk_virt = 0, nblk_probe = 0;
   bool do_sort = true, do_ghosts = true, do_grid = true, do_step = true;
   // graph capture per A/B parity
@@ -1565,6 +1566,10 @@ Plan &get_plan(gr_ctx *c, const std::vector<int> &integs) {
       }
     }
     P->blk_int.upload(bi, c->stream);
+    std::vector<int> hs(std::max<size_t>(c->sys.size(), 1), 0);
+    for (int ig : integs)
+      for (int s : c->integ[ig].sys) hs[s] = 1;
+    P->d_halfsys.upload(hs, c->stream);
     P->blk_virt.upload(bv, c->stream);
     P->blk_probe.upload(bp, c->stream);
     P->nblk_int = (int)bi.size();
@@ -1610,8 +1615,10 @@ void all_real(gr_ctx *c, const Blk *&blk, int &nblk) {
 // Step phases
 // ---------------------------------------------------------------------------
 
+// half: also take the leapfrog half step of the plan's integrated systems
+// in the permutation (run_step, when it follows the sort directly in data).
 template <typename R>
-void phase_sort_real(gr_ctx *c, Plan &P) {
+void phase_sort_real(gr_ctx *c, Plan &P, bool half = false) {
   if (P.real_list.empty()) return;
   hipStream_t s = c->stream;
   const XP xs = xp_pool(c, c->cur);
@@ -1637,7 +1644,8 @@ void phase_sort_real(gr_ctx *c, Plan &P) {
   Pool<R> A = pool_view<R>(c, c->cur), B = pool_view<R>(c, 1 - c->cur);
   launch_permute_real<R>(s, A, B, c->d_sd.as<SysDesc>(), arb, arn,
                          P.d_bbase_real.as<int>(), c->vals.as<int>(), c->vals2.as<int>(),
-                         c->d_flags.as<int>(), c->vadj_any ? 1 : 0, c->D);
+                         c->d_flags.as<int>(), c->vadj_any ? 1 : 0, c->D,
+                         c->d_st.as<StepSys>(), half ? P.d_halfsys.as<int>() : nullptr);
   c->cur = 1 - c->cur;
 }
 
@@ -1919,7 +1927,11 @@ void run_step(gr_ctx *c, Plan &P, bool cnt, std::vector<hipEvent_t> *ev) {
     if (ev) HIPCHK(hipEventRecord((*ev)[q], s));
   };
   rec(0);
-  phase_sort_real<R>(c, P);
+  // the leapfrog half step changes v and ρ only, which nothing reads before
+  // it (ghost generation, grids and lists read positions): it rides the
+  // sort's permutation
+  const bool half_in_sort = P.scheme != GR_SCHEME_RK4 && !P.real_list.empty();
+  phase_sort_real<R>(c, P, half_in_sort);
   rec(1);
   phase_ghosts<R>(c, P, true, true);
   rec(2);
@@ -1970,7 +1982,8 @@ void run_step(gr_ctx *c, Plan &P, bool cnt, std::vector<hipEvent_t> *ev) {
     }
     launch_rk4_apply<R>(s, A, sd, st, P.blk_int.as<Blk>(), P.nblk_int, c->D, !P.vadj_live);
   } else {
-    launch_halfstep<R>(s, A, sd, st, P.blk_int.as<Blk>(), P.nblk_int, c->D, 0.5, 0);
+    if (!half_in_sort)
+      launch_halfstep<R>(s, A, sd, st, P.blk_int.as<Blk>(), P.nblk_int, c->D, 0.5, 0);
     pass();
     launch_fullstep<R>(s, A, sd, st, P.blk_int.as<Blk>(), P.nblk_int, c->D, !P.vadj_live);
   }

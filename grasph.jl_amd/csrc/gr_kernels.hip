@@ -575,10 +575,16 @@ __device__ __forceinline__ const int *radix_sorted_vals(const int *vals, const i
 // every real particle; identity where the system is not sorted this step.
 // ---------------------------------------------------------------------------
 
-template <typename R, int D>
+__device__ inline double eos_p(int kind, double rho, double rho0, double gamma, double c);
+
+// HALF: the leapfrog half step of the systems flagged in `halfsys` is done
+// here, on the values as they are moved (the arithmetic of k_halfstep with
+// mult ½): one pass over the particles instead of two.
+template <typename R, int D, bool HALF>
 __global__ void k_permute_real(Pool<R> A, Pool<R> B, const SysDesc *sd, const Blk *blk,
                                const int *bbase, const int *vals, const int *vals2,
-                               const int *flags, int vadj) {
+                               const int *flags, int vadj, const StepSys *st,
+                               const int *halfsys) {
   const Blk b = blk[blockIdx.x];
   const int s = b.item;
   const int i = b.first + threadIdx.x;
@@ -587,17 +593,44 @@ __global__ void k_permute_real(Pool<R> A, Pool<R> B, const SysDesc *sd, const Bl
   const int64_t dst = S.off + i;
   int64_t src = dst;
   if (flags[0] != 0 && bbase[s] >= 0) src = radix_sorted_vals(vals, vals2, flags)[bbase[s] + i];
+  const bool half = HALF && halfsys[s] != 0;
+  if (half) {
+    const StepSys T = st[s];
+    const double cf = 0.5 * T.dt;
 #pragma unroll
-  for (int d = 0; d < D; ++d) {
-    B.x[d][dst] = A.x[d][src];
-    B.v[d][dst] = A.v[d][src];
-    B.dvdt[d][dst] = A.dvdt[d][src];
-    if (vadj) B.vadj[d][dst] = A.vadj[d][src];
+    for (int d = 0; d < D; ++d) {
+      const double v = A.v[d][src];
+      const R a = A.dvdt[d][src];
+      B.x[d][dst] = A.x[d][src];
+      B.v[d][dst] = v;
+      B.vs[d][dst] = (R)(v + cf * (double)a);
+      B.dvdt[d][dst] = T.reset_force ? (R)S.sv[d] : a;
+      if (vadj) B.vadj[d][dst] = A.vadj[d][src];
+    }
+    const double rho = A.rho[src];
+    const R dr = A.drhodt[src];
+    const double rh = fma(cf, (double)dr, rho);
+    B.rho[dst] = rho;
+    B.rho_h[dst] = rh;
+    B.rhos[dst] = (R)rh;
+    B.drhodt[dst] = T.reset_force ? (R)S.srho : dr;
+    if (T.fuse_eos)
+      B.p[dst] = (R)eos_p(T.fuse_eos, rh, T.eos_rho0, T.eos_gamma, S.c);
+    else if (S.kind >= GR_SYS_FLUID)
+      B.p[dst] = A.p[src];
+  } else {
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+      B.x[d][dst] = A.x[d][src];
+      B.v[d][dst] = A.v[d][src];
+      B.dvdt[d][dst] = A.dvdt[d][src];
+      if (vadj) B.vadj[d][dst] = A.vadj[d][src];
+    }
+    B.rho[dst] = A.rho[src];
+    B.drhodt[dst] = A.drhodt[src];
+    if (S.kind >= GR_SYS_FLUID) B.p[dst] = A.p[src];
   }
-  B.rho[dst] = A.rho[src];
-  B.drhodt[dst] = A.drhodt[src];
   B.id[dst] = A.id[src];
-  if (S.kind >= GR_SYS_FLUID) B.p[dst] = A.p[src];
   if (S.kind >= GR_SYS_STRESS) {
 #pragma unroll
     for (int k = 0; k < GR_MAXNS; ++k)
@@ -1701,20 +1734,33 @@ size_t radix_scratch_bytes(int N) {
 template <typename R>
 void launch_permute_real(hipStream_t s, const Pool<R> &A, const Pool<R> &B, const SysDesc *sd,
                          const Blk *blk, int nblk, const int *bbase, const int *vals,
-                         const int *vals2, const int *flags, int vadj, int D) {
+                         const int *vals2, const int *flags, int vadj, int D, const StepSys *st,
+                         const int *halfsys) {
   if (nblk <= 0) return;
+  if (halfsys) {
+    if (D == 2)
+      k_permute_real<R, 2, true><<<nblk, GR_BLOCK, 0, s>>>(A, B, sd, blk, bbase, vals, vals2,
+                                                           flags, vadj, st, halfsys);
+    else
+      k_permute_real<R, 3, true><<<nblk, GR_BLOCK, 0, s>>>(A, B, sd, blk, bbase, vals, vals2,
+                                                           flags, vadj, st, halfsys);
+    return;
+  }
   if (D == 2)
-    k_permute_real<R, 2><<<nblk, GR_BLOCK, 0, s>>>(A, B, sd, blk, bbase, vals, vals2, flags, vadj);
+    k_permute_real<R, 2, false><<<nblk, GR_BLOCK, 0, s>>>(A, B, sd, blk, bbase, vals, vals2,
+                                                          flags, vadj, st, halfsys);
   else
-    k_permute_real<R, 3><<<nblk, GR_BLOCK, 0, s>>>(A, B, sd, blk, bbase, vals, vals2, flags, vadj);
+    k_permute_real<R, 3, false><<<nblk, GR_BLOCK, 0, s>>>(A, B, sd, blk, bbase, vals, vals2,
+                                                          flags, vadj, st, halfsys);
 }
 template void launch_permute_real<float>(hipStream_t, const Pool<float> &, const Pool<float> &,
                                          const SysDesc *, const Blk *, int, const int *,
-                                         const int *, const int *, const int *, int, int);
+                                         const int *, const int *, const int *, int, int,
+                                         const StepSys *, const int *);
 template void launch_permute_real<double>(hipStream_t, const Pool<double> &,
                                           const Pool<double> &, const SysDesc *, const Blk *,
                                           int, const int *, const int *, const int *, const int *,
-                                          int, int);
+                                          int, int, const StepSys *, const int *);
 
 void launch_ghost_flags(hipStream_t s, XP x, const SysDesc *sd, const Bnd *bnd, const Blk *blk,
                         int nblk, const int *cand_off, int D, int *flags) {
