@@ -1618,7 +1618,7 @@ void all_real(gr_ctx *c, const Blk *&blk, int &nblk) {
 // half: also take the leapfrog half step of the plan's integrated systems
 // in the permutation (run_step, when it follows the sort directly in data).
 template <typename R>
-void phase_sort_real(gr_ctx *c, Plan &P, bool half = false) {
+void phase_sort_real(gr_ctx *c, Plan &P, bool half = false, int pack = 0) {
   if (P.real_list.empty()) return;
   hipStream_t s = c->stream;
   const XP xs = xp_pool(c, c->cur);
@@ -1645,7 +1645,7 @@ void phase_sort_real(gr_ctx *c, Plan &P, bool half = false) {
   launch_permute_real<R>(s, A, B, c->d_sd.as<SysDesc>(), arb, arn,
                          P.d_bbase_real.as<int>(), c->vals.as<int>(), c->vals2.as<int>(),
                          c->d_flags.as<int>(), c->vadj_any ? 1 : 0, c->D,
-                         c->d_st.as<StepSys>(), half ? P.d_halfsys.as<int>() : nullptr);
+                         c->d_st.as<StepSys>(), half ? P.d_halfsys.as<int>() : nullptr, pack);
   c->cur = 1 - c->cur;
 }
 
@@ -1717,7 +1717,7 @@ void phase_grids(gr_ctx *c, Plan &P) {
 }
 
 template <typename R>
-void phase_stage(gr_ctx *c, Plan &P, int k, bool cnt, hipEvent_t *mid) {
+void phase_stage(gr_ctx *c, Plan &P, int k, bool cnt, hipEvent_t *mid, bool real_packed = false) {
   hipStream_t s = c->stream;
   StagePlan &S = *P.stages[k];
   Pool<R> A = pool_view<R>(c, c->cur);
@@ -1736,8 +1736,9 @@ void phase_stage(gr_ctx *c, Plan &P, int k, bool cnt, hipEvent_t *mid) {
   // force sweeps without δ-SPH or surface tension
   const bool inv = c_only && S.fam[FAM_FORCE].inv && S.fam[FAM_STRAIN].nblk == 0;
   auto pack = [&]() {
-    launch_pack<R>(s, A, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), P.blk_real.as<Blk>(),
-                   P.nblk_real, c->D, c_only, with_s, inv);
+    if (!real_packed)
+      launch_pack<R>(s, A, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), P.blk_real.as<Blk>(),
+                     P.nblk_real, c->D, c_only, with_s, inv);
     launch_pack<R>(s, A, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), P.blk_ghost.as<Blk>(),
                    P.nblk_ghost, c->D, c_only, with_s, inv);
   };
@@ -1931,7 +1932,19 @@ void run_step(gr_ctx *c, Plan &P, bool cnt, std::vector<hipEvent_t> *ev) {
   // it (ghost generation, grids and lists read positions): it rides the
   // sort's permutation
   const bool half_in_sort = P.scheme != GR_SCHEME_RK4 && !P.real_list.empty();
-  phase_sort_real<R>(c, P, half_in_sort);
+  // and so can stage 0's compact partner records of the real systems, when
+  // no updater or virtual system changes their fields in between (the
+  // choice of records follows phase_stage's)
+  int pack_in_sort = 0;
+  if (half_in_sort && sizeof(R) == 4 && !P.stages.empty() && P.nblk_virt == 0) {
+    const StagePlan &S0 = *P.stages[0];
+    const bool c_only = S0.fam[FAM_INTERP].nblk == 0 && S0.fam[FAM_COUNT].nblk == 0;
+    const bool inv = c_only && S0.fam[FAM_FORCE].inv && S0.fam[FAM_STRAIN].nblk == 0;
+    if (c_only && !S0.fam[FAM_FORCE].cauchy && S0.nblk_upd == 0 &&
+        S0.fam[FAM_FORCE].nblk + S0.fam[FAM_STRAIN].nblk > 0)
+      pack_in_sort = inv ? 2 : 1;
+  }
+  phase_sort_real<R>(c, P, half_in_sort, pack_in_sort);
   rec(1);
   phase_ghosts<R>(c, P, true, true);
   rec(2);
@@ -1968,7 +1981,8 @@ void run_step(gr_ctx *c, Plan &P, bool cnt, std::vector<hipEvent_t> *ev) {
     launch_virtual<R>(s, A, sd, st, c->d_n.as<int>(), P.blk_virt.as<Blk>(), P.nblk_virt, 0, c->D);
     rec(5);
     for (int k = 0; k < nst; ++k) {
-      phase_stage<R>(c, P, k, cnt, ev ? &(*ev)[6 + 2 * k] : nullptr);
+      phase_stage<R>(c, P, k, cnt, ev ? &(*ev)[6 + 2 * k] : nullptr,
+                     k == 0 && pack_in_sort != 0);
       rec(7 + 2 * k);
     }
   };

@@ -6,6 +6,8 @@
 // explicit fma() is written (the reference's @fastmath scalar axpy).
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "../../include/grasph_hip.h"
 #include "gr_device.h"
 #include "gr_launch.h"
@@ -579,8 +581,10 @@ __device__ inline double eos_p(int kind, double rho, double rho0, double gamma, 
 
 // HALF: the leapfrog half step of the systems flagged in `halfsys` is done
 // here, on the values as they are moved (the arithmetic of k_halfstep with
-// mult ½): one pass over the particles instead of two.
-template <typename R, int D, bool HALF>
+// mult ½): one pass over the particles instead of two.  PACK (FP32, with
+// HALF): their compact partner records too, as k_pack writes them from the
+// half-step state (1: ρ and p, 2: 1/ρ and p/ρ²).
+template <typename R, int D, bool HALF, int PACK = 0>
 __global__ void k_permute_real(Pool<R> A, Pool<R> B, const SysDesc *sd, const Blk *blk,
                                const int *bbase, const int *vals, const int *vals2,
                                const int *flags, int vadj, const StepSys *st,
@@ -597,13 +601,17 @@ __global__ void k_permute_real(Pool<R> A, Pool<R> B, const SysDesc *sd, const Bl
   if (half) {
     const StepSys T = st[s];
     const double cf = 0.5 * T.dt;
+    double xn[D];
+    R vsn[D];
 #pragma unroll
     for (int d = 0; d < D; ++d) {
       const double v = A.v[d][src];
       const R a = A.dvdt[d][src];
-      B.x[d][dst] = A.x[d][src];
+      xn[d] = A.x[d][src];
+      vsn[d] = (R)(v + cf * (double)a);
+      B.x[d][dst] = xn[d];
       B.v[d][dst] = v;
-      B.vs[d][dst] = (R)(v + cf * (double)a);
+      B.vs[d][dst] = vsn[d];
       B.dvdt[d][dst] = T.reset_force ? (R)S.sv[d] : a;
       if (vadj) B.vadj[d][dst] = A.vadj[d][src];
     }
@@ -614,10 +622,34 @@ __global__ void k_permute_real(Pool<R> A, Pool<R> B, const SysDesc *sd, const Bl
     B.rho_h[dst] = rh;
     B.rhos[dst] = (R)rh;
     B.drhodt[dst] = T.reset_force ? (R)S.srho : dr;
-    if (T.fuse_eos)
-      B.p[dst] = (R)eos_p(T.fuse_eos, rh, T.eos_rho0, T.eos_gamma, S.c);
-    else if (S.kind >= GR_SYS_FLUID)
-      B.p[dst] = A.p[src];
+    R pn = (R)0;
+    if (T.fuse_eos) {
+      pn = (R)eos_p(T.fuse_eos, rh, T.eos_rho0, T.eos_gamma, S.c);
+      B.p[dst] = pn;
+    } else if (S.kind >= GR_SYS_FLUID) {
+      pn = A.p[src];
+      B.p[dst] = pn;
+    } else {
+      pn = B.p[dst];  // kept in place, as k_pack would read it
+    }
+    if constexpr (PACK > 0 && sizeof(R) == 4) {
+      RecC cc;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        cc.xq[d] = d < D ? (int32_t)quant_of(xn[d], B.iqs) : 0;
+        cc.v[d] = d < D ? (float)vsn[d] : 0.0f;
+      }
+      const float rs = (float)(R)rh;
+      if constexpr (PACK == 2) {
+        const float ir = __builtin_amdgcn_rcpf(rs);
+        cc.rho = ir;
+        cc.p = pn * (ir * ir);
+      } else {
+        cc.rho = rs;
+        cc.p = pn;
+      }
+      B.rc[dst] = cc;
+    }
   } else {
 #pragma unroll
     for (int d = 0; d < D; ++d) {
@@ -1735,15 +1767,29 @@ template <typename R>
 void launch_permute_real(hipStream_t s, const Pool<R> &A, const Pool<R> &B, const SysDesc *sd,
                          const Blk *blk, int nblk, const int *bbase, const int *vals,
                          const int *vals2, const int *flags, int vadj, int D, const StepSys *st,
-                         const int *halfsys) {
+                         const int *halfsys, int pack) {
   if (nblk <= 0) return;
   if (halfsys) {
-    if (D == 2)
-      k_permute_real<R, 2, true><<<nblk, GR_BLOCK, 0, s>>>(A, B, sd, blk, bbase, vals, vals2,
-                                                           flags, vadj, st, halfsys);
-    else
-      k_permute_real<R, 3, true><<<nblk, GR_BLOCK, 0, s>>>(A, B, sd, blk, bbase, vals, vals2,
-                                                           flags, vadj, st, halfsys);
+    auto go = [&](auto dc, auto pc) {
+      constexpr int DD = decltype(dc)::value, PP = decltype(pc)::value;
+      k_permute_real<R, DD, true, PP><<<nblk, GR_BLOCK, 0, s>>>(A, B, sd, blk, bbase, vals,
+                                                                vals2, flags, vadj, st, halfsys);
+    };
+    using I2 = std::integral_constant<int, 2>;
+    using I3 = std::integral_constant<int, 3>;
+    using P0 = std::integral_constant<int, 0>;
+    using P1 = std::integral_constant<int, 1>;
+    using P2 = std::integral_constant<int, 2>;
+    if (sizeof(R) != 4) pack = 0;
+    if (D == 2) {
+      if (pack == 2) go(I2{}, P2{});
+      else if (pack == 1) go(I2{}, P1{});
+      else go(I2{}, P0{});
+    } else {
+      if (pack == 2) go(I3{}, P2{});
+      else if (pack == 1) go(I3{}, P1{});
+      else go(I3{}, P0{});
+    }
     return;
   }
   if (D == 2)
@@ -1756,11 +1802,11 @@ void launch_permute_real(hipStream_t s, const Pool<R> &A, const Pool<R> &B, cons
 template void launch_permute_real<float>(hipStream_t, const Pool<float> &, const Pool<float> &,
                                          const SysDesc *, const Blk *, int, const int *,
                                          const int *, const int *, const int *, int, int,
-                                         const StepSys *, const int *);
+                                         const StepSys *, const int *, int);
 template void launch_permute_real<double>(hipStream_t, const Pool<double> &,
                                           const Pool<double> &, const SysDesc *, const Blk *,
                                           int, const int *, const int *, const int *, const int *,
-                                          int, int, const StepSys *, const int *);
+                                          int, int, const StepSys *, const int *, int);
 
 void launch_ghost_flags(hipStream_t s, XP x, const SysDesc *sd, const Bnd *bnd, const Blk *blk,
                         int nblk, const int *cand_off, int D, int *flags) {

@@ -73,7 +73,8 @@ template <typename R>
 void launch_permute_real(hipStream_t s, const Pool<R> &A, const Pool<R> &B, const SysDesc *sd,
                          const Blk *blk, int nblk, const int *bbase, const int *vals,
                          const int *vals2, const int *flags, int vadj, int D,
-                         const StepSys *st = nullptr, const int *halfsys = nullptr);
+                         const StepSys *st = nullptr, const int *halfsys = nullptr,
+                         int pack = 0);
 void launch_ghost_flags(hipStream_t s, XP x, const SysDesc *sd, const Bnd *bnd,
                         const Blk *blk, int nblk, const int *cand_off, int D, int *flags);
 void launch_scan(hipStream_t s, const int *in, int *out, int *bsum, int N);
