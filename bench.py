@@ -169,18 +169,22 @@ def sweep_bytes(cfg, grids):
 
 
 def sort_bytes(cfg, key_cells, precision):
-    """Algorithmic bytes of the sort of one step: the key pass reads x (8D)
-    and writes a key (4), the ranking reads and writes a pool index (8), the
-    histogram and its scan touch 8 B per key cell, and the permutation reads
-    and writes every per-particle array once (2·S_state: x 8D, v 8D, ρ 8, id
-    4, dvdt and drhodt in the state precision, p for fluids)."""
+    """Algorithmic bytes of the sort phase of one step: the key pass reads x
+    (8D) and writes a key (4), the ranking reads and writes a pool index (8),
+    the histogram and its scan touch 8 B per key cell, and the permutation
+    reads and writes every per-particle array once (2·S_state: x 8D, v 8D,
+    ρ 8, id 4, dvdt and drhodt in the state precision, p for fluids).  The
+    leapfrog half step and stage 0's partner records are written by the same
+    permutation: the half-step shadows (v_s in the state precision, ρ_h 8,
+    ρ_s) and, in FP32, the 32-byte compact record of every particle."""
     D = 3
     r = 4 if precision == "f32" else 8
     nf, nb = cfg["fluid_n"], cfg["boundary_n"]
     n = nf + nb
     s_fluid = 8 * D + 8 * D + 8 + 4 + r * D + r + r
     s_bnd = 8 * D + 8 * D + 8 + 4 + r * D + r
-    return n * (8 * D + 4) + 8 * n + 8 * key_cells + 2 * (nf * s_fluid + nb * s_bnd)
+    half = n * (r * D + 8 + r) + (n * 32 if precision == "f32" else 0)
+    return (n * (8 * D + 4) + 8 * n + 8 * key_cells + 2 * (nf * s_fluid + nb * s_bnd) + half)
 
 
 def load_traffic():
@@ -196,6 +200,56 @@ def load_traffic():
         return float(d["traffic_bytes_per_step"])
     except Exception:
         return None
+
+
+# FP32 operations of one FluidPfn pair (S = 2, Wendland C2, artificial
+# viscosity, the records' 1/ρ and p/ρ²), counted from the functor: x_i − x_j
+# from the fixed-point records (3), r² (5), 1/r and q (3), dW/dq and the
+# gradient factor (7), the gradient (3), v_ij and v_ij·x_ij (8), μ (4), Π (7),
+# the pressure term (3), the momentum factor (2), its FP64 sums (6), v_ij·∇W
+# and the density sum (8).
+FLOP_PER_PAIR = 59
+VALU_CLOCK_GHZ = 2.4  # peak engine clock: a wave64 VALU instruction issues in 4 cycles
+SIMDS = 1024          # 256 CUs x 4 SIMDs
+
+
+def load_sq():
+    """SQ counters of the pair kernels from the committed rocprofv3 --pmc
+    summary (per dispatch of the timed steps), or None."""
+    p = ROOT / "profiles" / "sweep_dambreak3d_pmc.json"
+    if not p.exists():
+        return None
+    try:
+        return json.loads(p.read_text())["sq"]["kernels"]
+    except Exception:
+        return None
+
+
+def valu_block(sq, t_nlist_ms, t_sweep_ms, ordered_pairs):
+    """VALU view of the two pair kernels: instructions per wavefront and per
+    ordered pair (counters), their issue rate against one wave64 instruction
+    per 4 cycles per SIMD at the peak clock (this run's kernel times), and the
+    pair functor's FLOP rate against the FP32 vector peak."""
+    if not sq:
+        return None
+    out = {"source": "SQ_INSTS_VALU / SQ_WAVES of the committed --pmc pass "
+                     "(profiles/sweep_dambreak3d_pmc.json); kernel times of this run",
+           "clock_ghz": VALU_CLOCK_GHZ, "simds": SIMDS}
+    for key, t in (("k_nlist", t_nlist_ms), ("k_sweep_list", t_sweep_ms)):
+        k = next((v for n, v in sq.items() if key + "<" in n), None)
+        if not k or not t:
+            continue
+        insts = k["SQ_INSTS_VALU"]
+        out[key] = {"valu_insts_per_wave": k["valu_insts_per_wave"],
+                    "lane_insts_per_ordered_pair": insts * 64 / ordered_pairs,
+                    "issue_frac": insts * 4 / (SIMDS * t * 1e-3 * VALU_CLOCK_GHZ * 1e9),
+                    "wait_frac": k["wait_fraction_of_wave_cycles"]}
+    if t_sweep_ms:
+        tf = FLOP_PER_PAIR * ordered_pairs / (t_sweep_ms * 1e-3) / 1e12
+        out["pair_flop"] = {"flop_per_ordered_pair": FLOP_PER_PAIR, "achieved_tflops": tf,
+                            "peak_tflops": FP32_PEAK_TFLOPS, "frac": tf / FP32_PEAK_TFLOPS,
+                            "kernel": "k_sweep_list"}
+    return out
 
 
 def cpu_baseline(cfg_builder, kw, nsteps, pairs_per_step):
@@ -335,13 +389,18 @@ def main():
                      "algorithmic_bytes_per_step": bytes_sweep,
                      "launch_ms": sweep_ms,
                      "k_nlist_ms": phases[8], "k_sweep_list_ms": phases[5],
-                     "ordered_pairs": self_ordered + coupled},
+                     "ordered_pairs": self_ordered + coupled,
+                     "valu": valu_block(load_sq(), phases[8], phases[5],
+                                        self_ordered + coupled)},
         "sort_roofline": {"bound": "hbm", "achieved": sort_gbs, "peak": HBM_PEAK_GBS,
                           "unit": "GB/s",
                           "frac": sort_gbs / HBM_PEAK_GBS if sort_gbs else None,
                           "algorithmic_bytes_per_step": bytes_sort, "phase_ms": phases[0],
-                          "kernels": "key min/max, key write, LSD radix (4 x 8 bits), "
-                                     "permute of every per-particle array"},
+                          "kernels": "key min/max, key write (with the first pass's tile "
+                                     "counts), LSD radix (a scan and a scatter kernel per "
+                                     "8-bit digit), permutation of every per-particle array "
+                                     "with the leapfrog half step and stage 0's partner "
+                                     "records"},
         "cpu_baseline": cpu,
         "steps_per_s": 1e3 / ms_step,
         "phase_ms": {k: round(v, 4) for k, v in zip(
