@@ -352,7 +352,8 @@ def main():
     # the sort's keys span the fluid's cells of side 2h (its self grid)
     bytes_sort = sort_bytes(cfg, grids[0] if grids else 0, args.precision)
     sort_gbs = bytes_sort / (phases[0] * 1e-3) / 1e9 if phases[0] > 0 else None
-    traffic = load_traffic()
+    # the committed counter passes are of the FP32 kernels
+    traffic = load_traffic() if args.precision == "f32" else None
     progress(f"3D dam break: {el / args.steps * 1e3:.3f} ms/step; CPU baseline")
     cpu = cpu_baseline(g.configs.dambreak_3d, kw, args.cpu_steps, pairs)
     ens = (ensemble_leg(args.ensemble, args.ensemble_steps, args.ensemble_serial_steps,
@@ -390,8 +391,8 @@ def main():
                      "launch_ms": sweep_ms,
                      "k_nlist_ms": phases[8], "k_sweep_list_ms": phases[5],
                      "ordered_pairs": self_ordered + coupled,
-                     "valu": valu_block(load_sq(), phases[8], phases[5],
-                                        self_ordered + coupled)},
+                     "valu": valu_block(load_sq() if args.precision == "f32" else None,
+                                        phases[8], phases[5], self_ordered + coupled)},
         "sort_roofline": {"bound": "hbm", "achieved": sort_gbs, "peak": HBM_PEAK_GBS,
                           "unit": "GB/s",
                           "frac": sort_gbs / HBM_PEAK_GBS if sort_gbs else None,
