@@ -1,9 +1,12 @@
 """Continuous-batching simulation server and the plain serial baseline.
 
-Jobs (a scene builder, a step count, an arrival time) are served by one
-device context per spatial dimension, each driven by its own host thread so
-2D and 3D work overlaps on the GPU (ctypes releases the GIL inside the
-library calls).  A context loop:
+Jobs (a scene builder, a step count, an arrival time) are served by device
+contexts ("lanes"), one or more per spatial dimension, each driven by its own
+host thread so their work overlaps on the GPU (ctypes releases the GIL inside
+the library calls).  A job goes to the lane of its dimension with the fewest
+steps assigned so far: a batch advances at the pace of all its particles, so
+spreading long jobs over lanes keeps a late long job from sharing every step
+with all the others.  A context loop:
 
   1. admits every queued job that has arrived and whose estimated HBM
      footprint fits the shared budget: builds the scene, registers it on the
@@ -112,10 +115,11 @@ class _Budget:
 
 
 class Server:
-    """Continuous batching over one context per spatial dimension."""
+    """Continuous batching over `lanes` contexts per spatial dimension (an
+    int for every dimension, or a dict D -> count)."""
 
     def __init__(self, precision="f32", hbm_budget=200e9, quantum=50, graph=False,
-                 keep_results=False, session_factory=None, builders=2):
+                 keep_results=False, session_factory=None, builders=2, lanes=1):
         self.precision = precision
         self._make_session = session_factory or (lambda D: Session(D, precision))
         self.budget = _Budget(hbm_budget)
@@ -129,14 +133,20 @@ class Server:
         # steps, in retirement, and the number of shared calls
         self.stats = {}
         self.builders = max(1, int(builders))
+        self.lanes = lanes
         self._pool = None
 
-    def _session(self, D):
-        if D not in self.sessions:
-            s = self._make_session(D)
+    def _nlanes(self, D):
+        n = self.lanes.get(D, 1) if isinstance(self.lanes, dict) else self.lanes
+        return max(1, int(n))
+
+    def _session(self, key):
+        """The context of lane key = (D, lane)."""
+        if key not in self.sessions:
+            s = self._make_session(key[0])
             s.backend.set_graph_mode(self.graph)
-            self.sessions[D] = s
-        return self.sessions[D]
+            self.sessions[key] = s
+        return self.sessions[key]
 
     def _fail(self, job, t0, why, taken=True):
         job.meta["failed"] = why
@@ -177,13 +187,13 @@ class Server:
         job.admitted = time.perf_counter() - t0
         return cfg, h
 
-    def _loop(self, D, jobs, t0):
+    def _loop(self, key, jobs, t0):
         queue = sorted(jobs, key=lambda j: (j.arrival, j.jid))
         running = []  # (job, cfg, handle)
-        st = self.stats.setdefault(D, {"admit_s": 0.0, "step_s": 0.0, "retire_s": 0.0,
-                                       "calls": 0, "max_running": 0})
+        st = self.stats.setdefault(key, {"admit_s": 0.0, "step_s": 0.0, "retire_s": 0.0,
+                                         "calls": 0, "max_running": 0, "jobs": len(jobs)})
         try:
-            s = self._session(D)
+            s = self._session(key)
             be, sc = s.backend, s.scene
             while queue or running:
                 now = time.perf_counter() - t0
@@ -272,14 +282,19 @@ class Server:
     def run(self, jobs):
         """Serve `jobs` (arrival times in seconds from now); returns a report."""
         t0 = time.perf_counter()
-        by_dim = {}
-        for j in jobs:
-            by_dim.setdefault(j.ndims, []).append(j)
-        for D in by_dim:  # create contexts before the clock matters
-            self._session(D)
+        by_lane = {}
+        assigned = {}
+        for j in sorted(jobs, key=lambda j: (j.arrival, j.jid)):
+            keys = [(j.ndims, k) for k in range(self._nlanes(j.ndims))]
+            key = min(keys, key=lambda q: (assigned.get(q, 0), q[1]))
+            assigned[key] = assigned.get(key, 0) + j.steps
+            by_lane.setdefault(key, []).append(j)
+        for key in by_lane:  # create contexts before the clock matters
+            self._session(key)
         self._pool = ThreadPoolExecutor(max_workers=self.builders)
         t0 = time.perf_counter()
-        threads = [threading.Thread(target=self._loop, args=(D, js, t0)) for D, js in by_dim.items()]
+        threads = [threading.Thread(target=self._loop, args=(key, js, t0))
+                   for key, js in by_lane.items()]
         for t in threads:
             t.start()
         for t in threads:
@@ -287,7 +302,8 @@ class Server:
         self._pool.shutdown(wait=True)
         rep = report(jobs, time.perf_counter() - t0)
         rep["errors"] = [str(e) for e in self.errors]
-        rep["contexts"] = {f"{D}d": v for D, v in self.stats.items()}
+        rep["contexts"] = {(f"{D}d" if self._nlanes(D) == 1 else f"{D}d.{k}"): v
+                           for (D, k), v in self.stats.items()}
         return rep
 
 

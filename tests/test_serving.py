@@ -86,9 +86,26 @@ def test_server_finishes_every_job_and_matches_alone():
         for f, v in b.result.items():
             assert np.array_equal(a.result[f], v), (a.jid, f)
     # quanta never overshoot a job's step count and batch several jobs
-    calls = srv.sessions[2].backend.calls
+    calls = srv.sessions[(2, 0)].backend.calls
     assert max(len(hs) for hs, _ in calls) > 1
     assert all(n <= 4 for _, n in calls)
+
+
+def test_server_lanes_split_the_jobs_and_match_alone():
+    """Two contexts per dimension: each job goes to the lane with the fewest
+    steps assigned, both lanes step, and every result equals the job alone."""
+    S = M.serving
+    served = small_jobs(6)
+    srv = S.Server(quantum=4, keep_results=True, session_factory=session, lanes=2)
+    rep = srv.run(served)
+    assert rep["jobs"] == len(served) and rep["failed"] == 0
+    assert set(rep["contexts"]) == {"2d.0", "2d.1", "3d.0", "3d.1"}
+    assert all(srv.sessions[(2, k)].backend.calls for k in (0, 1))
+    alone = small_jobs(6)
+    S.run_plain(alone, keep_results=True, session_factory=session)
+    for a, b in zip(served, alone):
+        for f, v in b.result.items():
+            assert np.array_equal(a.result[f], v), (a.jid, f)
 
 
 def test_budget_limits_residency_and_rejects_oversized():
@@ -153,7 +170,7 @@ def test_failed_jobs_do_not_disturb_the_others():
     assert "build" in by[11].meta["failed"]
     assert by[12].meta.get("rejected")
     # the NaN job ran batched with healthy jobs before it failed
-    calls = srv.sessions[2].backend.calls
+    calls = srv.sessions[(2, 0)].backend.calls
     assert any(len(hs) > 1 for hs, _ in calls)
     alone = small_jobs()
     S.run_plain(alone, keep_results=True, session_factory=session)
