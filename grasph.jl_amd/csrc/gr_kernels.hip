@@ -422,47 +422,33 @@ __device__ inline unsigned block_excl_scan256(unsigned v, unsigned *wsum) {
   return off + inc - v;
 }
 
-__device__ inline unsigned wave_incl_scan(unsigned v, int lane) {
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const unsigned u = __shfl_up(v, o, 64);
-    if (lane >= o) v += u;
-  }
-  return v;
-}
-
-// One wavefront per digit: the digit's tile counts become exclusive
-// prefixes in place, and its total goes to the totals of the pass.
+// One workgroup per digit: the digit's tile counts become exclusive prefixes
+// in place (each thread sums a run of consecutive tiles, a workgroup scan
+// places the runs), and its total goes to the totals of the pass.
 __global__ void __launch_bounds__(GR_BLOCK) k_radix_scan(int N, int pass, unsigned *scr,
                                                          const int *flags) {
   if (pass >= flags[3]) return;
+  __shared__ unsigned wsum[GR_BLOCK / 64];
   const int ntile = (N + OS_TILE - 1) / OS_TILE;
-  const int lane = threadIdx.x & 63;
-  const int d = blockIdx.x * (GR_BLOCK / 64) + (threadIdx.x >> 6);
+  const int t = threadIdx.x, d = blockIdx.x;
   unsigned *cnt = scr + RS_CNT + (size_t)(pass & 1) * ntile * 256 + d;
+  const int per = (ntile + GR_BLOCK - 1) / GR_BLOCK;
+  const int j0 = t * per, j1 = j0 + per < ntile ? j0 + per : ntile;
   if (flags[0] == 0) {
     // keys already in order: nothing runs, so clear what k_key_write counted
     if (pass == 0)
-      for (int j = lane; j < ntile; j += 64) cnt[(size_t)j * 256] = 0u;
+      for (int j = j0; j < j1; ++j) cnt[(size_t)j * 256] = 0u;
     return;
   }
-  unsigned carry = 0;
-  for (int j0 = 0; j0 < ntile; j0 += 64 * 8) {
-    unsigned c[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int j = j0 + k * 64 + lane;
-      c[k] = j < ntile ? cnt[(size_t)j * 256] : 0u;
-    }
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int j = j0 + k * 64 + lane;
-      const unsigned inc = wave_incl_scan(c[k], lane);
-      if (j < ntile) cnt[(size_t)j * 256] = carry + inc - c[k];
-      carry += __shfl(inc, 63, 64);
-    }
+  unsigned sum = 0;
+  for (int j = j0; j < j1; ++j) sum += cnt[(size_t)j * 256];
+  unsigned run = block_excl_scan256(sum, wsum);
+  for (int j = j0; j < j1; ++j) {
+    const unsigned c = cnt[(size_t)j * 256];
+    cnt[(size_t)j * 256] = run;
+    run += c;
   }
-  if (lane == 0) scr[RS_TOT + pass * 256 + d] = carry;
+  if (t == GR_BLOCK - 1) scr[RS_TOT + pass * 256 + d] = run;
 }
 
 // Stable scatter of one tile: wavefront w holds the tile's keys
@@ -1752,7 +1738,7 @@ void launch_radix_sort(hipStream_t s, unsigned *keys, int *vals, unsigned *keys2
   const unsigned ntile = nb_of(N, OS_TILE);
   for (int pass = 0; pass < 4; ++pass) {
     const bool even = (pass & 1) == 0;
-    k_radix_scan<<<256 / (GR_BLOCK / 64), GR_BLOCK, 0, s>>>(N, pass, rscr, flags);
+    k_radix_scan<<<256, GR_BLOCK, 0, s>>>(N, pass, rscr, flags);
     k_radix_scatter<<<ntile, GR_BLOCK, 0, s>>>(even ? keys : keys2, even ? vals : vals2,
                                                even ? keys2 : keys, even ? vals2 : vals, N, pass,
                                                rscr, flags);
