@@ -965,8 +965,11 @@ __global__ void __launch_bounds__(GR_BLOCK)
         const RecB<R> Bj = Bn;
         const RecC Cj = Cn;
         const RecS Sj = Sn;
+        // the next record is loaded whether or not there is a next partner
+        // (jn then still names this one): a conditional load would make the
+        // compiler keep a copy of the buffer for the path without it
         have = next(jn);
-        if (have) fetch();
+        fetch();
         double dx[D] = {};
         double r2 = 0.0;
         if constexpr (!CMP) {
