@@ -47,7 +47,8 @@ GR_WRAP_NONE, GR_WRAP_STATIC, GR_WRAP_DYNAMIC = range(3)
  GR_UPD_PRESCRIBED_V, GR_UPD_HOOKE) = range(9)
 GR_COPY_PLAIN, GR_COPY_HOUSEHOLDER = 0, 1
 GR_SCHEME_LEAPFROG, GR_SCHEME_RK4 = 0, 1
-GR_TUNE_NL_WIN, GR_TUNE_NL_SPAN, GR_TUNE_SWEEP_BLOCK, GR_TUNE_MAXNB, GR_TUNE_INV_RECORDS = range(5)
+(GR_TUNE_NL_WIN, GR_TUNE_NL_SPAN, GR_TUNE_SWEEP_BLOCK, GR_TUNE_MAXNB, GR_TUNE_INV_RECORDS,
+ GR_TUNE_FUSE_SORT) = range(6)
 
 
 class gr_pfn_desc(C.Structure):

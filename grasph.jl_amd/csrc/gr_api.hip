@@ -385,7 +385,7 @@ struct gr_ctx {
   int use_lists = 1;
   int list_maxnb = 0;  // mask entries kept per target (0: 48 in 3D, 24 in 2D)
   // launch shapes chosen by gr_tune_set (results do not depend on them)
-  int tune_win = 128, tune_span = 2, tune_block = GR_BLOCK, tune_inv = 1;
+  int tune_win = 128, tune_span = 2, tune_block = GR_BLOCK, tune_inv = 1, tune_fuse = 1;
   std::map<std::vector<int>, std::unique_ptr<Plan>> plans;
   // cached device tables derived from the layout
   DBuf allreal_blk;
@@ -1931,7 +1931,8 @@ void run_step(gr_ctx *c, Plan &P, bool cnt, std::vector<hipEvent_t> *ev) {
   // the leapfrog half step changes v and ρ only, which nothing reads before
   // it (ghost generation, grids and lists read positions): it rides the
   // sort's permutation
-  const bool half_in_sort = P.scheme != GR_SCHEME_RK4 && !P.real_list.empty();
+  const bool half_in_sort =
+      c->tune_fuse && P.scheme != GR_SCHEME_RK4 && !P.real_list.empty();
   // and so can stage 0's compact partner records of the real systems, when
   // no updater or virtual system changes their fields in between (the
   // choice of records follows phase_stage's)
@@ -2927,6 +2928,10 @@ int gr_tune_set(gr_ctx *c, int param, int value) {
         need(value == 0 || value == 1, GR_ERR_ARGUMENT, "GR_TUNE_INV_RECORDS must be 0 or 1");
         c->tune_inv = value;
         break;
+      case GR_TUNE_FUSE_SORT:
+        need(value == 0 || value == 1, GR_ERR_ARGUMENT, "GR_TUNE_FUSE_SORT must be 0 or 1");
+        c->tune_fuse = value;
+        break;
       default:
         throw GrError(GR_ERR_ARGUMENT, "unknown tuning parameter");
     }
@@ -2943,6 +2948,7 @@ int gr_tune_get(gr_ctx *c, int param, int *value) {
       case GR_TUNE_SWEEP_BLOCK: *value = c->tune_block; break;
       case GR_TUNE_MAXNB: *value = c->list_maxnb > 0 ? c->list_maxnb : (c->D == 3 ? 48 : 24); break;
       case GR_TUNE_INV_RECORDS: *value = c->tune_inv; break;
+      case GR_TUNE_FUSE_SORT: *value = c->tune_fuse; break;
       default: throw GrError(GR_ERR_ARGUMENT, "unknown tuning parameter");
     }
   });

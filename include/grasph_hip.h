@@ -316,11 +316,15 @@ enum {
   GR_TUNE_MAXNB = 3,        /* list entries kept per target before it walks
                                its cells directly: 0 = default (48 in 3D,
                                24 in 2D) or 4..1024                         */
-  GR_TUNE_INV_RECORDS = 4   /* 1 (default): in FP32 stages whose sweeps are
+  GR_TUNE_INV_RECORDS = 4,  /* 1 (default): in FP32 stages whose sweeps are
                                all FluidPfn S = 2 without δ-SPH or surface
                                tension, the partner records hold 1/ρ and
                                p/ρ² (the same FP32 operations, done once per
                                particle); 0: ρ and p                        */
+  GR_TUNE_FUSE_SORT = 5     /* 1 (default): a leapfrog step takes its half
+                               step, and stage 0's partner records when no
+                               updater or virtual system intervenes, in the
+                               sort's permutation; 0: separate kernels      */
 };
 int gr_tune_set(gr_ctx *ctx, int param, int value);
 int gr_tune_get(gr_ctx *ctx, int param, int *value);

@@ -40,7 +40,8 @@ def test_every_launch_shape_gives_the_same_state(builder, kw, D):
               ((A.GR_TUNE_NL_WIN, 64), (A.GR_TUNE_NL_SPAN, 1)),
               ((A.GR_TUNE_NL_WIN, 256), (A.GR_TUNE_NL_SPAN, 4)),
               ((A.GR_TUNE_MAXNB, 4),), ((A.GR_TUNE_INV_RECORDS, 0),),
-              ((A.GR_TUNE_MAXNB, 6), (A.GR_TUNE_SWEEP_BLOCK, 64), (A.GR_TUNE_NL_WIN, 64))]
+              ((A.GR_TUNE_MAXNB, 6), (A.GR_TUNE_SWEEP_BLOCK, 64), (A.GR_TUNE_NL_WIN, 64)),
+              ((A.GR_TUNE_FUSE_SORT, 0),), ((A.GR_TUNE_FUSE_SORT, 0), (A.GR_TUNE_INV_RECORDS, 0))]
     for knobs in shapes:
         got, _ = _run(builder, kw, D, 12, knobs)
         for f, v in ref.items():
