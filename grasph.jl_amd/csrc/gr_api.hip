@@ -94,24 +94,50 @@ struct GrError : std::runtime_error {
       throw GrError(GR_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_));     \
   } while (0)
 
-// Device blocks of one context kept for reuse instead of hipFree/hipMalloc
-// (each of which synchronises the device): plans are rebuilt whenever a
+// Device blocks of one context kept for reuse: plans are rebuilt whenever a
 // simulation is admitted or released, and their tables come and go.  Blocks
 // up to kCacheBlock bytes are kept in power-of-two size classes, at most
 // kCacheTotal bytes in all; a larger block is freed.  All work of a context
 // runs on its one stream, so a block returned while that stream may still
 // read it is only handed out again to work queued behind it.
+//
+// Blocks come from a memory pool of the context and go back to it in stream
+// order (hipMallocFromPoolAsync / hipFreeAsync on the context's stream).
+// hipFree waits for every stream of the device: a context that grows its
+// particle pool (dozens of arrays) while other contexts step would wait for
+// their queued steps once per array.  Without a pool (creation failed) the
+// blocks come from hipMalloc and go back with hipFree.
 struct DevCache {
   static constexpr size_t kCacheBlock = 32u << 20;
   static constexpr size_t kCacheTotal = 1u << 30;
   std::map<size_t, std::vector<void *>> bins;
   size_t cached = 0;
+  hipMemPool_t pool = nullptr;
+  hipStream_t stream = nullptr;
   DevCache() = default;
   DevCache(const DevCache &) = delete;
   DevCache &operator=(const DevCache &) = delete;
   ~DevCache() {
     for (auto &kv : bins)
-      for (void *q : kv.second) (void)hipFree(q);
+      for (void *q : kv.second) raw_free(q);
+    if (pool) {
+      (void)hipStreamSynchronize(stream);
+      (void)hipMemPoolDestroy(pool);
+    }
+  }
+  void *raw_alloc(size_t b) {
+    void *q = nullptr;
+    if (pool)
+      HIPCHK(hipMallocFromPoolAsync(&q, b, pool, stream));
+    else
+      HIPCHK(hipMalloc(&q, b));
+    return q;
+  }
+  void raw_free(void *q) {
+    if (pool)
+      (void)hipFreeAsync(q, stream);
+    else
+      (void)hipFree(q);
   }
   static size_t size_class(size_t b) {
     size_t c = 256;
@@ -131,16 +157,14 @@ struct DevCache {
     } else {
       got = b;
     }
-    void *q = nullptr;
-    HIPCHK(hipMalloc(&q, got));
-    return q;
+    return raw_alloc(got);
   }
   void give(void *q, size_t b) {
     if (b <= kCacheBlock && cached + b <= kCacheTotal) {
       bins[b].push_back(q);
       cached += b;
     } else {
-      (void)hipFree(q);
+      raw_free(q);
     }
   }
 };
@@ -2256,6 +2280,21 @@ int gr_ctx_create(gr_ctx **out, int device, int ndims, int precision) {
     delete c;
     g_create_err = "hipStreamCreate failed";
     return GR_ERR_HIP;
+  }
+  {
+    hipMemPoolProps pp;
+    memset(&pp, 0, sizeof(pp));
+    pp.allocType = hipMemAllocationTypePinned;
+    pp.handleTypes = hipMemHandleTypeNone;
+    pp.location.type = hipMemLocationTypeDevice;
+    pp.location.id = device;
+    hipMemPool_t pool = nullptr;
+    if (hipMemPoolCreate(&pool, &pp) == hipSuccess) {
+      c->cache.pool = pool;
+      c->cache.stream = c->stream;
+    } else {
+      (void)hipGetLastError();  // fall back to hipMalloc / hipFree
+    }
   }
   for (auto &e : c->ev) (void)hipEventCreate(&e);
   // GRASPH_NO_NLIST=1: walk the cells in every sweep; GRASPH_NLIST_MAXNB=k:
