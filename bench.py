@@ -64,7 +64,7 @@ def parse():
     ap.add_argument("--ensemble-steps", type=int, default=200,
                     help="timed steps of the batched ensemble")
     ap.add_argument("--ensemble-serial-steps", type=int, default=20,
-                    help="timed steps of each column run alone")
+                    help="timed steps of each column run alone (0: the batched run only)")
     ap.add_argument("--serve", type=int, default=100,
                     help="also serve this many column-collapse jobs (continuous batching) "
                          "and the same jobs serially (0 disables)")
@@ -75,18 +75,20 @@ def parse():
     return ap.parse_args()
 
 
-def ensemble_leg(n_sims, nsteps, serial_steps, precision, tune):
+def ensemble_leg(n_sims, nsteps, serial_steps, precision, tune, graph=True):
     """`n_sims` 2D granular columns (the ensemble workload: aspect, width, φ
     and ρ drawn per simulation, dx = 2.5 mm) advanced together (gr_step_many:
     one graph replay per step, one launch per phase over the packed pool)
     vs each column alone, one after another (graph replay too).  The
-    batched run times `nsteps` steps, each column alone `serial_steps`."""
+    batched run times `nsteps` steps, each column alone `serial_steps` (0:
+    the batched run only).  graph=False launches the phases one by one (for
+    a kernel trace)."""
     g = __graft_entry__.package()
     t0 = time.perf_counter()
     sims = g.configs.ensemble_columns(n_sims=n_sims)
     npart = sum(c["n_particles"] for c in sims)
     s = g.Session(2, precision)
-    s.backend.set_graph_mode(True)
+    s.backend.set_graph_mode(graph)
     hs = []
     for cfg in sims:
         h = s.scene.integrator(cfg["integrator"])
@@ -102,16 +104,19 @@ def ensemble_leg(n_sims, nsteps, serial_steps, precision, tune):
     t0 = time.perf_counter()
     s.backend.step_many(hs, nsteps)
     batched = time.perf_counter() - t0
-    progress(f"ensemble: batched {nsteps} steps in {batched:.2f} s; columns alone")
-    for h in hs:  # plan and capture of each column alone, untimed
-        s.backend.step_many([h], 1)
-    t0 = time.perf_counter()
-    for h in hs:
-        s.backend.step_many([h], serial_steps)
-    serial = time.perf_counter() - t0
+    progress(f"ensemble: batched {nsteps} steps in {batched:.2f} s")
+    serial = None
+    if serial_steps > 0:
+        progress("ensemble: columns alone")
+        for h in hs:  # plan and capture of each column alone, untimed
+            s.backend.step_many([h], 1)
+        t0 = time.perf_counter()
+        for h in hs:
+            s.backend.step_many([h], serial_steps)
+        serial = time.perf_counter() - t0
     s.close()
     b_rate = n_sims * nsteps / batched
-    s_rate = n_sims * serial_steps / serial
+    s_rate = n_sims * serial_steps / serial if serial_steps > 0 else None
     out = {"workload": f"{n_sims} 2D granular columns (EP, Drucker-Prager, ghost walls), "
                        f"aspect U[0.5,4], width U[0.1,0.4] m, phi U[20,40] deg, "
                        f"rho U[1500,2000], dx 2.5 mm",
@@ -120,7 +125,7 @@ def ensemble_leg(n_sims, nsteps, serial_steps, precision, tune):
            "sim_steps_per_s_batched": b_rate, "sim_steps_per_s_serial": s_rate,
            "particle_steps_per_s_batched": npart * nsteps / batched,
            "ms_per_batched_step": batched / nsteps * 1e3,
-           "speedup": b_rate / s_rate, "build_s": build_s,
+           "speedup": b_rate / s_rate if serial_steps > 0 else None, "build_s": build_s,
            "projected_5000_steps_s": 5000 * batched / nsteps}
     if rec:
         out["autotune"] = rec
@@ -362,7 +367,8 @@ def main():
     progress(f"3D dam break: {el / args.steps * 1e3:.3f} ms/step; CPU baseline")
     cpu = cpu_baseline(g.configs.dambreak_3d, kw, args.cpu_steps, pairs)
     ens = (ensemble_leg(args.ensemble, args.ensemble_steps, args.ensemble_serial_steps,
-                        args.precision, bool(args.autotune)) if args.ensemble else None)
+                        args.precision, bool(args.autotune), graph=bool(args.graph))
+           if args.ensemble else None)
     srv = (serve_leg(args.serve, args.serve_step_scale, args.serve_size_scale, args.precision)
            if args.serve else None)
     total_pairs = pairs * args.steps

@@ -305,6 +305,7 @@ struct FamPlan {
 struct StagePlan {
   DBuf blk_upd, urange, upd;
   int nblk_upd = 0;
+  int upd_kinds = 0;  // the updater kinds of the stage (bit GR_UPD_x)
   DBuf blk_gcopy, gcopy;
   int nblk_gcopy = 0;
   FamPlan fam[NFAM];
@@ -1458,7 +1459,10 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
     for (int s = 0; s < ns; ++s) {
       if (sys_integ[s] < 0 || k >= (int)upd[s].size() || upd[s][k].empty()) continue;
       ur[s] = make_int2((int)ud.size(), (int)(ud.size() + upd[s][k].size()));
-      for (auto &u : upd[s][k]) ud.push_back(u);
+      for (auto &u : upd[s][k]) {
+        ud.push_back(u);
+        sp->upd_kinds |= 1 << u.kind;
+      }
       add_blocks(ub, s, c->sys[s].cap, GR_BLOCK);
     }
     sp->upd.upload(ud, c->stream);
@@ -1746,7 +1750,7 @@ void phase_stage(gr_ctx *c, Plan &P, int k, bool cnt, hipEvent_t *mid, bool real
   StagePlan &S = *P.stages[k];
   Pool<R> A = pool_view<R>(c, c->cur);
   launch_updaters<R>(s, A, c->d_sd.as<SysDesc>(), c->d_st.as<StepSys>(), S.blk_upd.as<Blk>(),
-                     S.nblk_upd, S.urange.as<int2>(), S.upd.as<UpdDesc>(), c->D);
+                     S.nblk_upd, S.urange.as<int2>(), S.upd.as<UpdDesc>(), c->D, S.upd_kinds);
   launch_ghost_copy<R>(s, A, c->d_sd.as<SysDesc>(), c->d_bnd.as<Bnd>(), c->d_n.as<int>(),
                        S.blk_gcopy.as<Blk>(), S.nblk_gcopy, S.gcopy.as<int2>(), c->D);
   if (mid) HIPCHK(hipEventRecord(*mid, s));
@@ -3356,8 +3360,12 @@ int gr_update_state(gr_ctx *c, int sys, int stage, double dt) {
     c->stepsys[sys].dt = dt;
     upload_stepsys(c);
     std::vector<UpdDesc> ud;
+    int kinds = 0;
     for (auto &u : s.upd[stage - 1])
-      if (u.kind != GR_UPD_NOTHING) ud.push_back(u);
+      if (u.kind != GR_UPD_NOTHING) {
+        ud.push_back(u);
+        kinds |= 1 << u.kind;
+      }
     if (ud.empty()) return;
     std::vector<int2> ur(c->sys.size(), make_int2(0, 0));
     ur[sys] = make_int2(0, (int)ud.size());
@@ -3370,7 +3378,8 @@ int gr_update_state(gr_ctx *c, int sys, int stage, double dt) {
     by_prec(c, [&](auto *tag) {
       using R = std::remove_pointer_t<decltype(tag)>;
       launch_updaters<R>(c->stream, pool_view<R>(c, c->cur), c->d_sd.as<SysDesc>(), c->d_st.as<StepSys>(),
-                         db.as<Blk>(), (int)bv.size(), dr.as<int2>(), du.as<UpdDesc>(), c->D);
+                         db.as<Blk>(), (int)bv.size(), dr.as<int2>(), du.as<UpdDesc>(), c->D,
+                         kinds);
     });
     HIPCHK(hipStreamSynchronize(c->stream));
   });

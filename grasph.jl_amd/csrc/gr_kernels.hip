@@ -1463,7 +1463,11 @@ __device__ __forceinline__ void vpmc_update(Pool<R> P, int64_t g, int NS, double
     }
 }
 
-template <typename R>
+// The state updaters of one stage, in each system's order.  KM is the set of
+// updater kinds (bit GR_UPD_x) the kernel handles: a stage is launched with
+// the smallest instantiation that covers its kinds, so a stage of
+// Drucker-Prager updaters does not carry the registers of the others.
+template <typename R, int KM>
 __global__ void k_updaters(Pool<R> P, const SysDesc *sd, const StepSys *st, const Blk *blk,
                            const int2 *urange, const UpdDesc *ud, int D) {
   const Blk b = blk[blockIdx.x];
@@ -1471,13 +1475,18 @@ __global__ void k_updaters(Pool<R> P, const SysDesc *sd, const StepSys *st, cons
   const int i = b.first + threadIdx.x;
   const SysDesc S = sd[s];
   if (i >= S.cap) return;
-  const int64_t g = S.off + i;
+  const int64_t g0 = S.off + i;
   const int2 r = urange[s];
   const double dt = st[s].dt;
   for (int q = r.x; q < r.y; ++q) {
     const UpdDesc u = ud[q];
+    // the element address of each of the ~40 arrays would otherwise be
+    // hoisted out of the loop, two registers apiece (spilled): g is made
+    // opaque per iteration (an empty asm statement, no instruction)
+    int64_t g = g0;
+    asm volatile("" : "+v"(g));
     switch (u.kind) {
-      case GR_UPD_ZERO: {
+      case GR_UPD_ZERO: if constexpr ((KM >> GR_UPD_ZERO) & 1) {
         const long long m = u.mask;
 #pragma unroll
         for (int k = 0; k < GR_MAXNS; ++k) {
@@ -1501,18 +1510,21 @@ __global__ void k_updaters(Pool<R> P, const SysDesc *sd, const StepSys *st, cons
       }
       case GR_UPD_TAIT:
       case GR_UPD_LINEAR:
-        P.p[g] = (R)eos_p(u.kind, P.rho_h[g], u.prm[0], u.prm[1], S.c);
+        if constexpr ((KM >> GR_UPD_TAIT) & 1)
+          P.p[g] = (R)eos_p(u.kind, P.rho_h[g], u.prm[0], u.prm[1], S.c);
         break;
       case GR_UPD_EP:
-        ep_update(P, g, u.prm[0], u.prm[1], u.prm[2], u.prm[3], u.prm[4], dt);
+        if constexpr ((KM >> GR_UPD_EP) & 1)
+          ep_update(P, g, u.prm[0], u.prm[1], u.prm[2], u.prm[3], u.prm[4], dt);
         break;
       case GR_UPD_VPMC:
-        vpmc_update(P, g, S.ns, u.prm[0], u.prm[1], u.prm[2], u.prm[3], u.prm[4], S.c);
+        if constexpr ((KM >> GR_UPD_VPMC) & 1)
+          vpmc_update(P, g, S.ns, u.prm[0], u.prm[1], u.prm[2], u.prm[3], u.prm[4], S.c);
         break;
       case GR_UPD_HOOKE:
-        hooke_update(P, g, u.prm[0], u.prm[1], dt);
+        if constexpr ((KM >> GR_UPD_HOOKE) & 1) hooke_update(P, g, u.prm[0], u.prm[1], dt);
         break;
-      case GR_UPD_VNORM: {  // VirtualNormUpdater (src/StateUpdaters.jl:74-84)
+      case GR_UPD_VNORM: if constexpr ((KM >> GR_UPD_VNORM) & 1) {  // VirtualNormUpdater (src/StateUpdaters.jl:74-84)
         const long long m = u.mask;
         const double w = P.wsum[g];
         const bool z = w == 0.0;
@@ -1544,7 +1556,8 @@ __global__ void k_updaters(Pool<R> P, const SysDesc *sd, const StepSys *st, cons
         }
         break;
       }
-      case GR_UPD_PRESCRIBED_V: {  // PrescribedVelocityUpdater (:88-92)
+      case GR_UPD_PRESCRIBED_V: if constexpr ((KM >> GR_UPD_PRESCRIBED_V) & 1) {
+        // PrescribedVelocityUpdater (:88-92)
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
           if (d >= D) continue;
@@ -1931,8 +1944,25 @@ void launch_xsph(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepS
 }
 template <typename R>
 void launch_updaters(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
-                     const Blk *blk, int nblk, const int2 *urange, const UpdDesc *ud, int D) {
-  if (nblk > 0) k_updaters<R><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, blk, urange, ud, D);
+                     const Blk *blk, int nblk, const int2 *urange, const UpdDesc *ud, int D,
+                     int kinds) {
+  if (nblk <= 0) return;
+  constexpr int Z = 1 << GR_UPD_ZERO | 1 << GR_UPD_NOTHING;
+  constexpr int K_EP = Z | 1 << GR_UPD_EP, K_VPMC = Z | 1 << GR_UPD_VPMC,
+                K_EOS = Z | 1 << GR_UPD_TAIT | 1 << GR_UPD_LINEAR, K_HOOKE = Z | 1 << GR_UPD_HOOKE;
+  auto go = [&](auto km) {
+    k_updaters<R, decltype(km)::value><<<nblk, GR_BLOCK, 0, s>>>(P, sd, st, blk, urange, ud, D);
+  };
+  if ((kinds & ~K_EP) == 0)
+    go(std::integral_constant<int, K_EP>{});
+  else if ((kinds & ~K_VPMC) == 0)
+    go(std::integral_constant<int, K_VPMC>{});
+  else if ((kinds & ~K_EOS) == 0)
+    go(std::integral_constant<int, K_EOS>{});
+  else if ((kinds & ~K_HOOKE) == 0)
+    go(std::integral_constant<int, K_HOOKE>{});
+  else
+    go(std::integral_constant<int, -1>{});
 }
 #define INST(R)                                                                                \
   template void launch_halfstep<R>(hipStream_t, const Pool<R> &, const SysDesc *,              \
@@ -1947,7 +1977,7 @@ void launch_updaters(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const S
                                const int *, const Blk *, int, int, int);                       \
   template void launch_updaters<R>(hipStream_t, const Pool<R> &, const SysDesc *,              \
                                    const StepSys *, const Blk *, int, const int2 *,            \
-                                   const UpdDesc *, int);
+                                   const UpdDesc *, int, int);
 INST(float)
 INST(double)
 #undef INST

@@ -117,8 +117,10 @@ template <typename R>
 void launch_xsph(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
                  const int *n, const Blk *blk, int nblk, int D, int mode);
 template <typename R>
+// kinds: the set of updater kinds (bit GR_UPD_x) in ud
 void launch_updaters(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
-                     const Blk *blk, int nblk, const int2 *urange, const UpdDesc *ud, int D);
+                     const Blk *blk, int nblk, const int2 *urange, const UpdDesc *ud, int D,
+                     int kinds);
 template <typename R>
 void launch_sweep(hipStream_t s, const Pool<R> &P, const SweepArgs &a, int D, int fam, bool cnt);
 template <typename R>

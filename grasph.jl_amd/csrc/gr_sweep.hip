@@ -337,8 +337,9 @@ struct Gather {
     R irj;
     if constexpr (INV)  // the record's 1/ρ (a dynamic partner mirrors the target)
       irj = Q.pk == PK_DYNAMIC ? iri : rj;
-    else
-      irj = (FA && Q.pk == PK_DYNAMIC) ? iri : Fast<R>::rcp(rj);
+    else  // a dynamic partner has rj = ri, so this is iri bit for bit (a select
+          // of the two made the compiler keep them in scratch memory)
+      irj = Fast<R>::rcp(rj);
     if (FAM == FAM_STRAIN) {  // strain_rate_tensor / vorticity_tensor
       R dv[D];
 #pragma unroll
@@ -784,8 +785,9 @@ __global__ void __launch_bounds__(64)
         unsigned lo = 0u, hi = 0u;
         auto flush = [&]() {
           const unsigned sdl = (unsigned)(self_j - base);
-          if (sdl < 32u) lo &= ~(1u << sdl);
-          else if (sdl < 64u) hi &= ~(1u << (sdl - 32u));
+          const unsigned sbit = 1u << (sdl & 31u);
+          lo &= sdl < 32u ? ~sbit : ~0u;
+          hi &= sdl - 32u < 32u ? ~sbit : ~0u;
           if (live && (lo | hi) != 0u) {
             if (ne < L.maxnb) ent[(int64_t)ne * L.cap] = make_int4((int)lo, (int)hi, base, 0);
             ++ne;
@@ -866,10 +868,11 @@ __global__ void __launch_bounds__(64)
                 cur |= (unsigned)(r2 < c2) << (d & 31);
                 ++d;
               }
-              if (wsel == 0)
-                lo |= cur;
-              else
-                hi |= cur;
+              // selects, not branches (here and when the target's own bit
+              // is cleared): branches make the compiler index a pair
+              // {lo, hi} kept in scratch memory
+              lo |= wsel == 0 ? cur : 0u;
+              hi |= wsel != 0 ? cur : 0u;
             }
           }
           if (mem) pend = false;

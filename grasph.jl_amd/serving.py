@@ -279,9 +279,8 @@ class Server:
             for job in queue:
                 self._fail(job, t0, f"context error: {e}", taken=False)
 
-    def run(self, jobs):
-        """Serve `jobs` (arrival times in seconds from now); returns a report."""
-        t0 = time.perf_counter()
+    def _assign(self, jobs):
+        """Lane of every job, in arrival order, from the jobs before it only."""
         by_lane = {}
         assigned = {}
         for j in sorted(jobs, key=lambda j: (j.arrival, j.jid)):
@@ -289,6 +288,12 @@ class Server:
             key = min(keys, key=lambda q: (assigned.get(q, 0), q[1]))
             assigned[key] = assigned.get(key, 0) + j.steps
             by_lane.setdefault(key, []).append(j)
+        return by_lane
+
+    def run(self, jobs):
+        """Serve `jobs` (arrival times in seconds from now); returns a report."""
+        t0 = time.perf_counter()
+        by_lane = self._assign(jobs)
         for key in by_lane:  # create contexts before the clock matters
             self._session(key)
         self._pool = ThreadPoolExecutor(max_workers=self.builders)
