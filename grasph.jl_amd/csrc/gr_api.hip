@@ -1763,10 +1763,24 @@ void phase_stage(gr_ctx *c, Plan &P, int k, bool cnt, hipEvent_t *mid, bool real
   // records of 1/ρ and p/ρ² when the stage's only sweeps are FluidPfn S = 2
   // force sweeps without δ-SPH or surface tension
   const bool inv = c_only && S.fam[FAM_FORCE].inv && S.fam[FAM_STRAIN].nblk == 0;
+  // the compact records of the real systems packed for the previous stage
+  // still hold when they are of the same kind and nothing in between wrote
+  // x, v, ρ or p: the stage's only updaters are Drucker-Prager (stress and
+  // strains) and no virtual system is interpolated; then only the stress
+  // records are packed
+  auto rec_kind = [](const StagePlan &T) {  // 0: no compact records
+    const int nsw = T.fam[FAM_FORCE].nblk + T.fam[FAM_COUNT].nblk + T.fam[FAM_STRAIN].nblk +
+                    T.fam[FAM_INTERP].nblk;
+    if (sizeof(R) != 4 || nsw == 0 || T.fam[FAM_INTERP].nblk || T.fam[FAM_COUNT].nblk) return 0;
+    return T.fam[FAM_FORCE].inv && T.fam[FAM_STRAIN].nblk == 0 ? 2 : 1;
+  };
+  const bool keep_c = k > 0 && P.nblk_virt == 0 && rec_kind(S) != 0 &&
+                      rec_kind(S) == rec_kind(*P.stages[k - 1]) &&
+                      (S.upd_kinds & ~(1 << GR_UPD_EP | 1 << GR_UPD_NOTHING)) == 0;
   auto pack = [&]() {
     if (!real_packed)
       launch_pack<R>(s, A, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), P.blk_real.as<Blk>(),
-                     P.nblk_real, c->D, c_only, with_s, inv);
+                     P.nblk_real, c->D, c_only, with_s, inv, keep_c);
     launch_pack<R>(s, A, c->d_sd.as<SysDesc>(), c->d_n.as<int>(), P.blk_ghost.as<Blk>(),
                    P.nblk_ghost, c->D, c_only, with_s, inv);
   };

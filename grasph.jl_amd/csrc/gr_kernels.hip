@@ -1643,12 +1643,15 @@ __global__ void k_fill_i32(int *p, int v, long long n) {
 
 // Partner records of the live particles of the block's system (positions,
 // shadows of density and velocity, pressure), read by the pair sweeps.
-template <typename R, int D, bool C_ONLY, bool WITH_S, bool INV = false>
+// S_ONLY: the stress records alone (the compact records of the previous
+// stage still hold: nothing in between wrote x, v, ρ or p).
+template <typename R, int D, bool C_ONLY, bool WITH_S, bool INV = false, bool S_ONLY = false>
 __global__ void k_pack(Pool<R> P, const SysDesc *sd, const int *n_dev, const Blk *blk) {
   const Blk b = blk[blockIdx.x];
   const int i = b.first + threadIdx.x;
   if (i >= n_dev[b.item]) return;
   const int64_t g = sd[b.item].off + i;
+  if constexpr (!S_ONLY) {
   RecA<R> a;
   RecB<R> r;
 #pragma unroll
@@ -1683,6 +1686,7 @@ __global__ void k_pack(Pool<R> P, const SysDesc *sd, const int *n_dev, const Blk
       cc.p = a.p;
     }
     P.rc[g] = cc;
+  }
   }
   if constexpr (sizeof(R) == 4 && WITH_S) {  // stages with Cauchy sweeps
     const SysDesc S = sd[b.item];
@@ -2007,9 +2011,17 @@ static void launch_pack_d(hipStream_t s, const Pool<R> &P, const SysDesc *sd, co
 
 template <typename R>
 void launch_pack(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const int *n, const Blk *blk,
-                 int nblk, int D, bool c_only, bool with_s, bool inv) {
+                 int nblk, int D, bool c_only, bool with_s, bool inv, bool s_only) {
   if (nblk <= 0) return;
   c_only = c_only && sizeof(R) == 4;
+  if (s_only) {  // FP32 compact records only
+    if (!(c_only && with_s)) return;
+    if (D == 2)
+      k_pack<R, 2, true, true, false, true><<<nblk, GR_BLOCK, 0, s>>>(P, sd, n, blk);
+    else
+      k_pack<R, 3, true, true, false, true><<<nblk, GR_BLOCK, 0, s>>>(P, sd, n, blk);
+    return;
+  }
   if (inv && c_only && !with_s && sizeof(R) == 4) {
     if (D == 2)
       k_pack<R, 2, true, false, true><<<nblk, GR_BLOCK, 0, s>>>(P, sd, n, blk);
@@ -2027,9 +2039,9 @@ void launch_pack(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const int *
     launch_pack_d<R, false, false>(s, P, sd, n, blk, nblk, D);
 }
 template void launch_pack<float>(hipStream_t, const Pool<float> &, const SysDesc *, const int *,
-                                 const Blk *, int, int, bool, bool, bool);
+                                 const Blk *, int, int, bool, bool, bool, bool);
 template void launch_pack<double>(hipStream_t, const Pool<double> &, const SysDesc *, const int *,
-                                  const Blk *, int, int, bool, bool, bool);
+                                  const Blk *, int, int, bool, bool, bool, bool);
 
 // Probe slot id[i] − 1 takes the position of mirror particle i
 // (_measure_probes!, src/TimeIntegration.jl:469-479).

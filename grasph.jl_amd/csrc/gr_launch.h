@@ -127,8 +127,10 @@ template <typename R>
 void launch_virtual(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const StepSys *st,
                     const int *n, const Blk *blk, int nblk, int mode, int D);
 template <typename R>
+// s_only: the FP32 stress records alone (the compact records still hold)
 void launch_pack(hipStream_t s, const Pool<R> &P, const SysDesc *sd, const int *n, const Blk *blk,
-                 int nblk, int D, bool c_only = false, bool with_s = true, bool inv = false);
+                 int nblk, int D, bool c_only = false, bool with_s = true, bool inv = false,
+                 bool s_only = false);
 template <typename R>
 void launch_nlist(hipStream_t s, const Pool<R> &P, const NlistArgs &a, int D);
 void launch_fill_i32(hipStream_t s, int *p, int v, long long n);
