@@ -137,24 +137,25 @@ def serve_leg(n_jobs, step_scale, size_scale, precision):
     columns (the ensemble's parameter ranges) and 20 % 3D visco-plastic
     columns (50k-500k particles, dx 5 mm, phi U[25, 40] deg), step counts
     log-uniform in [500, 20000] times `step_scale`, particle counts times
-    `size_scale`, an HBM budget of 200 GB.  Continuous batching (one context
-    per dimension, shared launches, admission by footprint, retirement)
+    `size_scale`, an HBM budget of 200 GB.  Continuous batching (one 2D and
+    three 3D contexts, shared launches, admission by footprint, retirement)
     against the same jobs run one at a time, each alone on the GPU, in
     arrival order (graph replay and scene building overlapped in both)."""
     g = __graft_entry__.package()
     S = g.serving
     mk = lambda: S.column_jobs(n_jobs, step_scale=step_scale, size_scale=size_scale)  # noqa: E731
     progress(f"serving: {n_jobs} jobs, continuous batching")
-    # two 3D lanes: the 3D jobs are the long ones, and a late long job then
-    # shares its steps with half of them (a shorter makespan and lower
-    # latencies than one lane per dimension in this stream)
+    # three 3D lanes: the 3D jobs are the long ones, and a late long job then
+    # shares its steps with a third of them (a shorter makespan and lower
+    # latencies than one or two 3D lanes in this stream; four contexts use
+    # the four hardware queues)
     served = S.Server(precision=precision, quantum=50, graph=True, hbm_budget=200e9,
-                      lanes={2: 1, 3: 2}).run(mk())
+                      lanes={2: 1, 3: 3}).run(mk())
     progress(f"serving: served in {served['makespan_s']:.1f} s; the same jobs one at a time")
     plain = S.run_plain(mk(), precision=precision, graph=True,
                         progress=lambda m: progress("serving (plain): " + m))
     progress(f"serving: plain in {plain['makespan_s']:.1f} s")
-    return {"lanes": {"2d": 1, "3d": 2},
+    return {"lanes": {"2d": 1, "3d": 3},
             "workload": "served stream of 2D/3D column-collapse jobs (Poisson 20/s, 80% 2D / "
                         "20% 3D, steps log-U[500, 20000], HBM budget 200 GB)",
             "jobs": n_jobs, "step_scale": step_scale, "size_scale": size_scale,
