@@ -319,8 +319,10 @@ struct Plan {
   int nblk_real = 0, n_real = 0;
   // ghosts
   std::vector<int> ghost_list;
-  DBuf d_ghost_list, d_bbase_ghost, d_cand_off, blk_ghost, blk_cand;
-  int nblk_ghost = 0, nblk_cand = 0, n_ghost = 0, n_cand = 0;
+  // ghost generation: blocks of source particles per ghost system, and the
+  // first (boundary, source block) count slot of each ghost system
+  DBuf d_ghost_list, d_bbase_ghost, d_gslot, blk_ghost, blk_gsrc;
+  int nblk_ghost = 0, nblk_gsrc = 0, n_ghost = 0, n_gslot = 0;
   // grids
   std::vector<int> inters;
   std::vector<std::pair<int, bool>> carr_state;  // (cell array, built by this plan)
@@ -681,12 +683,13 @@ void commit(gr_ctx *c) {
     std::vector<unsigned long long> z2(2, 0);
     c->d_pairs.upload(z2, c->stream);
   }
-  // sort / candidate scratch
+  // sort / ghost generation scratch (one count per boundary and block of
+  // GR_BLOCK source particles)
   int64_t nreal = 0, nghost = 0, ncand = 0;
   for (auto &s : c->sys) {
     if (s.kind == GR_SYS_GHOST) {
       nghost += s.cap;
-      ncand += s.cap;
+      ncand += (int64_t)s.nb * ((c->sys[s.src].cap + GR_BLOCK - 1) / GR_BLOCK);
     } else {
       nreal += s.cap;
     }
@@ -1187,28 +1190,34 @@ void plan_sort_real(gr_ctx *c, Plan &P, const std::vector<int> &systems) {
   P.nblk_real = (int)blk.size();
 }
 
+// Count slots of ghost generation: one per (boundary, block of GR_BLOCK
+// source particles) of each ghost system.
+int64_t ghost_slots(const gr_ctx *c, const SysH &g) {
+  return (int64_t)g.nb * ((c->sys[g.src].cap + GR_BLOCK - 1) / GR_BLOCK);
+}
+
 void plan_ghosts(gr_ctx *c, Plan &P, const std::vector<int> &ghosts) {
   P.ghost_list = ghosts;
   P.d_ghost_list.upload(ghosts, c->stream);
-  std::vector<int> bbase(c->sys.size(), -1), cand(c->sys.size(), 0);
-  std::vector<Blk> blk, cblk;
-  int64_t acc = 0, cacc = 0;
+  std::vector<int> bbase(c->sys.size(), -1), slot(c->sys.size(), 0);
+  std::vector<Blk> blk, sblk;
+  int64_t acc = 0, sacc = 0;
   for (int g : ghosts) {
     bbase[g] = (int)acc;
     acc += c->sys[g].cap;
-    cand[g] = (int)cacc;
-    cacc += c->sys[g].cap;
+    slot[g] = (int)sacc;
+    sacc += ghost_slots(c, c->sys[g]);
     add_blocks(blk, g, c->sys[g].cap, GR_BLOCK);
-    add_blocks(cblk, g, c->sys[g].cap, GR_BLOCK);
+    add_blocks(sblk, g, c->sys[c->sys[g].src].cap, GR_BLOCK);
   }
   P.n_ghost = (int)acc;
-  P.n_cand = (int)cacc;
+  P.n_gslot = (int)sacc;
   P.d_bbase_ghost.upload(bbase, c->stream);
-  P.d_cand_off.upload(cand, c->stream);
+  P.d_gslot.upload(slot, c->stream);
   P.blk_ghost.upload(blk, c->stream);
-  P.blk_cand.upload(cblk, c->stream);
+  P.blk_gsrc.upload(sblk, c->stream);
   P.nblk_ghost = (int)blk.size();
-  P.nblk_cand = (int)cblk.size();
+  P.nblk_gsrc = (int)sblk.size();
 }
 
 // Stage plans of a batch of integrators: updater tables (with the fused EOS
@@ -1685,12 +1694,12 @@ void phase_ghosts(gr_ctx *c, Plan &P, bool generate, bool sort) {
   const XP x = xp_pool(c, c->cur);
   Pool<R> A = pool_view<R>(c, c->cur);
   if (generate) {
-    launch_ghost_flags(s, x, c->d_sd.as<SysDesc>(), c->d_bnd.as<Bnd>(), P.blk_cand.as<Blk>(),
-                       P.nblk_cand, P.d_cand_off.as<int>(), c->D, c->cand.as<int>());
-    launch_scan(s, c->cand.as<int>(), c->cand_scan.as<int>(), c->scan_bsum.as<int>(), P.n_cand);
+    launch_ghost_count(s, x, c->d_sd.as<SysDesc>(), c->d_bnd.as<Bnd>(), P.blk_gsrc.as<Blk>(),
+                       P.nblk_gsrc, P.d_gslot.as<int>(), c->D, c->cand.as<int>());
+    launch_scan(s, c->cand.as<int>(), c->cand_scan.as<int>(), c->scan_bsum.as<int>(), P.n_gslot);
     launch_ghost_scatter(s, x, xpw_gxs(c), c->gidx_s.as<int>(), c->gbnd_s.as<int>(),
                          c->gnrm.as<int>(), c->d_sd.as<SysDesc>(), c->d_bnd.as<Bnd>(),
-                         P.blk_cand.as<Blk>(), P.nblk_cand, P.d_cand_off.as<int>(),
+                         P.blk_gsrc.as<Blk>(), P.nblk_gsrc, P.d_gslot.as<int>(),
                          c->cand_scan.as<int>(), c->D, c->d_n.as<int>());
   }
   const XP gx = xp_gxs(c);

@@ -676,56 +676,90 @@ __global__ void k_permute_real(Pool<R> A, Pool<R> B, const SysDesc *sd, const Bl
 // Ghosts (src/GhostParticles.jl:344-421)
 // ---------------------------------------------------------------------------
 
-// Qualification of candidate (boundary b, source particle i) of ghost g,
-// laid out boundary-major: abs(da) <= cutoff && da > 0.
+// Ghost generation: candidate (boundary b, source particle i) qualifies when
+// abs(da) <= cutoff && da > 0, and the ghosts follow boundary-major, then
+// source order.  A thread takes one source particle and every boundary of its
+// ghost system: k_ghost_count counts the qualifying candidates of each
+// (boundary, block of GR_BLOCK source particles) slot, an exclusive scan of
+// the slots gives each its first ghost, and k_ghost_scatter writes the ghosts
+// there in source order.  Slots of ghost system g start at gslot[g],
+// boundary-major.
 template <int D>
-__global__ void k_ghost_flags(XP x, const SysDesc *sd, const Bnd *bnd, const Blk *blk,
-                              const int *cand_off, int *flags) {
-  const Blk b = blk[blockIdx.x];
-  const int g = b.item;
-  const int c = b.first + threadIdx.x;
-  const SysDesc G = sd[g];
-  const SysDesc S = sd[G.src];
-  if (c >= G.cap) return;
-  const int bi = c / S.cap, i = c % S.cap;
-  const Bnd B = bnd[G.bnd + bi];
-  const int64_t gi = S.off + i;
-  double da = (x[0][gi] - B.p[0]) * B.n[0];
+__device__ inline bool ghost_qualifies(const double *xi, const Bnd &B, double gcut, double &da) {
+  da = (xi[0] - B.p[0]) * B.n[0];
 #pragma unroll
-  for (int d = 1; d < D; ++d) da = da + (x[d][gi] - B.p[d]) * B.n[d];
-  flags[cand_off[g] + c] = (fabs(da) <= G.gcut && da > 0.0) ? 1 : 0;
+  for (int d = 1; d < D; ++d) da = da + (xi[d] - B.p[d]) * B.n[d];
+  return fabs(da) <= gcut && da > 0.0;
 }
 
-// Scatter qualifying candidates to generation order: mirrored position,
-// source index, boundary index, and the boundary whose normal slot k keeps
-// (normals are not permuted by the ghost sort, src/Sorting.jl:33-40).
 template <int D>
-__global__ void k_ghost_scatter(XP x, XPW gx, int *gidx, int *gbnd, int *gnrm, const SysDesc *sd,
-                                const Bnd *bnd, const Blk *blk, const int *cand_off,
-                                const int *scan, int *n_dev) {
+__global__ void __launch_bounds__(GR_BLOCK)
+    k_ghost_count(XP x, const SysDesc *sd, const Bnd *bnd, const Blk *blk, const int *gslot,
+                  int *cnt) {
   const Blk b = blk[blockIdx.x];
   const int g = b.item;
-  const int c = b.first + threadIdx.x;
   const SysDesc G = sd[g];
   const SysDesc S = sd[G.src];
-  if (c >= G.cap) return;
-  const int c0 = cand_off[g];
-  const int pos = scan[c0 + c], nxt = scan[c0 + c + 1];
-  if (c == G.cap - 1) n_dev[g] = nxt - scan[c0];
-  if (nxt == pos) return;
-  const int k = pos - scan[c0];
-  const int bi = c / S.cap, i = c % S.cap;
-  const Bnd B = bnd[G.bnd + bi];
-  const int64_t gi = S.off + i;
-  double da = (x[0][gi] - B.p[0]) * B.n[0];
+  const int i = b.first + threadIdx.x;
+  const bool live = i < S.cap;
+  const int64_t gi = S.off + (live ? i : 0);
+  double xi[D];
 #pragma unroll
-  for (int d = 1; d < D; ++d) da = da + (x[d][gi] - B.p[d]) * B.n[d];
-  const int64_t go = G.off + k;
+  for (int d = 0; d < D; ++d) xi[d] = x[d][gi];
+  const int nsb = (S.cap + GR_BLOCK - 1) / GR_BLOCK, sb = b.first / GR_BLOCK;
+  for (int bi = 0; bi < G.nb; ++bi) {
+    double da;
+    const bool q = live && ghost_qualifies<D>(xi, bnd[G.bnd + bi], G.gcut, da);
+    const int n = __syncthreads_count(q);
+    if (threadIdx.x == 0) cnt[gslot[g] + bi * nsb + sb] = n;
+  }
+}
+
+// Mirrored position, source index, boundary index, and the boundary whose
+// normal slot k keeps (normals are not permuted by the ghost sort,
+// src/Sorting.jl:33-40).
+template <int D>
+__global__ void __launch_bounds__(GR_BLOCK)
+    k_ghost_scatter(XP x, XPW gx, int *gidx, int *gbnd, int *gnrm, const SysDesc *sd,
+                    const Bnd *bnd, const Blk *blk, const int *gslot, const int *scan,
+                    int *n_dev) {
+  __shared__ int wc[GR_BLOCK / 64];
+  const Blk b = blk[blockIdx.x];
+  const int g = b.item;
+  const SysDesc G = sd[g];
+  const SysDesc S = sd[G.src];
+  const int i = b.first + threadIdx.x;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const bool live = i < S.cap;
+  const int64_t gi = S.off + (live ? i : 0);
+  double xi[D];
 #pragma unroll
-  for (int d = 0; d < D; ++d) gx[d][go] = x[d][gi] - (2.0 * da) * B.n[d];
-  gidx[go] = i;
-  gbnd[go] = bi;
-  gnrm[go] = bi;
+  for (int d = 0; d < D; ++d) xi[d] = x[d][gi];
+  const int nsb = (S.cap + GR_BLOCK - 1) / GR_BLOCK, sb = b.first / GR_BLOCK;
+  const int s0 = gslot[g];
+  const int first = scan[s0];
+  if (sb == 0 && threadIdx.x == 0) n_dev[g] = scan[s0 + G.nb * nsb] - first;
+  for (int bi = 0; bi < G.nb; ++bi) {
+    const Bnd B = bnd[G.bnd + bi];
+    double da;
+    const bool q = live && ghost_qualifies<D>(xi, B, G.gcut, da);
+    const unsigned long long bal = __ballot(q);
+    if (lane == 0) wc[w] = __popcll(bal);
+    __syncthreads();
+    int woff = 0;
+    for (int v = 0; v < w; ++v) woff += wc[v];
+    __syncthreads();  // wc is written again for the next boundary
+    if (q) {
+      const int k = scan[s0 + bi * nsb + sb] - first + woff +
+                    __popcll(bal & ((1ull << lane) - 1ull));
+      const int64_t go = G.off + k;
+#pragma unroll
+      for (int d = 0; d < D; ++d) gx[d][go] = xi[d] - (2.0 * da) * B.n[d];
+      gidx[go] = i;
+      gbnd[go] = bi;
+      gnrm[go] = bi;
+    }
+  }
 }
 
 // Ghost sort permutation: generation scratch → pool (x, idx_original,
@@ -1811,13 +1845,13 @@ template void launch_permute_real<double>(hipStream_t, const Pool<double> &,
                                           int, const int *, const int *, const int *, const int *,
                                           int, int, const StepSys *, const int *, int);
 
-void launch_ghost_flags(hipStream_t s, XP x, const SysDesc *sd, const Bnd *bnd, const Blk *blk,
-                        int nblk, const int *cand_off, int D, int *flags) {
+void launch_ghost_count(hipStream_t s, XP x, const SysDesc *sd, const Bnd *bnd, const Blk *blk,
+                        int nblk, const int *gslot, int D, int *cnt) {
   if (nblk <= 0) return;
   if (D == 2)
-    k_ghost_flags<2><<<nblk, GR_BLOCK, 0, s>>>(x, sd, bnd, blk, cand_off, flags);
+    k_ghost_count<2><<<nblk, GR_BLOCK, 0, s>>>(x, sd, bnd, blk, gslot, cnt);
   else
-    k_ghost_flags<3><<<nblk, GR_BLOCK, 0, s>>>(x, sd, bnd, blk, cand_off, flags);
+    k_ghost_count<3><<<nblk, GR_BLOCK, 0, s>>>(x, sd, bnd, blk, gslot, cnt);
 }
 void launch_scan(hipStream_t s, const int *in, int *out, int *bsum, int N) {
   const int nb = (int)nb_of(N > 0 ? N : 1, SCAN_TILE);
@@ -1828,13 +1862,13 @@ void launch_scan(hipStream_t s, const int *in, int *out, int *bsum, int N) {
 int scan_block_count(int N) { return (int)nb_of(N > 0 ? N : 1, SCAN_TILE); }
 void launch_ghost_scatter(hipStream_t s, XP x, XPW gx, int *gidx, int *gbnd, int *gnrm,
                           const SysDesc *sd, const Bnd *bnd, const Blk *blk, int nblk,
-                          const int *cand_off, const int *scan, int D, int *n) {
+                          const int *gslot, const int *scan, int D, int *n) {
   if (nblk <= 0) return;
   if (D == 2)
-    k_ghost_scatter<2><<<nblk, GR_BLOCK, 0, s>>>(x, gx, gidx, gbnd, gnrm, sd, bnd, blk, cand_off,
+    k_ghost_scatter<2><<<nblk, GR_BLOCK, 0, s>>>(x, gx, gidx, gbnd, gnrm, sd, bnd, blk, gslot,
                                                  scan, n);
   else
-    k_ghost_scatter<3><<<nblk, GR_BLOCK, 0, s>>>(x, gx, gidx, gbnd, gnrm, sd, bnd, blk, cand_off,
+    k_ghost_scatter<3><<<nblk, GR_BLOCK, 0, s>>>(x, gx, gidx, gbnd, gnrm, sd, bnd, blk, gslot,
                                                  scan, n);
 }
 void launch_permute_ghost(hipStream_t s, XP gxs, const int *gidx_s, const int *gbnd_s, XPW x,

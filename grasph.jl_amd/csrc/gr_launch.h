@@ -75,13 +75,15 @@ void launch_permute_real(hipStream_t s, const Pool<R> &A, const Pool<R> &B, cons
                          const int *vals2, const int *flags, int vadj, int D,
                          const StepSys *st = nullptr, const int *halfsys = nullptr,
                          int pack = 0);
-void launch_ghost_flags(hipStream_t s, XP x, const SysDesc *sd, const Bnd *bnd,
-                        const Blk *blk, int nblk, const int *cand_off, int D, int *flags);
+// ghost generation: per-(boundary, source block) counts at gslot[g] + ...,
+// then the scatter at the counts' exclusive scan
+void launch_ghost_count(hipStream_t s, XP x, const SysDesc *sd, const Bnd *bnd,
+                        const Blk *blk, int nblk, const int *gslot, int D, int *cnt);
 void launch_scan(hipStream_t s, const int *in, int *out, int *bsum, int N);
 int scan_block_count(int N);
 void launch_ghost_scatter(hipStream_t s, XP x, XPW gx, int *gidx,
                           int *gbnd, int *gnrm, const SysDesc *sd, const Bnd *bnd, const Blk *blk,
-                          int nblk, const int *cand_off, const int *scan, int D, int *n);
+                          int nblk, const int *gslot, const int *scan, int D, int *n);
 void launch_permute_ghost(hipStream_t s, XP gxs, const int *gidx_s,
                           const int *gbnd_s, XPW x, int *gidx, int *gbnd,
                           const SysDesc *sd, const int *n, const Blk *blk, int nblk,
