@@ -306,6 +306,12 @@ def main():
               file=sys.stderr)
         sys.exit(1)
     g = __graft_entry__.package()
+    # The served stream runs first, so that its four contexts' streams are the
+    # process's first streams.  When another context's stream had come and gone
+    # before, the same stream took 9.1 s instead of 7.2 s: two of the lanes
+    # stepped at about half their rate, as if they shared a hardware queue.
+    srv = (serve_leg(args.serve, args.serve_step_scale, args.serve_size_scale, args.precision)
+           if args.serve else None)
     kw = dict(dx=args.dx)
     progress("3D dam break: building the scene")
     cfg = g.configs.dambreak_3d(**kw)
@@ -365,13 +371,14 @@ def main():
     sort_gbs = bytes_sort / (phases[0] * 1e-3) / 1e9 if phases[0] > 0 else None
     # the committed counter passes are of the FP32 kernels
     traffic = load_traffic() if args.precision == "f32" else None
+    # the dam break's context (its stream, pool and graphs) is not needed by
+    # the legs below
+    s.close()
     progress(f"3D dam break: {el / args.steps * 1e3:.3f} ms/step; CPU baseline")
     cpu = cpu_baseline(g.configs.dambreak_3d, kw, args.cpu_steps, pairs)
     ens = (ensemble_leg(args.ensemble, args.ensemble_steps, args.ensemble_serial_steps,
                         args.precision, bool(args.autotune), graph=bool(args.graph))
            if args.ensemble else None)
-    srv = (serve_leg(args.serve, args.serve_step_scale, args.serve_size_scale, args.precision)
-           if args.serve else None)
     total_pairs = pairs * args.steps
     value = total_pairs / el
     res = {
