@@ -306,10 +306,11 @@ def main():
               file=sys.stderr)
         sys.exit(1)
     g = __graft_entry__.package()
-    # The served stream runs first, so that its four contexts' streams are the
-    # process's first streams.  When another context's stream had come and gone
-    # before, the same stream took 9.1 s instead of 7.2 s: two of the lanes
-    # stepped at about half their rate, as if they shared a hardware queue.
+    # The served stream runs first.  In a process where another context had
+    # run and been closed before, the same stream took 9.1 s instead of 7.2 s
+    # (two of the 3D lanes stepped about 30 % slower); reusing that context's
+    # stream for a lane did not change it, and eight hardware queues instead of
+    # four made it slower still.
     srv = (serve_leg(args.serve, args.serve_step_scale, args.serve_size_scale, args.precision)
            if args.serve else None)
     kw = dict(dx=args.dx)
