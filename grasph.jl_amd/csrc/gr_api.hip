@@ -1426,7 +1426,7 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
   std::vector<Blk> lbv;
   std::map<int, int> ltask_of;
   int64_t lacc = 0, cacc = 0;
-  // mask entries kept per target (16 B each): a strip of <= 64 candidates
+  // mask entries kept per target (12 B each): a strip of <= 64 candidates
   // takes one entry, so 9 (3D) or 3 (2D) per source for interior targets
   const int maxnb = c->list_maxnb > 0 ? c->list_maxnb : (c->D == 3 ? 48 : 24);
   for (auto &kv : tl) {
@@ -1440,7 +1440,7 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
     L.src_end = (int)lsv.size();
     L.list_off = lacc;
     L.cnt_off = cacc;
-    lacc += (int64_t)L.cap * maxnb * 4;
+    lacc += ((int64_t)L.cap * maxnb * 3 + 3) & ~(int64_t)3;  // 16-byte aligned lists
     cacc += (int64_t)L.cap * (L.src_end - L.src_begin + 1);
     add_blocks(lbv, (int)ltv.size(), L.cap, 64);  // one wavefront per workgroup
     ltask_of[kv.first] = (int)ltv.size();

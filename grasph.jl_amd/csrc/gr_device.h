@@ -132,10 +132,12 @@ struct Task {
 };
 
 // Neighbour list of one target system: sources [src_begin, src_end) of the
-// list source table; mask entry e of target i (int4 {mask_lo, mask_hi,
-// base, 0}) at lidx[list_off + 4·(e·cap + i)], entry counts per source at
-// lcnt[cnt_off + q·cap + i], overflow flag at lcnt[cnt_off + nsrc·cap + i];
-// maxnb = entries kept per target.
+// list source table; mask entry e of target i is the mask int2 {lo, hi} at
+// lidx[list_off + 2·(e·cap + i)] and its base at
+// lidx[list_off + 2·maxnb·cap + e·cap + i] (12 bytes an entry, list_off a
+// multiple of 4), entry counts per source at lcnt[cnt_off + q·cap + i],
+// overflow flag at lcnt[cnt_off + nsrc·cap + i]; maxnb = entries kept per
+// target.
 struct LTask {
   int32_t sys;
   int32_t src_begin, src_end;

@@ -679,15 +679,16 @@ __global__ void __launch_bounds__(GR_BLOCK)
 // ---------------------------------------------------------------------------
 
 // Accepted partners of every target of a list task, per source, in the
-// order the sources are listed, as acceptance masks: an entry
-// {mask_lo, mask_hi, base, 0} (int4; the mask first, so a load of the entry
-// puts it in an aligned register pair) says that candidates base + b of the
-// source (source-local indices, b = 0..63) with bit b set are partners.
-// Entries follow the cell walk (strips in order, ascending candidates), so
-// popping the bits in order visits the partners in the walk's order.  Entry e
-// of target i sits at lidx[L.list_off + 4·(e·L.cap + i)]: a lane writes one
-// 16-byte entry per (strip, 64 candidates) that holds a partner, and a
-// wavefront reads a row of entries as a contiguous 1 KiB.
+// order the sources are listed, as acceptance masks: an entry, a 64-bit mask
+// {lo, hi} and a base, says that candidates base + b of the source
+// (source-local indices, b = 0..63) with bit b set are partners.  Entries
+// follow the cell walk (strips in order, ascending candidates), so popping the
+// bits in order visits the partners in the walk's order.  The masks and the
+// bases are two arrays: entry e of target i has its mask at
+// lidx[L.list_off + 2·(e·L.cap + i)] (an int2) and its base at
+// lidx[L.list_off + 2·L.maxnb·L.cap + e·L.cap + i]; a lane writes one
+// 12-byte entry per (strip, 64 candidates) that holds a partner, and a
+// wavefront reads a row of entries as 512 and 256 contiguous bytes.
 // lcnt[L.cnt_off + q·L.cap + i] holds the number of entries of source q.
 // More than L.maxnb entries mean "overflow" (the physics pass then walks the
 // cells directly).
@@ -732,7 +733,8 @@ __global__ void __launch_bounds__(64)
   double xi[D];
 #pragma unroll
   for (int d = 0; d < D; ++d) xi[d] = P.x[d][gi];
-  int4 *ent = reinterpret_cast<int4 *>(lidx + L.list_off) + i;
+  int2 *emask = reinterpret_cast<int2 *>(lidx + L.list_off) + i;
+  int *ebase = lidx + L.list_off + 2 * (int64_t)L.maxnb * L.cap + i;
   int ne = 0;  // entries of this target
   int cq = 0;  // entries of the current source
   for (int q = L.src_begin; q < L.src_end; ++q) {
@@ -789,7 +791,10 @@ __global__ void __launch_bounds__(64)
           lo &= sdl < 32u ? ~sbit : ~0u;
           hi &= sdl - 32u < 32u ? ~sbit : ~0u;
           if (live && (lo | hi) != 0u) {
-            if (ne < L.maxnb) ent[(int64_t)ne * L.cap] = make_int4((int)lo, (int)hi, base, 0);
+            if (ne < L.maxnb) {
+              emask[(int64_t)ne * L.cap] = make_int2((int)lo, (int)hi);
+              ebase[(int64_t)ne * L.cap] = base;
+            }
             ++ne;
             ++cq;
           }
@@ -912,7 +917,8 @@ __global__ void __launch_bounds__(GR_BLOCK)
   if (overflow) {
     gather_direct<D, R, FAM>(Gt, P, sd, T, srcs, grids, carr, cells, i, ns, no);
   } else if (live) {
-    const int4 *ent = reinterpret_cast<const int4 *>(lidx + L.list_off) + i;
+    const int2 *emask = reinterpret_cast<const int2 *>(lidx + L.list_off) + i;
+    const int *ebase = lidx + L.list_off + 2 * (int64_t)L.maxnb * L.cap + i;
     for (int q = T.src_begin; q < T.src_end; ++q) {
       const SrcDesc Q = srcs[q];
       const int64_t poff = sd[Q.partner].off;
@@ -926,15 +932,22 @@ __global__ void __launch_bounds__(GR_BLOCK)
       const bool nb = needs_b<FAM>(Q);
       unsigned long long m = 0ull;
       int base = 0;
-      int4 epre = make_int4(0, 0, 0, 0);
-      if (e < eend) epre = ent[(int64_t)e * L.cap];
+      int2 mpre = make_int2(0, 0);
+      int bpre = 0;
+      if (e < eend) {
+        mpre = emask[(int64_t)e * L.cap];
+        bpre = ebase[(int64_t)e * L.cap];
+      }
       auto next = [&](int &j) -> bool {
         if (m == 0ull) {  // next entry (entries hold at least one partner)
           if (e >= eend) return false;
-          base = epre.z;
-          m = ((unsigned long long)(unsigned)epre.y << 32) | (unsigned)epre.x;
+          base = bpre;
+          m = ((unsigned long long)(unsigned)mpre.y << 32) | (unsigned)mpre.x;
           ++e;
-          if (e < eend) epre = ent[(int64_t)e * L.cap];
+          if (e < eend) {
+            mpre = emask[(int64_t)e * L.cap];
+            bpre = ebase[(int64_t)e * L.cap];
+          }
         }
         j = base + __builtin_ctzll(m);
         m &= m - 1ull;

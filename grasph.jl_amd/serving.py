@@ -80,7 +80,7 @@ def footprint_bytes(cfg, precision="f32", maxnb=None):
     lf = cfg["integrator"]
     D = lf.systems[0].ndims
     r = 4 if precision == "f32" else 8
-    maxnb = maxnb or (192 if D == 3 else 96)
+    maxnb = maxnb or (144 if D == 3 else 72)  # list words: 48 / 24 entries of 12 B
     per = 0
     n_real = 0
     for ps in lf.systems:
