@@ -1379,6 +1379,7 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
           zeroed = true;
     if (!zeroed) {
       c->stepsys[s].reset_force = 0;
+      c->stepsys[s].perm_skip |= 1;
       tasks[first * NFAM + firstfam][s].t.assign = 1;
     }
   }
@@ -1403,6 +1404,26 @@ void plan_stages(gr_ctx *c, Plan &P, bool count_pairs_unused) {
                 L.end());
       }
     }
+  // strain rate and vorticity are dead from the sort to the first strain-rate
+  // stage of a system when that stage assigns them and no updater up to it
+  // reads them (the stress updaters run before the stage's sweeps)
+  for (int s = 0; s < ns; ++s) {
+    if (sys_integ[s] < 0 || c->sys[s].kind == GR_SYS_GHOST) continue;
+    for (int k = 0; k < nst; ++k) {
+      auto it = tasks[k * NFAM + FAM_STRAIN].find(s);
+      bool reads = false;
+      if (k < (int)upd[s].size())
+        for (auto &u : upd[s][k])
+          reads = reads || u.kind == GR_UPD_EP || u.kind == GR_UPD_HOOKE || u.kind == GR_UPD_VPMC;
+      if (reads) break;
+      if (it == tasks[k * NFAM + FAM_STRAIN].end()) continue;
+      if (it->second.t.assign) {
+        c->stepsys[s].perm_skip |= 2;
+        if (it->second.t.write_vor) c->stepsys[s].perm_skip |= 4;
+      }
+      break;
+    }
+  }
   // neighbour lists: one per target, over the union of its sources
   std::map<int, std::vector<SrcDesc>> tl;  // target → list sources
   for (int k = 0; k <= nst; ++k)

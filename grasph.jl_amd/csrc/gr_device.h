@@ -60,7 +60,11 @@ struct StepSys {
   int32_t fuse_eos;       // 0 none, GR_UPD_TAIT, GR_UPD_LINEAR
   double eos_rho0, eos_gamma;
   int32_t damping;        // Γ != 0
-  int32_t pad;
+  // fields the step overwrites before reading them, which the sort's
+  // permutation with the half step need not move: 1 dvdt and drhodt (the
+  // first force stage assigns them), 2 strain rate, 4 vorticity (the first
+  // strain-rate stage assigns them and no updater before it reads them)
+  int32_t perm_skip;
 };
 
 struct Bnd {  // ghost boundary plane

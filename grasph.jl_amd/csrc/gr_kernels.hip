@@ -584,6 +584,9 @@ __global__ void k_permute_real(Pool<R> A, Pool<R> B, const SysDesc *sd, const Bl
   int64_t src = dst;
   if (flags[0] != 0 && bbase[s] >= 0) src = radix_sorted_vals(vals, vals2, flags)[bbase[s] + i];
   const bool half = HALF && halfsys[s] != 0;
+  // fields this step overwrites before reading them are not moved (only with
+  // the half step, which has read the old rates here)
+  const int skip = half ? st[s].perm_skip : 0;
   if (half) {
     const StepSys T = st[s];
     const double cf = 0.5 * T.dt;
@@ -598,7 +601,7 @@ __global__ void k_permute_real(Pool<R> A, Pool<R> B, const SysDesc *sd, const Bl
       B.x[d][dst] = xn[d];
       B.v[d][dst] = v;
       B.vs[d][dst] = vsn[d];
-      B.dvdt[d][dst] = T.reset_force ? (R)S.sv[d] : a;
+      if (!(skip & 1)) B.dvdt[d][dst] = T.reset_force ? (R)S.sv[d] : a;
       if (vadj) B.vadj[d][dst] = A.vadj[d][src];
     }
     const double rho = A.rho[src];
@@ -607,7 +610,7 @@ __global__ void k_permute_real(Pool<R> A, Pool<R> B, const SysDesc *sd, const Bl
     B.rho[dst] = rho;
     B.rho_h[dst] = rh;
     B.rhos[dst] = (R)rh;
-    B.drhodt[dst] = T.reset_force ? (R)S.srho : dr;
+    if (!(skip & 1)) B.drhodt[dst] = T.reset_force ? (R)S.srho : dr;
     R pn = (R)0;
     if (T.fuse_eos) {
       pn = (R)eos_p(T.fuse_eos, rh, T.eos_rho0, T.eos_gamma, S.c);
@@ -656,13 +659,13 @@ __global__ void k_permute_real(Pool<R> A, Pool<R> B, const SysDesc *sd, const Bl
         const double sg = A.stress[k][src];
         B.stress[k][dst] = sg;
         B.stress_s[k][dst] = (R)sg;
-        B.sr[k][dst] = A.sr[k][src];
+        if (!(skip & 2)) B.sr[k][dst] = A.sr[k][src];
       }
   }
   if (S.kind == GR_SYS_EP) {
 #pragma unroll
     for (int k = 0; k < 3; ++k)
-      if (k < S.nv) B.vor[k][dst] = A.vor[k][src];
+      if (k < S.nv && !(skip & 4)) B.vor[k][dst] = A.vor[k][src];
 #pragma unroll
     for (int k = 0; k < GR_MAXNS; ++k)
       if (k < S.ns) {
