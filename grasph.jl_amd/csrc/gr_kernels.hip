@@ -1733,14 +1733,12 @@ __global__ void k_pack(Pool<R> P, const SysDesc *sd, const int *n_dev, const Blk
       for (int k = 0; k < GR_MAXNS; ++k)
         if (k < S.ns) sv[k] = (float)P.stress_s[k][g];
     }
-    if constexpr (D == 3) {
-      RecS rs;
-      rs.a = make_float4(sv[0], sv[1], sv[2], sv[3]);
-      rs.b = make_float4(sv[4], sv[5], sv[6], sv[7]);
-      P.rs[g] = rs;
-    } else {  // NS <= 4 in 2D: the sweeps never read the second half
-      P.rs[g].a = make_float4(sv[0], sv[1], sv[2], sv[3]);
-    }
+    // the whole record also in 2D, where the sweeps read only its first half:
+    // writing half of every 32-byte record measured slower (partial lines)
+    RecS rs;
+    rs.a = make_float4(sv[0], sv[1], sv[2], sv[3]);
+    rs.b = make_float4(sv[4], sv[5], sv[6], sv[7]);
+    P.rs[g] = rs;
   }
 }
 
