@@ -333,7 +333,7 @@ def main():
         for kv in args.shapes.split(","):
             k, v = kv.split("=")
             be.tune_set(knob[k.strip()], int(v))
-    tune = (g.autotune.autotune(s, [lf], steps=3, reps=2)
+    tune = (g.autotune.autotune(s, [lf], steps=5, reps=3, passes=2)
             if args.autotune and not args.shapes else None)
     progress("3D dam break: tuned; timed steps")
     # warm-up (graph capture)

@@ -16,10 +16,14 @@ that matter per scene are (include/grasph_hip.h, GR_TUNE_*):
 None of them changes a result (bit-identical lists, sums and states), so
 the tuner only times.  It runs a coordinate descent: for each knob in turn,
 every candidate value is timed over a few steps of the caller's own
-integrators (HIP events around each step, the whole step's mean), the best
-value is kept, and the next knob is tried with it.  The simulations' state
-is saved to the host before the trials and restored afterwards, so tuning
-does not advance them.
+integrators (the wall time of the steps as they will run), the best
+value is kept, and the next knob is tried with it.  `passes` > 1 runs the
+descent again from the values found, since the knobs interact (the best
+workgroup size of the gather depends on the lists the search writes).  The
+simulations' state
+is saved to the host before the trials, restored before each trial (so
+every shape is timed on the same state) and afterwards, so tuning does not
+advance them.
 """
 from __future__ import annotations
 
@@ -41,15 +45,16 @@ def default_space(ndims):
 
 
 def _time_steps(be, integs, steps, reps):
-    """Mean ms per step (HIP events around each step), best of `reps`."""
+    """Mean ms per step, best of `reps`: the wall time of `steps` steps in
+    the context's own mode (graph replay when it is on; the call returns
+    with the stream synchronised), as the steps will run.  Timing each phase
+    with events instead launches eagerly and ranked the shapes differently."""
     best = float("inf")
-    be.step_many(integs, 1)  # plan and lists for the current shape
+    be.step_many(integs, 1)  # plan, lists and graph for the current shape
     for _ in range(reps):
-        be.set_timing(1)
+        t0 = time.perf_counter()
         be.step_many(integs, steps)
-        ms = be.get_timing()[7]
-        be.set_timing(0)
-        best = min(best, ms)
+        best = min(best, (time.perf_counter() - t0) / steps * 1e3)
     return best
 
 
@@ -57,7 +62,7 @@ def current(be):
     return {KNOB_NAMES[k]: be.tune_get(k) for k in KNOB_NAMES}
 
 
-def autotune(session, integrators, steps=3, reps=2, space=None, restore=True):
+def autotune(session, integrators, steps=3, reps=2, space=None, restore=True, passes=1):
     """Pick the launch shapes for `integrators` (host LeapFrogTimeIntegrator
     objects registered on `session`'s scene) and leave them set on the
     context.  Returns a record: the chosen values, the default's and the
@@ -70,21 +75,31 @@ def autotune(session, integrators, steps=3, reps=2, space=None, restore=True):
             sc.download_all(lf)
     space = space or default_space(sc.ndims)
     start = current(be)
-    ms_start = _time_steps(be, hs, steps, reps)
+
+    def timed():
+        # every trial starts from the state the tuner found (the flow would
+        # otherwise move on between trials and change what is timed)
+        if restore:
+            for lf in integrators:
+                sc.upload_all(lf)
+        return _time_steps(be, hs, steps, reps)
+
+    ms_start = timed()
     trials = []
-    for knob, values in space:
-        best_v, best_ms = be.tune_get(knob), None
-        for v in values:
-            be.tune_set(knob, v)
-            ms = _time_steps(be, hs, steps, reps)
-            trials.append([KNOB_NAMES[knob], int(v), ms])
-            if best_ms is None or ms < best_ms:
-                best_v, best_ms = v, ms
-        be.tune_set(knob, best_v)
-    ms_tuned = _time_steps(be, hs, steps, reps)
+    for _ in range(max(1, int(passes))):
+        for knob, values in space:
+            best_v, best_ms = be.tune_get(knob), None
+            for v in values:
+                be.tune_set(knob, v)
+                ms = timed()
+                trials.append([KNOB_NAMES[knob], int(v), ms])
+                if best_ms is None or ms < best_ms:
+                    best_v, best_ms = v, ms
+            be.tune_set(knob, best_v)
+    ms_tuned = timed()
     if restore:
         for lf in integrators:
             sc.upload_all(lf)
     return {"chosen": current(be), "start": start, "ms_per_step_start": ms_start,
             "ms_per_step_tuned": ms_tuned, "trials": trials, "steps_per_trial": steps,
-            "reps": reps, "tuning_s": time.perf_counter() - t0}
+            "reps": reps, "passes": max(1, int(passes)), "tuning_s": time.perf_counter() - t0}

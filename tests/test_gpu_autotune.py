@@ -10,7 +10,7 @@ pytestmark = pytest.mark.gpu
 A = M._abi
 
 
-def _run(builder, kw, D, nsteps, knobs=(), tune=False):
+def _run(builder, kw, D, nsteps, knobs=(), tune=False, passes=1):
     cfg = builder(**kw)
     lf = cfg["integrator"]
     s = M.Session(D, "f32")
@@ -22,7 +22,7 @@ def _run(builder, kw, D, nsteps, knobs=(), tune=False):
         s.backend.tune_set(k, v)
     rec = None
     if tune:
-        rec = M.autotune.autotune(s, [lf], steps=2, reps=1)
+        rec = M.autotune.autotune(s, [lf], steps=2, reps=1, passes=passes)
     s.backend.step_many([h], nsteps)
     s.scene.download_all(lf)
     f = cfg["fluid"]
@@ -57,3 +57,12 @@ def test_autotune_records_a_choice_and_leaves_the_state_alone():
     assert len(rec["trials"]) == 12 and rec["ms_per_step_tuned"] > 0
     with pytest.raises(ValueError):
         M.Session(3, "f32").backend.tune_set(A.GR_TUNE_NL_WIN, 100)
+
+
+def test_autotune_second_pass_leaves_the_state_alone():
+    ref, _ = _run(M.configs.ep_column_2d, dict(dx=0.005, width=0.1, height=0.06), 2, 10)
+    got, rec = _run(M.configs.ep_column_2d, dict(dx=0.005, width=0.1, height=0.06), 2, 10,
+                    tune=True, passes=2)
+    for f, v in ref.items():
+        assert np.array_equal(got[f], v), f
+    assert rec["passes"] == 2 and len(rec["trials"]) == 24
