@@ -29,11 +29,12 @@ def _run(builder, kw, D, nsteps, knobs=(), tune=False, passes=1):
     return {k: getattr(f, k).copy() for k in ["id"] + f.field_names()}, rec
 
 
-CASES = [(M.configs.dambreak_3d, dict(dx=0.03), 3), (M.configs.ep_column_2d,
-                                                      dict(dx=0.005, width=0.1, height=0.06), 2)]
+CASES = [(M.configs.dambreak_3d, dict(dx=0.03), 3),
+         (M.configs.ep_column_2d, dict(dx=0.005, width=0.1, height=0.06), 2),
+         (M.configs.granular_column_3d, dict(dx=0.02, width=0.12, depth=0.06, height=0.08), 3)]
 
 
-@pytest.mark.parametrize("builder,kw,D", CASES, ids=["dambreak3d", "ep_column2d"])
+@pytest.mark.parametrize("builder,kw,D", CASES, ids=["dambreak3d", "ep_column2d", "vpmc3d"])
 def test_every_launch_shape_gives_the_same_state(builder, kw, D):
     ref, _ = _run(builder, kw, D, 12)
     shapes = [((A.GR_TUNE_SWEEP_BLOCK, 64),), ((A.GR_TUNE_SWEEP_BLOCK, 128),),
