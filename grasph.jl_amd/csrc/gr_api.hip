@@ -412,7 +412,10 @@ struct gr_ctx {
   int use_lists = 1;
   int list_maxnb = 0;  // mask entries kept per target (0: 48 in 3D, 24 in 2D)
   // launch shapes chosen by gr_tune_set (results do not depend on them)
-  int tune_win = 128, tune_span = 2, tune_block = GR_BLOCK, tune_inv = 1, tune_fuse = 1;
+  // default shapes: nl_win 256 and span 4, the autotuner's choice for the dam
+  // break and the 2D column ensemble, 2–3 % faster than 128 and 2 on batches
+  // of 2D and 3D columns
+  int tune_win = 256, tune_span = 4, tune_block = GR_BLOCK, tune_inv = 1, tune_fuse = 1;
   std::map<std::vector<int>, std::unique_ptr<Plan>> plans;
   // cached device tables derived from the layout
   DBuf allreal_blk;

@@ -306,11 +306,11 @@ int gr_set_graph_mode(gr_ctx *ctx, int on);
  * order and every output are bit-identical for all of them. */
 enum {
   GR_TUNE_NL_WIN = 0,       /* candidates staged in LDS per window of the
-                               neighbour search: 64, 128 (default), 256,
+                               neighbour search: 64, 128, 256 (default),
                                512                                          */
   GR_TUNE_NL_SPAN = 1,      /* lanes whose candidate ranges start within
                                span × the widest range share one staging
-                               pass: 1..16 (default 2)                      */
+                               pass: 1..16 (default 4)                      */
   GR_TUNE_SWEEP_BLOCK = 2,  /* target particles per workgroup of the
                                functor gather: 64, 128, 256 (default)       */
   GR_TUNE_MAXNB = 3,        /* list entries kept per target before it walks
