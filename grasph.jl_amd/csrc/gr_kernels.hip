@@ -255,6 +255,9 @@ __global__ void k_bb_fold(const int *list, int nlist, long long *bbmin, long lon
   if (t < nlist * 3) bb_fold(bbmin, bbmax, rs, list[t / 3] * 3 + t % 3);
 }
 
+// Threads of k_key_base: one per listed system in each round.
+#define KB_THREADS 1024
+
 // Key space of each listed system: lexicographic linearisation of
 // (k1 - kmin1, ..., kD - kminD), bases prefix-summed in list order.  Ghost
 // systems reserve one extra key for padding slots.  flags[0] (the inversion
@@ -265,17 +268,18 @@ __global__ void k_bb_fold(const int *list, int nlist, long long *bbmin, long lon
 __device__ inline double unord_d(long long o) {
   return __longlong_as_double(o < 0 ? (o ^ 0x7FFFFFFFFFFFFFFFLL) : o);
 }
-__global__ void k_key_base(const int *list, int nlist, int D, int pad, const int *n_dev,
-                           const StepSys *st, long long *bbmin, long long *bbmax, int rs,
-                           int *kmin, int *kmax, unsigned *kbase, unsigned *krange, int *flags,
-                           int *sysfault) {
-  // one wavefront: the systems' key ranges in parallel, 64 at a time, and
-  // their exclusive prefix sum (each system's keys start after the ranges of
-  // the systems listed before it, plus one padding key each when `pad`)
-  const int lane = threadIdx.x;
+__global__ void __launch_bounds__(KB_THREADS)
+    k_key_base(const int *list, int nlist, int D, int pad, const int *n_dev, const StepSys *st,
+               long long *bbmin, long long *bbmax, int rs, int *kmin, int *kmax,
+               unsigned *kbase, unsigned *krange, int *flags, int *sysfault) {
+  // one workgroup: the systems' key ranges in parallel, KB_THREADS at a time,
+  // and their exclusive prefix sum (each system's keys start after the ranges
+  // of the systems listed before it, plus one padding key each when `pad`)
+  __shared__ unsigned long long wsum[KB_THREADS / 64];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   unsigned long long carry = 0;
-  for (int q0 = 0; q0 < nlist; q0 += 64) {
-    const int q = q0 + lane;
+  for (int q0 = 0; q0 < nlist; q0 += KB_THREADS) {
+    const int q = q0 + t;
     unsigned long long range = 0;
     int s = -1;
     if (q < nlist) {
@@ -327,10 +331,18 @@ __global__ void k_key_base(const int *list, int nlist, int D, int pad, const int
       const unsigned long long u = __shfl_up(inc, o, 64);
       if (lane >= o) inc += u;
     }
-    if (q < nlist) kbase[s] = (unsigned)(carry + inc - range);
-    carry += __shfl(inc, 63, 64);
+    if (lane == 63) wsum[w] = inc;
+    __syncthreads();
+    unsigned long long off = 0, tot = 0;
+    for (int v = 0; v < KB_THREADS / 64; ++v) {
+      off += v < w ? wsum[v] : 0ull;
+      tot += wsum[v];
+    }
+    if (q < nlist) kbase[s] = (unsigned)(carry + off + inc - range);
+    carry += tot;
+    __syncthreads();  // wsum is written again for the next systems
   }
-  if (lane == 0) {
+  if (t == 0) {
     flags[0] = 0;
     // the running base only grows, so overflowing at some system means the
     // total overflows
@@ -1773,8 +1785,8 @@ void launch_minmax_reset(hipStream_t s, const int *list, int nlist, long long *b
 void launch_key_base(hipStream_t s, const int *list, int nlist, int D, int pad, const int *n,
                      const StepSys *st, long long *bbmin, long long *bbmax, int rs, int *kmin,
                      int *kmax, unsigned *kbase, unsigned *krange, int *flags, int *sysfault) {
-  k_key_base<<<1, 64, 0, s>>>(list, nlist, D, pad, n, st, bbmin, bbmax, rs, kmin, kmax, kbase,
-                              krange, flags, sysfault);
+  k_key_base<<<1, KB_THREADS, 0, s>>>(list, nlist, D, pad, n, st, bbmin, bbmax, rs, kmin, kmax,
+                                      kbase, krange, flags, sysfault);
 }
 void launch_key_write(hipStream_t s, XP xs, const SysDesc *sd, const int *n, const StepSys *st,
                       const Blk *blk, int nblk, const int *bbase, int D, const int *kmin,
