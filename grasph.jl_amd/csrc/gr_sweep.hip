@@ -217,6 +217,9 @@ struct Gather {
 
   // One accepted pair (target i, partner j at pool index gj), dx = x_i − x_j
   // and r² as evaluated by _pair_self!/_pair_coupled! (src/Interaction.jl:546-553).
+  // DYN: 1 or 0 when the caller knows whether the source's partner is a
+  // dynamic boundary (Q.pk == PK_DYNAMIC), -1 to test it here.
+  template <int DYN = -1>
   __device__ __forceinline__ void pair(const Pool<R> P, const SrcDesc &Q, const SrcK<R> &K,
                                        int64_t gj, const RecA<R> &Aj, const RecB<R> &Bj,
                                        const RecC &Cj, const RecS &Sj, const double *dx,
@@ -305,7 +308,8 @@ struct Gather {
     }
     R vj[D], rj, pj = (R)0;
     const R m = K.m_q;
-    if (Q.pk == PK_DYNAMIC) {  // DynamicBoundarySystem: mirrored velocity
+    const bool dyn = DYN < 0 ? Q.pk == PK_DYNAMIC : DYN == 1;
+    if (dyn) {  // DynamicBoundarySystem: mirrored velocity
       double db;
       if constexpr (COMPACT) {  // (x_j − p)·n = (x_i − p)·n − (x_i − x_j)·n
         db = da;
@@ -336,7 +340,7 @@ struct Gather {
     }
     R irj;
     if constexpr (INV)  // the record's 1/ρ (a dynamic partner mirrors the target)
-      irj = Q.pk == PK_DYNAMIC ? iri : rj;
+      irj = dyn ? iri : rj;
     else  // a dynamic partner has rj = ri, so this is iri bit for bit (a select
           // of the two made the compiler keep them in scratch memory)
       irj = Fast<R>::rcp(rj);
@@ -407,7 +411,7 @@ struct Gather {
       const R pa = Q.psel == 2 ? pj : pi, pb = Q.psel == 1 ? pi : pj;
       R dh;
       if constexpr (INV) {  // S = 2; the record's p/ρ² for a real partner
-        dh = -(pa * (iri * iri) + (Q.pk == PK_DYNAMIC ? pb * (irj * irj) : pj));
+        dh = -(pa * (iri * iri) + (dyn ? pb * (irj * irj) : pj));
       } else if (FA) {
         if (K.sigma == 1)
           dh = -(pa + pb) * (iri * irj);
@@ -436,7 +440,7 @@ struct Gather {
       adr.add(m * (dr * coef + (FA ? psi * irj : psi / rj)));
     } else {  // cauchy_stress_force
       R sj[GR_MAXNS];
-      if (Q.pk == PK_DYNAMIC) {
+      if (dyn) {
 #pragma unroll
         for (int k = 0; k < GR_MAXNS; ++k) sj[k] = si[k];
       } else {
@@ -977,7 +981,7 @@ __global__ void __launch_bounds__(GR_BLOCK)
           if (nb) Bn = P.rb[poff + jn];
         }
       };
-      auto one = [&](int64_t gj, const RecA<R> &Aj, const RecB<R> &Bj, const RecC &Cj,
+      auto one = [&](auto dk, int64_t gj, const RecA<R> &Aj, const RecB<R> &Bj, const RecC &Cj,
                      const RecS &Sj) {
         double dx[D] = {};
         double r2 = 0.0;
@@ -994,35 +998,50 @@ __global__ void __launch_bounds__(GR_BLOCK)
           else
             ++no;
         }
-        Gt.pair(P, Q, K, gj, Aj, Bj, Cj, Sj, dx, r2, da);
+        Gt.template pair<decltype(dk)::value>(P, Q, K, gj, Aj, Bj, Cj, Sj, dx, r2, da);
       };
-      if constexpr (PREFETCH) {
-        // the next partner's records are loaded while the current pair is
-        // evaluated (the copy out of the prefetch registers costs a move per
-        // word)
-        bool have = next(jn);
-        if (have) fetch();
-        while (have) {
-          const int64_t gj = poff + jn;
-          const RecA<R> Aj = An;
-          const RecB<R> Bj = Bn;
-          const RecC Cj = Cn;
-          const RecS Sj = Sn;
-          // the next record is loaded whether or not there is a next
-          // partner (jn then still names this one): a conditional load
-          // would make the compiler keep a copy of the buffer for the path
-          // without it
-          have = next(jn);
-          fetch();
-          one(gj, Aj, Bj, Cj, Sj);
+      auto run = [&](auto dk) {
+        if constexpr (PREFETCH) {
+          // the next partner's records are loaded while the current pair is
+          // evaluated (the copy out of the prefetch registers costs a move per
+          // word; two buffers used in turn measured no faster, the compiler
+          // still copied one of them)
+          bool have = next(jn);
+          if (have) fetch();
+          while (have) {
+            const int64_t gj = poff + jn;
+            const RecA<R> Aj = An;
+            const RecB<R> Bj = Bn;
+            const RecC Cj = Cn;
+            const RecS Sj = Sn;
+            // the next record is loaded whether or not there is a next
+            // partner (jn then still names this one): a conditional load
+            // would make the compiler keep a copy of the buffer for the path
+            // without it
+            have = next(jn);
+            fetch();
+            one(dk, gj, Aj, Bj, Cj, Sj);
+          }
+        } else {
+          // loaded at the top of each iteration: the other wavefronts of the
+          // SIMD hide the latency, and no registers are copied
+          while (next(jn)) {
+            fetch();
+            one(dk, poff + jn, An, Bn, Cn, Sn);
+          }
         }
+      };
+      // the prefetching pair loop is compiled once for sources whose partner
+      // is a dynamic boundary and once for the others (dk), so that the
+      // pair's selects on the partner kind leave the loop (measured: 1 % off
+      // the 3D sweeps; the 2D sweeps without prefetch gained nothing)
+      if constexpr (PREFETCH && (Gather<D, R, FAM>::FORCE || FAM == FAM_STRAIN)) {
+        if (Q.pk == PK_DYNAMIC)
+          run(std::integral_constant<int, 1>{});
+        else
+          run(std::integral_constant<int, 0>{});
       } else {
-        // loaded at the top of each iteration: the other wavefronts of the
-        // SIMD hide the latency, and no registers are copied
-        while (next(jn)) {
-          fetch();
-          one(poff + jn, An, Bn, Cn, Sn);
-        }
+        run(std::integral_constant<int, -1>{});
       }
     }
   }
